@@ -1,0 +1,426 @@
+"""CPU oracle for the dense-disparity path -- TEST INFRASTRUCTURE ONLY.
+
+A float64 C restatement of the reference (stereonet.py) lives in
+``stereo_oracle.c``; this module builds it with gcc and exposes numpy
+wrappers.  Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+leg may import this package, and only as the checker or the timed CPU
+baseline; the shipped library never calls it.
+
+Parity status of each function (which check pins it) is listed in the C
+file's header and in the tests (tests/test_oracle.py).
+"""
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "stereo_oracle.c")
+LIB = os.path.join(HERE, "libstereo_oracle.so")
+
+_lib = None
+
+
+def build(force=True):
+    """Compile the oracle with gcc (binary64, no contraction)."""
+    if not force and os.path.exists(LIB) and os.path.getmtime(LIB) >= os.path.getmtime(SRC):
+        return LIB
+    cmd = ["gcc", "-O2", "-ffp-contract=off", "-fno-fast-math", "-fPIC",
+           "-shared", "-o", LIB + ".tmp", SRC, "-lm"]
+    subprocess.check_call(cmd)
+    os.replace(LIB + ".tmp", LIB)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(SRC):
+            build()
+        _lib = ctypes.CDLL(LIB)
+        _declare(_lib)
+    return _lib
+
+
+P = ctypes.c_void_p
+I = ctypes.c_int
+L_ = ctypes.c_long
+D_ = ctypes.c_double
+
+
+def _declare(l):
+    sig = {
+        "or_census": (I, [P, I, I, I, I, P]),
+        "or_band_costs": (None, [P, P, P, P, P, I, I, I, D_, D_, D_, P, D_, D_, P]),
+        "or_aggregate_path": (None, [P, P, I, I, I, D_, D_, I, I, P]),
+        "or_refine_fullres": (I, [P, P, P, I, I, I, D_, D_, D_, D_, I, D_, I, I, D_, D_, P, P]),
+        "or_left_to_right": (None, [P, I, I, P]),
+        "or_lr_refine": (I, [P, P, P, I, I, D_, I, D_, D_, D_, D_, I, D_, I, I, D_, D_, P, P]),
+        "or_median_filter": (None, [P, I, I, I, I, P]),
+        "or_refine_disparity_plain": (I, [P, P, P, I, I, I, D_, D_, I, I, D_, P, P]),
+        "or_conv2d_fwd": (None, [P, I, I, I, P, P, I, I, I, I, I, P]),
+        "or_conv2d_bwd": (None, [P, I, I, I, P, I, I, I, I, I, P, P, P, P]),
+        "or_conv3d_fwd": (None, [P, I, I, I, I, P, P, I, I, P]),
+        "or_conv3d_bwd": (None, [P, I, I, I, I, P, I, I, P, P, P, P]),
+        "or_costvol_fwd": (None, [P, P, I, I, I, I, P]),
+        "or_costvol_bwd": (None, [P, P, I, I, I, I, P, P, P]),
+        "or_softargmin_fwd": (None, [P, I, I, I, P, P, P]),
+        "or_softargmin_bwd": (None, [P, P, I, I, I, P, P]),
+        "or_bilinear_fwd": (None, [P, I, I, I, I, I, P]),
+        "or_bilinear_bwd": (None, [P, I, I, I, I, I, P]),
+        "or_avgpool2": (None, [P, I, I, I, P]),
+        "or_pseudo_huber": (D_, [P, P, P, L_, D_, P, P]),
+        "or_disparity_ce": (D_, [P, P, P, I, I, I, D_, P, P]),
+        "or_adam_step": (None, [P, P, P, P, L_, D_, D_, D_, D_, D_, D_, D_]),
+        "or_net_num_params": (L_, [I, I, I, I, I, P]),
+        "or_net_forward": (None, [I, I, I, I, I, P, P, P, P, I, I, P, P, P, P, P, P, P]),
+        "or_train_grads": (D_, [I, I, I, I, I, P, P, P, P, P, P, I, I, P, P]),
+        "or_predict": (None, [I, I, I, I, I, P, P, P, P, I, I, P]),
+        "or_peak": (D_, [P, L_, D_]),
+        "or_predict_at_scale": (None, [I, I, I, I, I, P, P, P, P, I, I, I, P, P, P]),
+        "or_predict_auto": (None, [I, I, I, I, I, P, P, D_, P, P, I, I, I, I, I, P, P, P, P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(l, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def _f64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+# ----------------------------------------------------------------- refinement
+
+def census(img, W, H, rx=2, ry=2):
+    img = _f64(img).ravel()
+    out = np.zeros(W * H, dtype=np.uint64)
+    if lib().or_census(_p(img), W, H, rx, ry, _p(out)) != 0:
+        raise ValueError("census window wider than 64 bits")
+    return out
+
+
+def band_costs(cl, cr, gl, gr, base, W, H, R, ad_weight, ad_trunc, invalid,
+               prior=None, prior_w=0.0, prior_t=4.0):
+    cl = np.ascontiguousarray(cl, dtype=np.uint64)
+    cr = np.ascontiguousarray(cr, dtype=np.uint64)
+    gl, gr = _f64(gl).ravel(), _f64(gr).ravel()
+    base = np.ascontiguousarray(base, dtype=np.int32)
+    prior = _f64(prior).ravel() if prior is not None else np.zeros(W * H)
+    C = np.zeros(W * H * (2 * R + 1))
+    lib().or_band_costs(_p(cl), _p(cr), _p(gl), _p(gr), _p(base), W, H, R,
+                        ad_weight, ad_trunc, invalid, _p(prior), prior_w,
+                        prior_t, _p(C))
+    return C
+
+
+def aggregate_path(C, base, W, H, K, P1, P2, dx, dy, S=None):
+    C = _f64(C).ravel()
+    base = np.ascontiguousarray(base, dtype=np.int32)
+    S = np.zeros(W * H * K) if S is None else S
+    lib().or_aggregate_path(_p(C), _p(base), W, H, K, P1, P2, dx, dy, _p(S))
+    return S
+
+
+def refine_fullres(left, right, disp0, W, H, R=3, P1=1.0, P2=12.0,
+                   ad_weight=0.0, ad_trunc=20.0, paths=8, uniq=0.0, rx=2,
+                   ry=2, prior_w=0.0, prior_t=4.0, return_conf=False):
+    left, right, disp0 = _f64(left).ravel(), _f64(right).ravel(), _f64(disp0).ravel()
+    out = np.zeros(W * H)
+    conf = np.zeros(W * H)
+    rc = lib().or_refine_fullres(_p(left), _p(right), _p(disp0), W, H, R, P1,
+                                 P2, ad_weight, ad_trunc, paths, uniq, rx, ry,
+                                 prior_w, prior_t, _p(out), _p(conf))
+    if rc != 0:
+        raise ValueError("census window wider than 64 bits")
+    return (out, conf) if return_conf else out
+
+
+def left_to_right(dl, W, H):
+    dl = _f64(dl).ravel()
+    dr = np.zeros(W * H)
+    lib().or_left_to_right(_p(dl), W, H, _p(dr))
+    return dr
+
+
+def lr_refine(left, right, disp0, W, H, tol=1.0, R=3, P1=1.0, P2=12.0,
+              ad_weight=0.0, ad_trunc=20.0, paths=8, uniq=0.0, rx=2, ry=2,
+              prior_w=0.0, prior_t=4.0):
+    left, right, disp0 = _f64(left).ravel(), _f64(right).ravel(), _f64(disp0).ravel()
+    out = np.zeros(W * H)
+    status = np.zeros(W * H, dtype=np.int32)
+    rc = lib().or_lr_refine(_p(left), _p(right), _p(disp0), W, H, tol, R, P1,
+                            P2, ad_weight, ad_trunc, paths, uniq, rx, ry,
+                            prior_w, prior_t, _p(out), _p(status))
+    if rc != 0:
+        raise ValueError("census window wider than 64 bits")
+    return out, status
+
+
+def median_filter(disp, W, H, radius=1, passes=1):
+    disp = _f64(disp).ravel()
+    out = np.zeros(W * H)
+    lib().or_median_filter(_p(disp), W, H, radius, passes, _p(out))
+    return out
+
+
+def refine_disparity_plain(left, right, disp0, W, H, R=10, P1=12.0, P2=150.0,
+                           rx=4, ry=3, tol=2.0):
+    left, right, disp0 = _f64(left).ravel(), _f64(right).ravel(), _f64(disp0).ravel()
+    out = np.zeros(W * H)
+    status = np.zeros(W * H, dtype=np.int32)
+    rc = lib().or_refine_disparity_plain(_p(left), _p(right), _p(disp0), W, H,
+                                         R, P1, P2, rx, ry, tol, _p(out),
+                                         _p(status))
+    if rc != 0:
+        raise ValueError("census window wider than 64 bits")
+    return out, status
+
+
+# -------------------------------------------------------------------- layers
+
+def conv_out(n, k, s, p, d):
+    return (n + 2 * p - (d * (k - 1) + 1)) // s + 1
+
+
+def conv2d_fwd(x, IC, H, W, w, b, OC, k, stride=1, pad=None, dil=1):
+    pad = (dil * (k - 1)) // 2 if pad is None else pad
+    x, w, b = _f64(x).ravel(), _f64(w).ravel(), _f64(b).ravel()
+    OH, OW = conv_out(H, k, stride, pad, dil), conv_out(W, k, stride, pad, dil)
+    y = np.zeros(OC * OH * OW)
+    lib().or_conv2d_fwd(_p(x), IC, H, W, _p(w), _p(b), OC, k, stride, pad, dil, _p(y))
+    return y, OH, OW
+
+
+def conv2d_bwd(x, IC, H, W, w, OC, k, gy, stride=1, pad=None, dil=1):
+    pad = (dil * (k - 1)) // 2 if pad is None else pad
+    x, w, gy = _f64(x).ravel(), _f64(w).ravel(), _f64(gy).ravel()
+    gx = np.zeros(IC * H * W)
+    gw = np.zeros(w.size)
+    gb = np.zeros(OC)
+    lib().or_conv2d_bwd(_p(x), IC, H, W, _p(w), OC, k, stride, pad, dil,
+                        _p(gy), _p(gx), _p(gw), _p(gb))
+    return gx, gw, gb
+
+
+def conv3d_fwd(x, IC, D, H, W, w, b, OC, k=3):
+    x, w, b = _f64(x).ravel(), _f64(w).ravel(), _f64(b).ravel()
+    y = np.zeros(OC * D * H * W)
+    lib().or_conv3d_fwd(_p(x), IC, D, H, W, _p(w), _p(b), OC, k, _p(y))
+    return y
+
+
+def conv3d_bwd(x, IC, D, H, W, w, OC, gy, k=3):
+    x, w, gy = _f64(x).ravel(), _f64(w).ravel(), _f64(gy).ravel()
+    gx = np.zeros(IC * D * H * W)
+    gw = np.zeros(w.size)
+    gb = np.zeros(OC)
+    lib().or_conv3d_bwd(_p(x), IC, D, H, W, _p(w), OC, k, _p(gy), _p(gx), _p(gw), _p(gb))
+    return gx, gw, gb
+
+
+def costvol_fwd(fl, fr, C, D, H, W):
+    fl, fr = _f64(fl).ravel(), _f64(fr).ravel()
+    vol = np.zeros(C * D * H * W)
+    lib().or_costvol_fwd(_p(fl), _p(fr), C, D, H, W, _p(vol))
+    return vol
+
+
+def costvol_bwd(fl, fr, C, D, H, W, gvol):
+    fl, fr, gvol = _f64(fl).ravel(), _f64(fr).ravel(), _f64(gvol).ravel()
+    gfl, gfr = np.zeros(C * H * W), np.zeros(C * H * W)
+    lib().or_costvol_bwd(_p(fl), _p(fr), C, D, H, W, _p(gvol), _p(gfl), _p(gfr))
+    return gfl, gfr
+
+
+def softargmin_fwd(cost, D, H, W, penalty=None):
+    cost = _f64(cost).ravel()
+    pen = None if penalty is None else _f64(penalty).ravel()
+    disp, probs = np.zeros(H * W), np.zeros(D * H * W)
+    lib().or_softargmin_fwd(_p(cost), D, H, W, _p(pen), _p(disp), _p(probs))
+    return disp, probs
+
+
+def softargmin_bwd(probs, disp, D, H, W, gdisp):
+    probs, disp, gdisp = _f64(probs).ravel(), _f64(disp).ravel(), _f64(gdisp).ravel()
+    g = np.zeros(D * H * W)
+    lib().or_softargmin_bwd(_p(probs), _p(disp), D, H, W, _p(gdisp), _p(g))
+    return g
+
+
+def bilinear_fwd(x, C, H, W, H2, W2):
+    x = _f64(x).ravel()
+    y = np.zeros(C * H2 * W2)
+    lib().or_bilinear_fwd(_p(x), C, H, W, H2, W2, _p(y))
+    return y
+
+
+def bilinear_bwd(gy, C, H, W, H2, W2):
+    gy = _f64(gy).ravel()
+    gx = np.zeros(C * H * W)
+    lib().or_bilinear_bwd(_p(gy), C, H, W, H2, W2, _p(gx))
+    return gx
+
+
+def avgpool2(img, C, H, W):
+    img = _f64(img).ravel()
+    out = np.zeros(C * (H // 2) * (W // 2))
+    lib().or_avgpool2(_p(img), C, H, W, _p(out))
+    return out, H // 2, W // 2
+
+
+def _u8(v):
+    return None if v is None else np.ascontiguousarray(np.asarray(v).ravel().astype(np.uint8))
+
+
+def pseudo_huber(pred, target, valid, alpha=1.0):
+    pred, target = _f64(pred).ravel(), _f64(target).ravel()
+    v = _u8(valid)
+    grad = np.zeros(pred.size)
+    cnt = ctypes.c_long(0)
+    loss = lib().or_pseudo_huber(_p(pred), _p(target), _p(v), pred.size, alpha,
+                                 _p(grad), ctypes.byref(cnt))
+    return loss, grad, cnt.value
+
+
+def disparity_ce(probs, target, valid, D, H, W, sigma=0.8):
+    probs, target = _f64(probs).ravel(), _f64(target).ravel()
+    v = _u8(valid)
+    g = np.zeros(D * H * W)
+    cnt = ctypes.c_long(0)
+    loss = lib().or_disparity_ce(_p(probs), _p(target), _p(v), D, H, W, sigma,
+                                 _p(g), ctypes.byref(cnt))
+    return loss, g, cnt.value
+
+
+class Adam:
+    """Adam.step of stereonet.py:1368-1387 over one flat float64 vector."""
+
+    def __init__(self, params, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+        self.p = params
+        self.m = np.zeros_like(params)
+        self.v = np.zeros_like(params)
+        self.lr, self.b1, self.b2, self.eps = lr, b1, b2, eps
+        self.b1t = 1.0
+        self.b2t = 1.0
+
+    def step(self, grads, clip=5.0):
+        self.b1t *= self.b1
+        self.b2t *= self.b2
+        g = _f64(grads)
+        lib().or_adam_step(_p(self.p), _p(g), _p(self.m), _p(self.v), self.p.size,
+                           self.lr, self.b1, self.b2, self.eps, self.b1t, self.b2t, clip)
+
+
+# ------------------------------------------------------------------- network
+
+class Arch:
+    def __init__(self, feat_C=12, cost_C=6, D=12, refine_C=8, dil=(1, 2, 4),
+                 target_peak=34.0):
+        self.feat_C, self.cost_C, self.D, self.refine_C = feat_C, cost_C, D, refine_C
+        self.dil = np.array(dil, dtype=np.int32)
+        self.target_peak = target_peak
+
+    def args(self):
+        return (self.feat_C, self.cost_C, self.D, self.refine_C, len(self.dil),
+                _p(self.dil))
+
+    def num_params(self):
+        return lib().or_net_num_params(*self.args())
+
+
+def net_forward(arch, params, left, right, H, W, guide_half, guide_full):
+    params = _f64(params)
+    left, right = _f64(left).ravel(), _f64(right).ravel()
+    gh, gf = _f64(guide_half).ravel(), _f64(guide_full).ravel()
+    h = conv_out(conv_out(H, 3, 2, 1, 1), 3, 2, 1, 1)
+    w = conv_out(conv_out(W, 3, 2, 1, 1), 3, 2, 1, 1)
+    d0 = np.zeros(h * w)
+    d1 = np.zeros((H // 2) * (W // 2))
+    d2 = np.zeros(H * W)
+    probs = np.zeros(arch.D * h * w)
+    dims = np.zeros(2, dtype=np.int32)
+    lib().or_net_forward(*arch.args(), _p(params), _p(left), _p(right), H, W,
+                         _p(gh), _p(gf), _p(d0), _p(d1), _p(d2), _p(probs), _p(dims))
+    return d0, d1, d2, probs, (h, w)
+
+
+def train_grads(arch, params, left, right, disp, valid, W, H):
+    params = _f64(params)
+    left, right, disp = _f64(left).ravel(), _f64(right).ravel(), _f64(disp).ravel()
+    v = _u8(valid)
+    G = np.zeros(params.size)
+    d2 = np.zeros(W * H)
+    loss = lib().or_train_grads(*arch.args(), _p(params), _p(left), _p(right),
+                                _p(disp), _p(v), W, H, _p(G), _p(d2))
+    return loss, G, d2
+
+
+def predict(arch, params, left, right, W, H):
+    params = _f64(params)
+    left, right = _f64(left).ravel(), _f64(right).ravel()
+    out = np.zeros(W * H)
+    lib().or_predict(*arch.args(), _p(params), _p(left), _p(right), W, H, _p(out))
+    return out
+
+
+def peak(disp, q=0.95):
+    d = _f64(disp).ravel()
+    return lib().or_peak(_p(d), d.size, q)
+
+
+def predict_at_scale(arch, params, left, right, W, H, work_width=96):
+    params = _f64(params)
+    left, right = _f64(left).ravel(), _f64(right).ravel()
+    full = np.zeros(W * H)
+    work = np.zeros(W * H)
+    dims = np.zeros(2, dtype=np.int32)
+    lib().or_predict_at_scale(*arch.args(), _p(params), _p(left), _p(right), W, H,
+                              work_width, _p(full), _p(work), _p(dims))
+    Ww, Hw = int(dims[0]), int(dims[1])
+    return full, work[:Ww * Hw].copy(), Ww, Hw
+
+
+def predict_auto(arch, params, left, right, W, H, target_peak=None,
+                 probe_width=64, max_width=320, passes=3):
+    tp = arch.target_peak if target_peak is None else target_peak
+    params = _f64(params)
+    left, right = _f64(left).ravel(), _f64(right).ravel()
+    full = np.zeros(W * H)
+    work = np.zeros(W * H)
+    dims = np.zeros(2, dtype=np.int32)
+    trace = np.zeros(8, dtype=np.int32)
+    lib().or_predict_auto(*arch.args(), _p(params), float(tp), _p(left), _p(right),
+                          W, H, probe_width, max_width, passes, _p(full), _p(work),
+                          _p(dims), _p(trace))
+    Ww, Hw = int(dims[0]), int(dims[1])
+    return full, work[:Ww * Hw].copy(), Ww, Hw, trace
+
+
+def load_weights_txt(path):
+    """Parse the text checkpoint format of stereonet.py:1766-1874 into
+    (config dict or None, list of float64 blocks)."""
+    lines = open(path).read().split("\n")
+    cfg = None
+    off = 0
+    if lines[0].startswith("stereonet "):
+        cfg = {}
+        for tok in lines[0].split()[1:]:
+            if "=" not in tok:
+                continue
+            k, v = tok.split("=", 1)
+            cfg[k] = tuple(int(d) for d in v.split(",")) if k == "refine_dil" else int(v)
+        off = 1
+    n = int(lines[off])
+    blocks = []
+    for k in range(n):
+        parts = lines[off + 1 + k].split()
+        cnt = int(parts[0])
+        blocks.append(np.array([float(v) for v in parts[1:cnt + 1]]))
+    return cfg, blocks

@@ -1,0 +1,63 @@
+"""Extract the network golden vectors embedded in the reference's demo page.
+
+    python tests/golden/extract_demo_page.py /root/reference
+
+web/stereonet_demo.html:997 holds the v5 checkpoint as JSON (``CKPT``) and
+:998 holds ``SAMPLES``: for each real Middlebury pair, the left/right PNGs
+(base64 data URIs) and ``raw``, the output of the reference's JavaScript
+``predictAuto`` (web/stereonet_infer.js, target peak 34) stored as
+round(d * 64) big-endian uint16 (web/build_web.py:55-56).  The JavaScript
+port agrees with the Python reference to 4e-14 px (README.md:54), so ``raw``
+pins predict_auto to +-1/128 px on real images.
+
+Gray is computed at test time exactly as web/build_web.py:27-29 does
+(``a / 255.0`` then 0.299 r + 0.587 g + 0.114 b in float64); the fixture
+stores the decoded uint8 RGB.  The trained weights file is copied as data.
+"""
+
+import base64
+import io
+import json
+import os
+import shutil
+import sys
+
+import numpy as np
+from PIL import Image
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def decode_png(uri):
+    b = base64.b64decode(uri.split(",", 1)[1])
+    return np.asarray(Image.open(io.BytesIO(b)).convert("RGB"), dtype=np.uint8)
+
+
+def main(ref_root):
+    page = os.path.join(ref_root, "web", "stereonet_demo.html")
+    lines = open(page).read().split("\n")
+    ck_line = lines[996]
+    sa_line = lines[997]
+    assert ck_line.startswith("const CKPT = ") and sa_line.startswith("const SAMPLES = ")
+    ckpt = json.loads(ck_line[len("const CKPT = "):].rstrip().rstrip(";"))
+    samples = json.loads(sa_line[len("const SAMPLES = "):].rstrip().rstrip(";"))
+    out = {"ckpt_sizes": np.array([len(b) for b in ckpt["blocks"]]),
+           "ckpt_flat": np.concatenate([np.array(b, dtype=np.float64) for b in ckpt["blocks"]])}
+    for key in ("cones", "art"):
+        s = samples[key]
+        l = decode_png(s["left"])
+        r = decode_png(s["right"])
+        raw = np.frombuffer(base64.b64decode(s["raw"]), dtype=">u2").astype(np.uint16)
+        H, W, _ = l.shape
+        assert raw.size == W * H
+        out[key + "_left_rgb"] = l
+        out[key + "_right_rgb"] = r
+        out[key + "_raw"] = raw
+    np.savez_compressed(os.path.join(HERE, "demo_page.npz"), **out)
+    shutil.copyfile(os.path.join(ref_root, "stereonet_weights.txt"),
+                    os.path.join(HERE, "stereonet_weights_v5.txt"))
+    print("wrote demo_page.npz and stereonet_weights_v5.txt")
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
