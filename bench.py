@@ -1,0 +1,218 @@
+#!/usr/bin/env python3
+"""Throughput of the dense-disparity path on one MI355X.
+
+Workload (one step): a batch of synthetic 460x370 rectified pairs
+(disparity 10-80 px, slanted planes + thin foreground boxes, sigma-2 noise,
+uint8, resident in HBM) through the full plain pipeline:
+predict_auto with the trained v5 StereoNet (probe + up to 3 passes, FP32
+network) followed by refine_disparity(planes=False): census 9x7, band SGM
+(R=10, P1=12, P2=150, 8 paths, both views), LR check (tol 2), 3x3 median.
+
+Printed (one JSON line): pairs/s (value), the full-range SGM control on the
+same pairs (eval_real.py:143-146 rule), accuracy of both (bad2 / EPE vs the
+generated ground truth on visible pixels), the roofline of the dominant
+kernel (band SGM path aggregation; live HIP-event timing of its launches)
+and the CPU baseline (the float64 C oracle on this host, one core, on a
+bounded sample of the same workload).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
+"""
+
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "pairs/s @460×370: net+band-SGM vs full-range SGM at equal bad2; HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+W, H = 460, 370
+R_BAND = 10
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def sgm_bytes_per_pair(views, K, HW):
+    """Algorithmic HBM bytes of the eight-path aggregation per pair: per
+    view, cell and path, read the u8 cost and read+write a u16 path sum
+    (V*HW*K*8*(1+2+2))."""
+    return views * HW * K * 8 * (1 + 2 + 2)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--cpu-pairs", type=int, default=6)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fullrange", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(pairs are independent; see DESIGN.md for how it would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    sv = importlib.import_module("from-scratch-stereo-vision_amd")
+    L = importlib.import_module("from-scratch-stereo-vision_amd._lib")
+    N = importlib.import_module("from-scratch-stereo-vision_amd.net")
+    dev = L.device()
+    B = args.batch
+    log("generating %d synthetic pairs %dx%d" % (B, W, H))
+    pairs = [sv.synth.pipeline_pair(s) for s in range(B)]
+    L8 = torch.from_numpy(np.stack([p["left"] for p in pairs])).to(dev)
+    R8 = torch.from_numpy(np.stack([p["right"] for p in pairs])).to(dev)
+    gt = np.stack([p["disp"] for p in pairs]).reshape(B, -1)
+    valid = np.stack([p["valid"] for p in pairs]).reshape(B, -1)
+    net = N.load_model(os.path.join(ROOT, "tests", "golden", "stereonet_weights_v5.txt"))
+    d0 = torch.empty(B, H * W, dtype=torch.float64, device=dev)
+    lv = N._Level([(H, W)] * B, dev)
+
+    t_net = [0.0]
+
+    def step():
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        res = N.predict_auto_batch(net, L8, R8, W, H)
+        N._call("sv_widen", N._p(res.full), N._p(lv.table), H * W, N._p(d0), H * W, None, B,
+                H * W, L.stream())
+        e1.record()
+        out, st = sv.refine_disparity(L8, R8, None, d0, W, H, planes=False)
+        return out, st, res, (e0, e1)
+
+    log("warmup %d" % args.warmup)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    L.lib().sv_profile_enable(1)
+    log("timing %d steps of %d pairs" % (args.steps, B))
+    net_events = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out, st, res, ev = step()
+        net_events.append(ev)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    import ctypes
+    stages = {}
+    for name in ("census", "costs", "sgm", "wta", "lr", "median"):
+        ms, cnt = ctypes.c_double(0), ctypes.c_int(0)
+        L.check(L.lib().sv_profile_read(name.encode(), ctypes.byref(ms), ctypes.byref(cnt)))
+        stages[name] = (ms.value, cnt.value)
+    L.lib().sv_profile_enable(0)
+    net_ms = sum(a.elapsed_time(b) for a, b in net_events)
+    pairs_s = B * args.steps / dt
+
+    # accuracy of net + band refinement
+    pred = out.reshape(B, -1).cpu().numpy()
+    err = np.abs(pred - gt)
+    bad2 = float((err > 2.0)[valid].mean() * 100.0)
+    epe = float(err[valid].mean())
+    net_err = np.abs(res.full.reshape(B, -1).double().cpu().numpy() - gt)
+    net_bad2 = float((net_err > 2.0)[valid].mean() * 100.0)
+    log("net+band: %.1f pairs/s, bad2 %.2f%%, EPE %.3f (network alone bad2 %.2f%%)"
+        % (pairs_s, bad2, epe, net_bad2))
+
+    # full-range SGM control (eval_real.py:143-146): centre R, K = 2R+1
+    fullrange = None
+    if not args.no_fullrange:
+        Rs = [int(float(g[v].max()) + 2) // 2 + 1 for g, v in zip(gt, valid)]
+        groups = {}
+        for i, r in enumerate(Rs):
+            groups.setdefault(r, []).append(i)
+
+        def fr_step():
+            outs = {}
+            for r, idx in groups.items():
+                sub_l = L8[idx] if len(idx) < B else L8
+                sub_r = R8[idx] if len(idx) < B else R8
+                c = torch.full((len(idx), H * W), float(r), dtype=torch.float64, device=dev)
+                outs[r] = (idx, sv.refine_fullres(sub_l, sub_r, c, W, H, R=r, P1=12.0, P2=150.0,
+                                                  rx=4, ry=3))
+            return outs
+
+        fr_step()
+        torch.cuda.synchronize()
+        fsteps = max(1, args.steps // 2)
+        t1 = time.perf_counter()
+        for _ in range(fsteps):
+            outs = fr_step()
+        torch.cuda.synchronize()
+        fdt = time.perf_counter() - t1
+        fpred = np.zeros_like(pred)
+        for r, (idx, o) in outs.items():
+            fpred[idx] = o.reshape(len(idx), -1).cpu().numpy()
+        ferr = np.abs(fpred - gt)
+        fullrange = {"pairs_per_s": B * fsteps / fdt,
+                     "bad2_pct": float((ferr > 2.0)[valid].mean() * 100.0),
+                     "epe_px": float(ferr[valid].mean()),
+                     "R": sorted(groups), "K": sorted(2 * r + 1 for r in groups)}
+        log("full-range: %.1f pairs/s, bad2 %.2f%%" % (fullrange["pairs_per_s"], fullrange["bad2_pct"]))
+
+    # roofline of the dominant kernel: band SGM aggregation (both views)
+    sgm_ms, sgm_n = stages["sgm"]
+    K = 2 * R_BAND + 1
+    bytes_launch = sgm_bytes_per_pair(2, K, W * H) * B
+    avg_s = (sgm_ms / max(sgm_n, 1)) / 1e3
+    achieved = bytes_launch / avg_s / 1e9 if avg_s > 0 else 0.0
+    roofline = {"bound": "hbm", "kernel": "k_sgm (8 paths, both views, batch of %d pairs)" % B,
+                "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "avg_launch_ms": round(avg_s * 1e3, 4),
+                "algorithmic_bytes_per_launch": bytes_launch}
+
+    # CPU baseline: the float64 C oracle, one core, same workload
+    cpu = None
+    if not args.no_cpu and args.cpu_pairs > 0:
+        import oracle
+        cfg, blocks = oracle.load_weights_txt(os.path.join(ROOT, "tests", "golden", "stereonet_weights_v5.txt"))
+        arch = oracle.Arch(cfg["feat_C"], cfg["cost_C"], cfg["D"], cfg["refine_C"], cfg["refine_dil"],
+                           cfg["target_peak"])
+        P = np.concatenate(blocks)
+        n = min(args.cpu_pairs, B)
+        log("CPU baseline on %d pairs" % n)
+        tc = time.perf_counter()
+        for i in range(n):
+            gl, gr = sv.synth.gray(pairs[i]["left"]), sv.synth.gray(pairs[i]["right"])
+            full, _, _, _, _ = oracle.predict_auto(arch, P, gl, gr, W, H)
+            oracle.refine_disparity_plain(gl, gr, full, W, H)
+        tcpu = time.perf_counter() - tc
+        cpu = {"value": round(n / tcpu, 4), "unit": "pairs/s", "cores": 1, "kind": "port",
+               "sample": "%d of the same 460x370 pairs through the float64 C oracle "
+                         "(predict_auto + refine_disparity planes=False), %.1f s" % (n, tcpu)}
+
+    rec = {
+        "metric": METRIC, "value": round(pairs_s, 2), "unit": "pairs/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 network / int SGM / f64 epilogue", "data": "synthetic",
+        "config": {"workload": "net+band-SGM 460x370: predict_auto (v5 StereoNet, FP32) + "
+                               "refine_disparity(planes=False), R=10 census 9x7",
+                   "pairs_per_step": B, "parallelism": "single GPU"},
+        "accuracy": {"bad2_pct": round(bad2, 3), "epe_px": round(epe, 4),
+                     "network_only_bad2_pct": round(net_bad2, 3)},
+        "fullrange": fullrange,
+        "speedup_vs_fullrange": (round(pairs_s / fullrange["pairs_per_s"], 3) if fullrange else None),
+        "stage_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in stages.items()},
+        "network_ms_per_step": round(net_ms / args.steps, 4),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "gpu_over_cpu": (round(pairs_s / cpu["value"], 1) if cpu else None),
+    }
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,24 @@
+"""MI355X-native (gfx950) dense-disparity path of
+Ray-5106/From-Scratch-Stereo-Vision.
+
+The package re-exports the reference's stereonet.py names on the path with
+the same signatures, so ``SN = vars(package)`` stands in for the namespace
+the reference's harnesses build with ``exec`` (eval_real.py:41-44).  All
+arithmetic runs in hand-written HIP kernels (libstereo_mi.so, built by
+build.py); torch supplies device memory and streams only.
+"""
+
+from .refine import (census, band_costs, _aggregate_path, refine_fullres, _mirror1,
+                     left_to_right, lr_refine, median_filter, refine_disparity)
+from .net import (StereoNet, Adam, LEVEL_WEIGHTS, CE_WEIGHT, train_grads, train_grads_batch,
+                  train_step, flat_params, set_flat_params, flat_grads, set_flat_grads,
+                  save_weights, read_checkpoint_config, load_weights, load_model, predict,
+                  predict_batch, predict_at_scale, predict_auto, predict_auto_batch)
+from . import synth
+
+__all__ = ["census", "band_costs", "_aggregate_path", "refine_fullres", "_mirror1",
+           "left_to_right", "lr_refine", "median_filter", "refine_disparity", "StereoNet", "Adam",
+           "LEVEL_WEIGHTS", "CE_WEIGHT", "train_grads", "train_grads_batch", "train_step",
+           "flat_params", "set_flat_params", "flat_grads", "set_flat_grads", "save_weights",
+           "read_checkpoint_config", "load_weights", "load_model", "predict", "predict_batch",
+           "predict_at_scale", "predict_auto", "predict_auto_batch", "synth"]

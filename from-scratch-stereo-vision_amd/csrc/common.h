@@ -1,0 +1,53 @@
+// Shared plumbing for the gfx950 stereo library: error codes, the thread-local
+// error message behind sv_last_error(), launch checking and small helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstddef>
+#include <cstdint>
+
+#define SV_OK 0
+#define SV_EINVAL -1        // bad argument (sizes, null pointers, ranges)
+#define SV_EUNSUPPORTED -2  // valid for the reference, outside what the GPU path supports
+#define SV_EHIP -3          // a HIP runtime error
+#define SV_EWORKSPACE -4    // workspace too small
+
+// element types of image inputs
+#define SV_U8 0
+#define SV_F32 1
+#define SV_F64 2
+
+namespace sv {
+
+int set_error(int code, const char* fmt, ...);
+
+inline size_t align_up(size_t n, size_t a = 256) { return (n + a - 1) / a * a; }
+
+// Carves aligned sub-buffers out of one caller-provided workspace.
+struct Carver {
+    char* base;
+    size_t used = 0;
+    explicit Carver(void* p) : base(static_cast<char*>(p)) {}
+    template <typename T>
+    T* take(size_t count) {
+        T* p = base ? reinterpret_cast<T*>(base + used) : nullptr;
+        used += align_up(count * sizeof(T));
+        return p;
+    }
+};
+
+}  // namespace sv
+
+#define SV_CHECK_LAUNCH()                                                          \
+    do {                                                                           \
+        hipError_t e_ = hipGetLastError();                                         \
+        if (e_ != hipSuccess)                                                      \
+            return sv::set_error(SV_EHIP, "%s: %s", __func__, hipGetErrorString(e_)); \
+    } while (0)
+
+#define SV_REQUIRE(cond, ...)                                   \
+    do {                                                        \
+        if (!(cond)) return sv::set_error(SV_EINVAL, __VA_ARGS__); \
+    } while (0)

@@ -1,0 +1,1025 @@
+// StereoNet layers for gfx950 in FP32 (stereonet.py:453-1388,
+// Ray-5106/From-Scratch-Stereo-Vision): convolutions (2D and 3D share one
+// kernel family; a 2D conv is depth 1), the cost volume (formed on the fly
+// inside the first 3D convolution, never stored), soft-argmin, bilinear
+// resampling, average pooling, the two losses with float64 reductions, Adam,
+// and a k-th-smallest selection for predict_auto's peak.
+//
+// Ragged batches: every kernel takes a per-sample dims table (int h, w per
+// sample) and a per-sample slot stride; sample n of a tensor lives at
+// ptr + n*stride, laid out densely as [C][D][h_n][w_n].  Grids are sized for
+// the largest sample and blocks outside a sample's extent exit, so pairs run
+// at different working resolutions share one launch per layer.
+//
+// Determinism: backward passes are gathers or fixed-order two-stage
+// reductions; no floating-point atomics.
+#include <cmath>
+
+#include "common.h"
+
+namespace {
+
+__device__ __forceinline__ int dh(const int* d, int n) { return d[2 * n]; }
+__device__ __forceinline__ int dw(const int* d, int n) { return d[2 * n + 1]; }
+
+// ---------------------------------------------------------------- conv fwd
+// Conv2d.forward (stereonet.py:489-526) and Conv3d.forward (:596-641) as one
+// direct convolution.  Weights [OC][IC][KD][3][3]; spatial stride s, zero
+// padding p, dilation dil; depth stride 1, padding KD/2.  Each thread makes
+// OCB output channels of one output voxel; the weight reads are uniform
+// across the wave and go through the scalar cache.
+// SRC == 1: the input is the cost volume |fl(x) - fr(x - d)|, 2.0 where
+// x < d (CostVolume.forward, :771-799), computed from the two feature maps.
+// mode 0: bias only; 1: LeakyReLU(slope); 2: max(resid + conv, 0) with the
+// residual read from `resid` (Refinement output, :1150-1163).
+template <int OCB, int SRC>
+__global__ void __launch_bounds__(256) k_conv_fwd(
+    const float* __restrict__ x, const float* __restrict__ xr, const int* __restrict__ idims,
+    long long xstride, int IC, int D, int KD, const float* __restrict__ w,
+    const float* __restrict__ b, int OC, int s, int p, int dil, float* __restrict__ y,
+    const int* __restrict__ odims, long long ystride, int tilesY, int mode, float slope,
+    const float* __restrict__ resid, long long rstride) {
+    const int n = blockIdx.z;
+    const int od = blockIdx.y / tilesY;
+    const int oy = (blockIdx.y - od * tilesY) * 4 + threadIdx.y;
+    const int ox = blockIdx.x * 64 + threadIdx.x;
+    const int hi = dh(idims, n), wi = dw(idims, n);
+    const int ho = dh(odims, n), wo = dw(odims, n);
+    if (ox >= wo || oy >= ho) return;
+    const int pd = KD / 2;
+    const long long iplane = (long long)hi * wi;
+    const long long ivol = iplane * D;
+    const long long oplane = (long long)ho * wo;
+    const float* xs = x + (long long)n * xstride;
+    const float* xrs = SRC == 1 ? xr + (long long)n * xstride : nullptr;
+    float* ys = y + (long long)n * ystride + (long long)od * oplane + (long long)oy * wo + ox;
+    const int iy0 = oy * s - p, ix0 = ox * s - p;
+    for (int oc0 = 0; oc0 < OC; oc0 += OCB) {
+        float acc[OCB];
+#pragma unroll
+        for (int j = 0; j < OCB; ++j) acc[j] = (oc0 + j < OC) ? b[oc0 + j] : 0.f;
+        for (int ic = 0; ic < IC; ++ic) {
+            for (int kd = 0; kd < KD; ++kd) {
+                const int dd = od + kd - pd;
+                if (dd < 0 || dd >= D) continue;
+#pragma unroll
+                for (int ky = 0; ky < 3; ++ky) {
+                    const int iy = iy0 + ky * dil;
+                    if (iy < 0 || iy >= hi) continue;
+#pragma unroll
+                    for (int kx = 0; kx < 3; ++kx) {
+                        const int ix = ix0 + kx * dil;
+                        if (ix < 0 || ix >= wi) continue;
+                        float v;
+                        if (SRC == 1) {
+                            // channel ic of the left/right feature maps, shift dd
+                            if (ix < dd) {
+                                v = 2.0f;
+                            } else {
+                                const long long o = (long long)ic * iplane + (long long)iy * wi;
+                                v = fabsf(xs[o + ix] - xrs[o + ix - dd]);
+                            }
+                        } else {
+                            v = xs[(long long)ic * ivol + (long long)dd * iplane + (long long)iy * wi + ix];
+                        }
+                        const int wt = ((ic * KD + kd) * 3 + ky) * 3 + kx;
+#pragma unroll
+                        for (int j = 0; j < OCB; ++j)
+                            if (oc0 + j < OC) acc[j] += v * w[(long long)(oc0 + j) * IC * KD * 9 + wt];
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < OCB; ++j) {
+            if (oc0 + j >= OC) break;
+            float v = acc[j];
+            if (mode == 1) {
+                v = v > 0.f ? v : v * slope;
+            } else if (mode == 2) {
+                const float r = resid[(long long)n * rstride + (long long)(oc0 + j) * D * oplane +
+                                      (long long)od * oplane + (long long)oy * wo + ox];
+                v = r + v;
+                v = v > 0.f ? v : 0.f;
+            }
+            ys[(long long)(oc0 + j) * D * oplane] = v;
+        }
+    }
+}
+
+// ----------------------------------------------------------- conv bwd data
+// Conv2d/Conv3d.backward input gradient (:528-568, :643-692) as a gather:
+// gx[ic][id][iy][ix] = sum over oc and taps whose output lands in range.
+template <int ICB>
+__global__ void __launch_bounds__(256) k_conv_bwd_data(
+    const float* __restrict__ g, const int* __restrict__ odims, long long gstride,
+    const float* __restrict__ w, int IC, int OC, int D, int KD, int s, int p, int dil,
+    float* __restrict__ gx, const int* __restrict__ idims, long long gxstride, int tilesY, int ICn) {
+    const int n = blockIdx.z;
+    const int id = blockIdx.y / tilesY;
+    const int iy = (blockIdx.y - id * tilesY) * 4 + threadIdx.y;
+    const int ix = blockIdx.x * 64 + threadIdx.x;
+    const int hi = dh(idims, n), wi = dw(idims, n);
+    const int ho = dh(odims, n), wo = dw(odims, n);
+    if (ix >= wi || iy >= hi) return;
+    const int pd = KD / 2;
+    const long long oplane = (long long)ho * wo;
+    const long long iplane = (long long)hi * wi;
+    const float* gs = g + (long long)n * gstride;
+    float* gxs = gx + (long long)n * gxstride + (long long)id * iplane + (long long)iy * wi + ix;
+    for (int ic0 = 0; ic0 < ICn; ic0 += ICB) {
+        float acc[ICB];
+#pragma unroll
+        for (int j = 0; j < ICB; ++j) acc[j] = 0.f;
+        for (int oc = 0; oc < OC; ++oc) {
+            for (int kd = 0; kd < KD; ++kd) {
+                const int od = id - kd + pd;
+                if (od < 0 || od >= D) continue;
+#pragma unroll
+                for (int ky = 0; ky < 3; ++ky) {
+                    const int ty = iy + p - ky * dil;
+                    if (ty < 0) continue;
+                    const int oy = ty / s;
+                    if (oy * s != ty || oy >= ho) continue;
+#pragma unroll
+                    for (int kx = 0; kx < 3; ++kx) {
+                        const int tx = ix + p - kx * dil;
+                        if (tx < 0) continue;
+                        const int ox = tx / s;
+                        if (ox * s != tx || ox >= wo) continue;
+                        const float gv = gs[(long long)oc * D * oplane + (long long)od * oplane +
+                                            (long long)oy * wo + ox];
+                        const int wt = (kd * 3 + ky) * 3 + kx;
+#pragma unroll
+                        for (int j = 0; j < ICB; ++j)
+                            if (ic0 + j < ICn)
+                                acc[j] += gv * w[((long long)oc * IC + ic0 + j) * KD * 9 + wt];
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < ICB; ++j)
+            if (ic0 + j < ICn) gxs[(long long)(ic0 + j) * D * iplane] = acc[j];
+    }
+}
+
+// --------------------------------------------------------- conv bwd weight
+// Weight and bias gradient as a split reduction.  Block (c, kt, n) owns the
+// weight columns kt*128 + t (one (ic, tap) pair per thread) for all output
+// channels, walks output positions c, c + nchunk*64, ... of sample n in
+// tiles of 64 (the gradient tile staged in LDS), and writes its partial sums;
+// k_reduce_partials adds the partials in a fixed order.
+constexpr int WG_COLS = 128;
+constexpr int POS_TILE = 64;
+
+template <int SRC, int OCMAX>
+__global__ void __launch_bounds__(WG_COLS) k_conv_bwd_weight(
+    const float* __restrict__ x, const float* __restrict__ xr, const int* __restrict__ idims,
+    long long xstride, int IC, int D, int KD, const float* __restrict__ g,
+    const int* __restrict__ odims, long long gstride, int OC, int s, int p, int dil, int nchunk,
+    float* __restrict__ partials) {
+    __shared__ float gt[OCMAX][POS_TILE];
+    const int n = blockIdx.z;
+    const int c = blockIdx.x;
+    const int t = threadIdx.x;
+    const int ncol = IC * KD * 9;
+    const int col = blockIdx.y * WG_COLS + t;
+    const int hi = dh(idims, n), wi = dw(idims, n);
+    const int ho = dh(odims, n), wo = dw(odims, n);
+    const long long oplane = (long long)ho * wo;
+    const long long npos = oplane * D;
+    const long long iplane = (long long)hi * wi;
+    const int pd = KD / 2;
+    int ic = 0, kd = 0, ky = 0, kx = 0;
+    if (col < ncol) {
+        ic = col / (KD * 9);
+        const int r = col - ic * KD * 9;
+        kd = r / 9;
+        ky = (r - kd * 9) / 3;
+        kx = r - kd * 9 - ky * 3;
+    }
+    const float* xs = x + (long long)n * xstride;
+    const float* xrs = SRC == 1 ? xr + (long long)n * xstride : nullptr;
+    const float* gs = g + (long long)n * gstride;
+    float acc[OCMAX];
+#pragma unroll
+    for (int o = 0; o < OCMAX; ++o) acc[o] = 0.f;
+    float bacc = 0.f;
+    for (long long base = (long long)c * POS_TILE; base < npos; base += (long long)nchunk * POS_TILE) {
+        __syncthreads();
+        for (int e = t; e < OC * POS_TILE; e += WG_COLS) {
+            const int o = e / POS_TILE, q = e - o * POS_TILE;
+            const long long pos = base + q;
+            gt[o][q] = pos < npos ? gs[(long long)o * npos + pos] : 0.f;
+        }
+        __syncthreads();
+        if (t < OC && blockIdx.y == 0) {
+            for (int q = 0; q < POS_TILE; ++q) bacc += gt[t][q];
+        }
+        if (col < ncol) {
+            const int lim = (int)min((long long)POS_TILE, npos - base);
+            for (int q = 0; q < lim; ++q) {
+                const long long pos = base + q;
+                const int od = (int)(pos / oplane);
+                const int rem = (int)(pos - (long long)od * oplane);
+                const int oy = rem / wo, ox = rem - (rem / wo) * wo;
+                const int dd = od + kd - pd;
+                const int iy = oy * s - p + ky * dil;
+                const int ix = ox * s - p + kx * dil;
+                float v = 0.f;
+                if (dd >= 0 && dd < D && iy >= 0 && iy < hi && ix >= 0 && ix < wi) {
+                    if (SRC == 1) {
+                        if (ix < dd) {
+                            v = 2.0f;
+                        } else {
+                            const long long off = (long long)ic * iplane + (long long)iy * wi;
+                            v = fabsf(xs[off + ix] - xrs[off + ix - dd]);
+                        }
+                    } else {
+                        v = xs[(long long)ic * iplane * D + (long long)dd * iplane + (long long)iy * wi + ix];
+                    }
+                }
+#pragma unroll
+                for (int o = 0; o < OCMAX; ++o)
+                    if (o < OC) acc[o] += gt[o][q] * v;
+            }
+        }
+    }
+    const int ncolp = ncol + 1;  // + one bias column
+    float* out = partials + ((long long)n * nchunk + c) * (long long)OC * ncolp;
+    if (col < ncol) {
+#pragma unroll
+        for (int o = 0; o < OCMAX; ++o)
+            if (o < OC) out[(long long)o * ncolp + col] = acc[o];
+    }
+    if (t < OC && blockIdx.y == 0) out[(long long)t * ncolp + ncol] = bacc;
+}
+
+// gw[o][col] += sum over (n, chunk) of partials, in a fixed order; the
+// bias column goes to gb.
+__global__ void k_reduce_partials(const float* __restrict__ partials, int nparts, int OC,
+                                  int ncol, float* __restrict__ gw, float* __restrict__ gb) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int ncolp = ncol + 1;
+    if (i >= OC * ncolp) return;
+    double s = 0.0;
+    for (int q = 0; q < nparts; ++q) s += (double)partials[(long long)q * OC * ncolp + i];
+    const int o = i / ncolp, col = i - o * ncolp;
+    if (col < ncol)
+        gw[(long long)o * ncol + col] += (float)s;
+    else
+        gb[o] += (float)s;
+}
+
+// ------------------------------------------------------------ elementwise
+// LeakyReLU.backward (:713-720) in place: the post-activation value has the
+// sign of the pre-activation for any slope >= 0.
+__global__ void k_act_bwd(float* __restrict__ g, const float* __restrict__ y, long long n,
+                          float slope) {
+    const long long i = blockIdx.x * 256LL + threadIdx.x;
+    if (i >= n) return;
+    g[i] = y[i] > 0.f ? g[i] : g[i] * slope;
+}
+
+// CostVolume.backward (:801-825) as a gather over disparities:
+// gfl[c][y][x] += sum_d gvol*sign, gfr[c][y][x] -= sum_d gvol(x+d)*sign.
+__global__ void k_costvol_bwd(const float* __restrict__ fl, const float* __restrict__ fr,
+                              const int* __restrict__ dims, long long fstride, int C, int D,
+                              const float* __restrict__ gvol, long long gvstride,
+                              float* __restrict__ gfl, float* __restrict__ gfr, int tilesY) {
+    const int n = blockIdx.z;
+    const int c = blockIdx.y / tilesY;
+    const int y = (blockIdx.y - c * tilesY) * 4 + threadIdx.y;
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int h = dh(dims, n), w = dw(dims, n);
+    if (x >= w || y >= h) return;
+    const long long plane = (long long)h * w;
+    const long long fo = (long long)n * fstride + (long long)c * plane + (long long)y * w;
+    const float* gv = gvol + (long long)n * gvstride + (long long)c * D * plane + (long long)y * w;
+    const float l = fl[fo + x];
+    const float r = fr[fo + x];
+    float al = 0.f, ar = 0.f;
+    for (int d = 0; d < D; ++d) {
+        if (x >= d) {
+            const float diff = l - fr[fo + x - d];
+            const float sg = diff < 0.f ? -1.f : 1.f;
+            al += gv[(long long)d * plane + x] * sg;
+        }
+        if (x + d < w) {
+            const float diff = fl[fo + x + d] - r;
+            const float sg = diff < 0.f ? -1.f : 1.f;
+            ar -= gv[(long long)d * plane + x + d] * sg;
+        }
+    }
+    gfl[fo + x] = al;
+    gfr[fo + x] = ar;
+}
+
+// SoftArgmin.forward (:861-889) with CostVolume.invalid_mask (:827-840):
+// p = softmax_d(-(cost + penalty*[d >= 1 and x < d])), disp = sum d*p.
+__global__ void k_softargmin_fwd(const float* __restrict__ cost, const int* __restrict__ dims,
+                                 long long cstride, int D, float penalty,
+                                 float* __restrict__ disp, long long dstride,
+                                 float* __restrict__ probs) {
+    const int n = blockIdx.y;
+    const int h = dh(dims, n), w = dw(dims, n);
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= h * w) return;
+    const int x = i % w;
+    const long long plane = (long long)h * w;
+    const float* cs = cost + (long long)n * cstride;
+    float* ps = probs + (long long)n * cstride;
+    float best = -INFINITY;
+    for (int d = 0; d < D; ++d) {
+        const float c = cs[(long long)d * plane + i] + ((d >= 1 && x < d) ? penalty : 0.f);
+        best = fmaxf(best, -c);
+    }
+    float total = 0.f;
+    for (int d = 0; d < D; ++d) {
+        const float c = cs[(long long)d * plane + i] + ((d >= 1 && x < d) ? penalty : 0.f);
+        const float e = __expf(-c - best);
+        ps[(long long)d * plane + i] = e;
+        total += e;
+    }
+    const float inv = 1.f / total;
+    float acc = 0.f;
+    for (int d = 0; d < D; ++d) {
+        const float pv = ps[(long long)d * plane + i] * inv;
+        ps[(long long)d * plane + i] = pv;
+        acc += (float)d * pv;
+    }
+    disp[(long long)n * dstride + i] = acc;
+}
+
+// SoftArgmin.backward (:891-901) plus an extra cost gradient (the
+// cross-entropy's q - p, train_grads :1641-1650).
+__global__ void k_softargmin_bwd(const float* __restrict__ probs, const float* __restrict__ disp,
+                                 const float* __restrict__ gdisp, const float* __restrict__ gextra,
+                                 const int* __restrict__ dims, long long cstride, int D,
+                                 long long dstride, float* __restrict__ gcost) {
+    const int n = blockIdx.y;
+    const int h = dh(dims, n), w = dw(dims, n);
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= h * w) return;
+    const long long plane = (long long)h * w;
+    const float g = gdisp[(long long)n * dstride + i];
+    const float di = disp[(long long)n * dstride + i];
+    const long long o = (long long)n * cstride + i;
+    for (int d = 0; d < D; ++d) {
+        float v = g * probs[o + (long long)d * plane] * (di - (float)d);
+        if (gextra) v += gextra[o + (long long)d * plane];
+        gcost[o + (long long)d * plane] = v;
+    }
+}
+
+// BilinearUpsample taps (:913-950), in float64 so the integer taps match
+// the reference's exactly.
+__device__ __forceinline__ void tap(int o, int in, int out, int& i0, int& i1, float& wt) {
+    const double sc = (double)in / (double)out;
+    double f = (o + 0.5) * sc - 0.5;
+    if (f < 0.0) f = 0.0;
+    int a = (int)f;
+    if (a > in - 1) a = in - 1;
+    i0 = a;
+    i1 = a + 1 < in ? a + 1 : a;
+    wt = (float)(f - a);
+}
+
+// BilinearUpsample.forward (:913-960) / resize_gray (:3843-3845): y =
+// mul * bilinear(x) into output channels [och, och + C).  T is the input
+// element type (uint8 images are read as value / 255).
+template <typename T>
+__global__ void k_bilinear_fwd(const T* __restrict__ x, const int* __restrict__ idims,
+                               long long xstride, int C, float* __restrict__ y,
+                               const int* __restrict__ odims, long long ystride, int och,
+                               float mul, const float* __restrict__ muln, int tilesY) {
+    const int n = blockIdx.z;
+    const int c = blockIdx.y / tilesY;
+    const int oy = (blockIdx.y - c * tilesY) * 4 + threadIdx.y;
+    const int ox = blockIdx.x * 64 + threadIdx.x;
+    const int hi = dh(idims, n), wi = dw(idims, n);
+    const int ho = dh(odims, n), wo = dw(odims, n);
+    if (ox >= wo || oy >= ho) return;
+    if (muln) mul = muln[n];
+    int y0, y1, x0, x1;
+    float wy, wx;
+    tap(oy, hi, ho, y0, y1, wy);
+    tap(ox, wi, wo, x0, x1, wx);
+    const T* xs = x + (long long)n * xstride + (long long)c * hi * wi;
+    auto rd = [&](int yy, int xx) -> float {
+        const T v = xs[(long long)yy * wi + xx];
+        if constexpr (sizeof(T) == 1)
+            return (float)v * (1.0f / 255.0f);
+        else
+            return (float)v;
+    };
+    const float w00 = (1.f - wy) * (1.f - wx), w01 = (1.f - wy) * wx;
+    const float w10 = wy * (1.f - wx), w11 = wy * wx;
+    const float v = rd(y0, x0) * w00 + rd(y0, x1) * w01 + rd(y1, x0) * w10 + rd(y1, x1) * w11;
+    y[(long long)n * ystride + (long long)(och + c) * ho * wo + (long long)oy * wo + ox] = v * mul;
+}
+
+// BilinearUpsample.backward (:962-973) as a gather: each input pixel sums
+// the output pixels whose taps reference it (found by scanning the bounded
+// window of rows/columns that map near it).
+__global__ void k_bilinear_bwd(const float* __restrict__ gy, const int* __restrict__ odims,
+                               long long gystride, int ich, int C, float* __restrict__ gx,
+                               const int* __restrict__ idims, long long gxstride, float mul,
+                               int tilesY) {
+    const int n = blockIdx.z;
+    const int c = blockIdx.y / tilesY;
+    const int iy = (blockIdx.y - c * tilesY) * 4 + threadIdx.y;
+    const int ix = blockIdx.x * 64 + threadIdx.x;
+    const int hi = dh(idims, n), wi = dw(idims, n);
+    const int ho = dh(odims, n), wo = dw(odims, n);
+    if (ix >= wi || iy >= hi) return;
+    const float* gs = gy + (long long)n * gystride + (long long)(ich + c) * ho * wo;
+    // output rows whose source coordinate lies within one pixel of iy
+    const double sy = (double)ho / hi, sx = (double)wo / wi;
+    int oy_lo = (int)floor((iy - 1.0 + 0.5) * sy - 0.5) - 1;
+    int oy_hi = (int)ceil((iy + 1.0 + 0.5) * sy - 0.5) + 1;
+    int ox_lo = (int)floor((ix - 1.0 + 0.5) * sx - 0.5) - 1;
+    int ox_hi = (int)ceil((ix + 1.0 + 0.5) * sx - 0.5) + 1;
+    oy_lo = max(oy_lo, 0);
+    ox_lo = max(ox_lo, 0);
+    oy_hi = min(oy_hi, ho - 1);
+    ox_hi = min(ox_hi, wo - 1);
+    // the clamp at 0 maps every output with f < 0 onto row/col 0
+    if (iy == 0) oy_lo = 0;
+    if (ix == 0) ox_lo = 0;
+    if (iy == hi - 1) oy_hi = ho - 1;
+    if (ix == wi - 1) ox_hi = wo - 1;
+    float acc = 0.f;
+    for (int oy = oy_lo; oy <= oy_hi; ++oy) {
+        int y0, y1;
+        float wy;
+        tap(oy, hi, ho, y0, y1, wy);
+        float fy = 0.f;
+        if (y0 == iy) fy += 1.f - wy;
+        if (y1 == iy) fy += wy;
+        if (fy == 0.f && y0 != iy && y1 != iy) continue;
+        for (int ox = ox_lo; ox <= ox_hi; ++ox) {
+            int x0, x1;
+            float wx;
+            tap(ox, wi, wo, x0, x1, wx);
+            float fx = 0.f;
+            if (x0 == ix) fx += 1.f - wx;
+            if (x1 == ix) fx += wx;
+            if (x0 != ix && x1 != ix) continue;
+            acc += gs[(long long)oy * wo + ox] * (fy * fx);
+        }
+    }
+    gx[(long long)n * gxstride + (long long)c * hi * wi + (long long)iy * wi + ix] = acc * mul;
+}
+
+// avgpool2 (:979-995) into output channel och; T as in k_bilinear_fwd.
+template <typename T>
+__global__ void k_avgpool2(const T* __restrict__ x, const int* __restrict__ idims,
+                           long long xstride, float* __restrict__ y, const int* __restrict__ odims,
+                           long long ystride, int och, float mul) {
+    const int n = blockIdx.z;
+    const int oy = blockIdx.y;
+    const int ox = blockIdx.x * 64 + threadIdx.x;
+    const int hi = dh(idims, n), wi = dw(idims, n);
+    const int ho = dh(odims, n), wo = dw(odims, n);
+    if (ox >= wo || oy >= ho) return;
+    const T* r0 = x + (long long)n * xstride + (long long)(2 * oy) * wi + 2 * ox;
+    const T* r1 = r0 + wi;
+    float s = (float)r0[0] + (float)r0[1] + (float)r1[0] + (float)r1[1];
+    if constexpr (sizeof(T) == 1) s *= (1.0f / 255.0f);
+    y[(long long)n * ystride + (long long)och * ho * wo + (long long)oy * wo + ox] = 0.25f * s * mul;
+}
+
+// copy (with uint8 -> value/255) into output channel och
+template <typename T>
+__global__ void k_copy_chan(const T* __restrict__ x, const int* __restrict__ dims,
+                            long long xstride, float* __restrict__ y, long long ystride, int och) {
+    const int n = blockIdx.y;
+    const int h = dh(dims, n), w = dw(dims, n);
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= h * w) return;
+    float v = (float)x[(long long)n * xstride + i];
+    if constexpr (sizeof(T) == 1) v *= (1.0f / 255.0f);
+    y[(long long)n * ystride + (long long)och * h * w + i] = v;
+}
+
+// ------------------------------------------------------------------ losses
+// pseudo_huber (:1326-1351) per sample: writes x/r into grad and per-block
+// float64 partial sums of (r - alpha) and the valid count.
+__global__ void k_ph_partial(const float* __restrict__ pred, const float* __restrict__ target,
+                             const uint8_t* __restrict__ valid, const int* __restrict__ dims,
+                             long long stride, float alpha, float* __restrict__ grad,
+                             double* __restrict__ part, int nblk) {
+    __shared__ double sl[256];
+    __shared__ double sc[256];
+    const int n = blockIdx.y;
+    const int hw = dh(dims, n) * dw(dims, n);
+    double l = 0.0, cnt = 0.0;
+    const long long o = (long long)n * stride;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < hw; i += nblk * 256) {
+        float g = 0.f;
+        if (valid[o + i]) {
+            const float xv = pred[o + i] - target[o + i];
+            const float r = sqrtf(xv * xv + alpha * alpha);
+            l += (double)(r - alpha);
+            g = xv / r;
+            cnt += 1.0;
+        }
+        grad[o + i] = g;
+    }
+    sl[threadIdx.x] = l;
+    sc[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+            sl[threadIdx.x] += sl[threadIdx.x + st];
+            sc[threadIdx.x] += sc[threadIdx.x + st];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[((long long)n * nblk + blockIdx.x) * 2] = sl[0];
+        part[((long long)n * nblk + blockIdx.x) * 2 + 1] = sc[0];
+    }
+}
+
+// per sample: loss = weight * sum / count, scale = weight * gscale / count
+// (0 when no pixel is valid); loss accumulates into out[n].
+__global__ void k_loss_finalize(const double* __restrict__ part, int nblk, int N, double weight,
+                                double gscale, double* __restrict__ loss_out,
+                                float* __restrict__ scale) {
+    const int n = blockIdx.x * 64 + threadIdx.x;
+    if (n >= N) return;
+    double s = 0.0, c = 0.0;
+    for (int b = 0; b < nblk; ++b) {
+        s += part[((long long)n * nblk + b) * 2];
+        c += part[((long long)n * nblk + b) * 2 + 1];
+    }
+    if (c > 0.0) {
+        loss_out[n] += weight * (s / c);
+        scale[n] = (float)(weight * gscale / c);
+    } else {
+        scale[n] = 0.f;
+    }
+}
+
+__global__ void k_scale_samples(float* __restrict__ g, const int* __restrict__ dims,
+                                long long stride, int chans, const float* __restrict__ scale) {
+    const int n = blockIdx.y;
+    const long long m = (long long)chans * dh(dims, n) * dw(dims, n);
+    const float sc = scale[n];
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < m; i += (long long)gridDim.x * 256)
+        g[(long long)n * stride + i] *= sc;
+}
+
+// disparity_ce (:1274-1323) per sample: gcost = q - p (unscaled), partial
+// sums of -q log p (p > 1e-12) and the count of valid pixels with qsum > 0.
+__global__ void k_ce_partial(const float* __restrict__ probs, const float* __restrict__ target,
+                             const uint8_t* __restrict__ valid, const int* __restrict__ dims,
+                             long long pstride, long long tstride, int D, float sigma,
+                             float* __restrict__ gcost, double* __restrict__ part, int nblk) {
+    __shared__ double sl[256];
+    __shared__ double sc[256];
+    const int n = blockIdx.y;
+    const int hw = dh(dims, n) * dw(dims, n);
+    const float inv2s2 = 1.f / (2.f * sigma * sigma);
+    double l = 0.0, cnt = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < hw; i += nblk * 256) {
+        const long long po = (long long)n * pstride + i;
+        bool use = valid[(long long)n * tstride + i] != 0;
+        float qsum = 0.f;
+        const float dg = target[(long long)n * tstride + i];
+        if (use) {
+            for (int d = 0; d < D; ++d) {
+                const float e = (float)d - dg;
+                qsum += __expf(-e * e * inv2s2);
+            }
+            use = qsum > 0.f;
+        }
+        if (!use) {
+            for (int d = 0; d < D; ++d) gcost[po + (long long)d * hw] = 0.f;
+            continue;
+        }
+        const float qinv = 1.f / qsum;
+        for (int d = 0; d < D; ++d) {
+            const float e = (float)d - dg;
+            const float q = __expf(-e * e * inv2s2) * qinv;
+            const float pk = probs[po + (long long)d * hw];
+            if (pk > 1.0e-12f) l -= (double)q * (double)__logf(pk);
+            gcost[po + (long long)d * hw] = q - pk;
+        }
+        cnt += 1.0;
+    }
+    sl[threadIdx.x] = l;
+    sc[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+            sl[threadIdx.x] += sl[threadIdx.x + st];
+            sc[threadIdx.x] += sc[threadIdx.x + st];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[((long long)n * nblk + blockIdx.x) * 2] = sl[0];
+        part[((long long)n * nblk + blockIdx.x) * 2 + 1] = sc[0];
+    }
+}
+
+// -------------------------------------------------------------------- Adam
+// Adam.step (:1368-1387): clip to +-clip, moments, bias-corrected update
+// (c1 = 1/(1-b1^t), c2 = 1/(1-b2^t) from the running products).
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, long long n, float lr, float b1, float b2, float eps,
+                       float c1, float c2, float clip) {
+    const long long i = blockIdx.x * 256LL + threadIdx.x;
+    if (i >= n) return;
+    float gi = g[i];
+    gi = gi > clip ? clip : (gi < -clip ? -clip : gi);
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+}
+
+// ------------------------------------------------------------- selection
+// k-th smallest value per sample (sorted(values)[k], _peak :3866-3869) by
+// a 4-pass 8-bit radix select over order-preserving keys, one block per
+// sample.
+__device__ __forceinline__ uint32_t fkey(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__global__ void __launch_bounds__(1024) k_select(const float* __restrict__ x,
+                                                 const int* __restrict__ dims, long long stride,
+                                                 const long long* __restrict__ kth,
+                                                 float* __restrict__ out) {
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t s_prefix, s_k;
+    const int n = blockIdx.x;
+    const long long m = (long long)dh(dims, n) * dw(dims, n);
+    const float* xs = x + (long long)n * stride;
+    if (threadIdx.x == 0) {
+        s_prefix = 0;
+        s_k = (uint32_t)kth[n];
+    }
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        for (int i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0;
+        __syncthreads();
+        const uint32_t prefix = s_prefix;
+        const uint32_t mask = shift == 24 ? 0u : (0xFFFFFFFFu << (shift + 8));
+        for (long long i = threadIdx.x; i < m; i += blockDim.x) {
+            const uint32_t k = fkey(xs[i]);
+            if ((k & mask) == (prefix & mask)) atomicAdd(&hist[(k >> shift) & 0xFFu], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t kk = s_k, acc = 0;
+            int b = 0;
+            for (; b < 256; ++b) {
+                if (acc + hist[b] > kk) break;
+                acc += hist[b];
+            }
+            s_k = kk - acc;
+            s_prefix = prefix | ((uint32_t)b << shift);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const uint32_t k = s_prefix;
+        const uint32_t u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+        out[n] = __uint_as_float(u);
+    }
+}
+
+// float -> double widening (network disparity -> refinement centre)
+__global__ void k_widen(const float* __restrict__ x, const int* __restrict__ dims,
+                        long long xstride, double* __restrict__ y, long long ystride,
+                        const float* __restrict__ mul) {
+    const int n = blockIdx.y;
+    const int hw = dh(dims, n) * dw(dims, n);
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= hw) return;
+    y[(long long)n * ystride + i] = (double)x[(long long)n * xstride + i] * (double)(mul ? mul[n] : 1.f);
+}
+
+// pool_valid (:1579-1592): a coarse pixel is valid only if all four of its
+// children are
+__global__ void k_pool_valid(const uint8_t* __restrict__ v, const int* __restrict__ idims,
+                             long long vstride, uint8_t* __restrict__ out,
+                             const int* __restrict__ odims, long long ostride) {
+    const int n = blockIdx.z;
+    const int oy = blockIdx.y;
+    const int ox = blockIdx.x * 64 + threadIdx.x;
+    const int wi = dw(idims, n);
+    const int ho = dh(odims, n), wo = dw(odims, n);
+    if (ox >= wo || oy >= ho) return;
+    const uint8_t* r0 = v + (long long)n * vstride + (long long)(2 * oy) * wi + 2 * ox;
+    const uint8_t* r1 = r0 + wi;
+    out[(long long)n * ostride + (long long)oy * wo + ox] = (r0[0] && r0[1] && r1[0] && r1[1]) ? 1 : 0;
+}
+
+// y = a*x + b*y over a flat range (slot tails hold unused values)
+__global__ void k_axpby(const float* __restrict__ x, float* __restrict__ y, long long n, float a,
+                        float b) {
+    const long long i = blockIdx.x * 256LL + threadIdx.x;
+    if (i >= n) return;
+    y[i] = a * x[i] + b * y[i];
+}
+
+inline int tiles4(int h) { return (h + 3) / 4; }
+
+}  // namespace
+
+// =============================================================== C ABI
+extern "C" {
+
+int sv_conv_fwd(const float* x, const float* xr, const int* idims, long long xstride, int IC,
+                int D, int KD, const float* w, const float* b, int OC, int s, int p, int dil,
+                float* y, const int* odims, long long ystride, int N, int max_ho, int max_wo,
+                int mode, float slope, const float* resid, long long rstride, void* stream) {
+    SV_REQUIRE(x && idims && w && b && y && odims && N > 0 && IC > 0 && OC > 0 && D > 0,
+               "sv_conv_fwd: bad arguments");
+    SV_REQUIRE(KD == 1 || KD == 3, "sv_conv_fwd: depth kernel must be 1 or 3");
+    SV_REQUIRE(mode != 2 || resid, "sv_conv_fwd: residual mode needs resid");
+    const int tY = tiles4(max_ho);
+    dim3 grid((max_wo + 63) / 64, tY * D, N), blk(64, 4);
+    hipStream_t st = (hipStream_t)stream;
+    const int src = xr ? 1 : 0;
+#define LAUNCH_FWD(OCB_, SRC_)                                                                \
+    hipLaunchKernelGGL((k_conv_fwd<OCB_, SRC_>), grid, blk, 0, st, x, xr, idims, xstride, IC, D, \
+                       KD, w, b, OC, s, p, dil, y, odims, ystride, tY, mode, slope, resid, rstride)
+    if (src) {
+        if (OC <= 8) LAUNCH_FWD(8, 1); else LAUNCH_FWD(16, 1);
+    } else {
+        if (OC == 1) LAUNCH_FWD(1, 0);
+        else if (OC <= 8) LAUNCH_FWD(8, 0);
+        else if (OC <= 12) LAUNCH_FWD(12, 0);
+        else LAUNCH_FWD(16, 0);
+    }
+#undef LAUNCH_FWD
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+int sv_conv_bwd_data(const float* g, const int* odims, long long gstride, const float* w, int IC,
+                     int OC, int D, int KD, int s, int p, int dil, float* gx, const int* idims,
+                     long long gxstride, int N, int max_hi, int max_wi, int ic_count,
+                     void* stream) {
+    SV_REQUIRE(g && odims && w && gx && idims && N > 0, "sv_conv_bwd_data: bad arguments");
+    const int ICn = ic_count > 0 ? ic_count : IC;
+    const int tY = tiles4(max_hi);
+    dim3 grid((max_wi + 63) / 64, tY * D, N), blk(64, 4);
+    hipStream_t st = (hipStream_t)stream;
+    if (ICn <= 1)
+        hipLaunchKernelGGL(k_conv_bwd_data<1>, grid, blk, 0, st, g, odims, gstride, w, IC, OC, D,
+                           KD, s, p, dil, gx, idims, gxstride, tY, ICn);
+    else if (ICn <= 8)
+        hipLaunchKernelGGL(k_conv_bwd_data<8>, grid, blk, 0, st, g, odims, gstride, w, IC, OC, D,
+                           KD, s, p, dil, gx, idims, gxstride, tY, ICn);
+    else
+        hipLaunchKernelGGL(k_conv_bwd_data<12>, grid, blk, 0, st, g, odims, gstride, w, IC, OC, D,
+                           KD, s, p, dil, gx, idims, gxstride, tY, ICn);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+int sv_conv_bwd_weight_workspace_bytes(int N, int IC, int OC, int KD, int nchunk, size_t* bytes) {
+    SV_REQUIRE(bytes && N > 0 && nchunk > 0, "bad arguments");
+    *bytes = (size_t)N * nchunk * OC * (IC * KD * 9 + 1) * sizeof(float);
+    return SV_OK;
+}
+
+int sv_conv_bwd_weight(const float* x, const float* xr, const int* idims, long long xstride,
+                       int IC, int D, int KD, const float* g, const int* odims, long long gstride,
+                       int OC, int s, int p, int dil, int N, int nchunk, float* gw, float* gb,
+                       float* partials, size_t ws_bytes, void* stream) {
+    SV_REQUIRE(x && idims && g && odims && gw && gb && partials && N > 0 && nchunk > 0,
+               "sv_conv_bwd_weight: bad arguments");
+    SV_REQUIRE(OC <= 48, "sv_conv_bwd_weight: at most 48 output channels");
+    size_t need = 0;
+    sv_conv_bwd_weight_workspace_bytes(N, IC, OC, KD, nchunk, &need);
+    if (ws_bytes < need) return sv::set_error(SV_EWORKSPACE, "conv weight gradient needs %zu bytes", need);
+    const int ncol = IC * KD * 9;
+    dim3 grid(nchunk, (ncol + WG_COLS - 1) / WG_COLS, N);
+    hipStream_t st = (hipStream_t)stream;
+#define LAUNCH_BW(SRC_, OCM_)                                                                  \
+    hipLaunchKernelGGL((k_conv_bwd_weight<SRC_, OCM_>), grid, dim3(WG_COLS), 0, st, x, xr, idims, \
+                       xstride, IC, D, KD, g, odims, gstride, OC, s, p, dil, nchunk, partials)
+    if (xr) {
+        if (OC <= 8) LAUNCH_BW(1, 8); else if (OC <= 24) LAUNCH_BW(1, 24); else LAUNCH_BW(1, 48);
+    } else {
+        if (OC <= 1) LAUNCH_BW(0, 1);
+        else if (OC <= 8) LAUNCH_BW(0, 8);
+        else if (OC <= 12) LAUNCH_BW(0, 12);
+        else if (OC <= 24) LAUNCH_BW(0, 24);
+        else LAUNCH_BW(0, 48);
+    }
+#undef LAUNCH_BW
+    SV_CHECK_LAUNCH();
+    const int tot = OC * (ncol + 1);
+    hipLaunchKernelGGL(k_reduce_partials, dim3((tot + 255) / 256), dim3(256), 0, st, partials,
+                       N * nchunk, OC, ncol, gw, gb);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+int sv_act_bwd(float* g, const float* y, long long n, float slope, void* stream) {
+    SV_REQUIRE(g && y && n >= 0, "sv_act_bwd: bad arguments");
+    if (n == 0) return SV_OK;
+    hipLaunchKernelGGL(k_act_bwd, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, g, y, n, slope);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+int sv_costvol_bwd(const float* fl, const float* fr, const int* dims, long long fstride, int C,
+                   int D, const float* gvol, long long gvstride, float* gfl, float* gfr, int N,
+                   int max_h, int max_w, void* stream) {
+    SV_REQUIRE(fl && fr && dims && gvol && gfl && gfr && N > 0, "sv_costvol_bwd: bad arguments");
+    const int tY = tiles4(max_h);
+    hipLaunchKernelGGL(k_costvol_bwd, dim3((max_w + 63) / 64, tY * C, N), dim3(64, 4), 0,
+                       (hipStream_t)stream, fl, fr, dims, fstride, C, D, gvol, gvstride, gfl, gfr,
+                       tY);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+int sv_softargmin_fwd(const float* cost, const int* dims, long long cstride, int D,
+                      float penalty, float* disp, long long dstride, float* probs, int N,
+                      int max_hw, void* stream) {
+    SV_REQUIRE(cost && dims && disp && probs && N > 0, "sv_softargmin_fwd: bad arguments");
+    hipLaunchKernelGGL(k_softargmin_fwd, dim3((max_hw + 255) / 256, N), dim3(256), 0,
+                       (hipStream_t)stream, cost, dims, cstride, D, penalty, disp, dstride, probs);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+int sv_softargmin_bwd(const float* probs, const float* disp, const float* gdisp,
+                      const float* gextra, const int* dims, long long cstride, int D,
+                      long long dstride, float* gcost, int N, int max_hw, void* stream) {
+    SV_REQUIRE(probs && disp && gdisp && dims && gcost && N > 0, "sv_softargmin_bwd: bad arguments");
+    hipLaunchKernelGGL(k_softargmin_bwd, dim3((max_hw + 255) / 256, N), dim3(256), 0,
+                       (hipStream_t)stream, probs, disp, gdisp, gextra, dims, cstride, D, dstride,
+                       gcost);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+int sv_bilinear_fwd(const void* x, int dtype, const int* idims, long long xstride, int C,
+                    float* y, const int* odims, long long ystride, int och, float mul,
+                    const float* muln, int N, int max_ho, int max_wo, void* stream) {
+    SV_REQUIRE(x && idims && y && odims && N > 0 && C > 0, "sv_bilinear_fwd: bad arguments");
+    const int tY = tiles4(max_ho);
+    dim3 grid((max_wo + 63) / 64, tY * C, N), blk(64, 4);
+    hipStream_t st = (hipStream_t)stream;
+    if (dtype == SV_U8)
+        hipLaunchKernelGGL(k_bilinear_fwd<uint8_t>, grid, blk, 0, st, (const uint8_t*)x, idims,
+                           xstride, C, y, odims, ystride, och, mul, muln, tY);
+    else if (dtype == SV_F32)
+        hipLaunchKernelGGL(k_bilinear_fwd<float>, grid, blk, 0, st, (const float*)x, idims, xstride,
+                           C, y, odims, ystride, och, mul, muln, tY);
+    else
+        return sv::set_error(SV_EINVAL, "sv_bilinear_fwd: dtype %d", dtype);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+int sv_bilinear_bwd(const float* gy, const int* odims, long long gystride, int ich, int C,
+                    float* gx, const int* idims, long long gxstride, float mul, int N, int max_hi,
+                    int max_wi, void* stream) {
+    SV_REQUIRE(gy && odims && gx && idims && N > 0, "sv_bilinear_bwd: bad arguments");
+    const int tY = tiles4(max_hi);
+    hipLaunchKernelGGL(k_bilinear_bwd, dim3((max_wi + 63) / 64, tY * C, N), dim3(64, 4), 0,
+                       (hipStream_t)stream, gy, odims, gystride, ich, C, gx, idims, gxstride, mul,
+                       tY);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+int sv_avgpool2(const void* x, int dtype, const int* idims, long long xstride, float* y,
+                const int* odims, long long ystride, int och, float mul, int N, int max_ho,
+                int max_wo, void* stream) {
+    SV_REQUIRE(x && idims && y && odims && N > 0, "sv_avgpool2: bad arguments");
+    dim3 grid((max_wo + 63) / 64, max_ho, N);
+    hipStream_t st = (hipStream_t)stream;
+    if (dtype == SV_U8)
+        hipLaunchKernelGGL(k_avgpool2<uint8_t>, grid, dim3(64), 0, st, (const uint8_t*)x, idims,
+                           xstride, y, odims, ystride, och, mul);
+    else if (dtype == SV_F32)
+        hipLaunchKernelGGL(k_avgpool2<float>, grid, dim3(64), 0, st, (const float*)x, idims, xstride,
+                           y, odims, ystride, och, mul);
+    else
+        return sv::set_error(SV_EINVAL, "sv_avgpool2: dtype %d", dtype);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+int sv_copy_channel(const void* x, int dtype, const int* dims, long long xstride, float* y,
+                    long long ystride, int och, int N, int max_hw, void* stream) {
+    SV_REQUIRE(x && dims && y && N > 0, "sv_copy_channel: bad arguments");
+    dim3 grid((max_hw + 255) / 256, N);
+    hipStream_t st = (hipStream_t)stream;
+    if (dtype == SV_U8)
+        hipLaunchKernelGGL(k_copy_chan<uint8_t>, grid, dim3(256), 0, st, (const uint8_t*)x, dims,
+                           xstride, y, ystride, och);
+    else if (dtype == SV_F32)
+        hipLaunchKernelGGL(k_copy_chan<float>, grid, dim3(256), 0, st, (const float*)x, dims,
+                           xstride, y, ystride, och);
+    else
+        return sv::set_error(SV_EINVAL, "sv_copy_channel: dtype %d", dtype);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+// pseudo_huber over N samples: grad (scaled by weight*gscale/count per
+// sample) and loss_out[n] += weight * loss_n.  part: 2*N*nblk doubles,
+// scale: N floats.
+int sv_pseudo_huber(const float* pred, const float* target, const uint8_t* valid,
+                    const int* dims, long long stride, int N, float alpha, double weight,
+                    double gscale, float* grad, double* loss_out, double* part, float* scale,
+                    int nblk, void* stream) {
+    SV_REQUIRE(pred && target && valid && dims && grad && loss_out && part && scale && N > 0 &&
+                   nblk > 0,
+               "sv_pseudo_huber: bad arguments");
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_ph_partial, dim3(nblk, N), dim3(256), 0, st, pred, target, valid, dims,
+                       stride, alpha, grad, part, nblk);
+    SV_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_loss_finalize, dim3((N + 63) / 64), dim3(64), 0, st, part, nblk, N,
+                       weight, gscale, loss_out, scale);
+    SV_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_scale_samples, dim3(64, N), dim3(256), 0, st, grad, dims, stride, 1, scale);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+// disparity_ce over N samples; gcost scaled by weight*gscale/count.
+int sv_disparity_ce(const float* probs, const float* target, const uint8_t* valid,
+                    const int* dims, long long pstride, long long tstride, int D, float sigma,
+                    int N, double weight, double gscale, float* gcost, double* loss_out,
+                    double* part, float* scale, int nblk, void* stream) {
+    SV_REQUIRE(probs && target && valid && dims && gcost && loss_out && part && scale && N > 0,
+               "sv_disparity_ce: bad arguments");
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_ce_partial, dim3(nblk, N), dim3(256), 0, st, probs, target, valid, dims,
+                       pstride, tstride, D, sigma, gcost, part, nblk);
+    SV_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_loss_finalize, dim3((N + 63) / 64), dim3(64), 0, st, part, nblk, N,
+                       weight, gscale, loss_out, scale);
+    SV_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_scale_samples, dim3(64, N), dim3(256), 0, st, gcost, dims, pstride, D, scale);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+int sv_adam_step(float* p, const float* g, float* m, float* v, long long n, float lr, float b1,
+                 float b2, float eps, float c1, float c2, float clip, void* stream) {
+    SV_REQUIRE(p && g && m && v && n > 0, "sv_adam_step: bad arguments");
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, p, g, m, v, n, lr, b1, b2, eps, c1, c2, clip);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+int sv_select_kth(const float* x, const int* dims, long long stride, const long long* kth,
+                  float* out, int N, void* stream) {
+    SV_REQUIRE(x && dims && kth && out && N > 0, "sv_select_kth: bad arguments");
+    hipLaunchKernelGGL(k_select, dim3(N), dim3(1024), 0, (hipStream_t)stream, x, dims, stride, kth,
+                       out);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+int sv_pool_valid(const uint8_t* v, const int* idims, long long vstride, uint8_t* out,
+                  const int* odims, long long ostride, int N, int max_ho, int max_wo,
+                  void* stream) {
+    SV_REQUIRE(v && idims && out && odims && N > 0, "sv_pool_valid: bad arguments");
+    hipLaunchKernelGGL(k_pool_valid, dim3((max_wo + 63) / 64, max_ho, N), dim3(64), 0,
+                       (hipStream_t)stream, v, idims, vstride, out, odims, ostride);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+int sv_axpby(const float* x, float* y, long long n, float a, float b, void* stream) {
+    SV_REQUIRE(x && y && n >= 0, "sv_axpby: bad arguments");
+    if (n == 0) return SV_OK;
+    hipLaunchKernelGGL(k_axpby, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, x, y, n, a, b);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+int sv_widen(const float* x, const int* dims, long long xstride, double* y, long long ystride,
+             const float* mul, int N, int max_hw, void* stream) {
+    SV_REQUIRE(x && dims && y && N > 0, "sv_widen: bad arguments");
+    hipLaunchKernelGGL(k_widen, dim3((max_hw + 255) / 256, N), dim3(256), 0, (hipStream_t)stream,
+                       x, dims, xstride, y, ystride, mul);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,1195 @@
+// Full-resolution census + band semi-global matching + left-right check for
+// gfx950 (MI355X).  Restates stereonet.py:5115-5391 and :2231-2258
+// (Ray-5106/From-Scratch-Stereo-Vision) as batched GPU kernels.
+//
+// Data layout in HBM (B pairs of W x H, V views, K = 2R+1 band entries,
+// Kp = G*KC padded band length):
+//   images      [B][H][W]            u8 / f32 / f64 (row-major, as read)
+//   census      [2][B][H][W]         u64, MSB = first window offset
+//   base        [V][B][H][W]         i32 band centre max(0, int(d0 + 0.5))
+//   costs       [V][B][H][W][Kp]     u8 (integer path) or f64 (general path)
+//   path costs  [dir][V][B][H][W][Kp] u8/u16 (integer) or f64 (general)
+//   disparity   [V][B][H][W]         f64 (parity with the float64 reference)
+//   status      [B][H][W]            u8 in {0, 1, 2}
+//
+// The right view of lr_refine is matched in the left image's coordinates:
+// census words of a mirrored image are a fixed bit permutation of the
+// unmirrored words, and Hamming distance is invariant under that
+// permutation, so costs, path sums and winners equal the reference's
+// mirrored computation without building mirrored copies.
+//
+// Two arithmetic paths: the integer path (ad_weight = prior_w = 0 and
+// integral, non-negative P1/P2: every cost is a popcount, every path cost an
+// integer, so results are bit-exact), and the float64 general path for
+// everything else, which evaluates the reference's expressions in the same
+// order so results are bit-exact as well.  This file is compiled with
+// -ffp-contract=off: no multiply-add is ever fused.
+#include <climits>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+
+#include "common.h"
+
+namespace {
+
+__constant__ int c_dx[8] = {1, -1, 0, 0, 1, -1, 1, -1};
+__constant__ int c_dy[8] = {0, 0, 1, -1, 1, 1, -1, -1};
+// summation order of the eight paths: the reference order for the left view;
+// for the right view the order the mirrored computation visits them in
+// (dx negated: 0<->1, 4<->5, 6<->7)
+__constant__ int c_order[2][8] = {{0, 1, 2, 3, 4, 5, 6, 7}, {1, 0, 2, 3, 5, 4, 7, 6}};
+
+template <typename T>
+__device__ __forceinline__ double to_gray(T v);
+template <>
+__device__ __forceinline__ double to_gray<uint8_t>(uint8_t v) {
+    return (double)v * (1.0 / 255.0);
+}
+template <>
+__device__ __forceinline__ double to_gray<float>(float v) {
+    return (double)v;
+}
+template <>
+__device__ __forceinline__ double to_gray<double>(double v) {
+    return v;
+}
+
+template <typename A>
+__device__ __forceinline__ A amin(A a, A b) {
+    return b < a ? b : a;
+}
+
+template <typename A>
+__device__ __forceinline__ A big();
+template <>
+__device__ __forceinline__ int big<int>() {
+    return 0x3FFFFFFF;
+}
+template <>
+__device__ __forceinline__ double big<double>() {
+    return HUGE_VAL;
+}
+
+// ------------------------------------------------------------------ census
+// census, stereonet.py:5115-5142: one thread per pixel; the window is read
+// through L1/L2 (each row segment is reused by 2*rx+1 neighbouring lanes).
+template <typename T>
+__global__ void __launch_bounds__(64) k_census(const T* __restrict__ img, int W, int H,
+                                               int rx, int ry, uint64_t* __restrict__ out) {
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= W) return;
+    const long long plane = (long long)W * H;
+    const T* im = img + (long long)blockIdx.z * plane;
+    const T c = im[(long long)y * W + x];
+    uint64_t s = 0;
+    for (int dy = -ry; dy <= ry; ++dy) {
+        const int yy = min(max(y + dy, 0), H - 1);
+        const T* row = im + (long long)yy * W;
+        for (int dx = -rx; dx <= rx; ++dx) {
+            if (dx == 0 && dy == 0) continue;
+            const int xx = min(max(x + dx, 0), W - 1);
+            s = (s << 1) | (uint64_t)(row[xx] < c);
+        }
+    }
+    out[(long long)blockIdx.z * plane + (long long)y * W + x] = s;
+}
+
+// band centre, stereonet.py:5254-5257: max(0, int(disp0 + 0.5)), int()
+// truncating toward zero.  Values beyond 2^30 (and NaN) put the whole band
+// outside the image, which is what the reference's huge centre does.
+__global__ void k_base(const double* __restrict__ d0, long long n, int* __restrict__ base) {
+    const long long i = blockIdx.x * 256LL + threadIdx.x;
+    if (i >= n) return;
+    const double t = d0[i] + 0.5;
+    int v;
+    if (!(t < 1073741824.0))
+        v = 1 << 30;
+    else if (t < 1.0)
+        v = 0;
+    else
+        v = (int)t;
+    base[i] = v;
+}
+
+// ------------------------------------------------------------- band costs
+// band_costs, stereonet.py:5156-5189, integer path: C = popcount(a ^ b) or
+// `invalid` where the candidate leaves the image.  RIGHT selects the right
+// view matched in left coordinates: partner column x + d instead of x - d.
+template <bool RIGHT>
+__global__ void __launch_bounds__(64) k_costs_int(const uint64_t* __restrict__ cA,
+                                                  const uint64_t* __restrict__ cB,
+                                                  const int* __restrict__ base, int W, int H,
+                                                  int R, int K, int Kp, int invalid,
+                                                  uint8_t* __restrict__ C) {
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= W) return;
+    const long long plane = (long long)W * H;
+    const long long pix = (long long)blockIdx.z * plane + (long long)y * W + x;
+    const uint64_t a = cA[pix];
+    const int b0 = base[pix] - R;
+    const uint64_t* rowB = cB + (long long)blockIdx.z * plane + (long long)y * W;
+    uint8_t* o = C + pix * Kp;
+    for (int k0 = 0; k0 < Kp; k0 += 4) {
+        uint32_t packed = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int k = k0 + j;
+            uint32_t v = 0;
+            if (k < K) {
+                const int d = b0 + k;
+                const int xr = RIGHT ? x + d : x - d;
+                const bool ok = d >= 0 && xr >= 0 && xr < W;
+                v = ok ? (uint32_t)__popcll(a ^ rowB[ok ? xr : 0]) : (uint32_t)invalid;
+            }
+            packed |= v << (8 * j);
+        }
+        *reinterpret_cast<uint32_t*>(o + k0) = packed;
+    }
+}
+
+// band_costs, general float64 path (absolute-difference and prior terms),
+// evaluated in the reference's operation order.
+template <bool RIGHT, typename T>
+__global__ void __launch_bounds__(64) k_costs_f64(const uint64_t* __restrict__ cA,
+                                                  const uint64_t* __restrict__ cB,
+                                                  const T* __restrict__ gA, const T* __restrict__ gB,
+                                                  const int* __restrict__ base,
+                                                  const double* __restrict__ prior, int W, int H,
+                                                  int R, int K, int Kp, double ad_weight,
+                                                  double ad_trunc, double invalid, double prior_w,
+                                                  double prior_t, double* __restrict__ C) {
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= W) return;
+    const long long plane = (long long)W * H;
+    const long long row0 = (long long)blockIdx.z * plane + (long long)y * W;
+    const long long pix = row0 + x;
+    const uint64_t a = cA[pix];
+    const double il = to_gray<T>(gA[pix]);
+    const int b0 = base[pix] - R;
+    const double pr = prior_w > 0.0 ? prior[pix] : 0.0;
+    double* o = C + pix * Kp;
+    for (int k = 0; k < Kp; ++k) {
+        double c = invalid;
+        if (k >= K) {
+            c = 0.0;
+        } else {
+            const int d = b0 + k;
+            const int xr = RIGHT ? x + d : x - d;
+            if (d >= 0 && xr >= 0 && xr < W) {
+                double ad = il - to_gray<T>(gB[row0 + xr]);
+                if (ad < 0) ad = -ad;
+                ad = ad * 255.0;
+                if (ad > ad_trunc) ad = ad_trunc;
+                c = (double)__popcll(a ^ cB[row0 + xr]) + ad_weight * ad;
+                if (prior_w > 0.0) {
+                    double e = (double)(b0 + k) - pr;
+                    if (e < 0) e = -e;
+                    if (e > prior_t) e = prior_t;
+                    c += prior_w * e;
+                }
+            }
+        }
+        o[k] = c;
+    }
+}
+
+// ------------------------------------------------------------- SGM paths
+template <typename CT, typename AT, int KC>
+__device__ __forceinline__ void load_chunk(const CT* __restrict__ p, AT (&c)[KC]);
+
+template <typename AT, int KC>
+__device__ __forceinline__ void load_chunk_u8(const uint8_t* __restrict__ p, AT (&c)[KC]) {
+#pragma unroll
+    for (int q = 0; q < KC / 8; ++q) {
+        const uint2 v = *reinterpret_cast<const uint2*>(p + 8 * q);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            c[8 * q + t] = (AT)((v.x >> (8 * t)) & 0xFFu);
+            c[8 * q + 4 + t] = (AT)((v.y >> (8 * t)) & 0xFFu);
+        }
+    }
+}
+
+template <typename AT, int KC>
+__device__ __forceinline__ void load_chunk_f64(const double* __restrict__ p, AT (&c)[KC]) {
+#pragma unroll
+    for (int q = 0; q < KC; ++q) c[q] = (AT)p[q];
+}
+
+template <typename LT, typename AT, int KC>
+__device__ __forceinline__ void store_chunk(LT* __restrict__ p, const AT (&L)[KC], int nvalid);
+
+template <typename AT, int KC>
+__device__ __forceinline__ void store_u8(uint8_t* __restrict__ p, const AT (&L)[KC], int nvalid) {
+#pragma unroll
+    for (int q = 0; q < KC / 8; ++q) {
+        uint2 v = {0u, 0u};
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const uint32_t a = (8 * q + t) < nvalid ? ((uint32_t)L[8 * q + t] & 0xFFu) : 0u;
+            const uint32_t b = (8 * q + 4 + t) < nvalid ? ((uint32_t)L[8 * q + 4 + t] & 0xFFu) : 0u;
+            v.x |= a << (8 * t);
+            v.y |= b << (8 * t);
+        }
+        *reinterpret_cast<uint2*>(p + 8 * q) = v;
+    }
+}
+
+template <typename AT, int KC>
+__device__ __forceinline__ void store_u16(uint16_t* __restrict__ p, const AT (&L)[KC], int nvalid) {
+#pragma unroll
+    for (int q = 0; q < KC / 4; ++q) {
+        uint2 v = {0u, 0u};
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const uint32_t a = (4 * q + t) < nvalid ? ((uint32_t)L[4 * q + t] & 0xFFFFu) : 0u;
+            const uint32_t b = (4 * q + 2 + t) < nvalid ? ((uint32_t)L[4 * q + 2 + t] & 0xFFFFu) : 0u;
+            v.x |= a << (16 * t);
+            v.y |= b << (16 * t);
+        }
+        *reinterpret_cast<uint2*>(p + 4 * q) = v;
+    }
+}
+
+template <typename AT, int KC>
+__device__ __forceinline__ void store_f64(double* __restrict__ p, const AT (&L)[KC], int nvalid) {
+#pragma unroll
+    for (int q = 0; q < KC; ++q) p[q] = q < nvalid ? (double)L[q] : 0.0;
+}
+
+template <typename CT, typename AT, int KC>
+__device__ __forceinline__ void load_any(const CT* p, AT (&c)[KC]) {
+    if constexpr (sizeof(CT) == 1)
+        load_chunk_u8<AT, KC>(reinterpret_cast<const uint8_t*>(p), c);
+    else
+        load_chunk_f64<AT, KC>(reinterpret_cast<const double*>(p), c);
+}
+
+template <typename LT, typename AT, int KC>
+__device__ __forceinline__ void store_any(LT* p, const AT (&L)[KC], int nvalid) {
+    if constexpr (sizeof(LT) == 1)
+        store_u8<AT, KC>(reinterpret_cast<uint8_t*>(p), L, nvalid);
+    else if constexpr (sizeof(LT) == 2)
+        store_u16<AT, KC>(reinterpret_cast<uint16_t*>(p), L, nvalid);
+    else
+        store_f64<AT, KC>(reinterpret_cast<double*>(p), L, nvalid);
+}
+
+// _aggregate_path, stereonet.py:5192-5241, for every direction of a launch at
+// once (blockIdx.y = direction).  G lanes own one scanline; lane j holds band
+// entries [j*KC, (j+1)*KC) of the current pixel in registers.  The previous
+// pixel's neighbour-minimum M[m] = min(L[m], L[m-1]+P1, L[m+1]+P1) (m in
+// [-1, K]) is staged in LDS so that the re-centring shift
+// s = base[p] - base[q] becomes an indexed read M[k+s], clamped into a
+// sentinel slot outside [-1, K]; then
+//   L[k] = C[k] + min(M[k+s], min_k' L_q[k'] + P2) - min_k' L_q[k'],
+// which is the reference's candidate set in one read.  Scanlines:
+// horizontal directions run one line per row; vertical and diagonal ones run
+// one line per column, stepping rows, and a diagonal that leaves the image
+// wraps to the one entering at the opposite edge (a border pixel, where the
+// recurrence restarts with L = C), so every line has exactly H steps.
+template <typename CT, typename LT, typename AT, int KC>
+__global__ void __launch_bounds__(64) k_sgm(const CT* __restrict__ C, const int* __restrict__ base,
+                                            LT* __restrict__ Lout, int W, int H, int K, int G,
+                                            AT P1, AT P2, int dir0, long long dirStride) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    AT* lds = reinterpret_cast<AT*>(smem_raw);
+    const int Kp = G * KC;
+    const int dir = dir0 + (int)blockIdx.y;
+    const int dx = c_dx[dir], dy = c_dy[dir];
+    const int lane = threadIdx.x;
+    const int lpw = 64 / G;
+    const int li = lane / G;
+    const int j = lane - li * G;
+    const bool horiz = dy == 0;
+    const int nLines = horiz ? H : W;
+    const int len = horiz ? W : H;
+    if ((int)blockIdx.x * lpw >= nLines) return;
+    const int line = blockIdx.x * lpw + li;
+    const bool active = line < nLines;
+    const int stride = Kp + 5;
+    AT* slot = lds + li * stride;
+    const AT INF = big<AT>();
+    if (j == 0) {
+        slot[0] = INF;
+        slot[K + 3] = INF;
+    }
+    const long long plane = (long long)W * H;
+    const CT* Cb = C + (long long)blockIdx.z * plane * Kp;
+    const int* Bb = base + (long long)blockIdx.z * plane;
+    LT* Lb = Lout + (long long)blockIdx.y * dirStride + (long long)blockIdx.z * plane * Kp;
+    const int kbase = j * KC;
+    AT L[KC];
+    AT mprev = 0;
+    int bprev = 0;
+    for (int i = 0; i < len; ++i) {
+        int x, y;
+        bool border;
+        if (horiz) {
+            y = line;
+            x = dx > 0 ? i : W - 1 - i;
+            border = i == 0;
+        } else {
+            y = dy > 0 ? i : H - 1 - i;
+            if (dx == 0) {
+                x = line;
+                border = i == 0;
+            } else {
+                int t = (dx > 0 ? line + i : line - i) % W;
+                if (t < 0) t += W;
+                x = t;
+                border = i == 0 || (dx > 0 ? x == 0 : x == W - 1);
+            }
+        }
+        if (!active) {
+            x = 0;
+            y = 0;
+        }
+        const long long pix = (long long)y * W + x;
+        AT c[KC];
+        load_any<CT, AT, KC>(Cb + pix * Kp + kbase, c);
+        const int b = Bb[pix];
+        if (border) {
+#pragma unroll
+            for (int kk = 0; kk < KC; ++kk) L[kk] = c[kk];
+        } else {
+            const int s = b - bprev;
+            const AT cap = mprev + P2;
+#pragma unroll
+            for (int kk = 0; kk < KC; ++kk) {
+                const int idx = min(max(kbase + kk + s + 2, 0), K + 3);
+                const AT best = amin(slot[idx], cap);
+                L[kk] = c[kk] + best - mprev;
+            }
+        }
+#pragma unroll
+        for (int kk = 0; kk < KC; ++kk)
+            if (kbase + kk >= K) L[kk] = INF;
+        AT ml = L[0];
+#pragma unroll
+        for (int kk = 1; kk < KC; ++kk) ml = amin(ml, L[kk]);
+        for (int off = G >> 1; off > 0; off >>= 1) ml = amin(ml, __shfl_xor(ml, off, G));
+        AT leftN = __shfl_up(L[KC - 1], 1, G);
+        AT rightN = __shfl_down(L[0], 1, G);
+        if (j == 0) leftN = INF;
+        if (j == G - 1) rightN = INF;
+        if (G > 1) __syncthreads();
+        if (active) {
+#pragma unroll
+            for (int kk = 0; kk < KC; ++kk) {
+                const int k = kbase + kk;
+                const AT lm = kk > 0 ? L[kk - 1] : leftN;
+                const AT lp = kk < KC - 1 ? L[kk + 1] : rightN;
+                const AT m = amin(L[kk], amin(lm, lp) + P1);
+                if (k < K) slot[k + 2] = m;
+                if (k == 0) slot[1] = L[kk] + P1;
+                if (k == K - 1) slot[K + 2] = L[kk] + P1;
+            }
+            store_any<LT, AT, KC>(Lb + pix * Kp + kbase, L, K - kbase);
+        }
+        if (G > 1) __syncthreads();
+        mprev = ml;
+        bprev = b;
+    }
+}
+
+// ------------------------------------------------------------------- WTA
+// refine_fullres epilogue, stereonet.py:5271-5302: path sums S, first
+// minimum, band-edge passthrough, second best outside bk+-1, confidence,
+// uniqueness, parabola.  G lanes per pixel, KC entries per lane.  All
+// float64 expressions follow the reference's order.
+template <typename LT, typename AT, int KC>
+__global__ void __launch_bounds__(64) k_wta(const LT* __restrict__ Lv, long long dirStride,
+                                            int ndir, int mirrorFrom, const int* __restrict__ base,
+                                            const double* __restrict__ d0a,
+                                            const double* __restrict__ d0b, int B, int HW, int K,
+                                            int G, int R, double uniq, double* __restrict__ out,
+                                            double* __restrict__ conf) {
+    const int lane = threadIdx.x;
+    const int lpw = 64 / G;
+    const int li = lane / G;
+    const int j = lane - li * G;
+    const int Kp = G * KC;
+    const int bv = blockIdx.y;
+    int p = blockIdx.x * lpw + li;
+    const bool active = p < HW;
+    if (!active) p = HW - 1;
+    const long long pix = (long long)bv * HW + p;
+    const int kbase = j * KC;
+    const int ord = bv >= mirrorFrom ? 1 : 0;
+    AT S[KC];
+#pragma unroll
+    for (int kk = 0; kk < KC; ++kk) S[kk] = 0;
+    for (int t = 0; t < ndir; ++t) {
+        const int dd = c_order[ord][t];
+        AT l[KC];
+        const LT* src = Lv + (long long)dd * dirStride + pix * Kp + kbase;
+        if constexpr (sizeof(LT) == 1) {
+            load_chunk_u8<AT, KC>(reinterpret_cast<const uint8_t*>(src), l);
+        } else if constexpr (sizeof(LT) == 2) {
+#pragma unroll
+            for (int kk = 0; kk < KC; ++kk) l[kk] = (AT)reinterpret_cast<const uint16_t*>(src)[kk];
+        } else {
+            load_chunk_f64<AT, KC>(reinterpret_cast<const double*>(src), l);
+        }
+#pragma unroll
+        for (int kk = 0; kk < KC; ++kk) S[kk] = S[kk] + l[kk];
+    }
+    const AT INF = big<AT>();
+    AT bv_ = INF;
+    int bk = INT_MAX;
+#pragma unroll
+    for (int kk = 0; kk < KC; ++kk)
+        if (kbase + kk < K && S[kk] < bv_) {
+            bv_ = S[kk];
+            bk = kbase + kk;
+        }
+    for (int off = G >> 1; off > 0; off >>= 1) {
+        const AT ov = __shfl_xor(bv_, off, G);
+        const int ok = __shfl_xor(bk, off, G);
+        if (ov < bv_ || (ov == bv_ && ok < bk)) {
+            bv_ = ov;
+            bk = ok;
+        }
+    }
+    AT sv = INF;
+    int found = 0;
+    AT a = INF, b = INF;
+#pragma unroll
+    for (int kk = 0; kk < KC; ++kk) {
+        const int k = kbase + kk;
+        if (k < K && (k < bk - 1 || k > bk + 1)) {
+            found = 1;
+            if (S[kk] < sv) sv = S[kk];
+        }
+        if (k == bk - 1) a = S[kk];
+        if (k == bk + 1) b = S[kk];
+    }
+    for (int off = G >> 1; off > 0; off >>= 1) {
+        sv = amin(sv, __shfl_xor(sv, off, G));
+        a = amin(a, __shfl_xor(a, off, G));
+        b = amin(b, __shfl_xor(b, off, G));
+        found |= __shfl_xor(found, off, G);
+    }
+    if (!active || j != 0) return;
+    const int v = bv >= B ? 1 : 0;
+    const double d0 = (v ? d0b : d0a)[(long long)(bv - v * B) * HW + p];
+    double o, cf;
+    if (bk == 0 || bk == K - 1) {
+        o = d0;
+        cf = 0.0;
+    } else {
+        const double bvd = (double)bv_;
+        cf = (found && (double)sv < 1.0e29) ? ((double)sv - bvd) / (bvd + 1.0e-6) : 1.0;
+        if (cf < uniq) {
+            o = d0;
+        } else {
+            const double ad = (double)a, bd = (double)b;
+            const double den = ad - 2.0 * bvd + bd;
+            double off = 0.0;
+            if (den > 1.0e-9) off = 0.5 * (ad - bd) / den;
+            o = (double)(base[pix] - R + bk) + off;
+        }
+    }
+    out[pix] = o;
+    if (conf) conf[pix] = cf;
+}
+
+// ------------------------------------------------------------ LR check
+// int(x - d + 0.5) in [0, W)  <=>  -1 < x - d + 0.5 < W (truncation).
+__device__ __forceinline__ int warp_col(int x, double d, int W) {
+    const double t = (double)x - d + 0.5;
+    if (!(t > -1.0) || !(t < (double)W)) return -1;
+    return (int)t;
+}
+
+// left_to_right, stereonet.py:5317-5340: one thread per row.  A z-buffer
+// forward warp (strict >, init -1), then each run of holes (value < 0) takes
+// min(nearest non-hole on the left, on the right), 0.0 if neither is below
+// 1e8 -- the closed form of the reference's in-place cascade.
+__global__ void k_left_to_right(const double* __restrict__ dl, int B, int W, int H,
+                                double* __restrict__ dr) {
+    const int y = blockIdx.x * 64 + threadIdx.x;
+    const int b = blockIdx.y;
+    if (y >= H) return;
+    const long long r = ((long long)b * H + y) * W;
+    for (int x = 0; x < W; ++x) dr[r + x] = -1.0;
+    for (int x = 0; x < W; ++x) {
+        const double d = dl[r + x];
+        const int xr = warp_col(x, d, W);
+        if (xr >= 0 && d > dr[r + xr]) dr[r + xr] = d;
+    }
+    int x = 0;
+    while (x < W) {
+        if (!(dr[r + x] < 0.0)) {
+            ++x;
+            continue;
+        }
+        int e = x;
+        while (e < W && dr[r + e] < 0.0) ++e;
+        const double va = x > 0 ? dr[r + x - 1] : 1.0e9;
+        const double vb = e < W ? dr[r + e] : 1.0e9;
+        double v = va < vb ? va : vb;
+        v = v < 1.0e8 ? v : 0.0;
+        for (int t = x; t < e; ++t) dr[r + t] = v;
+        x = e;
+    }
+}
+
+// lr_refine check and fills, stereonet.py:5358-5390: one thread per row.
+__global__ void k_lr_check_fill(const double* __restrict__ dl, const double* __restrict__ dr,
+                                const double* __restrict__ disp0, int B, int W, int H, double tol,
+                                double* __restrict__ out, uint8_t* __restrict__ status) {
+    const int y = blockIdx.x * 64 + threadIdx.x;
+    const int b = blockIdx.y;
+    if (y >= H) return;
+    const long long r = ((long long)b * H + y) * W;
+    for (int x = 0; x < W; ++x) {
+        const double d = dl[r + x];
+        const int xr = warp_col(x, d, W);
+        uint8_t st = 0;
+        if (xr < 0) {
+            st = 1;
+        } else {
+            const double e = dr[r + xr];
+            if (!(fabs(d - e) <= tol)) st = e > d ? 1 : 2;
+        }
+        status[r + x] = st;
+        out[r + x] = st == 2 ? disp0[r + x] : d;
+    }
+    int x = 0;
+    while (x < W) {
+        if (status[r + x] != 1) {
+            ++x;
+            continue;
+        }
+        int e = x;
+        while (e < W && status[r + e] == 1) ++e;
+        const double va = x > 0 ? out[r + x - 1] : 1.0e9;
+        const double vb = e < W ? out[r + e] : 1.0e9;
+        const double v = va < vb ? va : vb;
+        for (int t = x; t < e; ++t) out[r + t] = v < 1.0e8 ? v : disp0[r + t];
+        x = e;
+    }
+}
+
+// ------------------------------------------------------------- median
+// median_filter, stereonet.py:2231-2258: clamped (2R+1)^2 window, ascending
+// order, element n//2.  The window is padded with +inf to a fixed size and
+// sorted by an odd-even transposition network in registers.
+template <int RAD>
+__global__ void __launch_bounds__(64) k_median(const double* __restrict__ in, int W, int H,
+                                               double* __restrict__ out) {
+    constexpr int N = (2 * RAD + 1) * (2 * RAD + 1);
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= W) return;
+    const long long plane = (long long)W * H;
+    const double* im = in + (long long)blockIdx.z * plane;
+    const int y0 = max(y - RAD, 0), y1 = min(y + RAD, H - 1);
+    const int x0 = max(x - RAD, 0), x1 = min(x + RAD, W - 1);
+    double v[N];
+    int n = 0;
+#pragma unroll
+    for (int dy = 0; dy < 2 * RAD + 1; ++dy)
+#pragma unroll
+        for (int dx = 0; dx < 2 * RAD + 1; ++dx) {
+            const int yy = y0 + dy, xx = x0 + dx;
+            const bool in_ = yy <= y1 && xx <= x1;
+            v[dy * (2 * RAD + 1) + dx] = in_ ? im[(long long)yy * W + xx] : HUGE_VAL;
+            n += in_ ? 1 : 0;
+        }
+#pragma unroll
+    for (int pass = 0; pass < N; ++pass)
+#pragma unroll
+        for (int i = pass & 1; i + 1 < N; i += 2) {
+            const double a = v[i], b = v[i + 1];
+            const bool sw = b < a;
+            v[i] = sw ? b : a;
+            v[i + 1] = sw ? a : b;
+        }
+    double m = v[0];
+#pragma unroll
+    for (int i = 1; i < N; ++i)
+        if (i == n / 2) m = v[i];
+    out[(long long)blockIdx.z * plane + (long long)y * W + x] = m;
+}
+
+__global__ void k_add_into(const double* __restrict__ L, int Kp, int K, long long n,
+                           double* __restrict__ S) {
+    const long long i = blockIdx.x * 256LL + threadIdx.x;
+    if (i >= n) return;
+    const long long p = i / K;
+    const int k = (int)(i - p * K);
+    S[i] = S[i] + L[p * Kp + k];
+}
+
+__global__ void k_pad_costs(const double* __restrict__ C, int Kp, int K, long long n,
+                            double* __restrict__ Cp) {
+    const long long i = blockIdx.x * 256LL + threadIdx.x;
+    if (i >= n) return;
+    const long long p = i / Kp;
+    const int k = (int)(i - p * Kp);
+    Cp[i] = k < K ? C[p * K + k] : 0.0;
+}
+
+// ============================================================ host side
+
+// Opt-in stage timing: when enabled, each refinement stage records a pair
+// of HIP events around its launches on the caller's stream; sv_profile_read
+// sums the elapsed times once the work has completed.  Disabled by default
+// (then nothing is recorded).
+struct ProfPair {
+    int stage;
+    hipEvent_t a, b;
+};
+enum { ST_CENSUS, ST_COSTS, ST_SGM, ST_WTA, ST_LR, ST_MEDIAN, ST_COUNT };
+static const char* kStageNames[ST_COUNT] = {"census", "costs", "sgm", "wta", "lr", "median"};
+static bool g_prof_on = false;
+static ProfPair* g_prof = nullptr;
+static int g_prof_n = 0, g_prof_cap = 0;
+
+struct StageTimer {
+    hipEvent_t b = nullptr;
+    int idx = -1;
+    hipStream_t st;
+    StageTimer(int stage, hipStream_t s) : st(s) {
+        if (!g_prof_on) return;
+        if (g_prof_n == g_prof_cap) {
+            const int cap = g_prof_cap ? 2 * g_prof_cap : 256;
+            ProfPair* np = (ProfPair*)realloc(g_prof, sizeof(ProfPair) * cap);
+            if (!np) return;
+            g_prof = np;
+            g_prof_cap = cap;
+        }
+        ProfPair& pp = g_prof[g_prof_n];
+        if (hipEventCreate(&pp.a) != hipSuccess) return;
+        if (hipEventCreate(&pp.b) != hipSuccess) {
+            (void)hipEventDestroy(pp.a);
+            return;
+        }
+        pp.stage = stage;
+        idx = g_prof_n++;
+        (void)hipEventRecord(pp.a, st);
+    }
+    void stop() {
+        if (idx >= 0) (void)hipEventRecord(g_prof[idx].b, st);
+        idx = -1;
+    }
+    ~StageTimer() { stop(); }
+};
+
+struct Plan {
+    int K, G, KC, Kp;
+    int nbits;
+    bool integer;  // integer path
+    int lt;        // path-cost element bytes: 1, 2 or 8
+    double invalid;
+    int ndir;
+};
+
+int choose_layout(int K, bool f64, int* G, int* KC) {
+    static const int kc_int[] = {8, 16, 24, 32};
+    static const int kc_f64[] = {8, 16};
+    const int* kcs = f64 ? kc_f64 : kc_int;
+    const int nk = f64 ? 2 : 4;
+    int best = INT_MAX;
+    for (int g = 1; g <= 64; g <<= 1)
+        for (int t = 0; t < nk; ++t) {
+            const int kp = g * kcs[t];
+            if (kp >= K && kp < best) {
+                best = kp;
+                *G = g;
+                *KC = kcs[t];
+            }
+        }
+    if (best == INT_MAX) return sv::set_error(SV_EUNSUPPORTED, "band of %d entries is wider than the GPU path supports", K);
+    return SV_OK;
+}
+
+int make_plan(int W, int H, int R, double P1, double P2, double ad_weight, double prior_w,
+              double ad_trunc, double prior_t, int paths, int rx, int ry, Plan* pl) {
+    SV_REQUIRE(W > 0 && H > 0, "image size must be positive (W=%d, H=%d)", W, H);
+    SV_REQUIRE(R >= 0, "band half-width R must be >= 0 (R=%d)", R);
+    SV_REQUIRE(rx >= 0 && ry >= 0, "census radii must be >= 0");
+    pl->nbits = (2 * rx + 1) * (2 * ry + 1) - 1;
+    if (pl->nbits > 64)
+        return sv::set_error(SV_EUNSUPPORTED, "census window of %d bits exceeds 64", pl->nbits);
+    pl->K = 2 * R + 1;
+    pl->ndir = paths == 8 ? 8 : 4;
+    pl->invalid = (double)pl->nbits + ad_weight * ad_trunc + prior_w * prior_t + 4.0;
+    const bool integral = P1 == std::floor(P1) && P2 == std::floor(P2) && P1 >= 0 && P2 >= 0 &&
+                          P1 <= 65535 && P2 <= 65535;
+    pl->integer = ad_weight == 0.0 && prior_w == 0.0 && integral;
+    if (pl->integer) {
+        const double lmax = pl->invalid + P2;  // L <= C + P2
+        pl->lt = lmax <= 255.0 ? 1 : 2;
+    } else {
+        pl->lt = 8;
+    }
+    return choose_layout(pl->K, !pl->integer, &pl->G, &pl->KC);
+}
+
+struct RefineBufs {
+    uint64_t* cen;   // [2][B][HW]
+    int* base;       // [V][B][HW]
+    void* C;         // [V][B][HW][Kp]
+    void* L;         // [ndir][V][B][HW][Kp]
+    double* d0r;     // [B][HW]            (lr only)
+    double* disp;    // [V][B][HW]         (lr only: dl, dr)
+    double* tmp;     // [B][HW]            (median)
+    size_t bytes;
+};
+
+RefineBufs carve(void* ws, const Plan& pl, int V, int B, int W, int H, bool lr, bool median) {
+    sv::Carver cv(ws);
+    const size_t hw = (size_t)W * H;
+    RefineBufs r;
+    r.cen = cv.take<uint64_t>(2 * (size_t)B * hw);
+    r.base = cv.take<int>((size_t)V * B * hw);
+    const size_t csz = pl.integer ? 1 : 8;
+    r.C = cv.take<unsigned char>((size_t)V * B * hw * pl.Kp * csz);
+    r.L = cv.take<unsigned char>((size_t)pl.ndir * V * B * hw * pl.Kp * pl.lt);
+    r.d0r = lr ? cv.take<double>((size_t)B * hw) : nullptr;
+    r.disp = lr ? cv.take<double>((size_t)V * B * hw) : nullptr;
+    r.tmp = median ? cv.take<double>((size_t)B * hw) : nullptr;
+    r.bytes = cv.used;
+    return r;
+}
+
+template <typename T>
+int launch_census(const T* img, int n, int W, int H, int rx, int ry, uint64_t* out,
+                  hipStream_t st) {
+    dim3 grid((W + 63) / 64, H, n);
+    hipLaunchKernelGGL(k_census<T>, grid, dim3(64), 0, st, img, W, H, rx, ry, out);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+int census_any(const void* img, int dtype, int n, int W, int H, int rx, int ry, uint64_t* out,
+               hipStream_t st) {
+    switch (dtype) {
+        case SV_U8: return launch_census((const uint8_t*)img, n, W, H, rx, ry, out, st);
+        case SV_F32: return launch_census((const float*)img, n, W, H, rx, ry, out, st);
+        case SV_F64: return launch_census((const double*)img, n, W, H, rx, ry, out, st);
+    }
+    return sv::set_error(SV_EINVAL, "unknown image dtype %d", dtype);
+}
+
+int launch_base(const double* d0, long long n, int* base, hipStream_t st) {
+    hipLaunchKernelGGL(k_base, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d0, n, base);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+template <typename T>
+int costs_f64_t(bool right, const uint64_t* cA, const uint64_t* cB, const void* gA,
+                const void* gB, const int* base, const double* prior, int B, int W, int H, int R,
+                int K, int Kp, double aw, double at, double inv, double pw, double pt, double* C,
+                hipStream_t st) {
+    dim3 grid((W + 63) / 64, H, B);
+    if (right)
+        hipLaunchKernelGGL((k_costs_f64<true, T>), grid, dim3(64), 0, st, cA, cB, (const T*)gA,
+                           (const T*)gB, base, prior, W, H, R, K, Kp, aw, at, inv, pw, pt, C);
+    else
+        hipLaunchKernelGGL((k_costs_f64<false, T>), grid, dim3(64), 0, st, cA, cB, (const T*)gA,
+                           (const T*)gB, base, prior, W, H, R, K, Kp, aw, at, inv, pw, pt, C);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+int launch_costs(const Plan& pl, bool right, const uint64_t* cA, const uint64_t* cB,
+                 const void* gA, const void* gB, int dtype, const int* base, const double* prior,
+                 int B, int W, int H, int R, double aw, double at, double pw, double pt, void* C,
+                 hipStream_t st) {
+    if (pl.integer) {
+        dim3 grid((W + 63) / 64, H, B);
+        if (right)
+            hipLaunchKernelGGL(k_costs_int<true>, grid, dim3(64), 0, st, cA, cB, base, W, H, R,
+                               pl.K, pl.Kp, (int)pl.invalid, (uint8_t*)C);
+        else
+            hipLaunchKernelGGL(k_costs_int<false>, grid, dim3(64), 0, st, cA, cB, base, W, H, R,
+                               pl.K, pl.Kp, (int)pl.invalid, (uint8_t*)C);
+        SV_CHECK_LAUNCH();
+        return SV_OK;
+    }
+    switch (dtype) {
+        case SV_U8:
+            return costs_f64_t<uint8_t>(right, cA, cB, gA, gB, base, prior, B, W, H, R, pl.K,
+                                        pl.Kp, aw, at, pl.invalid, pw, pt, (double*)C, st);
+        case SV_F32:
+            return costs_f64_t<float>(right, cA, cB, gA, gB, base, prior, B, W, H, R, pl.K, pl.Kp,
+                                      aw, at, pl.invalid, pw, pt, (double*)C, st);
+        case SV_F64:
+            return costs_f64_t<double>(right, cA, cB, gA, gB, base, prior, B, W, H, R, pl.K,
+                                       pl.Kp, aw, at, pl.invalid, pw, pt, (double*)C, st);
+    }
+    return sv::set_error(SV_EINVAL, "unknown image dtype %d", dtype);
+}
+
+template <typename CT, typename LT, typename AT, int KC>
+int sgm_t(const Plan& pl, const void* C, const int* base, void* L, int BV, int W, int H,
+          double P1, double P2, int dir0, int ndir, hipStream_t st) {
+    const int lpw = 64 / pl.G;
+    const int nl = W > H ? W : H;
+    dim3 grid((nl + lpw - 1) / lpw, ndir, BV);
+    const size_t shm = (size_t)lpw * (pl.Kp + 5) * sizeof(AT);
+    const long long dirStride = (long long)BV * W * H * pl.Kp;
+    hipLaunchKernelGGL((k_sgm<CT, LT, AT, KC>), grid, dim3(64), shm, st, (const CT*)C, base,
+                       (LT*)L, W, H, pl.K, pl.G, (AT)P1, (AT)P2, dir0, dirStride);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+template <typename LT, typename AT, int KC>
+int wta_t(const Plan& pl, const void* L, int BV, int mirrorFrom, const int* base,
+          const double* d0a, const double* d0b, int B, int W, int H, int R, double uniq,
+          double* out, double* conf, hipStream_t st) {
+    const int lpw = 64 / pl.G;
+    const int HW = W * H;
+    dim3 grid((HW + lpw - 1) / lpw, BV);
+    const long long dirStride = (long long)BV * HW * pl.Kp;
+    hipLaunchKernelGGL((k_wta<LT, AT, KC>), grid, dim3(64), 0, st, (const LT*)L, dirStride,
+                       pl.ndir, mirrorFrom, base, d0a, d0b, B, HW, pl.K, pl.G, R, uniq, out, conf);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+#define SV_DISPATCH_KC(KCV, CALL)                                              \
+    switch (KCV) {                                                             \
+        case 8: { constexpr int KC_ = 8; return CALL; }                        \
+        case 16: { constexpr int KC_ = 16; return CALL; }                      \
+        case 24: { constexpr int KC_ = 24; return CALL; }                      \
+        case 32: { constexpr int KC_ = 32; return CALL; }                      \
+    }                                                                          \
+    return sv::set_error(SV_EUNSUPPORTED, "band chunk %d", KCV);
+
+int launch_sgm(const Plan& pl, const void* C, const int* base, void* L, int BV, int W, int H,
+               double P1, double P2, int dir0, int ndir, hipStream_t st) {
+    if (!pl.integer) {
+        switch (pl.KC) {
+            case 8: return sgm_t<double, double, double, 8>(pl, C, base, L, BV, W, H, P1, P2, dir0, ndir, st);
+            case 16: return sgm_t<double, double, double, 16>(pl, C, base, L, BV, W, H, P1, P2, dir0, ndir, st);
+        }
+        return sv::set_error(SV_EUNSUPPORTED, "band chunk %d", pl.KC);
+    }
+    if (pl.lt == 1) {
+        SV_DISPATCH_KC(pl.KC, (sgm_t<uint8_t, uint8_t, int, KC_>(pl, C, base, L, BV, W, H, P1, P2, dir0, ndir, st)));
+    }
+    SV_DISPATCH_KC(pl.KC, (sgm_t<uint8_t, uint16_t, int, KC_>(pl, C, base, L, BV, W, H, P1, P2, dir0, ndir, st)));
+}
+
+int launch_wta(const Plan& pl, const void* L, int BV, int mirrorFrom, const int* base,
+               const double* d0a, const double* d0b, int B, int W, int H, int R, double uniq,
+               double* out, double* conf, hipStream_t st) {
+    if (!pl.integer) {
+        switch (pl.KC) {
+            case 8: return wta_t<double, double, 8>(pl, L, BV, mirrorFrom, base, d0a, d0b, B, W, H, R, uniq, out, conf, st);
+            case 16: return wta_t<double, double, 16>(pl, L, BV, mirrorFrom, base, d0a, d0b, B, W, H, R, uniq, out, conf, st);
+        }
+        return sv::set_error(SV_EUNSUPPORTED, "band chunk %d", pl.KC);
+    }
+    if (pl.lt == 1) {
+        SV_DISPATCH_KC(pl.KC, (wta_t<uint8_t, int, KC_>(pl, L, BV, mirrorFrom, base, d0a, d0b, B, W, H, R, uniq, out, conf, st)));
+    }
+    SV_DISPATCH_KC(pl.KC, (wta_t<uint16_t, int, KC_>(pl, L, BV, mirrorFrom, base, d0a, d0b, B, W, H, R, uniq, out, conf, st)));
+}
+
+int launch_median(const double* in, int B, int W, int H, int radius, int passes, double* out,
+                  double* tmp, hipStream_t st) {
+    SV_REQUIRE(passes >= 0, "passes must be >= 0");
+    if (passes == 0) {
+        hipError_t e = hipMemcpyAsync(out, in, sizeof(double) * B * W * H, hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) return sv::set_error(SV_EHIP, "median copy: %s", hipGetErrorString(e));
+        return SV_OK;
+    }
+    if (radius < 0 || radius > 2)
+        return sv::set_error(SV_EUNSUPPORTED, "median radius %d (the GPU path supports 0..2)", radius);
+    SV_REQUIRE(passes == 1 || tmp, "multi-pass median needs a workspace");
+    dim3 grid((W + 63) / 64, H, B);
+    const double* src = in;
+    for (int it = 0; it < passes; ++it) {
+        // the last pass lands in out; earlier passes alternate so that the
+        // final one does
+        double* dst = ((passes - 1 - it) % 2 == 0) ? out : tmp;
+        switch (radius) {
+            case 0: hipLaunchKernelGGL(k_median<0>, grid, dim3(64), 0, st, src, W, H, dst); break;
+            case 1: hipLaunchKernelGGL(k_median<1>, grid, dim3(64), 0, st, src, W, H, dst); break;
+            case 2: hipLaunchKernelGGL(k_median<2>, grid, dim3(64), 0, st, src, W, H, dst); break;
+        }
+        SV_CHECK_LAUNCH();
+        src = dst;
+    }
+    return SV_OK;
+}
+
+int check_ws(size_t need, void* ws, size_t have) {
+    if (!ws || have < need)
+        return sv::set_error(SV_EWORKSPACE, "workspace of %zu bytes needed, %zu given", need, have);
+    return SV_OK;
+}
+
+// One band refinement of V views over B pairs.  View 0 matches left against
+// right with centre disp0; view 1 (lr only) matches right against left with
+// centre left_to_right(disp0).  Results land in outs[v*B*HW ...].
+int refine_views(const Plan& pl, const RefineBufs& bf, const void* left, const void* right,
+                 int dtype, const double* disp0, const double* d0r, int V, int B, int W, int H,
+                 int R, double P1, double P2, double aw, double at, double uniq, int rx, int ry,
+                 double pw, double pt, double* outs, double* conf, hipStream_t st) {
+    const size_t hw = (size_t)W * H;
+    int rc;
+    uint64_t* cL = bf.cen;
+    uint64_t* cR = bf.cen + (size_t)B * hw;
+    {
+        StageTimer t(ST_CENSUS, st);
+        if ((rc = census_any(left, dtype, B, W, H, rx, ry, cL, st))) return rc;
+        if ((rc = census_any(right, dtype, B, W, H, rx, ry, cR, st))) return rc;
+    }
+    const size_t csz = pl.integer ? 1 : 8;
+    StageTimer tc(ST_COSTS, st);
+    for (int v = 0; v < V; ++v) {
+        const double* d0 = v == 0 ? disp0 : d0r;
+        int* base = bf.base + (size_t)v * B * hw;
+        if ((rc = launch_base(d0, (long long)B * hw, base, st))) return rc;
+        void* C = (char*)bf.C + (size_t)v * B * hw * pl.Kp * csz;
+        if (v == 0)
+            rc = launch_costs(pl, false, cL, cR, left, right, dtype, base, d0, B, W, H, R, aw, at, pw, pt, C, st);
+        else
+            rc = launch_costs(pl, true, cR, cL, right, left, dtype, base, d0, B, W, H, R, aw, at, pw, pt, C, st);
+        if (rc) return rc;
+    }
+    tc.stop();
+    {
+        StageTimer t(ST_SGM, st);
+        if ((rc = launch_sgm(pl, bf.C, bf.base, bf.L, V * B, W, H, P1, P2, 0, pl.ndir, st))) return rc;
+    }
+    StageTimer t(ST_WTA, st);
+    return launch_wta(pl, bf.L, V * B, V == 2 ? B : V * B, bf.base, disp0, d0r, B, W, H, R, uniq,
+                      outs, conf, st);
+}
+
+}  // namespace
+
+// =============================================================== C ABI
+extern "C" {
+
+int sv_census(const void* img, int dtype, int n_images, int W, int H, int rx, int ry,
+              uint64_t* out, void* stream) {
+    SV_REQUIRE(img && out && n_images > 0 && W > 0 && H > 0, "sv_census: bad arguments");
+    const int nbits = (2 * rx + 1) * (2 * ry + 1) - 1;
+    if (rx < 0 || ry < 0 || nbits > 64)
+        return sv::set_error(SV_EUNSUPPORTED, "census window of %d bits exceeds 64", nbits);
+    return census_any(img, dtype, n_images, W, H, rx, ry, out, (hipStream_t)stream);
+}
+
+int sv_band_costs(const uint64_t* cl, const uint64_t* cr, const void* gl, const void* gr,
+                  int dtype, const int* base, int B, int W, int H, int R, double ad_weight,
+                  double ad_trunc, double invalid, const double* prior, double prior_w,
+                  double prior_t, double* C, void* stream) {
+    SV_REQUIRE(cl && cr && gl && gr && base && C && B > 0 && W > 0 && H > 0 && R >= 0,
+               "sv_band_costs: bad arguments");
+    SV_REQUIRE(prior_w <= 0.0 || prior, "sv_band_costs: prior needed when prior_w > 0");
+    const int K = 2 * R + 1;
+    hipStream_t st = (hipStream_t)stream;
+    // costs are written unpadded ([p][K])
+    const void* ga = gl;
+    const void* gb = gr;
+    switch (dtype) {
+        case SV_U8:
+            return costs_f64_t<uint8_t>(false, cl, cr, ga, gb, base, prior, B, W, H, R, K, K,
+                                        ad_weight, ad_trunc, invalid, prior_w, prior_t, C, st);
+        case SV_F32:
+            return costs_f64_t<float>(false, cl, cr, ga, gb, base, prior, B, W, H, R, K, K,
+                                      ad_weight, ad_trunc, invalid, prior_w, prior_t, C, st);
+        case SV_F64:
+            return costs_f64_t<double>(false, cl, cr, ga, gb, base, prior, B, W, H, R, K, K,
+                                       ad_weight, ad_trunc, invalid, prior_w, prior_t, C, st);
+    }
+    return sv::set_error(SV_EINVAL, "unknown image dtype %d", dtype);
+}
+
+int sv_aggregate_path_workspace_bytes(int B, int W, int H, int K, size_t* bytes) {
+    SV_REQUIRE(bytes && B > 0 && W > 0 && H > 0 && K > 0, "bad arguments");
+    int G = 1, KC = 8;
+    int rc = choose_layout(K, true, &G, &KC);
+    if (rc) return rc;
+    const size_t n = (size_t)B * W * H * G * KC;
+    *bytes = sv::align_up(n * 8) * 2;
+    return SV_OK;
+}
+
+int sv_aggregate_path(const double* C, const int* base, int B, int W, int H, int K, double P1,
+                      double P2, int dx, int dy, double* S, void* ws, size_t ws_bytes,
+                      void* stream) {
+    SV_REQUIRE(C && base && S && B > 0 && W > 0 && H > 0 && K > 0, "sv_aggregate_path: bad arguments");
+    int dir = -1;
+    for (int d = 0; d < 8; ++d) {
+        static const int DX[8] = {1, -1, 0, 0, 1, -1, 1, -1};
+        static const int DY[8] = {0, 0, 1, -1, 1, 1, -1, -1};
+        if (DX[d] == dx && DY[d] == dy) dir = d;
+    }
+    SV_REQUIRE(dir >= 0, "direction (%d, %d) is not one of the eight SGM paths", dx, dy);
+    Plan pl{};
+    pl.K = K;
+    pl.integer = false;
+    pl.lt = 8;
+    pl.ndir = 1;
+    int rc = choose_layout(K, true, &pl.G, &pl.KC);
+    if (rc) return rc;
+    pl.Kp = pl.G * pl.KC;
+    size_t need = 0;
+    sv_aggregate_path_workspace_bytes(B, W, H, K, &need);
+    if ((rc = check_ws(need, ws, ws_bytes))) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const long long n = (long long)B * W * H * pl.Kp;
+    double* Cp = (double*)ws;
+    double* L = (double*)((char*)ws + need / 2);
+    hipLaunchKernelGGL(k_pad_costs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, C, pl.Kp, K, n, Cp);
+    SV_CHECK_LAUNCH();
+    if ((rc = launch_sgm(pl, Cp, base, L, B, W, H, P1, P2, dir, 1, st))) return rc;
+    const long long m = (long long)B * W * H * K;
+    hipLaunchKernelGGL(k_add_into, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, L, pl.Kp, K, m, S);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+// mode: 0 refine_fullres, 1 lr_refine, 2 refine_disparity (planes=False)
+int sv_refine_workspace_bytes(int mode, int B, int W, int H, int R, double P1, double P2,
+                              double ad_weight, double ad_trunc, int paths, int rx, int ry,
+                              double prior_w, double prior_t, size_t* bytes) {
+    SV_REQUIRE(bytes && B > 0, "bad arguments");
+    Plan pl{};
+    int rc = make_plan(W, H, R, P1, P2, ad_weight, prior_w, ad_trunc, prior_t, paths, rx, ry, &pl);
+    if (rc) return rc;
+    pl.Kp = pl.G * pl.KC;
+    const int V = mode == 0 ? 1 : 2;
+    RefineBufs r = carve(nullptr, pl, V, B, W, H, mode != 0, mode == 2);
+    *bytes = r.bytes;
+    return SV_OK;
+}
+
+int sv_refine_fullres(const void* left, const void* right, int dtype, const double* disp0, int B,
+                      int W, int H, int R, double P1, double P2, double ad_weight,
+                      double ad_trunc, int paths, double uniq, int rx, int ry, double prior_w,
+                      double prior_t, double* out, double* conf, void* ws, size_t ws_bytes,
+                      void* stream) {
+    SV_REQUIRE(left && right && disp0 && out && B > 0, "sv_refine_fullres: bad arguments");
+    Plan pl{};
+    int rc = make_plan(W, H, R, P1, P2, ad_weight, prior_w, ad_trunc, prior_t, paths, rx, ry, &pl);
+    if (rc) return rc;
+    pl.Kp = pl.G * pl.KC;
+    RefineBufs bf = carve(ws, pl, 1, B, W, H, false, false);
+    if ((rc = check_ws(bf.bytes, ws, ws_bytes))) return rc;
+    return refine_views(pl, bf, left, right, dtype, disp0, nullptr, 1, B, W, H, R, P1, P2,
+                        ad_weight, ad_trunc, uniq, rx, ry, prior_w, prior_t, out, conf,
+                        (hipStream_t)stream);
+}
+
+int sv_left_to_right(const double* dl, int B, int W, int H, double* dr, void* stream) {
+    SV_REQUIRE(dl && dr && B > 0 && W > 0 && H > 0, "sv_left_to_right: bad arguments");
+    hipLaunchKernelGGL(k_left_to_right, dim3((H + 63) / 64, B), dim3(64), 0, (hipStream_t)stream,
+                       dl, B, W, H, dr);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+static int lr_core(const void* left, const void* right, int dtype, const double* disp0, int B,
+                   int W, int H, double tol, int R, double P1, double P2, double ad_weight,
+                   double ad_trunc, int paths, double uniq, int rx, int ry, double prior_w,
+                   double prior_t, double* out, uint8_t* status, void* ws, size_t ws_bytes,
+                   bool median, hipStream_t st) {
+    SV_REQUIRE(left && right && disp0 && out && status && B > 0, "lr_refine: bad arguments");
+    Plan pl{};
+    int rc = make_plan(W, H, R, P1, P2, ad_weight, prior_w, ad_trunc, prior_t, paths, rx, ry, &pl);
+    if (rc) return rc;
+    pl.Kp = pl.G * pl.KC;
+    RefineBufs bf = carve(ws, pl, 2, B, W, H, true, median);
+    if ((rc = check_ws(bf.bytes, ws, ws_bytes))) return rc;
+    {
+        StageTimer t(ST_LR, st);
+        hipLaunchKernelGGL(k_left_to_right, dim3((H + 63) / 64, B), dim3(64), 0, st, disp0, B, W, H, bf.d0r);
+        SV_CHECK_LAUNCH();
+    }
+    rc = refine_views(pl, bf, left, right, dtype, disp0, bf.d0r, 2, B, W, H, R, P1, P2, ad_weight,
+                      ad_trunc, uniq, rx, ry, prior_w, prior_t, bf.disp, nullptr, st);
+    if (rc) return rc;
+    const size_t hw = (size_t)W * H;
+    double* lrout = median ? bf.tmp : out;
+    {
+        StageTimer t(ST_LR, st);
+        hipLaunchKernelGGL(k_lr_check_fill, dim3((H + 63) / 64, B), dim3(64), 0, st, bf.disp,
+                           bf.disp + (size_t)B * hw, disp0, B, W, H, tol, lrout, status);
+        SV_CHECK_LAUNCH();
+    }
+    if (median) {
+        StageTimer t(ST_MEDIAN, st);
+        return launch_median(lrout, B, W, H, 1, 1, out, nullptr, st);
+    }
+    return SV_OK;
+}
+
+int sv_lr_refine(const void* left, const void* right, int dtype, const double* disp0, int B,
+                 int W, int H, double tol, int R, double P1, double P2, double ad_weight,
+                 double ad_trunc, int paths, double uniq, int rx, int ry, double prior_w,
+                 double prior_t, double* out, uint8_t* status, void* ws, size_t ws_bytes,
+                 void* stream) {
+    return lr_core(left, right, dtype, disp0, B, W, H, tol, R, P1, P2, ad_weight, ad_trunc, paths,
+                   uniq, rx, ry, prior_w, prior_t, out, status, ws, ws_bytes, false,
+                   (hipStream_t)stream);
+}
+
+int sv_refine_disparity(const void* left, const void* right, int dtype, const double* disp0,
+                        int B, int W, int H, int R, double P1, double P2, int rx, int ry,
+                        double tol, double* out, uint8_t* status, void* ws, size_t ws_bytes,
+                        void* stream) {
+    return lr_core(left, right, dtype, disp0, B, W, H, tol, R, P1, P2, 0.0, 20.0, 8, 0.0, rx, ry,
+                   0.0, 4.0, out, status, ws, ws_bytes, true, (hipStream_t)stream);
+}
+
+// Stage timing (see StageTimer): on != 0 clears and starts collecting.
+int sv_profile_enable(int on) {
+    for (int i = 0; i < g_prof_n; ++i) {
+        (void)hipEventDestroy(g_prof[i].a);
+        (void)hipEventDestroy(g_prof[i].b);
+    }
+    g_prof_n = 0;
+    g_prof_on = on != 0;
+    return SV_OK;
+}
+
+// Total milliseconds and number of recorded launches of one stage ("census",
+// "costs", "sgm", "wta", "lr", "median"); waits for the recorded events.
+int sv_profile_read(const char* stage, double* total_ms, int* launches) {
+    SV_REQUIRE(stage && total_ms && launches, "sv_profile_read: bad arguments");
+    int id = -1;
+    for (int i = 0; i < ST_COUNT; ++i)
+        if (strcmp(stage, kStageNames[i]) == 0) id = i;
+    SV_REQUIRE(id >= 0, "unknown stage '%s'", stage);
+    double tot = 0.0;
+    int cnt = 0;
+    for (int i = 0; i < g_prof_n; ++i) {
+        if (g_prof[i].stage != id) continue;
+        hipError_t e = hipEventSynchronize(g_prof[i].b);
+        if (e != hipSuccess) return sv::set_error(SV_EHIP, "profile: %s", hipGetErrorString(e));
+        float ms = 0.f;
+        e = hipEventElapsedTime(&ms, g_prof[i].a, g_prof[i].b);
+        if (e != hipSuccess) return sv::set_error(SV_EHIP, "profile: %s", hipGetErrorString(e));
+        tot += ms;
+        ++cnt;
+    }
+    *total_ms = tot;
+    *launches = cnt;
+    return SV_OK;
+}
+
+int sv_median_filter(const double* disp, int B, int W, int H, int radius, int passes,
+                     double* out, double* tmp, void* stream) {
+    SV_REQUIRE(disp && out && B > 0 && W > 0 && H > 0, "sv_median_filter: bad arguments");
+    return launch_median(disp, B, W, H, radius, passes, out, tmp, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,264 @@
+"""Full-resolution census + band SGM + left-right refinement on the GPU.
+
+Drop-in counterparts of stereonet.py:5115-5391, :5865-5884 and :2231-2258
+with the reference's names, argument order and defaults.  Inputs may be
+Python lists (returned as lists, like the reference), numpy arrays (returned
+as numpy) or torch tensors (returned as CUDA tensors, no host round trip).
+Images may be flat (W*H), (H, W) or a batch (B, H, W); uint8 images are
+read as gray = value / 255 (the reference's PGM convention,
+stereonet.py:284,300), float images are used as they are.
+
+Every call runs hand-written HIP kernels from libstereo_mi.so; there is no
+CPU fallback.
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+_DTYPES = {torch.uint8: L.SV_U8, torch.float32: L.SV_F32, torch.float64: L.SV_F64}
+
+
+def _kind(a):
+    if isinstance(a, torch.Tensor):
+        return "torch"
+    if isinstance(a, np.ndarray):
+        return "numpy"
+    return "list"
+
+
+def _dev(a, dtype=None):
+    dev = L.device()
+    if isinstance(a, torch.Tensor):
+        t = a
+    elif isinstance(a, np.ndarray):
+        t = torch.from_numpy(np.ascontiguousarray(a))
+    else:
+        t = torch.tensor(a, dtype=torch.float64 if dtype is None else dtype)
+    if dtype is not None and t.dtype != dtype:
+        t = t.to(dtype)
+    return t.to(dev).contiguous()
+
+
+def _image(a):
+    t = _dev(a)
+    if t.dtype not in _DTYPES:
+        t = t.to(torch.float64)
+    return t, _DTYPES[t.dtype]
+
+
+def _batch(t, W, H):
+    n = t.numel()
+    if W <= 0 or H <= 0 or n % (W * H) != 0 or n == 0:
+        raise ValueError("array of %d values does not hold W x H = %d x %d images" % (n, W, H))
+    return n // (W * H)
+
+
+def _ret(t, kind, shape=None):
+    if kind == "torch":
+        return t.reshape(shape) if shape is not None else t
+    a = t.cpu().numpy()
+    if shape is not None:
+        a = a.reshape(shape)
+    return a if kind == "numpy" else a.ravel().tolist()
+
+
+def _shape_of(a):
+    return tuple(a.shape) if isinstance(a, (torch.Tensor, np.ndarray)) else None
+
+
+def census(img, W, H, rx=2, ry=2):
+    """census, stereonet.py:5115-5142.  Returns the 64-bit words (as Python
+    ints, uint64 numpy, or an int64 tensor holding the same bits)."""
+    kind = _kind(img)
+    t, dt = _image(img)
+    B = _batch(t, W, H)
+    out = torch.empty(B * W * H, dtype=torch.int64, device=t.device)
+    L.check(L.lib().sv_census(L.ptr(t), dt, B, W, H, rx, ry, L.ptr(out), L.stream()))
+    if kind == "torch":
+        return out.reshape(_shape_of(img))
+    a = out.cpu().numpy().view(np.uint64)
+    if kind == "numpy":
+        return a.reshape(_shape_of(img))
+    return [int(v) for v in a]
+
+
+def _words(c):
+    if isinstance(c, torch.Tensor):
+        return _dev(c, torch.int64)
+    a = np.asarray(c if isinstance(c, np.ndarray) else [int(v) for v in c], dtype=np.uint64)
+    return _dev(a.view(np.int64))
+
+
+def band_costs(cl, cr, gl, gr, base, W, H, R, ad_weight, ad_trunc, invalid,
+               prior=None, prior_w=0.0, prior_t=4.0):
+    """band_costs, stereonet.py:5156-5189: costs for d = base[p] + k - R,
+    pixel-major [p][k] (W*H*(2R+1) values)."""
+    kind = _kind(cl)
+    wl, wr = _words(cl), _words(cr)
+    B = _batch(wl, W, H)
+    g_l, dt = _image(gl)
+    g_r, dt2 = _image(gr)
+    if dt2 != dt:
+        g_r = g_r.to(g_l.dtype)
+    b = _dev(base, torch.int32)
+    pr = _dev(prior, torch.float64) if (prior is not None and prior_w > 0.0) else None
+    K = 2 * R + 1
+    C = torch.empty(B * W * H * K, dtype=torch.float64, device=wl.device)
+    L.check(L.lib().sv_band_costs(L.ptr(wl), L.ptr(wr), L.ptr(g_l), L.ptr(g_r), dt, L.ptr(b), B, W,
+                                  H, R, float(ad_weight), float(ad_trunc), float(invalid), L.ptr(pr),
+                                  float(prior_w), float(prior_t), L.ptr(C), L.stream()))
+    return _ret(C, kind)
+
+
+def _aggregate_path(C, base, W, H, K, P1, P2, dx, dy, S):
+    """_aggregate_path, stereonet.py:5192-5241: adds one path's costs into S
+    in place (S is a list, a float64 numpy array or a CUDA float64 tensor)."""
+    c = _dev(C, torch.float64)
+    b = _dev(base, torch.int32)
+    B = _batch(b, W, H)
+    if isinstance(S, torch.Tensor) and S.is_cuda and S.dtype == torch.float64 and S.is_contiguous():
+        s = S
+    else:
+        s = _dev(S, torch.float64)
+    nbytes = ctypes.c_size_t(0)
+    L.check(L.lib().sv_aggregate_path_workspace_bytes(B, W, H, K, ctypes.byref(nbytes)))
+    ws = L.workspace.get(nbytes.value)
+    L.check(L.lib().sv_aggregate_path(L.ptr(c), L.ptr(b), B, W, H, K, float(P1), float(P2), dx, dy,
+                                      L.ptr(s), L.ptr(ws), nbytes.value, L.stream()))
+    if s is not S:
+        res = s.cpu().numpy()
+        if isinstance(S, np.ndarray):
+            S.reshape(-1)[:] = res
+        elif isinstance(S, torch.Tensor):
+            S.copy_(torch.from_numpy(res).reshape(S.shape))
+        else:
+            S[:] = res.tolist()
+    return None
+
+
+def _ws_bytes(mode, B, W, H, R, P1, P2, ad_weight, ad_trunc, paths, rx, ry, prior_w, prior_t):
+    n = ctypes.c_size_t(0)
+    L.check(L.lib().sv_refine_workspace_bytes(mode, B, W, H, R, float(P1), float(P2),
+                                              float(ad_weight), float(ad_trunc), int(paths), rx,
+                                              ry, float(prior_w), float(prior_t), ctypes.byref(n)))
+    return n.value
+
+
+def _pair(left, right, disp0, W, H):
+    l, dt = _image(left)
+    r, dt2 = _image(right)
+    if dt2 != dt:
+        r = r.to(l.dtype)
+    d0 = _dev(disp0, torch.float64)
+    B = _batch(l, W, H)
+    if r.numel() != l.numel() or d0.numel() != l.numel():
+        raise ValueError("left, right and disp0 must have the same size")
+    return l, r, dt, d0, B
+
+
+def refine_fullres(left, right, disp0, W, H, R=3, P1=1.0, P2=12.0,
+                   ad_weight=0.0, ad_trunc=20.0, paths=8, uniq=0.0,
+                   rx=2, ry=2, prior_w=0.0, prior_t=4.0, return_conf=False):
+    """refine_fullres, stereonet.py:5244-5305 (same defaults)."""
+    kind, shape = _kind(disp0), _shape_of(disp0)
+    l, r, dt, d0, B = _pair(left, right, disp0, W, H)
+    out = torch.empty(B * W * H, dtype=torch.float64, device=l.device)
+    conf = torch.empty_like(out) if return_conf else None
+    nb = _ws_bytes(0, B, W, H, R, P1, P2, ad_weight, ad_trunc, paths, rx, ry, prior_w, prior_t)
+    ws = L.workspace.get(nb)
+    L.check(L.lib().sv_refine_fullres(L.ptr(l), L.ptr(r), dt, L.ptr(d0), B, W, H, R, float(P1),
+                                      float(P2), float(ad_weight), float(ad_trunc), int(paths),
+                                      float(uniq), rx, ry, float(prior_w), float(prior_t),
+                                      L.ptr(out), L.ptr(conf), L.ptr(ws), nb, L.stream()))
+    if return_conf:
+        return _ret(out, kind, shape), _ret(conf, kind, shape)
+    return _ret(out, kind, shape)
+
+
+def _mirror1(img, W, H):
+    """_mirror1, stereonet.py:5308-5314 (horizontal flip)."""
+    kind, shape = _kind(img), _shape_of(img)
+    t = _dev(img)
+    B = _batch(t, W, H)
+    return _ret(t.reshape(B, H, W).flip(-1).contiguous().reshape(-1), kind, shape)
+
+
+def left_to_right(dl, W, H):
+    """left_to_right, stereonet.py:5317-5340."""
+    kind, shape = _kind(dl), _shape_of(dl)
+    d = _dev(dl, torch.float64)
+    B = _batch(d, W, H)
+    dr = torch.empty_like(d)
+    L.check(L.lib().sv_left_to_right(L.ptr(d), B, W, H, L.ptr(dr), L.stream()))
+    return _ret(dr, kind, shape)
+
+
+_LR_KW = ("R", "P1", "P2", "ad_weight", "ad_trunc", "paths", "uniq", "rx", "ry", "prior_w",
+          "prior_t")
+
+
+def lr_refine(left, right, disp0, W, H, tol=1.0, **kw):
+    """lr_refine, stereonet.py:5343-5391.  Returns (disparity, status) with
+    status 0 ok, 1 occluded, 2 mismatch."""
+    bad = set(kw) - set(_LR_KW)
+    if bad:
+        raise TypeError("refine_fullres() got unexpected keyword argument(s) %s" % sorted(bad))
+    p = dict(R=3, P1=1.0, P2=12.0, ad_weight=0.0, ad_trunc=20.0, paths=8, uniq=0.0, rx=2, ry=2,
+             prior_w=0.0, prior_t=4.0)
+    p.update(kw)
+    kind, shape = _kind(disp0), _shape_of(disp0)
+    l, r, dt, d0, B = _pair(left, right, disp0, W, H)
+    out = torch.empty(B * W * H, dtype=torch.float64, device=l.device)
+    status = torch.empty(B * W * H, dtype=torch.uint8, device=l.device)
+    nb = _ws_bytes(1, B, W, H, p["R"], p["P1"], p["P2"], p["ad_weight"], p["ad_trunc"],
+                   p["paths"], p["rx"], p["ry"], p["prior_w"], p["prior_t"])
+    ws = L.workspace.get(nb)
+    L.check(L.lib().sv_lr_refine(L.ptr(l), L.ptr(r), dt, L.ptr(d0), B, W, H, float(tol), p["R"],
+                                 float(p["P1"]), float(p["P2"]), float(p["ad_weight"]),
+                                 float(p["ad_trunc"]), int(p["paths"]), float(p["uniq"]), p["rx"],
+                                 p["ry"], float(p["prior_w"]), float(p["prior_t"]), L.ptr(out),
+                                 L.ptr(status), L.ptr(ws), nb, L.stream()))
+    st = status if kind == "torch" else status.to(torch.int32)
+    return _ret(out, kind, shape), _ret(st, kind, shape)
+
+
+def median_filter(disp, W, H, radius=1, passes=1):
+    """median_filter, stereonet.py:2231-2258 (radius 0..2 on the GPU)."""
+    kind, shape = _kind(disp), _shape_of(disp)
+    d = _dev(disp, torch.float64)
+    B = _batch(d, W, H)
+    out = torch.empty_like(d)
+    tmp = torch.empty_like(d) if passes > 1 else None
+    L.check(L.lib().sv_median_filter(L.ptr(d), B, W, H, radius, passes, L.ptr(out), L.ptr(tmp),
+                                     L.stream()))
+    return _ret(out, kind, shape)
+
+
+def refine_disparity(left, right, rgb, disp0, W, H, R=10, P1=12.0, P2=150.0,
+                     rx=4, ry=3, tol=2.0, planes=True, plane_step=12,
+                     plane_thr=1.0, plane_min_inl=0.4, verbose=False):
+    """refine_disparity, stereonet.py:5865-5884: lr_refine (both views), then
+    one 3x3 median.  Plane snapping (planes=True with an RGB image) is not
+    part of this library; pass planes=False or rgb=None."""
+    if planes and rgb is not None:
+        raise NotImplementedError("plane_snap (SLIC + RANSAC planes) is not implemented on the "
+                                  "GPU path; call refine_disparity(..., planes=False)")
+    kind, shape = _kind(disp0), _shape_of(disp0)
+    l, r, dt, d0, B = _pair(left, right, disp0, W, H)
+    out = torch.empty(B * W * H, dtype=torch.float64, device=l.device)
+    status = torch.empty(B * W * H, dtype=torch.uint8, device=l.device)
+    nb = _ws_bytes(2, B, W, H, R, P1, P2, 0.0, 20.0, 8, rx, ry, 0.0, 4.0)
+    ws = L.workspace.get(nb)
+    L.check(L.lib().sv_refine_disparity(L.ptr(l), L.ptr(r), dt, L.ptr(d0), B, W, H, R, float(P1),
+                                        float(P2), rx, ry, float(tol), L.ptr(out), L.ptr(status),
+                                        L.ptr(ws), nb, L.stream()))
+    if verbose:
+        n = float(W * H * B)
+        print("  full-res refinement: %.1f%% occluded, %.1f%% mismatched"
+              % (100.0 * int((status == 1).sum()) / n, 100.0 * int((status == 2).sum()) / n))
+    st = status if kind == "torch" else status.to(torch.int32)
+    return _ret(out, kind, shape), _ret(st, kind, shape)

@@ -75,7 +75,9 @@ def _add_boxes(rng, d, n_boxes, dmax, wmin=3, wmax=8):
 
 
 def _warp_right(left, disp):
-    """Forward warp with z-buffer; returns (right, zbuf)."""
+    """Forward warp with z-buffer; returns (right, zbuf).  Holes take the
+    nearest filled pixel on the background side (the smaller disparity of
+    the two nearest filled neighbours, left on a tie), 0 if none."""
     H, W = left.shape
     right = np.full((H, W), -1.0)
     zbuf = np.full((H, W), -1.0)
@@ -84,30 +86,24 @@ def _warp_right(left, disp):
         d = disp[y]
         xr = np.trunc(xs - d + 0.5).astype(np.int64)
         ok = (xr >= 0) & (xr < W)
-        # process in increasing disparity so the nearest surface is written last
+        # increasing disparity, so the nearest surface is written last
         order = np.argsort(d[ok], kind="stable")
         src = np.nonzero(ok)[0][order]
         dst = xr[ok][order]
         right[y, dst] = left[y, src]
         zbuf[y, dst] = d[src]
-        holes = np.nonzero(zbuf[y] < 0.0)[0]
-        for x in holes:
-            a = x - 1
-            while a >= 0 and zbuf[y, a] < 0.0:
-                a -= 1
-            b = x + 1
-            while b < W and zbuf[y, b] < 0.0:
-                b += 1
-            cand = []
-            if a >= 0:
-                cand.append((zbuf[y, a], right[y, a]))
-            if b < W:
-                cand.append((zbuf[y, b], right[y, b]))
-            if cand:
-                cand.sort(key=lambda t: t[0])      # background = smaller disparity
-                right[y, x] = cand[0][1]
-            else:
-                right[y, x] = 0.0
+    filled = zbuf >= 0.0
+    cols = np.broadcast_to(np.arange(W), (H, W))
+    li = np.maximum.accumulate(np.where(filled, cols, -1), axis=1)
+    ri = np.minimum.accumulate(np.where(filled, cols, W)[:, ::-1], axis=1)[:, ::-1]
+    rows = np.arange(H)[:, None]
+    zl = np.where(li >= 0, zbuf[rows, np.clip(li, 0, W - 1)], np.inf)
+    zr = np.where(ri < W, zbuf[rows, np.clip(ri, 0, W - 1)], np.inf)
+    vl = right[rows, np.clip(li, 0, W - 1)]
+    vr = right[rows, np.clip(ri, 0, W - 1)]
+    fill = np.where(zl <= zr, vl, vr)
+    fill = np.where(np.isinf(zl) & np.isinf(zr), 0.0, fill)
+    right = np.where(filled, right, fill)
     return right, zbuf
 
 
