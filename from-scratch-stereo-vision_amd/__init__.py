@@ -13,7 +13,7 @@ from .refine import (census, band_costs, _aggregate_path, refine_fullres, _mirro
 from .net import (StereoNet, Adam, LEVEL_WEIGHTS, CE_WEIGHT, train_grads, train_grads_batch,
                   train_step, flat_params, set_flat_params, flat_grads, set_flat_grads,
                   save_weights, read_checkpoint_config, load_weights, load_model, predict,
-                  predict_batch, predict_at_scale, predict_auto, predict_auto_batch)
+                  predict_batch, predict_at_scale, predict_auto, predict_auto_batch, widen)
 from . import synth
 
 __all__ = ["census", "band_costs", "_aggregate_path", "refine_fullres", "_mirror1",
@@ -21,4 +21,4 @@ __all__ = ["census", "band_costs", "_aggregate_path", "refine_fullres", "_mirror
            "LEVEL_WEIGHTS", "CE_WEIGHT", "train_grads", "train_grads_batch", "train_step",
            "flat_params", "set_flat_params", "flat_grads", "set_flat_grads", "save_weights",
            "read_checkpoint_config", "load_weights", "load_model", "predict", "predict_batch",
-           "predict_at_scale", "predict_auto", "predict_auto_batch", "synth"]
+           "predict_at_scale", "predict_auto", "predict_auto_batch", "widen", "synth"]

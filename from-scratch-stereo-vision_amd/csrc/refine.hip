@@ -397,6 +397,93 @@ __global__ void __launch_bounds__(64) k_sgm(const CT* __restrict__ C, const int*
     }
 }
 
+// Specialisation of k_sgm for one lane per scanline (G = 1) with the band
+// length K known at compile time (the pipeline's K = 21, the signature
+// default K = 7), integer path.  Same recurrence and LDS staging as k_sgm;
+// with K static every padding test and slot index resolves at compile time.
+// A lane owns its scanline alone, so no barrier is needed: its LDS reads of
+// the previous pixel's M precede its writes of the current one in program
+// order.
+template <int K, typename LT>
+__global__ void __launch_bounds__(64) k_sgm_k(const uint8_t* __restrict__ C,
+                                              const int* __restrict__ base,
+                                              LT* __restrict__ Lout, int W, int H, int Kp, int P1,
+                                              int P2, int dir0, long long dirStride) {
+    constexpr int SL = ((K + 4 + 3) / 4) * 4 + 1;  // odd stride spreads lanes over banks
+    __shared__ int lds[64 * SL];
+    const int dir = dir0 + (int)blockIdx.y;
+    const int dx = c_dx[dir], dy = c_dy[dir];
+    const int lane = threadIdx.x;
+    const bool horiz = dy == 0;
+    const int nLines = horiz ? H : W;
+    const int len = horiz ? W : H;
+    const int line = blockIdx.x * 64 + lane;
+    if (line >= nLines) return;
+    int* slot = lds + lane * SL;
+    constexpr int INF = 0x3FFFFFFF;
+    slot[0] = INF;
+    slot[K + 3] = INF;
+    const long long plane = (long long)W * H;
+    const uint8_t* Cb = C + (long long)blockIdx.z * plane * Kp;
+    const int* Bb = base + (long long)blockIdx.z * plane;
+    LT* Lb = Lout + (long long)blockIdx.y * dirStride + (long long)blockIdx.z * plane * Kp;
+    int L[K];
+    int mprev = 0, bprev = 0;
+    for (int i = 0; i < len; ++i) {
+        int x, y;
+        bool border;
+        if (horiz) {
+            y = line;
+            x = dx > 0 ? i : W - 1 - i;
+            border = i == 0;
+        } else {
+            y = dy > 0 ? i : H - 1 - i;
+            if (dx == 0) {
+                x = line;
+                border = i == 0;
+            } else {
+                int t = (dx > 0 ? line + i : line - i) % W;
+                if (t < 0) t += W;
+                x = t;
+                border = i == 0 || (dx > 0 ? x == 0 : x == W - 1);
+            }
+        }
+        const long long pix = (long long)y * W + x;
+        int c[(K + 7) / 8 * 8];
+        load_chunk_u8<int, (K + 7) / 8 * 8>(Cb + pix * Kp, c);
+        const int b = Bb[pix];
+        if (border) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) L[k] = c[k];
+        } else {
+            const int s = b - bprev;
+            const int cap = mprev + P2;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int idx = min(max(k + s + 2, 0), K + 3);
+                L[k] = c[k] + min(slot[idx], cap) - mprev;
+            }
+        }
+        int m = L[0];
+#pragma unroll
+        for (int k = 1; k < K; ++k) m = min(m, L[k]);
+        slot[1] = L[0] + P1;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int lm = k > 0 ? L[k - 1] : INF;
+            const int lp = k < K - 1 ? L[k + 1] : INF;
+            slot[k + 2] = min(L[k], min(lm, lp) + P1);
+        }
+        slot[K + 2] = L[K - 1] + P1;
+        int Ls[(K + 7) / 8 * 8];
+#pragma unroll
+        for (int k = 0; k < (K + 7) / 8 * 8; ++k) Ls[k] = k < K ? L[k] : 0;
+        store_any<LT, int, (K + 7) / 8 * 8>(Lb + pix * Kp, Ls, K);
+        mprev = m;
+        bprev = b;
+    }
+}
+
 // ------------------------------------------------------------------- WTA
 // refine_fullres epilogue, stereonet.py:5271-5302: path sums S, first
 // minimum, band-edge passthrough, second best outside bk+-1, confidence,
@@ -489,6 +576,75 @@ __global__ void __launch_bounds__(64) k_wta(const LT* __restrict__ Lv, long long
             o = d0;
         } else {
             const double ad = (double)a, bd = (double)b;
+            const double den = ad - 2.0 * bvd + bd;
+            double off = 0.0;
+            if (den > 1.0e-9) off = 0.5 * (ad - bd) / den;
+            o = (double)(base[pix] - R + bk) + off;
+        }
+    }
+    out[pix] = o;
+    if (conf) conf[pix] = cf;
+}
+
+// k_wta with G = 1 and K known at compile time (integer path).
+template <int K, typename LT>
+__global__ void __launch_bounds__(64) k_wta_k(const LT* __restrict__ Lv, long long dirStride,
+                                              int ndir, int mirrorFrom, const int* __restrict__ base,
+                                              const double* __restrict__ d0a,
+                                              const double* __restrict__ d0b, int B, int HW, int Kp,
+                                              int R, double uniq, double* __restrict__ out,
+                                              double* __restrict__ conf) {
+    constexpr int K8 = (K + 7) / 8 * 8;
+    const int bv = blockIdx.y;
+    const int p = blockIdx.x * 64 + threadIdx.x;
+    if (p >= HW) return;
+    const long long pix = (long long)bv * HW + p;
+    int S[K8];
+#pragma unroll
+    for (int k = 0; k < K8; ++k) S[k] = 0;
+    for (int t = 0; t < ndir; ++t) {
+        int l[K8];
+        const LT* src = Lv + (long long)t * dirStride + pix * Kp;
+        if constexpr (sizeof(LT) == 1) {
+            load_chunk_u8<int, K8>(reinterpret_cast<const uint8_t*>(src), l);
+        } else {
+#pragma unroll
+            for (int k = 0; k < K8; ++k) l[k] = reinterpret_cast<const uint16_t*>(src)[k];
+        }
+#pragma unroll
+        for (int k = 0; k < K8; ++k) S[k] += l[k];
+    }
+    int bvv = S[0], bk = 0;
+#pragma unroll
+    for (int k = 1; k < K; ++k)
+        if (S[k] < bvv) {
+            bvv = S[k];
+            bk = k;
+        }
+    const int v = bv >= B ? 1 : 0;
+    const double d0 = (v ? d0b : d0a)[(long long)(bv - v * B) * HW + p];
+    double o, cf;
+    if (bk == 0 || bk == K - 1) {
+        o = d0;
+        cf = 0.0;
+    } else {
+        int sv = 0x7FFFFFFF, a = 0, bb = 0;
+        bool found = false;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (k < bk - 1 || k > bk + 1) {
+                found = true;
+                sv = min(sv, S[k]);
+            }
+            if (k == bk - 1) a = S[k];
+            if (k == bk + 1) bb = S[k];
+        }
+        const double bvd = (double)bvv;
+        cf = found ? ((double)sv - bvd) / (bvd + 1.0e-6) : 1.0;
+        if (cf < uniq) {
+            o = d0;
+        } else {
+            const double ad = (double)a, bd = (double)bb;
             const double den = ad - 2.0 * bvd + bd;
             double off = 0.0;
             if (den > 1.0e-9) off = 0.5 * (ad - bd) / den;
@@ -868,8 +1024,32 @@ int wta_t(const Plan& pl, const void* L, int BV, int mirrorFrom, const int* base
     }                                                                          \
     return sv::set_error(SV_EUNSUPPORTED, "band chunk %d", KCV);
 
+template <int K, typename LT>
+int sgm_k_t(const Plan& pl, const void* C, const int* base, void* L, int BV, int W, int H,
+            double P1, double P2, int dir0, int ndir, hipStream_t st) {
+    const int nl = W > H ? W : H;
+    dim3 grid((nl + 63) / 64, ndir, BV);
+    const long long dirStride = (long long)BV * W * H * pl.Kp;
+    hipLaunchKernelGGL((k_sgm_k<K, LT>), grid, dim3(64), 0, st, (const uint8_t*)C, base, (LT*)L, W,
+                       H, pl.Kp, (int)P1, (int)P2, dir0, dirStride);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+// band lengths with a compile-time specialisation (G = 1, integer path)
+inline bool has_k_specialisation(const Plan& pl) {
+    return pl.integer && pl.G == 1 && (pl.K == 21 || pl.K == 7) && pl.Kp >= ((pl.K + 7) / 8) * 8;
+}
+
 int launch_sgm(const Plan& pl, const void* C, const int* base, void* L, int BV, int W, int H,
                double P1, double P2, int dir0, int ndir, hipStream_t st) {
+    if (has_k_specialisation(pl)) {
+        if (pl.K == 21)
+            return pl.lt == 1 ? sgm_k_t<21, uint8_t>(pl, C, base, L, BV, W, H, P1, P2, dir0, ndir, st)
+                              : sgm_k_t<21, uint16_t>(pl, C, base, L, BV, W, H, P1, P2, dir0, ndir, st);
+        return pl.lt == 1 ? sgm_k_t<7, uint8_t>(pl, C, base, L, BV, W, H, P1, P2, dir0, ndir, st)
+                          : sgm_k_t<7, uint16_t>(pl, C, base, L, BV, W, H, P1, P2, dir0, ndir, st);
+    }
     if (!pl.integer) {
         switch (pl.KC) {
             case 8: return sgm_t<double, double, double, 8>(pl, C, base, L, BV, W, H, P1, P2, dir0, ndir, st);
@@ -883,9 +1063,29 @@ int launch_sgm(const Plan& pl, const void* C, const int* base, void* L, int BV, 
     SV_DISPATCH_KC(pl.KC, (sgm_t<uint8_t, uint16_t, int, KC_>(pl, C, base, L, BV, W, H, P1, P2, dir0, ndir, st)));
 }
 
+template <int K, typename LT>
+int wta_k_t(const Plan& pl, const void* L, int BV, int mirrorFrom, const int* base,
+            const double* d0a, const double* d0b, int B, int W, int H, int R, double uniq,
+            double* out, double* conf, hipStream_t st) {
+    const int HW = W * H;
+    dim3 grid((HW + 63) / 64, BV);
+    const long long dirStride = (long long)BV * HW * pl.Kp;
+    hipLaunchKernelGGL((k_wta_k<K, LT>), grid, dim3(64), 0, st, (const LT*)L, dirStride, pl.ndir,
+                       mirrorFrom, base, d0a, d0b, B, HW, pl.Kp, R, uniq, out, conf);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
 int launch_wta(const Plan& pl, const void* L, int BV, int mirrorFrom, const int* base,
                const double* d0a, const double* d0b, int B, int W, int H, int R, double uniq,
                double* out, double* conf, hipStream_t st) {
+    if (has_k_specialisation(pl)) {
+        if (pl.K == 21)
+            return pl.lt == 1 ? wta_k_t<21, uint8_t>(pl, L, BV, mirrorFrom, base, d0a, d0b, B, W, H, R, uniq, out, conf, st)
+                              : wta_k_t<21, uint16_t>(pl, L, BV, mirrorFrom, base, d0a, d0b, B, W, H, R, uniq, out, conf, st);
+        return pl.lt == 1 ? wta_k_t<7, uint8_t>(pl, L, BV, mirrorFrom, base, d0a, d0b, B, W, H, R, uniq, out, conf, st)
+                          : wta_k_t<7, uint16_t>(pl, L, BV, mirrorFrom, base, d0a, d0b, B, W, H, R, uniq, out, conf, st);
+    }
     if (!pl.integer) {
         switch (pl.KC) {
             case 8: return wta_t<double, double, 8>(pl, L, BV, mirrorFrom, base, d0a, d0b, B, W, H, R, uniq, out, conf, st);

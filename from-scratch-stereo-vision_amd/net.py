@@ -48,7 +48,32 @@ L.extra_signatures({
     "sv_pool_valid": (_I, [_P, _P, _LL, _P, _P, _LL, _I, _I, _I, _P]),
     "sv_axpby": (_I, [_P, _P, _LL, _F, _F, _P]),
     "sv_widen": (_I, [_P, _P, _LL, _P, _LL, _P, _I, _I, _P]),
+    "sv_conv_mfma_pack_bytes": (_I, [_I, _I, _I, ctypes.POINTER(_SZ)]),
+    "sv_conv_fwd_mfma": (_I, [_P, _P, _P, _LL, _I, _I, _I, _P, _P, _I, _I, _I, _I, _P, _P, _LL, _I,
+                              _LL, _I, _F, _P, _LL, _P]),
 })
+
+
+def _bf16_bits(a):
+    """float -> bfloat16 bit patterns, round to nearest even."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return u.astype(np.uint16)
+
+
+def _pack_mfma(w, ic, oc, kd):
+    """Weight [OC][IC][KD*9] -> the B operand of v_mfma_f32_16x16x16_bf16
+    for k = tap*IC + ic: pack[ks][t][lane][i] = B[16ks + 4(lane/16) + i]
+    [16t + lane%16] (csrc/mfma_conv.hip)."""
+    K = ic * kd * 9
+    ks = (K + 15) // 16
+    nt = (oc + 15) // 16
+    B = np.zeros((ks * 16, nt * 16), dtype=np.float32)
+    wr = w.reshape(oc, ic, kd * 9)
+    k = np.arange(K)
+    B[:K, :oc] = wr[:, k % ic, k // ic].T
+    arr = B.reshape(ks, 4, 4, nt, 16).transpose(0, 3, 1, 4, 2).reshape(ks, nt, 64, 4)
+    return _bf16_bits(arr)
 
 LEVEL_WEIGHTS = [0.5, 0.7, 1.0]   # stereonet.py:1615
 CE_WEIGHT = 2.0                   # stereonet.py:1616
@@ -124,6 +149,24 @@ class StereoNet:
         self._init(seed if seed is not None else (rng if isinstance(rng, int) else 1))
         self._plans = {}
         self._cache = None
+        self.bf16 = False
+        self._packs = {}
+
+    # -- precision -------------------------------------------------------
+    def use_bf16(self, on=True):
+        """Run every convolution with IC % 4 == 0 and OC >= 8 on the matrix
+        cores (BF16 operands, FP32 accumulation); the rest, soft-argmin,
+        losses and Adam stay FP32.  Weights are packed now; call again after
+        changing the parameters."""
+        self.bf16 = bool(on)
+        self._packs = {}
+        if not on:
+            return
+        host = self.params.detach().cpu().numpy()
+        for c in self.convs():
+            if c.ic % 4 == 0 and c.oc >= 8:
+                pk = _pack_mfma(host[c.w_off:c.b_off], c.ic, c.oc, c.kd)
+                self._packs[c.w_off] = torch.from_numpy(pk.view(np.int16).copy()).to(self.params.device)
 
     # -- parameters ------------------------------------------------------
     def convs(self):
@@ -360,7 +403,15 @@ class _Plan:
 
     def _conv(self, c, x, xlv, y, ylv, N, mode, D=1, xr=None, resid=None, rstride=0):
         P = self.net.params
-        _call("sv_conv_fwd", _p(x), _p(xr), _p(xlv.table), c.ic * D * xlv.slot, c.ic, D, c.kd,
+        pk = self.net._packs.get(c.w_off) if self.net.bf16 else None
+        xst = c.ic * (1 if xr is not None else D) * xlv.slot
+        if pk is not None:
+            _call("sv_conv_fwd_mfma", _p(x), _p(xr), _p(xlv.table), xst, c.ic, D, c.kd, _p(pk),
+                  _p(P[c.b_off:]), c.oc, c.s, c.p, c.dil, _p(y), _p(ylv.table),
+                  c.oc * D * ylv.slot, N, D * ylv.slot, mode, SLOPE, _p(resid), rstride,
+                  L.stream())
+            return
+        _call("sv_conv_fwd", _p(x), _p(xr), _p(xlv.table), xst, c.ic, D, c.kd,
               _p(P[c.w_off:]), _p(P[c.b_off:]), c.oc, c.s, c.p, c.dil, _p(y), _p(ylv.table),
               c.oc * D * ylv.slot, N, ylv.maxh, ylv.maxw, mode, SLOPE, _p(resid), rstride,
               L.stream())
@@ -378,10 +429,7 @@ class _Plan:
         fl = self.fa[3]
         fr = self.fa[3][N * C * self.f4.slot:]
         c1, c2, c3 = net.filt
-        _call("sv_conv_fwd", _p(fl), _p(fr), _p(self.f4.table), C * self.f4.slot, C, D, 3,
-              _p(net.params[c1.w_off:]), _p(net.params[c1.b_off:]), Cc, 1, 1, 1,
-              _p(self.ca[0]), _p(self.f4.table), Cc * D * self.f4.slot, N, self.f4.maxh,
-              self.f4.maxw, 1, SLOPE, None, 0, L.stream())
+        self._conv(c1, fl, self.f4, self.ca[0], self.f4, N, 1, D=D, xr=fr)
         self._conv(c2, self.ca[0], self.f4, self.ca[1], self.f4, N, 1, D=D)
         self._conv(c3, self.ca[1], self.f4, self.ca[2], self.f4, N, 0, D=D)
         # soft-argmin with the invalid-disparity penalty (:1216-1218)
@@ -886,3 +934,30 @@ def predict_auto(net, left, right, W, H, target_peak=None, probe_width=64,
     else:
         dw = net.forward_resized(l, r, W, H, [(Hw, Ww)]).out(2, 0).clone()
     return _ret(full, kind), _ret(dw, kind), Ww, Hw
+
+
+def widen(old, feat_C, cost_C, refine_C, seed=77):
+    """Net2Net widening as widen_net.py:18-41 does it: every convolution of
+    the wider network receives the old weights in its top-left block,
+    weights from new input channels into old output channels are zero (so
+    the wider net computes exactly the old output), new output channels keep
+    their He-initialised weights and get zero bias."""
+    new = StereoNet(None, feat_C=feat_C, cost_C=cost_C, D=old.D, max_disp=old.max_disp,
+                    refine_C=refine_C, refine_dilations=old.refine_dilations, seed=seed)
+    new.target_peak = old.target_peak
+    po = old.params.detach().cpu().double().numpy()
+    pn = new.params.detach().cpu().double().numpy().copy()
+    co, cn = old.convs(), new.convs()
+    assert len(co) == len(cn)
+    for a, b in zip(co, cn):
+        kk = a.kd * 9
+        wa = po[a.w_off:a.b_off].reshape(a.oc, a.ic, kk)
+        wb = pn[b.w_off:b.b_off].reshape(b.oc, b.ic, kk)
+        wb[:a.oc, :, :] = 0.0
+        wb[:a.oc, :a.ic, :] = wa
+        bb = np.zeros(b.oc)
+        bb[:a.oc] = po[a.b_off:a.end]
+        pn[b.w_off:b.b_off] = wb.ravel()
+        pn[b.b_off:b.end] = bb
+    new.params.copy_(torch.from_numpy(pn).float())
+    return new

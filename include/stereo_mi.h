@@ -111,6 +111,115 @@ int sv_refine_disparity(const void* left, const void* right, int dtype, const do
                         double tol, double* out, uint8_t* status, void* ws, size_t ws_bytes,
                         void* stream);
 
+/* Opt-in stage timing of the refinement entry points: sv_profile_enable(1)
+ * clears and starts recording HIP events around each stage on the caller's
+ * stream; sv_profile_read sums a stage's ("census", "costs", "sgm", "wta",
+ * "lr", "median") elapsed milliseconds.  Instrumentation, no reference
+ * counterpart. */
+int sv_profile_enable(int on);
+int sv_profile_read(const char* stage, double* total_ms, int* launches);
+
+/* ------------------------------------------------------ StereoNet (FP32)
+ *
+ * Ragged batches: a tensor of N samples is N slots of `stride` floats;
+ * sample n is dense [C][D][h_n][w_n] at ptr + n*stride, with (h_n, w_n)
+ * read from a device table `dims` of 2N ints.  max_h/max_w size the grid.
+ * Weights follow the reference layout [OC][IC][KD][3][3] (KD = 1 for 2D).
+ */
+
+/* Conv2d.forward (stereonet.py:489-526) and Conv3d.forward (:596-641).
+ * xr != NULL: the input is CostVolume.forward(x, xr) (:771-799), formed on
+ * the fly.  mode 0: bias; 1: LeakyReLU(slope) (:705-710); 2: max(resid +
+ * conv, 0), the Refinement output (:1150-1163). */
+int sv_conv_fwd(const float* x, const float* xr, const int* idims, long long xstride, int IC,
+                int D, int KD, const float* w, const float* b, int OC, int s, int p, int dil,
+                float* y, const int* odims, long long ystride, int N, int max_ho, int max_wo,
+                int mode, float slope, const float* resid, long long rstride, void* stream);
+
+/* Conv2d/Conv3d.backward input gradient (:528-568, :643-692); ic_count > 0
+ * computes only the first ic_count input channels. */
+int sv_conv_bwd_data(const float* g, const int* odims, long long gstride, const float* w, int IC,
+                     int OC, int D, int KD, int s, int p, int dil, float* gx, const int* idims,
+                     long long gxstride, int N, int max_hi, int max_wi, int ic_count,
+                     void* stream);
+
+/* Conv2d/Conv3d.backward weight and bias gradients (accumulated into gw,
+ * gb, like the reference's +=); partials is a workspace of the size the
+ * query returns. */
+int sv_conv_bwd_weight_workspace_bytes(int N, int IC, int OC, int KD, int nchunk, size_t* bytes);
+int sv_conv_bwd_weight(const float* x, const float* xr, const int* idims, long long xstride,
+                       int IC, int D, int KD, const float* g, const int* odims, long long gstride,
+                       int OC, int s, int p, int dil, int N, int nchunk, float* gw, float* gb,
+                       float* partials, size_t ws_bytes, void* stream);
+
+/* LeakyReLU.backward (:713-720) in place, from the post-activation values. */
+int sv_act_bwd(float* g, const float* y, long long n, float slope, void* stream);
+
+/* CostVolume.backward (:801-825). */
+int sv_costvol_bwd(const float* fl, const float* fr, const int* dims, long long fstride, int C,
+                   int D, const float* gvol, long long gvstride, float* gfl, float* gfr, int N,
+                   int max_h, int max_w, void* stream);
+
+/* SoftArgmin.forward (:861-889) with CostVolume.invalid_mask(penalty)
+ * (:827-840); SoftArgmin.backward (:891-901) plus an optional direct cost
+ * gradient (the cross-entropy term of train_grads, :1641-1650). */
+int sv_softargmin_fwd(const float* cost, const int* dims, long long cstride, int D,
+                      float penalty, float* disp, long long dstride, float* probs, int N,
+                      int max_hw, void* stream);
+int sv_softargmin_bwd(const float* probs, const float* disp, const float* gdisp,
+                      const float* gextra, const int* dims, long long cstride, int D,
+                      long long dstride, float* gcost, int N, int max_hw, void* stream);
+
+/* BilinearUpsample.forward/backward (:913-973) and resize_gray
+ * (:3843-3845): y = mul * bilinear(x) (muln, when not NULL, gives one
+ * multiplier per sample) into output channel och; uint8 input reads as
+ * value/255. */
+int sv_bilinear_fwd(const void* x, int dtype, const int* idims, long long xstride, int C,
+                    float* y, const int* odims, long long ystride, int och, float mul,
+                    const float* muln, int N, int max_ho, int max_wo, void* stream);
+int sv_bilinear_bwd(const float* gy, const int* odims, long long gystride, int ich, int C,
+                    float* gx, const int* idims, long long gxstride, float mul, int N, int max_hi,
+                    int max_wi, void* stream);
+
+/* avgpool2 (:979-995), scaled by mul (pool_disp uses 0.5, :1572-1576). */
+int sv_avgpool2(const void* x, int dtype, const int* idims, long long xstride, float* y,
+                const int* odims, long long ystride, int och, float mul, int N, int max_ho,
+                int max_wo, void* stream);
+/* pool_valid (:1579-1592). */
+int sv_pool_valid(const uint8_t* v, const int* idims, long long vstride, uint8_t* out,
+                  const int* odims, long long ostride, int N, int max_ho, int max_wo,
+                  void* stream);
+/* Channel copy into a multi-channel slot (the guide channel of
+ * Refinement's input, :1140-1146); uint8 reads as value/255. */
+int sv_copy_channel(const void* x, int dtype, const int* dims, long long xstride, float* y,
+                    long long ystride, int och, int N, int max_hw, void* stream);
+
+/* pseudo_huber (:1326-1351) and disparity_ce (:1274-1323) per sample, with
+ * float64 reductions: gradients scaled by weight*gscale/count_n, and
+ * loss_out[n] += weight * loss_n.  part: 2*N*nblk doubles, scale: N floats. */
+int sv_pseudo_huber(const float* pred, const float* target, const uint8_t* valid,
+                    const int* dims, long long stride, int N, float alpha, double weight,
+                    double gscale, float* grad, double* loss_out, double* part, float* scale,
+                    int nblk, void* stream);
+int sv_disparity_ce(const float* probs, const float* target, const uint8_t* valid,
+                    const int* dims, long long pstride, long long tstride, int D, float sigma,
+                    int N, double weight, double gscale, float* gcost, double* loss_out,
+                    double* part, float* scale, int nblk, void* stream);
+
+/* Adam.step (:1368-1387) with c1 = 1/(1-b1^t), c2 = 1/(1-b2^t). */
+int sv_adam_step(float* p, const float* g, float* m, float* v, long long n, float lr, float b1,
+                 float b2, float eps, float c1, float c2, float clip, void* stream);
+
+/* sorted(values)[k] per sample: the selection inside _peak (:3866-3869). */
+int sv_select_kth(const float* x, const int* dims, long long stride, const long long* kth,
+                  float* out, int N, void* stream);
+
+/* Elementwise helpers of the backward pass and of the network-to-
+ * refinement hand-off (no reference counterpart: list arithmetic there). */
+int sv_axpby(const float* x, float* y, long long n, float a, float b, void* stream);
+int sv_widen(const float* x, const int* dims, long long xstride, double* y, long long ystride,
+             const float* mul, int N, int max_hw, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

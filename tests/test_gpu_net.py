@@ -196,3 +196,54 @@ def test_loss_falls_on_one_scene(sv, N):
         first = loss if first is None else first
     print("loss %.4f -> %.4f" % (first, loss))
     assert loss < 0.6 * first
+
+
+def test_widened_net_computes_the_old_output(sv, N):
+    """widen_net.py:27-40: the 4x-widened net (48/24/32 channels) starts
+    out computing exactly what the v5 net computes (up to FP32 summation
+    order)."""
+    v5 = _v5(N)
+    wide = N.widen(v5, 48, 24, 32)
+    p = sv.synth.make_pair(96, 64, 4, dmin=2.0, dmax=30.0)
+    l = torch.from_numpy(p["left"]).cuda().reshape(-1)
+    r = torch.from_numpy(p["right"]).cuda().reshape(-1)
+    a = v5.forward_batch([l], [r], [(64, 96)]).out(2, 0).clone()
+    b = wide.forward_batch([l], [r], [(64, 96)]).out(2, 0).clone()
+    err = float((a - b).abs().max())
+    print("widened vs v5 max |diff| %.2e px" % err)
+    assert err <= 1e-3
+
+
+@pytest.mark.parametrize("wide", [False, True])
+def test_bf16_mfma_error_vs_float64(sv, N, wide):
+    """BF16 operands on the matrix cores with FP32 accumulation: per-pixel
+    disparity error and EPE shift against the float64 oracle (stated)."""
+    net = _v5(N)
+    if wide:
+        net = N.widen(net, 48, 24, 32)
+        # give the new channels weight so they take part in the output
+        g = np.random.default_rng(1)
+        host = net.params.detach().cpu().double().numpy()
+        for c in net.convs():
+            w = host[c.w_off:c.b_off].reshape(c.oc, c.ic, c.kd * 9)
+            old_ic = {48: 12, 24: 6, 32: 8}.get(c.ic, c.ic)
+            w[:, old_ic:, :] += g.normal(size=w[:, old_ic:, :].shape) * 0.02
+        N.set_flat_params(net, host)
+    p = sv.synth.make_pair(128, 96, 6, dmin=2.0, dmax=40.0)
+    gl, gr = sv.synth.gray(p["left"]), sv.synth.gray(p["right"])
+    host = _host_params(net)
+    ref = oracle.predict(_arch(net), host, gl, gr, 128, 96)
+    l = torch.from_numpy(gl.astype(np.float32)).cuda().reshape(-1)
+    r = torch.from_numpy(gr.astype(np.float32)).cuda().reshape(-1)
+    f32 = net.forward_batch([l], [r], [(96, 128)]).out(2, 0).double().cpu().numpy()
+    net.use_bf16(True)
+    b16 = net.forward_batch([l], [r], [(96, 128)]).out(2, 0).double().cpu().numpy()
+    net.use_bf16(False)
+    gt, v = p["disp"].ravel(), p["valid"].ravel()
+    e32, e16 = np.abs(f32 - ref), np.abs(b16 - ref)
+    epe = lambda d: float(np.abs(d - gt)[v].mean())
+    print("%s: fp32 max/mean |err| %.2e/%.2e px; bf16 max/mean %.3f/%.4f px; EPE fp64 %.4f "
+          "fp32 %.4f bf16 %.4f" % ("widened" if wide else "v5", e32.max(), e32.mean(), e16.max(),
+                                   e16.mean(), epe(ref), epe(f32), epe(b16)))
+    assert e32.max() <= 2e-3
+    assert e16.mean() <= 0.05 and abs(epe(b16) - epe(ref)) <= 0.05
