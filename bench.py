@@ -54,6 +54,9 @@ def main():
     ap.add_argument("--cpu-pairs", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fullrange", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after the run, copy 256 MiB device-to-device once: a kernel of known "
+                         "traffic to scale rocprofv3 FETCH_SIZE/WRITE_SIZE readings")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -212,6 +215,13 @@ def main():
         "gpu_over_cpu": (round(pairs_s / cpu["value"], 1) if cpu else None),
     }
     print(json.dumps(rec), flush=True)
+    if args.copy_probe:
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy probe: 268435456 bytes read + 268435456 bytes written")
 
 
 if __name__ == "__main__":

@@ -13,6 +13,7 @@
 //
 // Determinism: backward passes are gathers or fixed-order two-stage
 // reductions; no floating-point atomics.
+#include <algorithm>
 #include <cmath>
 
 #include "common.h"
@@ -103,6 +104,134 @@ __global__ void __launch_bounds__(256) k_conv_fwd(
                 v = v > 0.f ? v : 0.f;
             }
             ys[(long long)(oc0 + j) * D * oplane] = v;
+        }
+    }
+}
+
+// LDS-tiled form of k_conv_fwd.  A block makes a 16 x 64 tile of output
+// pixels at one output depth; for each input channel it stages the input
+// halo tile of all KD depth taps in LDS (zero outside the image, the cost
+// volume formed while staging when SRC == 1), double-buffered so the next
+// channel loads while this one is consumed.  Each thread accumulates four
+// output rows x OCB channels; weights come through the scalar cache.
+constexpr int TT_W = 64;
+
+template <int KD, int SRC, int OCB, int RP>
+__global__ void __launch_bounds__(256) k_conv_tile(
+    const float* __restrict__ x, const float* __restrict__ xr, const int* __restrict__ idims,
+    long long xstride, int IC, int D, const float* __restrict__ w, const float* __restrict__ b,
+    int OC, int s, int p, int dil, float* __restrict__ y, const int* __restrict__ odims,
+    long long ystride, int tilesY, int mode, float slope, const float* __restrict__ resid,
+    long long rstride, int IHt, int IWt) {
+    // OCB padded to a multiple of 4 for float4 weight reads
+    constexpr int OCP = (OCB + 3) / 4 * 4;
+    constexpr int WSZ = KD * 9 * OCP;
+    constexpr int TT_H = 4 * RP;  // rows per tile: 4 threads high, RP rows each
+    extern __shared__ __align__(16) float smem_f[];
+    const int n = blockIdx.z;
+    const int od = blockIdx.y / tilesY;
+    const int ty0 = (blockIdx.y - od * tilesY) * TT_H;
+    const int tx0 = blockIdx.x * TT_W;
+    const int hi = dh(idims, n), wi = dw(idims, n);
+    const int ho = dh(odims, n), wo = dw(odims, n);
+    if (tx0 >= wo || ty0 >= ho) return;  // uniform per block
+    const int tx = threadIdx.x, ty = threadIdx.y;  // 64 x 4
+    const int tid = ty * 64 + tx;
+    const int plane_t = IHt * IWt;
+    const int tsz = KD * plane_t;
+    float* wsm = smem_f;                   // [2][KD*9][OCP]
+    float* tile = smem_f + 2 * WSZ;        // [2][KD][IHt][IWt]
+    const int iy0 = ty0 * s - p, ix0 = tx0 * s - p;
+    const long long iplane = (long long)hi * wi;
+    const float* xs = x + (long long)n * xstride;
+    const float* xrs = SRC == 1 ? xr + (long long)n * xstride : nullptr;
+    constexpr int PD = KD / 2;
+    float acc[RP][OCP];
+    auto stage = [&](int ic, int oc0, int slotb) {
+        float* buf = tile + slotb * tsz;
+        for (int e = tid; e < tsz; e += 256) {
+            const int kd = e / plane_t;
+            const int r = e - kd * plane_t;
+            const int yy = r / IWt, xx = r - yy * IWt;
+            const int dd = od + kd - PD, iy = iy0 + yy, ix = ix0 + xx;
+            float v = 0.f;
+            if (dd >= 0 && dd < D && iy >= 0 && iy < hi && ix >= 0 && ix < wi) {
+                if (SRC == 1) {
+                    if (ix < dd) {
+                        v = 2.0f;
+                    } else {
+                        const long long o = (long long)ic * iplane + (long long)iy * wi;
+                        v = fabsf(xs[o + ix] - xrs[o + ix - dd]);
+                    }
+                } else {
+                    v = xs[((long long)ic * D + dd) * iplane + (long long)iy * wi + ix];
+                }
+            }
+            buf[e] = v;
+        }
+        float* wb = wsm + slotb * WSZ;
+        for (int e = tid; e < WSZ; e += 256) {
+            const int t = e / OCP, j = e - t * OCP;
+            wb[e] = (j < OCB && oc0 + j < OC) ? w[((long long)(oc0 + j) * IC + ic) * KD * 9 + t] : 0.f;
+        }
+    };
+    for (int oc0 = 0; oc0 < OC; oc0 += OCB) {
+#pragma unroll
+        for (int i = 0; i < RP; ++i)
+#pragma unroll
+            for (int j = 0; j < OCP; ++j) acc[i][j] = 0.f;
+        stage(0, oc0, 0);
+        __syncthreads();
+        for (int ic = 0; ic < IC; ++ic) {
+            const float* cur = tile + (ic & 1) * tsz;
+            const float4* wc = reinterpret_cast<const float4*>(wsm + (ic & 1) * WSZ);
+            if (ic + 1 < IC) stage(ic + 1, oc0, (ic + 1) & 1);
+#pragma unroll
+            for (int kd = 0; kd < KD; ++kd)
+#pragma unroll
+                for (int ky = 0; ky < 3; ++ky)
+#pragma unroll
+                    for (int kx = 0; kx < 3; ++kx) {
+                        float v[RP];
+#pragma unroll
+                        for (int i = 0; i < RP; ++i)
+                            v[i] = cur[kd * plane_t + ((ty * RP + i) * s + ky * dil) * IWt + tx * s + kx * dil];
+                        const int wt = (kd * 3 + ky) * 3 + kx;
+#pragma unroll
+                        for (int q = 0; q < OCP / 4; ++q) {
+                            const float4 wv = wc[wt * (OCP / 4) + q];
+#pragma unroll
+                            for (int i = 0; i < RP; ++i) {
+                                acc[i][4 * q + 0] += v[i] * wv.x;
+                                acc[i][4 * q + 1] += v[i] * wv.y;
+                                acc[i][4 * q + 2] += v[i] * wv.z;
+                                acc[i][4 * q + 3] += v[i] * wv.w;
+                            }
+                        }
+                    }
+            __syncthreads();
+        }
+        const int ox = tx0 + tx;
+        const long long oplane = (long long)ho * wo;
+        float* ys = y + (long long)n * ystride + (long long)od * oplane;
+#pragma unroll
+        for (int i = 0; i < RP; ++i) {
+            const int oy = ty0 + ty * RP + i;
+            if (ox >= wo || oy >= ho) continue;
+#pragma unroll
+            for (int j = 0; j < OCB; ++j) {
+                const int oc = oc0 + j;
+                if (oc >= OC) break;
+                float v = acc[i][j] + b[oc];
+                if (mode == 1) {
+                    v = v > 0.f ? v : v * slope;
+                } else if (mode == 2) {
+                    v = resid[(long long)n * rstride + (long long)oc * D * oplane +
+                              (long long)od * oplane + (long long)oy * wo + ox] + v;
+                    v = v > 0.f ? v : 0.f;
+                }
+                ys[(long long)oc * D * oplane + (long long)oy * wo + ox] = v;
+            }
         }
     }
 }
@@ -695,6 +824,66 @@ __global__ void __launch_bounds__(1024) k_select(const float* __restrict__ x,
     }
 }
 
+// Multi-block form: per 8-bit pass, every block of a sample adds its LDS
+// histogram of the keys matching the prefix into the sample's global
+// histogram (integer atomics, so the counts are exact); k_sel_pick then
+// moves the prefix to the bucket holding the k-th key.
+struct SelState {
+    uint32_t prefix, k;
+};
+
+__global__ void k_sel_init(const long long* __restrict__ kth, SelState* __restrict__ st,
+                           uint32_t* __restrict__ ghist, int N) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < N * 256) ghist[i] = 0;
+    if (i < N) {
+        st[i].prefix = 0;
+        st[i].k = (uint32_t)kth[i];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_sel_hist(const float* __restrict__ x,
+                                                  const int* __restrict__ dims, long long stride,
+                                                  const SelState* __restrict__ st, int shift,
+                                                  uint32_t* __restrict__ ghist) {
+    __shared__ uint32_t h[256];
+    const int n = blockIdx.y;
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const long long m = (long long)dh(dims, n) * dw(dims, n);
+    const float* xs = x + (long long)n * stride;
+    const uint32_t prefix = st[n].prefix;
+    const uint32_t mask = shift == 24 ? 0u : (0xFFFFFFFFu << (shift + 8));
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < m; i += (long long)gridDim.x * 256) {
+        const uint32_t k = fkey(xs[i]);
+        if ((k & mask) == (prefix & mask)) atomicAdd(&h[(k >> shift) & 0xFFu], 1u);
+    }
+    __syncthreads();
+    const uint32_t c = h[threadIdx.x];
+    if (c) atomicAdd(&ghist[n * 256 + threadIdx.x], c);
+}
+
+__global__ void k_sel_pick(SelState* __restrict__ st, uint32_t* __restrict__ ghist, int shift,
+                           float* __restrict__ out, int N) {
+    const int n = blockIdx.x * 64 + threadIdx.x;
+    if (n >= N) return;
+    uint32_t* h = ghist + n * 256;
+    uint32_t kk = st[n].k, acc = 0;
+    int b = 0;
+    for (; b < 255; ++b) {
+        if (acc + h[b] > kk) break;
+        acc += h[b];
+    }
+    for (int i = 0; i < 256; ++i) h[i] = 0;
+    st[n].k = kk - acc;
+    const uint32_t pre = st[n].prefix | ((uint32_t)b << shift);
+    st[n].prefix = pre;
+    if (shift == 0) {
+        const uint32_t u = (pre & 0x80000000u) ? (pre & 0x7FFFFFFFu) : ~pre;
+        out[n] = __uint_as_float(u);
+    }
+}
+
 // float -> double widening (network disparity -> refinement centre)
 __global__ void k_widen(const float* __restrict__ x, const int* __restrict__ dims,
                         long long xstride, double* __restrict__ y, long long ystride,
@@ -745,10 +934,52 @@ int sv_conv_fwd(const float* x, const float* xr, const int* idims, long long xst
                "sv_conv_fwd: bad arguments");
     SV_REQUIRE(KD == 1 || KD == 3, "sv_conv_fwd: depth kernel must be 1 or 3");
     SV_REQUIRE(mode != 2 || resid, "sv_conv_fwd: residual mode needs resid");
-    const int tY = tiles4(max_ho);
-    dim3 grid((max_wo + 63) / 64, tY * D, N), blk(64, 4);
     hipStream_t st = (hipStream_t)stream;
     const int src = xr ? 1 : 0;
+    {
+        // LDS-tiled path: the input halo tile of one channel (all KD planes),
+        // double-buffered.  Four rows per thread unless that leaves the GPU
+        // with fewer than ~2 blocks per CU, then one.
+        const long long blocks4 = (long long)((max_wo + TT_W - 1) / TT_W) * ((max_ho + 15) / 16) * D * N;
+        const int RP = blocks4 >= 512 ? 4 : 1;
+        const int TT_H = 4 * RP;
+        const int IHt = (TT_H - 1) * s + 2 * dil + 1, IWt = (TT_W - 1) * s + 2 * dil + 1;
+        const int ocp = OC == 1 ? 4 : (OC <= 4 ? 4 : (OC <= 8 ? 8 : (OC <= 12 ? 12 : 16)));
+        const size_t shm = 2 * ((size_t)KD * IHt * IWt + (size_t)KD * 9 * ocp) * sizeof(float);
+        if (shm <= 64 * 1024) {
+            const int tYt = (max_ho + TT_H - 1) / TT_H;
+            dim3 g((max_wo + TT_W - 1) / TT_W, tYt * D, N), bk(64, 4);
+#define LAUNCH_T(KD_, SRC_, OCB_)                                                                \
+    do {                                                                                         \
+        if (RP == 4)                                                                             \
+            hipLaunchKernelGGL((k_conv_tile<KD_, SRC_, OCB_, 4>), g, bk, shm, st, x, xr, idims,    \
+                               xstride, IC, D, w, b, OC, s, p, dil, y, odims, ystride, tYt, mode,  \
+                               slope, resid, rstride, IHt, IWt);                                  \
+        else                                                                                     \
+            hipLaunchKernelGGL((k_conv_tile<KD_, SRC_, OCB_, 1>), g, bk, shm, st, x, xr, idims,    \
+                               xstride, IC, D, w, b, OC, s, p, dil, y, odims, ystride, tYt, mode,  \
+                               slope, resid, rstride, IHt, IWt);                                  \
+    } while (0)
+#define LAUNCH_T_OC(KD_, SRC_)                                                                   \
+    do {                                                                                         \
+        if (OC == 1) LAUNCH_T(KD_, SRC_, 1);                                                     \
+        else if (OC <= 4) LAUNCH_T(KD_, SRC_, 4);                                                \
+        else if (OC <= 8) LAUNCH_T(KD_, SRC_, 8);                                                \
+        else if (OC <= 12) LAUNCH_T(KD_, SRC_, 12);                                              \
+        else LAUNCH_T(KD_, SRC_, 16);                                                            \
+    } while (0)
+            if (KD == 1 && !src) LAUNCH_T_OC(1, 0);
+            else if (KD == 3 && !src) LAUNCH_T_OC(3, 0);
+            else if (KD == 3) LAUNCH_T_OC(3, 1);
+            else LAUNCH_T_OC(1, 1);
+#undef LAUNCH_T_OC
+#undef LAUNCH_T
+            SV_CHECK_LAUNCH();
+            return SV_OK;
+        }
+    }
+    const int tY = tiles4(max_ho);
+    dim3 grid((max_wo + 63) / 64, tY * D, N), blk(64, 4);
 #define LAUNCH_FWD(OCB_, SRC_)                                                                \
     hipLaunchKernelGGL((k_conv_fwd<OCB_, SRC_>), grid, blk, 0, st, x, xr, idims, xstride, IC, D, \
                        KD, w, b, OC, s, p, dil, y, odims, ystride, tY, mode, slope, resid, rstride)
@@ -985,12 +1216,33 @@ int sv_adam_step(float* p, const float* g, float* m, float* v, long long n, floa
     return SV_OK;
 }
 
+int sv_select_kth_workspace_bytes(int N, size_t* bytes) {
+    SV_REQUIRE(bytes && N > 0, "bad arguments");
+    *bytes = (size_t)N * (sizeof(SelState) + 256 * sizeof(uint32_t));
+    return SV_OK;
+}
+
+// ws: sv_select_kth_workspace_bytes(N) bytes; max_count: the largest sample
+// size, which sizes the number of blocks per sample.
 int sv_select_kth(const float* x, const int* dims, long long stride, const long long* kth,
-                  float* out, int N, void* stream) {
+                  float* out, int N, long long max_count, void* ws, size_t ws_bytes,
+                  void* stream) {
     SV_REQUIRE(x && dims && kth && out && N > 0, "sv_select_kth: bad arguments");
-    hipLaunchKernelGGL(k_select, dim3(N), dim3(1024), 0, (hipStream_t)stream, x, dims, stride, kth,
-                       out);
+    size_t need = 0;
+    sv_select_kth_workspace_bytes(N, &need);
+    if (!ws || ws_bytes < need) return sv::set_error(SV_EWORKSPACE, "sv_select_kth needs %zu bytes", need);
+    hipStream_t st = (hipStream_t)stream;
+    uint32_t* ghist = (uint32_t*)ws;
+    SelState* state = (SelState*)(ghist + (size_t)N * 256);
+    hipLaunchKernelGGL(k_sel_init, dim3((N * 256 + 255) / 256), dim3(256), 0, st, kth, state, ghist, N);
     SV_CHECK_LAUNCH();
+    const int nblk = (int)std::min<long long>(64, std::max<long long>(1, max_count / 4096));
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        hipLaunchKernelGGL(k_sel_hist, dim3(nblk, N), dim3(256), 0, st, x, dims, stride, state, shift, ghist);
+        SV_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_sel_pick, dim3((N + 63) / 64), dim3(64), 0, st, state, ghist, shift, out, N);
+        SV_CHECK_LAUNCH();
+    }
     return SV_OK;
 }
 

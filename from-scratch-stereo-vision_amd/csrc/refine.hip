@@ -484,6 +484,186 @@ __global__ void __launch_bounds__(64) k_sgm_k(const uint8_t* __restrict__ C,
     }
 }
 
+// Packed form of k_sgm_k for the uint8 path-cost case (odd K >= 3, every
+// value below 0xFFFF): band entries travel as u16 pairs, so each add / sub
+// / min is one v_pk_*_u16 for two entries.  The LDS slot holds u16 M values
+// with INF (0xFFFF) guard zones of K+2 entries on both sides, so the
+// shifted read M[k+s] needs no per-entry clamp once s is clamped to
+// [-(K+2), K+2] (a larger shift reads only INF, leaving the P2 candidate,
+// as in the reference).  Neighbour pairs (L[2j-1], L[2j]) and
+// (L[2j+1], L[2j+2]) come from v_alignbit.  The pad entry (index K) is
+// kept at INF, which makes the computed M[K] equal L[K-1] + P1.
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+// the slot is written as u32 pairs and read as single u16 entries: both
+// pointer types must be allowed to alias
+typedef uint32_t __attribute__((may_alias)) u32_alias;
+typedef uint16_t __attribute__((may_alias)) u16_alias;
+
+__device__ __forceinline__ u16x2 as_u16x2(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
+__device__ __forceinline__ uint32_t as_u32(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+
+// Position of step i of a scanline (see k_sgm for the line geometry).
+struct LinePos {
+    int W, H, dx, dy, line;
+    bool horiz;
+    __device__ __forceinline__ long long at(int i, bool& border) const {
+        int x, y;
+        if (horiz) {
+            y = line;
+            x = dx > 0 ? i : W - 1 - i;
+            border = i == 0;
+        } else {
+            y = dy > 0 ? i : H - 1 - i;
+            if (dx == 0) {
+                x = line;
+                border = i == 0;
+            } else {
+                int t = (dx > 0 ? line + i : line - i) % W;
+                if (t < 0) t += W;
+                x = t;
+                border = i == 0 || (dx > 0 ? x == 0 : x == W - 1);
+            }
+        }
+        return (long long)y * W + x;
+    }
+};
+
+template <int K>
+__global__ void __launch_bounds__(64) k_sgm_p(const uint8_t* __restrict__ C,
+                                              const int* __restrict__ base,
+                                              uint8_t* __restrict__ Lout, int W, int H, int Kp,
+                                              int P1, int P2, int dir0, long long dirStride) {
+    static_assert(K % 2 == 1 && K >= 3, "odd band length");
+    constexpr int NP = (K + 1) / 2;        // u16 pairs, the last one padded
+    constexpr int NB = (2 * NP + 7) / 8;   // uint2 (8-byte) cost/path chunks
+    constexpr int OFF = K + 3;             // u16 index of M[0]; M[-1] at OFF-1
+    constexpr int SLOT16 = 3 * K + 6;
+    constexpr int SD = ((SLOT16 + 1) / 2) | 1;  // dwords per lane, odd
+    constexpr int PF = 4;                  // steps of cost prefetch in flight
+    __shared__ uint32_t lds[64 * SD];
+    const int dir = dir0 + (int)blockIdx.y;
+    LinePos lp;
+    lp.dx = c_dx[dir];
+    lp.dy = c_dy[dir];
+    lp.W = W;
+    lp.H = H;
+    lp.horiz = lp.dy == 0;
+    const int lane = threadIdx.x;
+    const int nLines = lp.horiz ? H : W;
+    const int len = lp.horiz ? W : H;
+    lp.line = blockIdx.x * 64 + lane;
+    if (lp.line >= nLines) return;
+    u32_alias* sl32 = reinterpret_cast<u32_alias*>(lds + lane * SD);
+    u16_alias* s16 = reinterpret_cast<u16_alias*>(sl32);
+    for (int i = 0; i < SD; ++i) sl32[i] = 0xFFFFFFFFu;
+    const long long plane = (long long)W * H;
+    const uint8_t* Cb = C + (long long)blockIdx.z * plane * Kp;
+    const int* Bb = base + (long long)blockIdx.z * plane;
+    uint8_t* Lb = Lout + (long long)blockIdx.y * dirStride + (long long)blockIdx.z * plane * Kp;
+    const u16x2 p1v = {(unsigned short)P1, (unsigned short)P1};
+    // prefetch ring: cost chunks and band centres of the next PF steps
+    uint2 ring[PF][NB];
+    int rb[PF];
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+        bool bd;
+        const long long pq = lp.at(u < len ? u : 0, bd);
+#pragma unroll
+        for (int q = 0; q < NB; ++q) ring[u][q] = *reinterpret_cast<const uint2*>(Cb + pq * Kp + 8 * q);
+        rb[u] = Bb[pq];
+    }
+    u16x2 L[NP];
+    uint32_t mprev = 0;
+    int bprev = 0;
+    // one step of the recurrence from prefetched costs (cr) and centre (b)
+    auto step = [&](int i, const uint2 (&cr)[NB], int b) {
+        bool border;
+        const long long pix = lp.at(i, border);
+        uint32_t cw[2 * NB];
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+            cw[2 * q] = cr[q].x;
+            cw[2 * q + 1] = cr[q].y;
+        }
+        u16x2 c[NP];
+#pragma unroll
+        for (int j = 0; j < NP; ++j)
+            c[j] = as_u16x2(__builtin_amdgcn_perm(0u, cw[j >> 1], (j & 1) ? 0x0c030c02u : 0x0c010c00u));
+        if (border) {
+#pragma unroll
+            for (int j = 0; j < NP; ++j) L[j] = c[j];
+        } else {
+            const int sh = min(max(b - bprev, -(K + 2)), K + 2);
+            const uint32_t cap = mprev + (uint32_t)P2;
+            const u16x2 capv = {(unsigned short)cap, (unsigned short)cap};
+            const u16x2 mpv = {(unsigned short)mprev, (unsigned short)mprev};
+            const u16_alias* g = s16 + OFF + sh;
+#pragma unroll
+            for (int j = 0; j < NP; ++j) {
+                u16x2 m2;
+                m2.x = g[2 * j];
+                m2.y = g[2 * j + 1];
+                const u16x2 best = __builtin_elementwise_min(m2, capv);
+                L[j] = c[j] + best - mpv;
+            }
+        }
+        L[NP - 1].y = 0xFFFF;  // pad entry K
+        u16x2 mm = L[0];
+#pragma unroll
+        for (int j = 1; j < NP; ++j) mm = __builtin_elementwise_min(mm, L[j]);
+        const uint32_t m = min((uint32_t)mm.x, (uint32_t)mm.y);
+        s16[OFF - 1] = (uint16_t)(L[0].x + P1);
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+            const uint32_t lo = j > 0 ? as_u32(L[j - 1]) : 0xFFFFFFFFu;
+            const uint32_t hi = j + 1 < NP ? as_u32(L[j + 1]) : 0xFFFFFFFFu;
+            const u16x2 left = as_u16x2(__builtin_amdgcn_alignbit(as_u32(L[j]), lo, 16));
+            const u16x2 right = as_u16x2(__builtin_amdgcn_alignbit(hi, as_u32(L[j]), 16));
+            const u16x2 nb = __builtin_elementwise_min(left, right) + p1v;
+            sl32[OFF / 2 + j] = as_u32(__builtin_elementwise_min(L[j], nb));
+        }
+        // path costs: low bytes of the pairs (entries >= K written as 0)
+        u16x2 Ls[4 * NB];
+#pragma unroll
+        for (int j = 0; j < 4 * NB; ++j) Ls[j] = j < NP ? L[j] : (u16x2){0, 0};
+        Ls[NP - 1].y = 0;
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+            uint2 v;
+            v.x = __builtin_amdgcn_perm(as_u32(Ls[4 * q + 1]), as_u32(Ls[4 * q]), 0x06040200u);
+            v.y = __builtin_amdgcn_perm(as_u32(Ls[4 * q + 3]), as_u32(Ls[4 * q + 2]), 0x06040200u);
+            *reinterpret_cast<uint2*>(Lb + pix * Kp + 8 * q) = v;
+        }
+        mprev = m;
+        // a value of its own (empty asm): otherwise bprev shares the ring
+        // register and keeps it alive past its refill, which costs a full
+        // memory wait per group of steps
+        int bc = b;
+        asm volatile("" : "+v"(bc));
+        bprev = bc;
+    };
+    auto refill = [&](int u, int inext) {
+        bool bd;
+        const long long pq = lp.at(inext, bd);
+#pragma unroll
+        for (int q = 0; q < NB; ++q) ring[u][q] = *reinterpret_cast<const uint2*>(Cb + pq * Kp + 8 * q);
+        rb[u] = Bb[pq];
+    };
+    // full groups of PF steps: no exit inside a group, so the ring stays in
+    // place and the loads of step i + PF overlap steps i+1 .. i+PF-1
+    int i0 = 0;
+    for (; i0 + PF <= len; i0 += PF) {
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            step(i0 + u, ring[u], rb[u]);
+            refill(u, min(i0 + u + PF, len - 1));
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < PF; ++u)
+        if (i0 + u < len) step(i0 + u, ring[u], rb[u]);
+}
+
 // ------------------------------------------------------------------- WTA
 // refine_fullres epilogue, stereonet.py:5271-5302: path sums S, first
 // minimum, band-edge passthrough, second best outside bk+-1, confidence,
@@ -731,6 +911,161 @@ __global__ void k_lr_check_fill(const double* __restrict__ dl, const double* __r
         for (int t = x; t < e; ++t) out[r + t] = v < 1.0e8 ? v : disp0[r + t];
         x = e;
     }
+}
+
+// ---------------------------------------------------- row-parallel LR
+// One 256-thread block per image row.  nearest_valid() gives, for every
+// column x, the nearest column at or left of x whose flag is set (-1 if
+// none) and the nearest at or right of x (n if none), by per-thread chunk
+// scans plus a block-wide prefix max / suffix min.
+constexpr int ROW_THREADS = 256;
+
+__device__ void nearest_valid(const uint8_t* __restrict__ flag, int n, int* __restrict__ lidx,
+                              int* __restrict__ ridx, int* __restrict__ sc) {
+    const int t = threadIdx.x;
+    const int chunk = (n + ROW_THREADS - 1) / ROW_THREADS;
+    const int a = min(t * chunk, n), b = min(a + chunk, n);
+    int last = -1, first = n;
+    for (int x = a; x < b; ++x)
+        if (flag[x]) {
+            if (first == n) first = x;
+            last = x;
+        }
+    int* sl = sc;
+    int* sr = sc + ROW_THREADS;
+    sl[t] = last;
+    sr[t] = first;
+    __syncthreads();
+    // inclusive prefix max of sl, suffix min of sr (Hillis-Steele)
+    for (int off = 1; off < ROW_THREADS; off <<= 1) {
+        const int vl = t >= off ? sl[t - off] : -1;
+        const int vr = t + off < ROW_THREADS ? sr[t + off] : n;
+        __syncthreads();
+        sl[t] = max(sl[t], vl);
+        sr[t] = min(sr[t], vr);
+        __syncthreads();
+    }
+    int run = t > 0 ? sl[t - 1] : -1;
+    for (int x = a; x < b; ++x) {
+        if (flag[x]) run = x;
+        lidx[x] = run;
+    }
+    run = t + 1 < ROW_THREADS ? sr[t + 1] : n;
+    for (int x = b - 1; x >= a; --x) {
+        if (flag[x]) run = x;
+        ridx[x] = run;
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ unsigned long long dkey(double d) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(d);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double dval(unsigned long long k) {
+    const unsigned long long u = (k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
+    return __longlong_as_double((long long)u);
+}
+
+// left_to_right, stereonet.py:5317-5340.  The z-buffer keeps the largest
+// disparity landing on a column (and -1 where none does), which a max over
+// order-preserving keys computes exactly in any order; holes (< 0) then
+// take min(nearest non-hole left, right), 0.0 if neither is below 1e8.
+__global__ void __launch_bounds__(ROW_THREADS) k_left_to_right_row(const double* __restrict__ dl,
+                                                                   int W, int H,
+                                                                   double* __restrict__ dr) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    unsigned long long* key = reinterpret_cast<unsigned long long*>(smem_raw);
+    int* lidx = reinterpret_cast<int*>(key + W);
+    int* ridx = lidx + W;
+    int* sc = ridx + W;
+    uint8_t* flag = reinterpret_cast<uint8_t*>(sc + 2 * ROW_THREADS);
+    const int y = blockIdx.x, b = blockIdx.y;
+    const long long r = ((long long)b * H + y) * W;
+    const unsigned long long kinit = dkey(-1.0);
+    for (int x = threadIdx.x; x < W; x += ROW_THREADS) key[x] = kinit;
+    __syncthreads();
+    for (int x = threadIdx.x; x < W; x += ROW_THREADS) {
+        const double d = dl[r + x];
+        const int xr = warp_col(x, d, W);
+        if (xr >= 0 && d > -1.0) atomicMax(&key[xr], dkey(d));
+    }
+    __syncthreads();
+    for (int x = threadIdx.x; x < W; x += ROW_THREADS) flag[x] = dval(key[x]) < 0.0 ? 0 : 1;
+    __syncthreads();
+    nearest_valid(flag, W, lidx, ridx, sc);
+    for (int x = threadIdx.x; x < W; x += ROW_THREADS) {
+        double v = dval(key[x]);
+        if (!flag[x]) {
+            const double va = lidx[x] >= 0 ? dval(key[lidx[x]]) : 1.0e9;
+            const double vb = ridx[x] < W ? dval(key[ridx[x]]) : 1.0e9;
+            v = va < vb ? va : vb;
+            v = v < 1.0e8 ? v : 0.0;
+        }
+        dr[r + x] = v;
+    }
+}
+
+// lr_refine's check and fills, stereonet.py:5358-5390, one block per row.
+__global__ void __launch_bounds__(ROW_THREADS) k_lr_check_fill_row(
+    const double* __restrict__ dl, const double* __restrict__ dr, const double* __restrict__ disp0,
+    int W, int H, double tol, double* __restrict__ out, uint8_t* __restrict__ status) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double* o = reinterpret_cast<double*>(smem_raw);
+    int* lidx = reinterpret_cast<int*>(o + W);
+    int* ridx = lidx + W;
+    int* sc = ridx + W;
+    uint8_t* flag = reinterpret_cast<uint8_t*>(sc + 2 * ROW_THREADS);
+    const int y = blockIdx.x, b = blockIdx.y;
+    const long long r = ((long long)b * H + y) * W;
+    for (int x = threadIdx.x; x < W; x += ROW_THREADS) {
+        const double d = dl[r + x];
+        const int xr = warp_col(x, d, W);
+        uint8_t st = 0;
+        if (xr < 0) {
+            st = 1;
+        } else {
+            const double e = dr[r + xr];
+            if (!(fabs(d - e) <= tol)) st = e > d ? 1 : 2;
+        }
+        status[r + x] = st;
+        o[x] = st == 2 ? disp0[r + x] : d;
+        flag[x] = st != 1;
+    }
+    __syncthreads();
+    nearest_valid(flag, W, lidx, ridx, sc);
+    for (int x = threadIdx.x; x < W; x += ROW_THREADS) {
+        double v = o[x];
+        if (!flag[x]) {
+            const double va = lidx[x] >= 0 ? o[lidx[x]] : 1.0e9;
+            const double vb = ridx[x] < W ? o[ridx[x]] : 1.0e9;
+            const double m = va < vb ? va : vb;
+            v = m < 1.0e8 ? m : disp0[r + x];
+        }
+        out[r + x] = v;
+    }
+}
+
+inline size_t row_smem(int W) { return (size_t)W * (8 + 4 + 4 + 1) + 2 * ROW_THREADS * 4 + 16; }
+// rows wider than this use the one-thread-per-row kernels above
+constexpr int ROW_PAR_MAX_W = 3500;
+
+void launch_l2r(const double* dl, int B, int W, int H, double* dr, hipStream_t st) {
+    if (W <= ROW_PAR_MAX_W)
+        hipLaunchKernelGGL(k_left_to_right_row, dim3(H, B), dim3(ROW_THREADS), row_smem(W), st, dl,
+                           W, H, dr);
+    else
+        hipLaunchKernelGGL(k_left_to_right, dim3((H + 63) / 64, B), dim3(64), 0, st, dl, B, W, H, dr);
+}
+
+void launch_lr_check(const double* dl, const double* dr, const double* d0, int B, int W, int H,
+                     double tol, double* out, uint8_t* status, hipStream_t st) {
+    if (W <= ROW_PAR_MAX_W)
+        hipLaunchKernelGGL(k_lr_check_fill_row, dim3(H, B), dim3(ROW_THREADS), row_smem(W), st, dl,
+                           dr, d0, W, H, tol, out, status);
+    else
+        hipLaunchKernelGGL(k_lr_check_fill, dim3((H + 63) / 64, B), dim3(64), 0, st, dl, dr, d0, B,
+                           W, H, tol, out, status);
 }
 
 // ------------------------------------------------------------- median
@@ -1030,6 +1365,13 @@ int sgm_k_t(const Plan& pl, const void* C, const int* base, void* L, int BV, int
     const int nl = W > H ? W : H;
     dim3 grid((nl + 63) / 64, ndir, BV);
     const long long dirStride = (long long)BV * W * H * pl.Kp;
+    // packed u16 kernel when every value stays below the u16 sentinel
+    if (sizeof(LT) == 1 && pl.invalid + P2 + P1 < 65000.0 && pl.Kp >= 8 * ((K + 1 + 7) / 8)) {
+        hipLaunchKernelGGL((k_sgm_p<K>), grid, dim3(64), 0, st, (const uint8_t*)C, base,
+                           (uint8_t*)L, W, H, pl.Kp, (int)P1, (int)P2, dir0, dirStride);
+        SV_CHECK_LAUNCH();
+        return SV_OK;
+    }
     hipLaunchKernelGGL((k_sgm_k<K, LT>), grid, dim3(64), 0, st, (const uint8_t*)C, base, (LT*)L, W,
                        H, pl.Kp, (int)P1, (int)P2, dir0, dirStride);
     SV_CHECK_LAUNCH();
@@ -1291,8 +1633,7 @@ int sv_refine_fullres(const void* left, const void* right, int dtype, const doub
 
 int sv_left_to_right(const double* dl, int B, int W, int H, double* dr, void* stream) {
     SV_REQUIRE(dl && dr && B > 0 && W > 0 && H > 0, "sv_left_to_right: bad arguments");
-    hipLaunchKernelGGL(k_left_to_right, dim3((H + 63) / 64, B), dim3(64), 0, (hipStream_t)stream,
-                       dl, B, W, H, dr);
+    launch_l2r(dl, B, W, H, dr, (hipStream_t)stream);
     SV_CHECK_LAUNCH();
     return SV_OK;
 }
@@ -1311,7 +1652,7 @@ static int lr_core(const void* left, const void* right, int dtype, const double*
     if ((rc = check_ws(bf.bytes, ws, ws_bytes))) return rc;
     {
         StageTimer t(ST_LR, st);
-        hipLaunchKernelGGL(k_left_to_right, dim3((H + 63) / 64, B), dim3(64), 0, st, disp0, B, W, H, bf.d0r);
+        launch_l2r(disp0, B, W, H, bf.d0r, st);
         SV_CHECK_LAUNCH();
     }
     rc = refine_views(pl, bf, left, right, dtype, disp0, bf.d0r, 2, B, W, H, R, P1, P2, ad_weight,
@@ -1321,8 +1662,7 @@ static int lr_core(const void* left, const void* right, int dtype, const double*
     double* lrout = median ? bf.tmp : out;
     {
         StageTimer t(ST_LR, st);
-        hipLaunchKernelGGL(k_lr_check_fill, dim3((H + 63) / 64, B), dim3(64), 0, st, bf.disp,
-                           bf.disp + (size_t)B * hw, disp0, B, W, H, tol, lrout, status);
+        launch_lr_check(bf.disp, bf.disp + (size_t)B * hw, disp0, B, W, H, tol, lrout, status, st);
         SV_CHECK_LAUNCH();
     }
     if (median) {

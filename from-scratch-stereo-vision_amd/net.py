@@ -44,7 +44,8 @@ L.extra_signatures({
     "sv_pseudo_huber": (_I, [_P, _P, _P, _P, _LL, _I, _F, _D, _D, _P, _P, _P, _P, _I, _P]),
     "sv_disparity_ce": (_I, [_P, _P, _P, _P, _LL, _LL, _I, _F, _I, _D, _D, _P, _P, _P, _P, _I, _P]),
     "sv_adam_step": (_I, [_P, _P, _P, _P, _LL, _F, _F, _F, _F, _F, _F, _F, _P]),
-    "sv_select_kth": (_I, [_P, _P, _LL, _P, _P, _I, _P]),
+    "sv_select_kth_workspace_bytes": (_I, [_I, ctypes.POINTER(_SZ)]),
+    "sv_select_kth": (_I, [_P, _P, _LL, _P, _P, _I, _LL, _P, _SZ, _P]),
     "sv_pool_valid": (_I, [_P, _P, _LL, _P, _P, _LL, _I, _I, _I, _P]),
     "sv_axpby": (_I, [_P, _P, _LL, _F, _F, _P]),
     "sv_widen": (_I, [_P, _P, _LL, _P, _LL, _P, _I, _I, _P]),
@@ -283,6 +284,22 @@ def _ret(t, kind):
     return a if kind == "numpy" else a.ravel().tolist()
 
 
+_TABLES = {}
+
+
+def _dims_table(hw, dev):
+    """Device copy of a dims table, cached (host-to-device copies cost a
+    synchronising transfer each)."""
+    key = (tuple(hw), str(dev))
+    t = _TABLES.get(key)
+    if t is None:
+        if len(_TABLES) > 512:
+            _TABLES.clear()
+        t = torch.tensor([v for d in hw for v in d], dtype=torch.int32, device=dev)
+        _TABLES[key] = t
+    return t
+
+
 class _Level:
     """Per-sample dims of one resolution level: a device table and the slot
     size (largest h*w)."""
@@ -290,7 +307,7 @@ class _Level:
     def __init__(self, hw, dev, reps=1):
         hw = list(hw) * reps
         self.hw = hw
-        self.table = torch.tensor([v for d in hw for v in d], dtype=torch.int32, device=dev)
+        self.table = _dims_table(hw, dev)
         self.maxh = max(h for h, _ in hw)
         self.maxw = max(w for _, w in hw)
         self.slot = max(h * w for h, w in hw)
@@ -784,7 +801,11 @@ def _select(buf, lv, N, ks):
     """sorted(values)[k] per sample (the k-th smallest), as float32."""
     kt = torch.tensor(ks, dtype=torch.int64, device=buf.device)
     out = torch.empty(N, dtype=torch.float32, device=buf.device)
-    _call("sv_select_kth", _p(buf), _p(lv.table), lv.slot, _p(kt), _p(out), N, L.stream())
+    nb = ctypes.c_size_t(0)
+    _call("sv_select_kth_workspace_bytes", N, ctypes.byref(nb))
+    ws = torch.empty(nb.value, dtype=torch.uint8, device=buf.device)
+    _call("sv_select_kth", _p(buf), _p(lv.table), lv.slot, _p(kt), _p(out), N, lv.slot, _p(ws),
+          nb.value, L.stream())
     return out
 
 

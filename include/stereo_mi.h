@@ -210,9 +210,13 @@ int sv_disparity_ce(const float* probs, const float* target, const uint8_t* vali
 int sv_adam_step(float* p, const float* g, float* m, float* v, long long n, float lr, float b1,
                  float b2, float eps, float c1, float c2, float clip, void* stream);
 
-/* sorted(values)[k] per sample: the selection inside _peak (:3866-3869). */
+/* sorted(values)[k] per sample: the selection inside _peak (:3866-3869).
+ * max_count is the largest sample's element count; ws holds
+ * sv_select_kth_workspace_bytes(N) bytes. */
+int sv_select_kth_workspace_bytes(int N, size_t* bytes);
 int sv_select_kth(const float* x, const int* dims, long long stride, const long long* kth,
-                  float* out, int N, void* stream);
+                  float* out, int N, long long max_count, void* ws, size_t ws_bytes,
+                  void* stream);
 
 /* Elementwise helpers of the backward pass and of the network-to-
  * refinement hand-off (no reference counterpart: list arithmetic there). */

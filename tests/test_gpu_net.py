@@ -247,3 +247,25 @@ def test_bf16_mfma_error_vs_float64(sv, N, wide):
                                    e16.mean(), epe(ref), epe(f32), epe(b16)))
     assert e32.max() <= 2e-3
     assert e16.mean() <= 0.05 and abs(epe(b16) - epe(ref)) <= 0.05
+
+
+def test_select_kth_matches_sort(sv, N):
+    """sorted(values)[k] per sample (the selection inside _peak,
+    stereonet.py:3866-3869) on a ragged batch: ties, negatives, +-0."""
+    rng = np.random.default_rng(3)
+    sizes = [(370, 460), (51, 64), (7, 5)]
+    vals = [np.round(rng.normal(20.0, 15.0, size=h * w).astype(np.float32) * 4) / 4 for h, w in sizes]
+    vals[1][:10] = -0.0
+    vals[1][10:20] = 0.0
+    lv = N._Level(sizes, N.L.device())
+    buf = torch.zeros(len(sizes) * lv.slot, dtype=torch.float32, device="cuda")
+    for n, v in enumerate(vals):
+        buf[n * lv.slot:n * lv.slot + v.size] = torch.from_numpy(v).cuda()
+    ks = [int(v.size * 0.95) for v in vals]
+    got = N._select(buf, lv, len(sizes), ks).cpu().numpy()
+    want = np.array([np.sort(v)[k] for v, k in zip(vals, ks)], dtype=np.float32)
+    assert np.array_equal(got, want)
+    ks = [0, vals[1].size - 1, 17]
+    got = N._select(buf, lv, len(sizes), ks).cpu().numpy()
+    want = np.array([np.sort(v)[k] for v, k in zip(vals, ks)], dtype=np.float32)
+    assert np.array_equal(got, want)

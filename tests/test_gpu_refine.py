@@ -204,3 +204,31 @@ def test_errors(sv):
     with pytest.raises(NotImplementedError):
         sv.refine_disparity(img, img, [0.5] * 600, torch.zeros(200, dtype=torch.float64,
                                                                    device="cuda"), 20, 10)
+
+
+@pytest.mark.parametrize("W,H", [(700, 9), (3600, 3)])
+def test_left_to_right_random_rows(sv, W, H):
+    """Row-parallel z-buffer warp and hole fill (and the one-thread-per-row
+    kernel past 3500 columns) against the oracle: collisions, negative and
+    huge disparities, long hole runs."""
+    rng = np.random.default_rng(W)
+    dl = rng.uniform(-3.0, 60.0, size=(H, W))
+    dl[:, ::7] = -0.75
+    dl[0, 100:300] = 1e9
+    dl = np.round(dl * 8) / 8
+    got = sv.left_to_right(_cu(dl), W, H).cpu().numpy().ravel()
+    assert np.array_equal(got, oracle.left_to_right(dl, W, H))
+
+
+def test_lr_check_random_rows(sv):
+    """Status and fills of lr_refine at a width the row kernel handles,
+    against the oracle on a synthetic pair (both views refined)."""
+    p = sv.synth.make_pair(640, 24, 12, dmin=5.0, dmax=60.0)
+    W, H = 640, 24
+    gl, gr = sv.synth.gray(p["left"]), sv.synth.gray(p["right"])
+    kw = dict(R=10, P1=12.0, P2=150.0, rx=4, ry=3)
+    lo, st = sv.lr_refine(_cu(p["left"], torch.uint8), _cu(p["right"], torch.uint8),
+                          _cu(p["disp0"]), W, H, tol=2.0, **kw)
+    rlo, rst = oracle.lr_refine(gl, gr, p["disp0"], W, H, tol=2.0, **kw)
+    assert np.array_equal(st.cpu().numpy().ravel().astype(np.int32), rst)
+    assert np.array_equal(lo.cpu().numpy().ravel(), rlo)
