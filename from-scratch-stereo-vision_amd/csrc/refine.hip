@@ -528,18 +528,97 @@ struct LinePos {
     }
 };
 
+// Band geometry of the packed kernels for band length K.
+template <int K>
+struct PackedBand {
+    static_assert(K % 2 == 1 && K >= 3, "odd band length");
+    static constexpr int NP = (K + 1) / 2;       // u16 pairs, the last one padded
+    static constexpr int NB = (2 * NP + 7) / 8;  // uint2 (8-byte) cost/path chunks
+    static constexpr int OFF = K + 3;            // u16 index of M[0]; M[-1] at OFF-1
+    static constexpr int SLOT16 = 3 * K + 6;
+    static constexpr int SD = ((SLOT16 + 1) / 2) | 1;  // slot dwords per lane, odd
+};
+
+// One step of the packed recurrence for one scanline: costs cw (u8, 2*NB
+// words) and band centre b of the current pixel in, path costs out (u8,
+// entries >= K written as 0).  L, mprev and bprev carry the line's state;
+// sl32 / s16 are the line's LDS slot (INF-initialised).
+template <int K>
+__device__ __forceinline__ void packed_step(u16x2 (&L)[PackedBand<K>::NP], uint32_t& mprev,
+                                            int& bprev, u32_alias* sl32, u16_alias* s16,
+                                            const uint32_t (&cw)[2 * PackedBand<K>::NB], int b,
+                                            bool border, int P1, int P2,
+                                            uint2 (&out)[PackedBand<K>::NB]) {
+    constexpr int NP = PackedBand<K>::NP, NB = PackedBand<K>::NB, OFF = PackedBand<K>::OFF;
+    const u16x2 p1v = {(unsigned short)P1, (unsigned short)P1};
+    u16x2 c[NP];
+#pragma unroll
+    for (int j = 0; j < NP; ++j)
+        c[j] = as_u16x2(__builtin_amdgcn_perm(0u, cw[j >> 1], (j & 1) ? 0x0c030c02u : 0x0c010c00u));
+    if (border) {
+#pragma unroll
+        for (int j = 0; j < NP; ++j) L[j] = c[j];
+    } else {
+        const int sh = min(max(b - bprev, -(K + 2)), K + 2);
+        const uint32_t cap = mprev + (uint32_t)P2;
+        const u16x2 capv = {(unsigned short)cap, (unsigned short)cap};
+        const u16x2 mpv = {(unsigned short)mprev, (unsigned short)mprev};
+        const u16_alias* g = s16 + OFF + sh;
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+            u16x2 m2;
+            m2.x = g[2 * j];
+            m2.y = g[2 * j + 1];
+            const u16x2 best = __builtin_elementwise_min(m2, capv);
+            L[j] = c[j] + best - mpv;
+        }
+    }
+    L[NP - 1].y = 0xFFFF;  // pad entry K
+    u16x2 mm = L[0];
+#pragma unroll
+    for (int j = 1; j < NP; ++j) mm = __builtin_elementwise_min(mm, L[j]);
+    const uint32_t m = min((uint32_t)mm.x, (uint32_t)mm.y);
+    s16[OFF - 1] = (uint16_t)(L[0].x + P1);
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+        const uint32_t lo = j > 0 ? as_u32(L[j - 1]) : 0xFFFFFFFFu;
+        const uint32_t hi = j + 1 < NP ? as_u32(L[j + 1]) : 0xFFFFFFFFu;
+        const u16x2 left = as_u16x2(__builtin_amdgcn_alignbit(as_u32(L[j]), lo, 16));
+        const u16x2 right = as_u16x2(__builtin_amdgcn_alignbit(hi, as_u32(L[j]), 16));
+        const u16x2 nb = __builtin_elementwise_min(left, right) + p1v;
+        sl32[OFF / 2 + j] = as_u32(__builtin_elementwise_min(L[j], nb));
+    }
+    // path costs: low bytes of the pairs
+    u16x2 Ls[4 * NB];
+#pragma unroll
+    for (int j = 0; j < 4 * NB; ++j) Ls[j] = j < NP ? L[j] : (u16x2){0, 0};
+    Ls[NP - 1].y = 0;
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        out[q].x = __builtin_amdgcn_perm(as_u32(Ls[4 * q + 1]), as_u32(Ls[4 * q]), 0x06040200u);
+        out[q].y = __builtin_amdgcn_perm(as_u32(Ls[4 * q + 3]), as_u32(Ls[4 * q + 2]), 0x06040200u);
+    }
+    mprev = m;
+    // a value of its own (empty asm): otherwise bprev shares the register
+    // of the prefetch ring and keeps it alive past its refill, which costs a
+    // full memory wait per group of steps
+    int bc = b;
+    asm volatile("" : "+v"(bc));
+    bprev = bc;
+}
+
+// Vertical and diagonal directions (and horizontal ones as a fallback):
+// one lane per scanline, 64 lines per wave.  Neighbouring lanes hold
+// neighbouring columns of one row, so cost reads and path-cost writes are
+// coalesced; the costs of the next PF steps are prefetched into registers.
 template <int K>
 __global__ void __launch_bounds__(64) k_sgm_p(const uint8_t* __restrict__ C,
                                               const int* __restrict__ base,
                                               uint8_t* __restrict__ Lout, int W, int H, int Kp,
                                               int P1, int P2, int dir0, long long dirStride) {
-    static_assert(K % 2 == 1 && K >= 3, "odd band length");
-    constexpr int NP = (K + 1) / 2;        // u16 pairs, the last one padded
-    constexpr int NB = (2 * NP + 7) / 8;   // uint2 (8-byte) cost/path chunks
-    constexpr int OFF = K + 3;             // u16 index of M[0]; M[-1] at OFF-1
-    constexpr int SLOT16 = 3 * K + 6;
-    constexpr int SD = ((SLOT16 + 1) / 2) | 1;  // dwords per lane, odd
-    constexpr int PF = 4;                  // steps of cost prefetch in flight
+    using PB = PackedBand<K>;
+    constexpr int NP = PB::NP, NB = PB::NB, SD = PB::SD;
+    constexpr int PF = 4;  // steps of cost prefetch in flight
     __shared__ uint32_t lds[64 * SD];
     const int dir = dir0 + (int)blockIdx.y;
     LinePos lp;
@@ -560,7 +639,6 @@ __global__ void __launch_bounds__(64) k_sgm_p(const uint8_t* __restrict__ C,
     const uint8_t* Cb = C + (long long)blockIdx.z * plane * Kp;
     const int* Bb = base + (long long)blockIdx.z * plane;
     uint8_t* Lb = Lout + (long long)blockIdx.y * dirStride + (long long)blockIdx.z * plane * Kp;
-    const u16x2 p1v = {(unsigned short)P1, (unsigned short)P1};
     // prefetch ring: cost chunks and band centres of the next PF steps
     uint2 ring[PF][NB];
     int rb[PF];
@@ -575,7 +653,6 @@ __global__ void __launch_bounds__(64) k_sgm_p(const uint8_t* __restrict__ C,
     u16x2 L[NP];
     uint32_t mprev = 0;
     int bprev = 0;
-    // one step of the recurrence from prefetched costs (cr) and centre (b)
     auto step = [&](int i, const uint2 (&cr)[NB], int b) {
         bool border;
         const long long pix = lp.at(i, border);
@@ -585,62 +662,10 @@ __global__ void __launch_bounds__(64) k_sgm_p(const uint8_t* __restrict__ C,
             cw[2 * q] = cr[q].x;
             cw[2 * q + 1] = cr[q].y;
         }
-        u16x2 c[NP];
+        uint2 o[NB];
+        packed_step<K>(L, mprev, bprev, sl32, s16, cw, b, border, P1, P2, o);
 #pragma unroll
-        for (int j = 0; j < NP; ++j)
-            c[j] = as_u16x2(__builtin_amdgcn_perm(0u, cw[j >> 1], (j & 1) ? 0x0c030c02u : 0x0c010c00u));
-        if (border) {
-#pragma unroll
-            for (int j = 0; j < NP; ++j) L[j] = c[j];
-        } else {
-            const int sh = min(max(b - bprev, -(K + 2)), K + 2);
-            const uint32_t cap = mprev + (uint32_t)P2;
-            const u16x2 capv = {(unsigned short)cap, (unsigned short)cap};
-            const u16x2 mpv = {(unsigned short)mprev, (unsigned short)mprev};
-            const u16_alias* g = s16 + OFF + sh;
-#pragma unroll
-            for (int j = 0; j < NP; ++j) {
-                u16x2 m2;
-                m2.x = g[2 * j];
-                m2.y = g[2 * j + 1];
-                const u16x2 best = __builtin_elementwise_min(m2, capv);
-                L[j] = c[j] + best - mpv;
-            }
-        }
-        L[NP - 1].y = 0xFFFF;  // pad entry K
-        u16x2 mm = L[0];
-#pragma unroll
-        for (int j = 1; j < NP; ++j) mm = __builtin_elementwise_min(mm, L[j]);
-        const uint32_t m = min((uint32_t)mm.x, (uint32_t)mm.y);
-        s16[OFF - 1] = (uint16_t)(L[0].x + P1);
-#pragma unroll
-        for (int j = 0; j < NP; ++j) {
-            const uint32_t lo = j > 0 ? as_u32(L[j - 1]) : 0xFFFFFFFFu;
-            const uint32_t hi = j + 1 < NP ? as_u32(L[j + 1]) : 0xFFFFFFFFu;
-            const u16x2 left = as_u16x2(__builtin_amdgcn_alignbit(as_u32(L[j]), lo, 16));
-            const u16x2 right = as_u16x2(__builtin_amdgcn_alignbit(hi, as_u32(L[j]), 16));
-            const u16x2 nb = __builtin_elementwise_min(left, right) + p1v;
-            sl32[OFF / 2 + j] = as_u32(__builtin_elementwise_min(L[j], nb));
-        }
-        // path costs: low bytes of the pairs (entries >= K written as 0)
-        u16x2 Ls[4 * NB];
-#pragma unroll
-        for (int j = 0; j < 4 * NB; ++j) Ls[j] = j < NP ? L[j] : (u16x2){0, 0};
-        Ls[NP - 1].y = 0;
-#pragma unroll
-        for (int q = 0; q < NB; ++q) {
-            uint2 v;
-            v.x = __builtin_amdgcn_perm(as_u32(Ls[4 * q + 1]), as_u32(Ls[4 * q]), 0x06040200u);
-            v.y = __builtin_amdgcn_perm(as_u32(Ls[4 * q + 3]), as_u32(Ls[4 * q + 2]), 0x06040200u);
-            *reinterpret_cast<uint2*>(Lb + pix * Kp + 8 * q) = v;
-        }
-        mprev = m;
-        // a value of its own (empty asm): otherwise bprev shares the ring
-        // register and keeps it alive past its refill, which costs a full
-        // memory wait per group of steps
-        int bc = b;
-        asm volatile("" : "+v"(bc));
-        bprev = bc;
+        for (int q = 0; q < NB; ++q) *reinterpret_cast<uint2*>(Lb + pix * Kp + 8 * q) = o[q];
     };
     auto refill = [&](int u, int inext) {
         bool bd;
@@ -662,6 +687,127 @@ __global__ void __launch_bounds__(64) k_sgm_p(const uint8_t* __restrict__ C,
 #pragma unroll
     for (int u = 0; u < PF; ++u)
         if (i0 + u < len) step(i0 + u, ring[u], rb[u]);
+}
+
+// Horizontal directions (dy = 0): one lane per row, 64 rows per wave.  A
+// lane's consecutive pixels are consecutive in memory but neighbouring lanes
+// are a row apart, so direct per-lane access would touch 64 cache lines per
+// instruction.  Instead the wave stages T columns of its 64 rows through LDS:
+// each row's T*Kp cost bytes are one contiguous run, loaded and later stored
+// by the whole wave; every lane then steps through its row in LDS, writing
+// the path costs over the costs it has consumed.
+template <int K>
+struct HorizTile {
+    static constexpr int NB = PackedBand<K>::NB;
+    // columns per tile: one load instruction per row carries the row's
+    // T*NB cost units and its T band centres
+    static constexpr int T = 64 / (NB + 1) > 32 ? 32 : 64 / (NB + 1);
+    static constexpr int ROWD = T * NB * 2;                 // dwords of one row's run
+    static constexpr int RS = ROWD + ((34 - ROWD % 32) % 32);  // row stride, = 2 mod 32
+    static constexpr int BS = T + 1;                        // band-centre row stride
+};
+
+template <int K>
+__global__ void __launch_bounds__(64) k_sgm_ph(const uint8_t* __restrict__ C,
+                                               const int* __restrict__ base,
+                                               uint8_t* __restrict__ Lout, int W, int H, int Kp,
+                                               int P1, int P2, int dir0, long long dirStride) {
+    using PB = PackedBand<K>;
+    using HT = HorizTile<K>;
+    constexpr int NP = PB::NP, NB = PB::NB, SD = PB::SD;
+    constexpr int T = HT::T, RS = HT::RS, BS = HT::BS;
+    static_assert(RS % 2 == 0, "8-byte aligned rows");
+    __shared__ __align__(16) uint32_t tile[64 * RS];
+    __shared__ int btile[64 * BS];
+    __shared__ uint32_t lds[64 * SD];
+    const int dx = c_dx[dir0 + (int)blockIdx.y];
+    const int lane = threadIdx.x;
+    const int y0 = blockIdx.x * 64;
+    const int rows = min(64, H - y0);
+    u32_alias* sl32 = reinterpret_cast<u32_alias*>(lds + lane * SD);
+    u16_alias* s16 = reinterpret_cast<u16_alias*>(sl32);
+    for (int i = 0; i < SD; ++i) sl32[i] = 0xFFFFFFFFu;
+    const long long plane = (long long)W * H;
+    const uint8_t* Cb = C + (long long)blockIdx.z * plane * Kp + (long long)y0 * W * Kp;
+    const int* Bb = base + (long long)blockIdx.z * plane + (long long)y0 * W;
+    uint8_t* Lb = Lout + (long long)blockIdx.y * dirStride + (long long)blockIdx.z * plane * Kp +
+                  (long long)y0 * W * Kp;
+    const long long rowBytes = (long long)W * Kp;
+    u16x2 L[NP];
+    uint32_t mprev = 0;
+    int bprev = 0;
+    uint2* mine = reinterpret_cast<uint2*>(tile + lane * RS);
+    const int* bmine = btile + lane * BS;
+    // staging walks a (row, unit) grid of the tile, 64 units per
+    // instruction; units past the chunk or rows past the image are clamped
+    // onto the last valid one (a duplicate load and an identical LDS write)
+    constexpr int CU = T * NB;  // 8-byte cost units of a full row run
+    static_assert(CU % 8 == 0, "staging runs in groups of 8 instructions");
+    for (int c0 = 0; c0 < W; c0 += T) {
+        const int n = min(T, W - c0);
+        const int xlo = dx > 0 ? c0 : W - c0 - n;
+        const int units = n * NB;
+        const uint8_t* Cx = Cb + (long long)xlo * Kp;
+        const int* Bx = Bb + xlo;
+#pragma unroll 1
+        for (int it = 0; it < CU; it += 8) {
+            uint2 v[8];
+            int off[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int idx = (it + u) * 64 + lane;
+                const int r = min(idx / CU, rows - 1);
+                const int q = min(idx % CU, units - 1);
+                off[u] = r * RS + 2 * q;
+                v[u] = *reinterpret_cast<const uint2*>(Cx + r * rowBytes + 8 * q);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) *reinterpret_cast<uint2*>(tile + off[u]) = v[u];
+        }
+#pragma unroll
+        for (int it = 0; it < T; it += 8) {
+            int v[8], off[8];
+#pragma unroll
+            for (int u = 0; u < 8 && it + u < T; ++u) {
+                const int idx = (it + u) * 64 + lane;
+                const int r = min(idx / T, rows - 1);
+                const int c = min(idx % T, n - 1);
+                off[u] = r * BS + c;
+                v[u] = Bx[(long long)r * W + c];
+            }
+#pragma unroll
+            for (int u = 0; u < 8 && it + u < T; ++u) btile[off[u]] = v[u];
+        }
+        __syncthreads();
+        for (int j = 0; j < n; ++j) {
+            const int col = dx > 0 ? j : n - 1 - j;
+            uint32_t cw[2 * NB];
+#pragma unroll
+            for (int q = 0; q < NB; ++q) {
+                const uint2 v = mine[col * NB + q];
+                cw[2 * q] = v.x;
+                cw[2 * q + 1] = v.y;
+            }
+            uint2 o[NB];
+            packed_step<K>(L, mprev, bprev, sl32, s16, cw, bmine[col], c0 + j == 0, P1, P2, o);
+#pragma unroll
+            for (int q = 0; q < NB; ++q) mine[col * NB + q] = o[q];
+        }
+        __syncthreads();
+        uint8_t* Lx = Lb + (long long)xlo * Kp;
+#pragma unroll 1
+        for (int it = 0; it < CU; it += 8) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int idx = (it + u) * 64 + lane;
+                const int r = min(idx / CU, rows - 1);
+                const int q = min(idx % CU, units - 1);
+                *reinterpret_cast<uint2*>(Lx + r * rowBytes + 8 * q) =
+                    *reinterpret_cast<const uint2*>(tile + r * RS + 2 * q);
+            }
+        }
+        __syncthreads();
+    }
 }
 
 // ------------------------------------------------------------------- WTA
@@ -1365,11 +1511,25 @@ int sgm_k_t(const Plan& pl, const void* C, const int* base, void* L, int BV, int
     const int nl = W > H ? W : H;
     dim3 grid((nl + 63) / 64, ndir, BV);
     const long long dirStride = (long long)BV * W * H * pl.Kp;
-    // packed u16 kernel when every value stays below the u16 sentinel
+    // packed u16 kernels when every value stays below the u16 sentinel:
+    // the horizontal directions (0, 1) through the LDS-staged kernel, the
+    // others through the per-lane one
     if (sizeof(LT) == 1 && pl.invalid + P2 + P1 < 65000.0 && pl.Kp >= 8 * ((K + 1 + 7) / 8)) {
-        hipLaunchKernelGGL((k_sgm_p<K>), grid, dim3(64), 0, st, (const uint8_t*)C, base,
-                           (uint8_t*)L, W, H, pl.Kp, (int)P1, (int)P2, dir0, dirStride);
-        SV_CHECK_LAUNCH();
+        const int hEnd = min(dir0 + ndir, 2);
+        if (hEnd > dir0) {
+            dim3 hgrid((H + 63) / 64, hEnd - dir0, BV);
+            hipLaunchKernelGGL((k_sgm_ph<K>), hgrid, dim3(64), 0, st, (const uint8_t*)C, base,
+                               (uint8_t*)L, W, H, pl.Kp, (int)P1, (int)P2, dir0, dirStride);
+            SV_CHECK_LAUNCH();
+        }
+        const int vBeg = max(dir0, 2), vEnd = dir0 + ndir;
+        if (vEnd > vBeg) {
+            dim3 vgrid((W + 63) / 64, vEnd - vBeg, BV);
+            hipLaunchKernelGGL((k_sgm_p<K>), vgrid, dim3(64), 0, st, (const uint8_t*)C, base,
+                               (uint8_t*)L + (long long)(vBeg - dir0) * dirStride, W, H, pl.Kp,
+                               (int)P1, (int)P2, vBeg, dirStride);
+            SV_CHECK_LAUNCH();
+        }
         return SV_OK;
     }
     hipLaunchKernelGGL((k_sgm_k<K, LT>), grid, dim3(64), 0, st, (const uint8_t*)C, base, (LT*)L, W,
