@@ -775,56 +775,13 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 
 // ------------------------------------------------------------- selection
 // k-th smallest value per sample (sorted(values)[k], _peak :3866-3869) by
-// a 4-pass 8-bit radix select over order-preserving keys, one block per
-// sample.
+// a 4-pass 8-bit radix select over order-preserving keys.
 __device__ __forceinline__ uint32_t fkey(float f) {
     const uint32_t u = __float_as_uint(f);
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
-__global__ void __launch_bounds__(1024) k_select(const float* __restrict__ x,
-                                                 const int* __restrict__ dims, long long stride,
-                                                 const long long* __restrict__ kth,
-                                                 float* __restrict__ out) {
-    __shared__ uint32_t hist[256];
-    __shared__ uint32_t s_prefix, s_k;
-    const int n = blockIdx.x;
-    const long long m = (long long)dh(dims, n) * dw(dims, n);
-    const float* xs = x + (long long)n * stride;
-    if (threadIdx.x == 0) {
-        s_prefix = 0;
-        s_k = (uint32_t)kth[n];
-    }
-    for (int shift = 24; shift >= 0; shift -= 8) {
-        for (int i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0;
-        __syncthreads();
-        const uint32_t prefix = s_prefix;
-        const uint32_t mask = shift == 24 ? 0u : (0xFFFFFFFFu << (shift + 8));
-        for (long long i = threadIdx.x; i < m; i += blockDim.x) {
-            const uint32_t k = fkey(xs[i]);
-            if ((k & mask) == (prefix & mask)) atomicAdd(&hist[(k >> shift) & 0xFFu], 1u);
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t kk = s_k, acc = 0;
-            int b = 0;
-            for (; b < 256; ++b) {
-                if (acc + hist[b] > kk) break;
-                acc += hist[b];
-            }
-            s_k = kk - acc;
-            s_prefix = prefix | ((uint32_t)b << shift);
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        const uint32_t k = s_prefix;
-        const uint32_t u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
-        out[n] = __uint_as_float(u);
-    }
-}
-
-// Multi-block form: per 8-bit pass, every block of a sample adds its LDS
+// Per 8-bit pass, every block of a sample adds its LDS
 // histogram of the keys matching the prefix into the sample's global
 // histogram (integer atomics, so the counts are exact); k_sel_pick then
 // moves the prefix to the bucket holding the k-th key.
@@ -863,24 +820,38 @@ __global__ void __launch_bounds__(256) k_sel_hist(const float* __restrict__ x,
     if (c) atomicAdd(&ghist[n * 256 + threadIdx.x], c);
 }
 
-__global__ void k_sel_pick(SelState* __restrict__ st, uint32_t* __restrict__ ghist, int shift,
-                           float* __restrict__ out, int N) {
-    const int n = blockIdx.x * 64 + threadIdx.x;
-    if (n >= N) return;
-    uint32_t* h = ghist + n * 256;
-    uint32_t kk = st[n].k, acc = 0;
-    int b = 0;
-    for (; b < 255; ++b) {
-        if (acc + h[b] > kk) break;
-        acc += h[b];
+// One block of 256 threads per sample: an exclusive scan of the 256 bucket
+// counts finds the bucket holding the k-th key (the last bucket when k is
+// past the end, as a serial scan that stops at 255 would).
+__global__ void __launch_bounds__(256) k_sel_pick(SelState* __restrict__ st,
+                                                  uint32_t* __restrict__ ghist, int shift,
+                                                  float* __restrict__ out, int N) {
+    __shared__ uint32_t sc[256];
+    const int n = blockIdx.x;
+    const int t = threadIdx.x;
+    uint32_t* h = ghist + (long long)n * 256;
+    const uint32_t kk = st[n].k, pre0 = st[n].prefix;  // read before any thread updates them
+    const uint32_t c = h[t];
+    sc[t] = c;
+    __syncthreads();
+    // Hillis-Steele inclusive scan
+    for (int o = 1; o < 256; o <<= 1) {
+        const uint32_t v = t >= o ? sc[t - o] : 0u;
+        __syncthreads();
+        sc[t] += v;
+        __syncthreads();
     }
-    for (int i = 0; i < 256; ++i) h[i] = 0;
-    st[n].k = kk - acc;
-    const uint32_t pre = st[n].prefix | ((uint32_t)b << shift);
-    st[n].prefix = pre;
-    if (shift == 0) {
-        const uint32_t u = (pre & 0x80000000u) ? (pre & 0x7FFFFFFFu) : ~pre;
-        out[n] = __uint_as_float(u);
+    const uint32_t incl = sc[t], excl = incl - c;
+    const bool mine = (excl <= kk && kk < incl) || (t == 255 && incl <= kk);
+    h[t] = 0;  // ready for the next pass
+    if (mine) {
+        const uint32_t pre = pre0 | ((uint32_t)t << shift);
+        st[n].k = kk - excl;
+        st[n].prefix = pre;
+        if (shift == 0) {
+            const uint32_t u = (pre & 0x80000000u) ? (pre & 0x7FFFFFFFu) : ~pre;
+            out[n] = __uint_as_float(u);
+        }
     }
 }
 
@@ -1240,7 +1211,7 @@ int sv_select_kth(const float* x, const int* dims, long long stride, const long 
     for (int shift = 24; shift >= 0; shift -= 8) {
         hipLaunchKernelGGL(k_sel_hist, dim3(nblk, N), dim3(256), 0, st, x, dims, stride, state, shift, ghist);
         SV_CHECK_LAUNCH();
-        hipLaunchKernelGGL(k_sel_pick, dim3((N + 63) / 64), dim3(64), 0, st, state, ghist, shift, out, N);
+        hipLaunchKernelGGL(k_sel_pick, dim3(N), dim3(256), 0, st, state, ghist, shift, out, N);
         SV_CHECK_LAUNCH();
     }
     return SV_OK;

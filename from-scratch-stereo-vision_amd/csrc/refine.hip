@@ -72,26 +72,60 @@ __device__ __forceinline__ double big<double>() {
 }
 
 // ------------------------------------------------------------------ census
-// census, stereonet.py:5115-5142: one thread per pixel; the window is read
-// through L1/L2 (each row segment is reused by 2*rx+1 neighbouring lanes).
-template <typename T>
-__global__ void __launch_bounds__(64) k_census(const T* __restrict__ img, int W, int H,
-                                               int rx, int ry, uint64_t* __restrict__ out) {
-    const int x = blockIdx.x * 64 + threadIdx.x;
-    const int y = blockIdx.y;
-    if (x >= W) return;
+// census, stereonet.py:5115-5142.  A block makes 64 x 4 words: it stages
+// the window-padded tile of its pixels in LDS with border-clamped
+// coordinates (the reference clamps every read the same way), then each
+// thread compares its window from LDS.  Bit order: MSB = first offset, dy
+// outer, dx inner.  RX / RY >= 0 fix the window at compile time; -1 takes
+// the runtime radii.
+template <typename T, int RX, int RY>
+__global__ void __launch_bounds__(256) k_census(const T* __restrict__ img, int W, int H,
+                                                int rx_, int ry_, uint64_t* __restrict__ out) {
+    extern __shared__ __align__(16) unsigned char smem_c[];
+    T* tile = reinterpret_cast<T*>(smem_c);
+    const int rx = RX >= 0 ? RX : rx_, ry = RY >= 0 ? RY : ry_;
+    const int TW = 64 + 2 * rx, TH = 4 + 2 * ry;
+    const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 4;
     const long long plane = (long long)W * H;
     const T* im = img + (long long)blockIdx.z * plane;
-    const T c = im[(long long)y * W + x];
-    uint64_t s = 0;
-    for (int dy = -ry; dy <= ry; ++dy) {
-        const int yy = min(max(y + dy, 0), H - 1);
-        const T* row = im + (long long)yy * W;
-        for (int dx = -rx; dx <= rx; ++dx) {
-            if (dx == 0 && dy == 0) continue;
-            const int xx = min(max(x + dx, 0), W - 1);
-            s = (s << 1) | (uint64_t)(row[xx] < c);
-        }
+    const int tid = threadIdx.y * 64 + threadIdx.x;
+    for (int e = tid; e < TW * TH; e += 256) {
+        const int ty = e / TW, tx = e - ty * TW;
+        const int yy = min(max(y0 - ry + ty, 0), H - 1);
+        const int xx = min(max(x0 - rx + tx, 0), W - 1);
+        tile[e] = im[(long long)yy * W + xx];
+    }
+    __syncthreads();
+    const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
+    if (x >= W || y >= H) return;
+    const T* t0 = tile + threadIdx.y * TW + threadIdx.x;  // window corner (-rx, -ry)
+    const T c = t0[ry * TW + rx];
+    uint64_t s;
+    if constexpr (RX >= 0 && RY >= 0) {
+        constexpr int NBITS = (2 * RX + 1) * (2 * RY + 1) - 1;
+        uint32_t hi = 0, lo = 0;
+        int i = 0;
+#pragma unroll
+        for (int dy = 0; dy <= 2 * RY; ++dy)
+#pragma unroll
+            for (int dx = 0; dx <= 2 * RX; ++dx) {
+                if (dy == RY && dx == RX) continue;
+                const int pos = NBITS - 1 - i;  // bit of this offset
+                const uint32_t bit = t0[dy * (64 + 2 * RX) + dx] < c ? 1u : 0u;
+                if (pos >= 32)
+                    hi |= bit << (pos - 32);
+                else
+                    lo |= bit << pos;
+                ++i;
+            }
+        s = ((uint64_t)hi << 32) | lo;
+    } else {
+        s = 0;
+        for (int dy = 0; dy <= 2 * ry; ++dy)
+            for (int dx = 0; dx <= 2 * rx; ++dx) {
+                if (dy == ry && dx == rx) continue;
+                s = (s << 1) | (uint64_t)(t0[dy * TW + dx] < c);
+            }
     }
     out[(long long)blockIdx.z * plane + (long long)y * W + x] = s;
 }
@@ -123,31 +157,42 @@ __global__ void __launch_bounds__(64) k_costs_int(const uint64_t* __restrict__ c
                                                   const int* __restrict__ base, int W, int H,
                                                   int R, int K, int Kp, int invalid,
                                                   uint8_t* __restrict__ C) {
-    const int x = blockIdx.x * 64 + threadIdx.x;
+    // the block's 64 pixels x Kp bytes, written back as contiguous 8-byte
+    // units (a lane's own Kp bytes are Kp apart from its neighbour's)
+    extern __shared__ __align__(16) uint32_t cstage[];
+    const int lane = threadIdx.x;
+    const int x0 = blockIdx.x * 64;
+    const int x = x0 + lane;
     const int y = blockIdx.y;
-    if (x >= W) return;
     const long long plane = (long long)W * H;
-    const long long pix = (long long)blockIdx.z * plane + (long long)y * W + x;
-    const uint64_t a = cA[pix];
-    const int b0 = base[pix] - R;
-    const uint64_t* rowB = cB + (long long)blockIdx.z * plane + (long long)y * W;
-    uint8_t* o = C + pix * Kp;
-    for (int k0 = 0; k0 < Kp; k0 += 4) {
-        uint32_t packed = 0;
+    const long long row0 = (long long)blockIdx.z * plane + (long long)y * W;
+    if (x < W) {
+        const long long pix = row0 + x;
+        const uint64_t a = cA[pix];
+        const int b0 = base[pix] - R;
+        const uint64_t* rowB = cB + row0;
+        for (int k0 = 0; k0 < Kp; k0 += 4) {
+            uint32_t packed = 0;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int k = k0 + j;
-            uint32_t v = 0;
-            if (k < K) {
-                const int d = b0 + k;
-                const int xr = RIGHT ? x + d : x - d;
-                const bool ok = d >= 0 && xr >= 0 && xr < W;
-                v = ok ? (uint32_t)__popcll(a ^ rowB[ok ? xr : 0]) : (uint32_t)invalid;
+            for (int j = 0; j < 4; ++j) {
+                const int k = k0 + j;
+                uint32_t v = 0;
+                if (k < K) {
+                    const int d = b0 + k;
+                    const int xr = RIGHT ? x + d : x - d;
+                    const bool ok = d >= 0 && xr >= 0 && xr < W;
+                    v = ok ? (uint32_t)__popcll(a ^ rowB[ok ? xr : 0]) : (uint32_t)invalid;
+                }
+                packed |= v << (8 * j);
             }
-            packed |= v << (8 * j);
+            cstage[lane * (Kp / 4) + k0 / 4] = packed;
         }
-        *reinterpret_cast<uint32_t*>(o + k0) = packed;
     }
+    __syncthreads();
+    const int units = (min(64, W - x0) * Kp) / 8;
+    const uint2* src = reinterpret_cast<const uint2*>(cstage);
+    uint2* dst = reinterpret_cast<uint2*>(C + (row0 + x0) * Kp);
+    for (int u = lane; u < units; u += 64) dst[u] = src[u];
 }
 
 // band_costs, general float64 path (absolute-difference and prior terms),
@@ -912,33 +957,48 @@ __global__ void __launch_bounds__(64) k_wta(const LT* __restrict__ Lv, long long
     if (conf) conf[pix] = cf;
 }
 
-// k_wta with G = 1 and K known at compile time (integer path).
-template <int K, typename LT>
+// k_wta with G = 1 and K and the path count known at compile time (integer
+// path).  The NDIR path-cost chunks of a pixel are all requested before the
+// first is summed.
+template <int K, typename LT, int NDIR>
 __global__ void __launch_bounds__(64) k_wta_k(const LT* __restrict__ Lv, long long dirStride,
-                                              int ndir, int mirrorFrom, const int* __restrict__ base,
+                                              int mirrorFrom, const int* __restrict__ base,
                                               const double* __restrict__ d0a,
                                               const double* __restrict__ d0b, int B, int HW, int Kp,
                                               int R, double uniq, double* __restrict__ out,
                                               double* __restrict__ conf) {
     constexpr int K8 = (K + 7) / 8 * 8;
+    constexpr int NU = K8 * (int)sizeof(LT) / 8;  // 8-byte units per chunk
     const int bv = blockIdx.y;
     const int p = blockIdx.x * 64 + threadIdx.x;
     if (p >= HW) return;
     const long long pix = (long long)bv * HW + p;
+    uint2 raw[NDIR][NU];
+#pragma unroll
+    for (int t = 0; t < NDIR; ++t) {
+        const uint2* src = reinterpret_cast<const uint2*>(Lv + (long long)t * dirStride + pix * Kp);
+#pragma unroll
+        for (int q = 0; q < NU; ++q) raw[t][q] = src[q];
+    }
     int S[K8];
 #pragma unroll
     for (int k = 0; k < K8; ++k) S[k] = 0;
-    for (int t = 0; t < ndir; ++t) {
-        int l[K8];
-        const LT* src = Lv + (long long)t * dirStride + pix * Kp;
-        if constexpr (sizeof(LT) == 1) {
-            load_chunk_u8<int, K8>(reinterpret_cast<const uint8_t*>(src), l);
-        } else {
 #pragma unroll
-            for (int k = 0; k < K8; ++k) l[k] = reinterpret_cast<const uint16_t*>(src)[k];
+    for (int t = 0; t < NDIR; ++t) {
+#pragma unroll
+        for (int q = 0; q < NU; ++q) {
+            const uint32_t w2[2] = {raw[t][q].x, raw[t][q].y};
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                if constexpr (sizeof(LT) == 1) {
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) S[8 * q + 4 * h + b] += (w2[h] >> (8 * b)) & 0xFFu;
+                } else {
+#pragma unroll
+                    for (int b = 0; b < 2; ++b) S[4 * q + 2 * h + b] += (w2[h] >> (16 * b)) & 0xFFFFu;
+                }
+            }
         }
-#pragma unroll
-        for (int k = 0; k < K8; ++k) S[k] += l[k];
     }
     int bvv = S[0], bk = 0;
 #pragma unroll
@@ -1401,8 +1461,12 @@ RefineBufs carve(void* ws, const Plan& pl, int V, int B, int W, int H, bool lr, 
 template <typename T>
 int launch_census(const T* img, int n, int W, int H, int rx, int ry, uint64_t* out,
                   hipStream_t st) {
-    dim3 grid((W + 63) / 64, H, n);
-    hipLaunchKernelGGL(k_census<T>, grid, dim3(64), 0, st, img, W, H, rx, ry, out);
+    dim3 grid((W + 63) / 64, (H + 3) / 4, n);
+    const size_t shm = (size_t)(64 + 2 * rx) * (4 + 2 * ry) * sizeof(T);
+    if (rx == 4 && ry == 3)
+        hipLaunchKernelGGL((k_census<T, 4, 3>), grid, dim3(64, 4), shm, st, img, W, H, rx, ry, out);
+    else
+        hipLaunchKernelGGL((k_census<T, -1, -1>), grid, dim3(64, 4), shm, st, img, W, H, rx, ry, out);
     SV_CHECK_LAUNCH();
     return SV_OK;
 }
@@ -1445,11 +1509,12 @@ int launch_costs(const Plan& pl, bool right, const uint64_t* cA, const uint64_t*
                  hipStream_t st) {
     if (pl.integer) {
         dim3 grid((W + 63) / 64, H, B);
+        const size_t shm = (size_t)64 * pl.Kp;
         if (right)
-            hipLaunchKernelGGL(k_costs_int<true>, grid, dim3(64), 0, st, cA, cB, base, W, H, R,
+            hipLaunchKernelGGL(k_costs_int<true>, grid, dim3(64), shm, st, cA, cB, base, W, H, R,
                                pl.K, pl.Kp, (int)pl.invalid, (uint8_t*)C);
         else
-            hipLaunchKernelGGL(k_costs_int<false>, grid, dim3(64), 0, st, cA, cB, base, W, H, R,
+            hipLaunchKernelGGL(k_costs_int<false>, grid, dim3(64), shm, st, cA, cB, base, W, H, R,
                                pl.K, pl.Kp, (int)pl.invalid, (uint8_t*)C);
         SV_CHECK_LAUNCH();
         return SV_OK;
@@ -1572,8 +1637,12 @@ int wta_k_t(const Plan& pl, const void* L, int BV, int mirrorFrom, const int* ba
     const int HW = W * H;
     dim3 grid((HW + 63) / 64, BV);
     const long long dirStride = (long long)BV * HW * pl.Kp;
-    hipLaunchKernelGGL((k_wta_k<K, LT>), grid, dim3(64), 0, st, (const LT*)L, dirStride, pl.ndir,
-                       mirrorFrom, base, d0a, d0b, B, HW, pl.Kp, R, uniq, out, conf);
+    if (pl.ndir == 8)
+        hipLaunchKernelGGL((k_wta_k<K, LT, 8>), grid, dim3(64), 0, st, (const LT*)L, dirStride,
+                           mirrorFrom, base, d0a, d0b, B, HW, pl.Kp, R, uniq, out, conf);
+    else
+        hipLaunchKernelGGL((k_wta_k<K, LT, 4>), grid, dim3(64), 0, st, (const LT*)L, dirStride,
+                           mirrorFrom, base, d0a, d0b, B, HW, pl.Kp, R, uniq, out, conf);
     SV_CHECK_LAUNCH();
     return SV_OK;
 }
