@@ -529,7 +529,7 @@ __global__ void k_bilinear_fwd(const T* __restrict__ x, const int* __restrict__ 
     const int ox = blockIdx.x * 64 + threadIdx.x;
     const int hi = dh(idims, n), wi = dw(idims, n);
     const int ho = dh(odims, n), wo = dw(odims, n);
-    if (ox >= wo || oy >= ho) return;
+    if (ox >= wo || oy >= ho || hi <= 0 || wi <= 0) return;  // empty source: pair skipped
     if (muln) mul = muln[n];
     int y0, y1, x0, x1;
     float wy, wx;
@@ -864,6 +864,83 @@ __global__ void k_widen(const float* __restrict__ x, const int* __restrict__ dim
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= hw) return;
     y[(long long)n * ystride + i] = (double)x[(long long)n * xstride + i] * (double)(mul ? mul[n] : 1.f);
+}
+
+// predict_auto's width rule on the device (stereonet.py:3897-3926 with
+// _peak :3866-3869 and predict_at_scale's working size :3943-3947), in the
+// reference's float64 arithmetic.  One thread per pair.  pass 0 takes the
+// probe's peak (times peak_scale = W / probe width) and activates every
+// pair; a later pass keeps a pair active only if its width moved by more
+// than 8%.  Writes the width used in this pass to trace[n][pass] (-1 when
+// the pair is done) and the per-level dims tables of the pass (zero dims
+// for a finished pair, which every kernel of the pass skips): table block
+// [full N][half N][f4 N][full2 2N][f1_2 2N][f4_2 2N] of (h, w) pairs.
+__device__ __forceinline__ int co_s2(int n) { return (n - 1) / 2 + 1; }
+
+__global__ void k_auto_widths(const float* __restrict__ peaks, int N, int W, int H, double tp,
+                              int max_width, double peak_scale, int pass, int passes,
+                              int* __restrict__ works, int* __restrict__ active,
+                              int* __restrict__ trace, int* __restrict__ tables,
+                              float* __restrict__ inv) {
+    const int n = blockIdx.x * 64 + threadIdx.x;
+    if (n >= N) return;
+    double v = (double)peaks[n];
+    v = v > 0.5 ? v : 0.5;
+    const double pk = v * peak_scale;
+    const double den = pk > 1.0e-6 ? pk : 1.0e-6;
+    int w = (int)((double)W * tp / den + 0.5);
+    if (w > W) w = W;
+    if (w > max_width) w = max_width;
+    if (w < 48) w = 48;
+    bool act;
+    if (pass == 0) {
+        act = true;
+        works[n] = w;
+    } else {
+        const int cur = works[n];
+        act = active[n] != 0 && !((double)abs(w - cur) <= 0.08 * (double)cur);
+        if (act) works[n] = w;
+    }
+    active[n] = act ? 1 : 0;
+    trace[(long long)n * passes + pass] = act ? works[n] : -1;
+    int hw = 0, ww = 0;
+    double scale = 1.0;
+    if (act) {
+        const int wk = works[n];
+        if (wk >= W) {
+            hw = H;
+            ww = W;
+        } else {
+            scale = (double)wk / (double)W;
+            ww = wk;
+            hw = (int)((double)H * scale + 0.5);
+            if (hw < 8) hw = 8;
+        }
+    }
+    inv[n] = act ? (float)(1.0 / scale) : 0.f;
+    const int f1h = act ? co_s2(hw) : 0, f1w = act ? co_s2(ww) : 0;
+    const int f4h = act ? co_s2(f1h) : 0, f4w = act ? co_s2(f1w) : 0;
+    int* full = tables;
+    int* half = tables + 2 * N;
+    int* f4 = tables + 4 * N;
+    int* full2 = tables + 6 * N;
+    int* f1_2 = tables + 10 * N;
+    int* f4_2 = tables + 14 * N;
+    full[2 * n] = hw;
+    full[2 * n + 1] = ww;
+    half[2 * n] = hw / 2;
+    half[2 * n + 1] = ww / 2;
+    f4[2 * n] = f4h;
+    f4[2 * n + 1] = f4w;
+    for (int r = 0; r < 2; ++r) {
+        const int m = r * N + n;
+        full2[2 * m] = hw;
+        full2[2 * m + 1] = ww;
+        f1_2[2 * m] = f1h;
+        f1_2[2 * m + 1] = f1w;
+        f4_2[2 * m] = f4h;
+        f4_2[2 * m + 1] = f4w;
+    }
 }
 
 // pool_valid (:1579-1592): a coarse pixel is valid only if all four of its
@@ -1241,6 +1318,20 @@ int sv_widen(const float* x, const int* dims, long long xstride, double* y, long
     SV_REQUIRE(x && dims && y && N > 0, "sv_widen: bad arguments");
     hipLaunchKernelGGL(k_widen, dim3((max_hw + 255) / 256, N), dim3(256), 0, (hipStream_t)stream,
                        x, dims, xstride, y, ystride, mul);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+// predict_auto's per-pair width decision and dims tables (k_auto_widths).
+int sv_auto_widths(const float* peaks, int N, int W, int H, double target_peak, int max_width,
+                   double peak_scale, int pass, int passes, int* works, int* active, int* trace,
+                   int* tables, float* inv, void* stream) {
+    SV_REQUIRE(peaks && works && active && trace && tables && inv && N > 0 && W > 0 && H > 0,
+               "sv_auto_widths: bad arguments");
+    SV_REQUIRE(pass >= 0 && pass < passes, "sv_auto_widths: pass %d outside [0, %d)", pass, passes);
+    hipLaunchKernelGGL(k_auto_widths, dim3((N + 63) / 64), dim3(64), 0, (hipStream_t)stream, peaks,
+                       N, W, H, target_peak, max_width, peak_scale, pass, passes, works, active,
+                       trace, tables, inv);
     SV_CHECK_LAUNCH();
     return SV_OK;
 }

@@ -49,6 +49,7 @@ L.extra_signatures({
     "sv_pool_valid": (_I, [_P, _P, _LL, _P, _P, _LL, _I, _I, _I, _P]),
     "sv_axpby": (_I, [_P, _P, _LL, _F, _F, _P]),
     "sv_widen": (_I, [_P, _P, _LL, _P, _LL, _P, _I, _I, _P]),
+    "sv_auto_widths": (_I, [_P, _I, _I, _I, _D, _I, _D, _I, _I, _P, _P, _P, _P, _P, _P]),
     "sv_conv_mfma_pack_bytes": (_I, [_I, _I, _I, ctypes.POINTER(_SZ)]),
     "sv_conv_fwd_mfma": (_I, [_P, _P, _P, _LL, _I, _I, _I, _P, _P, _I, _I, _I, _I, _P, _P, _LL, _I,
                               _LL, _I, _F, _P, _LL, _P]),
@@ -302,12 +303,13 @@ def _dims_table(hw, dev):
 
 class _Level:
     """Per-sample dims of one resolution level: a device table and the slot
-    size (largest h*w)."""
+    size (largest h*w).  With `table` given, the table is that device view
+    (filled on the device) and `hw` only bounds each sample's dims."""
 
-    def __init__(self, hw, dev, reps=1):
+    def __init__(self, hw, dev, reps=1, table=None):
         hw = list(hw) * reps
         self.hw = hw
-        self.table = _dims_table(hw, dev)
+        self.table = _dims_table(hw, dev) if table is None else table
         self.maxh = max(h for h, _ in hw)
         self.maxw = max(w for _, w in hw)
         self.slot = max(h * w for h, w in hw)
@@ -317,7 +319,11 @@ class _Plan:
     """Buffers and dims tables of one forward/backward over a batch of pairs
     with given sizes."""
 
-    def __init__(self, net, dims):
+    def __init__(self, net, dims, tables=None):
+        """dims: [(h, w)] per pair.  tables: optional device int32 block of
+        18N entries ([full N][half N][f4 N][full2 2N][f1_2 2N][f4_2 2N]
+        (h, w) pairs) that the device fills before each forward; dims then
+        bound every pair's size and size the buffers and grids."""
         self.net = net
         self.dims = list(dims)
         self.N = N = len(dims)
@@ -329,9 +335,12 @@ class _Plan:
         f4 = [(_co(h, s=2), _co(w, s=2)) for h, w in f1]
         if any(h < 1 or w < 1 for h, w in half):
             raise ValueError("image too small for the network: %s" % (dims,))
-        self.full, self.half = _Level(full, dev), _Level(half, dev)
-        self.f4 = _Level(f4, dev)
-        self.full2, self.f1_2, self.f4_2 = _Level(full, dev, 2), _Level(f1, dev, 2), _Level(f4, dev, 2)
+        t = (lambda a, b: None) if tables is None else (lambda a, b: tables[2 * a * N:2 * b * N])
+        self.full, self.half = _Level(full, dev, table=t(0, 1)), _Level(half, dev, table=t(1, 2))
+        self.f4 = _Level(f4, dev, table=t(2, 3))
+        self.full2 = _Level(full, dev, 2, table=t(3, 5))
+        self.f1_2 = _Level(f1, dev, 2, table=t(5, 7))
+        self.f4_2 = _Level(f4, dev, 2, table=t(7, 9))
         C, Cc, D, RC = net.feat_C, net.cost_C, net.D, net.refine_C
         z = lambda n: torch.empty(max(int(n), 1), dtype=torch.float32, device=dev)
         # activations (the forward tape)
@@ -617,12 +626,13 @@ class _Targets:
               _p(q.table), pl.f4.slot, N, q.maxh, q.maxw, L.stream())
 
 
-def train_grads_batch(net, lefts, rights, disps, valids, dims):
+def train_grads_batch(net, lefts, rights, disps, valids, dims, sync=True):
     """train_grads (stereonet.py:1619-1653) over a batch: per-sample losses
     (three pseudo-Huber levels with LEVEL_WEIGHTS, cross-entropy with
     CE_WEIGHT), gradients averaged over the batch like train_parallel.py's
     host average (:146-153).  Leaves the mean gradient in net.grads; returns
-    the per-sample losses (float64 numpy) and the plan."""
+    the per-sample losses (float64 numpy, or the device tensor when
+    sync=False) and the plan."""
     for (h, w) in dims:
         if _co(_co(h, s=2), s=2) != (h // 2) // 2 or _co(_co(w, s=2), s=2) != (w // 2) // 2:
             raise ValueError("training needs H and W divisible by 4 (got %dx%d)" % (w, h))
@@ -646,7 +656,7 @@ def train_grads_batch(net, lefts, rights, disps, valids, dims):
           _p(loss), _p(part), _p(scale), nblk, L.stream())
     pl.has_extra = True
     pl.backward()
-    return loss.cpu().numpy(), pl
+    return (loss.cpu().numpy() if sync else loss), pl
 
 
 def train_grads(net, sample):
@@ -797,9 +807,18 @@ def load_model(path, rng=None):
 
 # ------------------------------------------------------------- inference
 
+_KTH = {}
+
+
 def _select(buf, lv, N, ks):
-    """sorted(values)[k] per sample (the k-th smallest), as float32."""
-    kt = torch.tensor(ks, dtype=torch.int64, device=buf.device)
+    """sorted(values)[k] per sample (the k-th smallest), as float32 on the
+    device (no host round trip once the k table is cached)."""
+    key = (tuple(ks), str(buf.device))
+    kt = _KTH.get(key)
+    if kt is None:
+        if len(_KTH) > 256:
+            _KTH.clear()
+        kt = _KTH[key] = torch.tensor(ks, dtype=torch.int64, device=buf.device)
     out = torch.empty(N, dtype=torch.float32, device=buf.device)
     nb = ctypes.c_size_t(0)
     _call("sv_select_kth_workspace_bytes", N, ctypes.byref(nb))
@@ -852,10 +871,44 @@ def _work_size(W, H, work_width):
 
 class AutoResult:
     """Result of predict_auto_batch: full-resolution disparities [N][H][W]
-    (float32, on the GPU) plus the decisions taken per pair."""
+    (float32, on the GPU) plus the decisions taken per pair.  The decisions
+    stay on the GPU until `works` or `trace` is read."""
 
-    def __init__(self, full, works, trace):
-        self.full, self.works, self.trace = full, works, trace
+    def __init__(self, full, trace_dev, passes):
+        self.full = full
+        self._trace_dev = trace_dev
+        self._passes = passes
+        self._trace = None
+
+    @property
+    def trace(self):
+        """Working width of every pass each pair ran ([[w0, w1, ...]])."""
+        if self._trace is None:
+            t = self._trace_dev.reshape(-1, self._passes).cpu().numpy()
+            self._trace = [[int(w) for w in row if w > 0] for row in t]
+        return self._trace
+
+    @property
+    def works(self):
+        """Final working width per pair."""
+        return [t[-1] for t in self.trace]
+
+
+class _AutoState:
+    """Device state of predict_auto_batch for one (N, W, H, settings): the
+    plan sized for the largest working size, its dims tables, and the
+    per-pair widths, flags, trace and 1/scale."""
+
+    def __init__(self, net, N, W, H, max_width, passes):
+        dev = L.device()
+        cw = max(48, min(W, max_width))
+        Wc, Hc, _ = _work_size(W, H, cw)
+        self.tables = torch.zeros(18 * N, dtype=torch.int32, device=dev)
+        self.plan = _Plan(net, [(Hc, Wc)] * N, tables=self.tables)
+        self.works = torch.zeros(N, dtype=torch.int32, device=dev)
+        self.active = torch.zeros(N, dtype=torch.int32, device=dev)
+        self.inv = torch.zeros(N, dtype=torch.float32, device=dev)
+        self.full_lv = _Level([(H, W)] * N, dev)
 
 
 def predict_auto_batch(net, left, right, W, H, target_peak=None, probe_width=64,
@@ -863,55 +916,44 @@ def predict_auto_batch(net, left, right, W, H, target_peak=None, probe_width=64,
     """predict_auto (stereonet.py:3872-3929) over N pairs of the same size
     (left/right: [N][H][W] uint8 or float32 on the GPU): probe at
     probe_width, choose each pair's working width from its 95th-percentile
-    disparity, rerun the pairs whose width moved by more than 8%.  Width
-    decisions use float64 host arithmetic on the GPU-selected peaks."""
-    dev = L.device()
+    disparity, rerun the pairs whose width moved by more than 8%.
+
+    Nothing returns to the host: peaks are selected on the GPU, the width
+    rule runs there in float64 (sv_auto_widths), and every pass is launched
+    for all N pairs at the largest possible working size, pairs that are
+    done carrying zero dims so that each kernel skips them.  The sequence
+    is therefore fixed for given (N, W, H, settings) and can be captured in
+    a graph."""
     N = left.shape[0]
     tp = float(getattr(net, "target_peak", 20)) if target_peak is None else float(target_peak)
     pw = probe_width if probe_width < W else W
     ph = int(H * pw / float(W) + 0.5)
     if ph < 8:
         ph = 8
+    key = ("auto", N, W, H, max_width, passes)
+    stt = net._plans.get(key)
+    if stt is None:
+        stt = net._plans[key] = _AutoState(net, N, W, H, max_width, passes)
+    trace = torch.empty(N * passes, dtype=torch.int32, device=left.device)
+    full = torch.empty(N, H * W, dtype=torch.float32, device=left.device)
     pl = net.forward_resized(left, right, W, H, [(ph, pw)] * N)
-    k = _peak_index(pw * ph)
-    peaks = _select(pl.d2, pl.full, N, [k] * N).double().cpu().numpy()
-    works = []
-    for v in peaks:
-        pk = (v if v > 0.5 else 0.5) * (W / float(pw))
-        works.append(_width_for(pk, W, tp, max_width))
-    trace = [[w] for w in works]
-    full = torch.empty(N, H * W, dtype=torch.float32, device=dev)
-    todo = list(range(N))
+    peaks = _select(pl.d2, pl.full, N, [_peak_index(pw * ph)] * N)
+    plw = stt.plan
     for it in range(passes):
-        if not todo:
-            break
-        sizes = [_work_size(W, H, works[n]) for n in todo]
-        dims = [(hw, ww) for (ww, hw, _) in sizes]
-        sub_l = left[todo] if len(todo) < N else left
-        sub_r = right[todo] if len(todo) < N else right
-        plw = net.forward_resized(sub_l, sub_r, W, H, dims)
-        # back to full resolution, values scaled by 1/scale (:3957-3959)
-        inv = torch.tensor([1.0 / s for (_, _, s) in sizes], dtype=torch.float32, device=dev)
-        sub_lv = _Level([(H, W)] * len(todo), dev)
-        tmp = torch.empty(len(todo), H * W, dtype=torch.float32, device=dev)
+        _call("sv_auto_widths", _p(peaks), N, W, H, tp, max_width,
+              (W / float(pw)) if it == 0 else 1.0, it, passes, _p(stt.works), _p(stt.active),
+              _p(trace), _p(stt.tables), _p(stt.inv), L.stream())
+        plw.load_resized(left, right, W, H)
+        plw.forward()
+        net._cache = plw
+        # back to full resolution, values scaled by 1/scale (:3957-3959);
+        # pairs that are done keep their previous result
         _call("sv_bilinear_fwd", _p(plw.d2), L.SV_F32, _p(plw.full.table), plw.full.slot, 1,
-              _p(tmp), _p(sub_lv.table), H * W, 0, 1.0, _p(inv), len(todo), H, W, L.stream())
-        full[todo] = tmp
+              _p(full), _p(stt.full_lv.table), H * W, 0, 1.0, _p(stt.inv), N, H, W, L.stream())
         if it == passes - 1:
             break
-        ks = [_peak_index(W * H)] * len(todo)
-        newp = _select(tmp, sub_lv, len(todo), ks).double().cpu().numpy()
-        nxt = []
-        for j, n in enumerate(todo):
-            v = newp[j]
-            nw = _width_for(v if v > 0.5 else 0.5, W, tp, max_width)
-            if abs(nw - works[n]) <= 0.08 * works[n]:
-                continue
-            works[n] = nw
-            trace[n].append(nw)
-            nxt.append(n)
-        todo = nxt
-    return AutoResult(full.reshape(N, H, W), works, trace)
+        peaks = _select(full, stt.full_lv, N, [_peak_index(W * H)] * N)
+    return AutoResult(full.reshape(N, H, W), trace, passes)
 
 
 def predict_at_scale(net, left, right, W, H, work_width=96, verbose=True):

@@ -218,6 +218,17 @@ int sv_select_kth(const float* x, const int* dims, long long stride, const long 
                   float* out, int N, long long max_count, void* ws, size_t ws_bytes,
                   void* stream);
 
+/* predict_auto's scale decision (:3897-3926, widths by width_for, working
+ * size by predict_at_scale :3943-3947) for N pairs without a host round
+ * trip: from the selected peaks (pass 0: probe peaks scaled by
+ * peak_scale = W / probe width) writes works[n], active[n], trace[n*passes
+ * + pass] (-1 once a pair is done), inv[n] = 1/scale, and the dims tables
+ * of the pass: int block of 18N = [full N][half N][f4 N][full2 2N][f1_2 2N]
+ * [f4_2 2N] (h, w) pairs, all zero for a finished pair. */
+int sv_auto_widths(const float* peaks, int N, int W, int H, double target_peak, int max_width,
+                   double peak_scale, int pass, int passes, int* works, int* active, int* trace,
+                   int* tables, float* inv, void* stream);
+
 /* Elementwise helpers of the backward pass and of the network-to-
  * refinement hand-off (no reference counterpart: list arithmetic there). */
 int sv_axpby(const float* x, float* y, long long n, float a, float b, void* stream);
