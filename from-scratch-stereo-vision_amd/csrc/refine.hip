@@ -40,6 +40,18 @@ __constant__ int c_dy[8] = {0, 0, 1, -1, 1, 1, -1, -1};
 // (dx negated: 0<->1, 4<->5, 6<->7)
 __constant__ int c_order[2][8] = {{0, 1, 2, 3, 4, 5, 6, 7}, {1, 0, 2, 3, 5, 4, 7, 6}};
 
+// Geometry of a batch: image i (views of pair i % B) is h x w with rows
+// packed at its own width, and starts i * slot pixels into every per-pixel
+// buffer.  dims == nullptr: every image is W x H (slot = W * H); otherwise
+// dims holds (h, w) per pair on the device and W, H bound every pair.
+struct Geom {
+    const int* dims;
+    int W, H, B;
+    long long slot;
+    __device__ __forceinline__ int w(int i) const { return dims ? dims[2 * (i % B) + 1] : W; }
+    __device__ __forceinline__ int h(int i) const { return dims ? dims[2 * (i % B)] : H; }
+};
+
 template <typename T>
 __device__ __forceinline__ double to_gray(T v);
 template <>
@@ -79,14 +91,16 @@ __device__ __forceinline__ double big<double>() {
 // outer, dx inner.  RX / RY >= 0 fix the window at compile time; -1 takes
 // the runtime radii.
 template <typename T, int RX, int RY>
-__global__ void __launch_bounds__(256) k_census(const T* __restrict__ img, int W, int H,
-                                                int rx_, int ry_, uint64_t* __restrict__ out) {
+__global__ void __launch_bounds__(256) k_census(const T* __restrict__ img, Geom g, int rx_,
+                                                int ry_, uint64_t* __restrict__ out) {
     extern __shared__ __align__(16) unsigned char smem_c[];
     T* tile = reinterpret_cast<T*>(smem_c);
     const int rx = RX >= 0 ? RX : rx_, ry = RY >= 0 ? RY : ry_;
     const int TW = 64 + 2 * rx, TH = 4 + 2 * ry;
     const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 4;
-    const long long plane = (long long)W * H;
+    const int W = g.w(blockIdx.z), H = g.h(blockIdx.z);
+    if (x0 >= W || y0 >= H) return;  // whole block outside this image
+    const long long plane = g.slot;
     const T* im = img + (long long)blockIdx.z * plane;
     const int tid = threadIdx.y * 64 + threadIdx.x;
     for (int e = tid; e < TW * TH; e += 256) {
@@ -154,8 +168,8 @@ __global__ void k_base(const double* __restrict__ d0, long long n, int* __restri
 template <bool RIGHT>
 __global__ void __launch_bounds__(64) k_costs_int(const uint64_t* __restrict__ cA,
                                                   const uint64_t* __restrict__ cB,
-                                                  const int* __restrict__ base, int W, int H,
-                                                  int R, int K, int Kp, int invalid,
+                                                  const int* __restrict__ base, Geom g, int R,
+                                                  int K, int Kp, int invalid,
                                                   uint8_t* __restrict__ C) {
     // the block's 64 pixels x Kp bytes, written back as contiguous 8-byte
     // units (a lane's own Kp bytes are Kp apart from its neighbour's)
@@ -164,7 +178,9 @@ __global__ void __launch_bounds__(64) k_costs_int(const uint64_t* __restrict__ c
     const int x0 = blockIdx.x * 64;
     const int x = x0 + lane;
     const int y = blockIdx.y;
-    const long long plane = (long long)W * H;
+    const int W = g.w(blockIdx.z);
+    if (x0 >= W || y >= g.h(blockIdx.z)) return;  // whole block outside this image
+    const long long plane = g.slot;
     const long long row0 = (long long)blockIdx.z * plane + (long long)y * W;
     if (x < W) {
         const long long pix = row0 + x;
@@ -202,14 +218,15 @@ __global__ void __launch_bounds__(64) k_costs_f64(const uint64_t* __restrict__ c
                                                   const uint64_t* __restrict__ cB,
                                                   const T* __restrict__ gA, const T* __restrict__ gB,
                                                   const int* __restrict__ base,
-                                                  const double* __restrict__ prior, int W, int H,
-                                                  int R, int K, int Kp, double ad_weight,
+                                                  const double* __restrict__ prior, Geom g, int R,
+                                                  int K, int Kp, double ad_weight,
                                                   double ad_trunc, double invalid, double prior_w,
                                                   double prior_t, double* __restrict__ C) {
     const int x = blockIdx.x * 64 + threadIdx.x;
     const int y = blockIdx.y;
-    if (x >= W) return;
-    const long long plane = (long long)W * H;
+    const int W = g.w(blockIdx.z);
+    if (x >= W || y >= g.h(blockIdx.z)) return;
+    const long long plane = g.slot;
     const long long row0 = (long long)blockIdx.z * plane + (long long)y * W;
     const long long pix = row0 + x;
     const uint64_t a = cA[pix];
@@ -339,10 +356,11 @@ __device__ __forceinline__ void store_any(LT* p, const AT (&L)[KC], int nvalid) 
 // recurrence restarts with L = C), so every line has exactly H steps.
 template <typename CT, typename LT, typename AT, int KC>
 __global__ void __launch_bounds__(64) k_sgm(const CT* __restrict__ C, const int* __restrict__ base,
-                                            LT* __restrict__ Lout, int W, int H, int K, int G,
+                                            LT* __restrict__ Lout, Geom g, int K, int G,
                                             AT P1, AT P2, int dir0, long long dirStride) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     AT* lds = reinterpret_cast<AT*>(smem_raw);
+    const int W = g.w(blockIdx.z), H = g.h(blockIdx.z);
     const int Kp = G * KC;
     const int dir = dir0 + (int)blockIdx.y;
     const int dx = c_dx[dir], dy = c_dy[dir];
@@ -363,7 +381,7 @@ __global__ void __launch_bounds__(64) k_sgm(const CT* __restrict__ C, const int*
         slot[0] = INF;
         slot[K + 3] = INF;
     }
-    const long long plane = (long long)W * H;
+    const long long plane = g.slot;
     const CT* Cb = C + (long long)blockIdx.z * plane * Kp;
     const int* Bb = base + (long long)blockIdx.z * plane;
     LT* Lb = Lout + (long long)blockIdx.y * dirStride + (long long)blockIdx.z * plane * Kp;
@@ -452,10 +470,11 @@ __global__ void __launch_bounds__(64) k_sgm(const CT* __restrict__ C, const int*
 template <int K, typename LT>
 __global__ void __launch_bounds__(64) k_sgm_k(const uint8_t* __restrict__ C,
                                               const int* __restrict__ base,
-                                              LT* __restrict__ Lout, int W, int H, int Kp, int P1,
+                                              LT* __restrict__ Lout, Geom g, int Kp, int P1,
                                               int P2, int dir0, long long dirStride) {
     constexpr int SL = ((K + 4 + 3) / 4) * 4 + 1;  // odd stride spreads lanes over banks
     __shared__ int lds[64 * SL];
+    const int W = g.w(blockIdx.z), H = g.h(blockIdx.z);
     const int dir = dir0 + (int)blockIdx.y;
     const int dx = c_dx[dir], dy = c_dy[dir];
     const int lane = threadIdx.x;
@@ -468,7 +487,7 @@ __global__ void __launch_bounds__(64) k_sgm_k(const uint8_t* __restrict__ C,
     constexpr int INF = 0x3FFFFFFF;
     slot[0] = INF;
     slot[K + 3] = INF;
-    const long long plane = (long long)W * H;
+    const long long plane = g.slot;
     const uint8_t* Cb = C + (long long)blockIdx.z * plane * Kp;
     const int* Bb = base + (long long)blockIdx.z * plane;
     LT* Lb = Lout + (long long)blockIdx.y * dirStride + (long long)blockIdx.z * plane * Kp;
@@ -659,12 +678,13 @@ __device__ __forceinline__ void packed_step(u16x2 (&L)[PackedBand<K>::NP], uint3
 template <int K>
 __global__ void __launch_bounds__(64) k_sgm_p(const uint8_t* __restrict__ C,
                                               const int* __restrict__ base,
-                                              uint8_t* __restrict__ Lout, int W, int H, int Kp,
+                                              uint8_t* __restrict__ Lout, Geom g, int Kp,
                                               int P1, int P2, int dir0, long long dirStride) {
     using PB = PackedBand<K>;
     constexpr int NP = PB::NP, NB = PB::NB, SD = PB::SD;
     constexpr int PF = 4;  // steps of cost prefetch in flight
     __shared__ uint32_t lds[64 * SD];
+    const int W = g.w(blockIdx.z), H = g.h(blockIdx.z);
     const int dir = dir0 + (int)blockIdx.y;
     LinePos lp;
     lp.dx = c_dx[dir];
@@ -680,7 +700,7 @@ __global__ void __launch_bounds__(64) k_sgm_p(const uint8_t* __restrict__ C,
     u32_alias* sl32 = reinterpret_cast<u32_alias*>(lds + lane * SD);
     u16_alias* s16 = reinterpret_cast<u16_alias*>(sl32);
     for (int i = 0; i < SD; ++i) sl32[i] = 0xFFFFFFFFu;
-    const long long plane = (long long)W * H;
+    const long long plane = g.slot;
     const uint8_t* Cb = C + (long long)blockIdx.z * plane * Kp;
     const int* Bb = base + (long long)blockIdx.z * plane;
     uint8_t* Lb = Lout + (long long)blockIdx.y * dirStride + (long long)blockIdx.z * plane * Kp;
@@ -755,10 +775,12 @@ struct HorizTile {
 template <int K>
 __global__ void __launch_bounds__(64) k_sgm_ph(const uint8_t* __restrict__ C,
                                                const int* __restrict__ base,
-                                               uint8_t* __restrict__ Lout, int W, int H, int Kp,
+                                               uint8_t* __restrict__ Lout, Geom g, int Kp,
                                                int P1, int P2, int dir0, long long dirStride) {
     using PB = PackedBand<K>;
     using HT = HorizTile<K>;
+    const int W = g.w(blockIdx.z), H = g.h(blockIdx.z);
+    if ((int)blockIdx.x * 64 >= H) return;  // whole block below this image
     constexpr int NP = PB::NP, NB = PB::NB, SD = PB::SD;
     constexpr int T = HT::T, RS = HT::RS, BS = HT::BS;
     static_assert(RS % 2 == 0, "8-byte aligned rows");
@@ -772,7 +794,7 @@ __global__ void __launch_bounds__(64) k_sgm_ph(const uint8_t* __restrict__ C,
     u32_alias* sl32 = reinterpret_cast<u32_alias*>(lds + lane * SD);
     u16_alias* s16 = reinterpret_cast<u16_alias*>(sl32);
     for (int i = 0; i < SD; ++i) sl32[i] = 0xFFFFFFFFu;
-    const long long plane = (long long)W * H;
+    const long long plane = g.slot;
     const uint8_t* Cb = C + (long long)blockIdx.z * plane * Kp + (long long)y0 * W * Kp;
     const int* Bb = base + (long long)blockIdx.z * plane + (long long)y0 * W;
     uint8_t* Lb = Lout + (long long)blockIdx.y * dirStride + (long long)blockIdx.z * plane * Kp +
@@ -864,7 +886,7 @@ template <typename LT, typename AT, int KC>
 __global__ void __launch_bounds__(64) k_wta(const LT* __restrict__ Lv, long long dirStride,
                                             int ndir, int mirrorFrom, const int* __restrict__ base,
                                             const double* __restrict__ d0a,
-                                            const double* __restrict__ d0b, int B, int HW, int K,
+                                            const double* __restrict__ d0b, Geom g, int K,
                                             int G, int R, double uniq, double* __restrict__ out,
                                             double* __restrict__ conf) {
     const int lane = threadIdx.x;
@@ -873,10 +895,13 @@ __global__ void __launch_bounds__(64) k_wta(const LT* __restrict__ Lv, long long
     const int j = lane - li * G;
     const int Kp = G * KC;
     const int bv = blockIdx.y;
+    const int B = g.B;
+    const int HW = g.w(bv) * g.h(bv);
+    if ((int)blockIdx.x * lpw >= HW) return;  // whole block past this image
     int p = blockIdx.x * lpw + li;
     const bool active = p < HW;
     if (!active) p = HW - 1;
-    const long long pix = (long long)bv * HW + p;
+    const long long pix = (long long)bv * g.slot + p;
     const int kbase = j * KC;
     const int ord = bv >= mirrorFrom ? 1 : 0;
     AT S[KC];
@@ -935,7 +960,7 @@ __global__ void __launch_bounds__(64) k_wta(const LT* __restrict__ Lv, long long
     }
     if (!active || j != 0) return;
     const int v = bv >= B ? 1 : 0;
-    const double d0 = (v ? d0b : d0a)[(long long)(bv - v * B) * HW + p];
+    const double d0 = (v ? d0b : d0a)[(long long)(bv - v * B) * g.slot + p];
     double o, cf;
     if (bk == 0 || bk == K - 1) {
         o = d0;
@@ -964,15 +989,16 @@ template <int K, typename LT, int NDIR>
 __global__ void __launch_bounds__(64) k_wta_k(const LT* __restrict__ Lv, long long dirStride,
                                               int mirrorFrom, const int* __restrict__ base,
                                               const double* __restrict__ d0a,
-                                              const double* __restrict__ d0b, int B, int HW, int Kp,
+                                              const double* __restrict__ d0b, Geom g, int Kp,
                                               int R, double uniq, double* __restrict__ out,
                                               double* __restrict__ conf) {
     constexpr int K8 = (K + 7) / 8 * 8;
     constexpr int NU = K8 * (int)sizeof(LT) / 8;  // 8-byte units per chunk
     const int bv = blockIdx.y;
+    const int B = g.B;
     const int p = blockIdx.x * 64 + threadIdx.x;
-    if (p >= HW) return;
-    const long long pix = (long long)bv * HW + p;
+    if (p >= g.w(bv) * g.h(bv)) return;
+    const long long pix = (long long)bv * g.slot + p;
     uint2 raw[NDIR][NU];
 #pragma unroll
     for (int t = 0; t < NDIR; ++t) {
@@ -1008,7 +1034,7 @@ __global__ void __launch_bounds__(64) k_wta_k(const LT* __restrict__ Lv, long lo
             bk = k;
         }
     const int v = bv >= B ? 1 : 0;
-    const double d0 = (v ? d0b : d0a)[(long long)(bv - v * B) * HW + p];
+    const double d0 = (v ? d0b : d0a)[(long long)(bv - v * B) * g.slot + p];
     double o, cf;
     if (bk == 0 || bk == K - 1) {
         o = d0;
@@ -1053,12 +1079,12 @@ __device__ __forceinline__ int warp_col(int x, double d, int W) {
 // forward warp (strict >, init -1), then each run of holes (value < 0) takes
 // min(nearest non-hole on the left, on the right), 0.0 if neither is below
 // 1e8 -- the closed form of the reference's in-place cascade.
-__global__ void k_left_to_right(const double* __restrict__ dl, int B, int W, int H,
-                                double* __restrict__ dr) {
+__global__ void k_left_to_right(const double* __restrict__ dl, Geom g, double* __restrict__ dr) {
     const int y = blockIdx.x * 64 + threadIdx.x;
     const int b = blockIdx.y;
+    const int W = g.w(b), H = g.h(b);
     if (y >= H) return;
-    const long long r = ((long long)b * H + y) * W;
+    const long long r = (long long)b * g.slot + (long long)y * W;
     for (int x = 0; x < W; ++x) dr[r + x] = -1.0;
     for (int x = 0; x < W; ++x) {
         const double d = dl[r + x];
@@ -1084,12 +1110,13 @@ __global__ void k_left_to_right(const double* __restrict__ dl, int B, int W, int
 
 // lr_refine check and fills, stereonet.py:5358-5390: one thread per row.
 __global__ void k_lr_check_fill(const double* __restrict__ dl, const double* __restrict__ dr,
-                                const double* __restrict__ disp0, int B, int W, int H, double tol,
+                                const double* __restrict__ disp0, Geom g, double tol,
                                 double* __restrict__ out, uint8_t* __restrict__ status) {
     const int y = blockIdx.x * 64 + threadIdx.x;
     const int b = blockIdx.y;
+    const int W = g.w(b), H = g.h(b);
     if (y >= H) return;
-    const long long r = ((long long)b * H + y) * W;
+    const long long r = (long long)b * g.slot + (long long)y * W;
     for (int x = 0; x < W; ++x) {
         const double d = dl[r + x];
         const int xr = warp_col(x, d, W);
@@ -1178,16 +1205,18 @@ __device__ __forceinline__ double dval(unsigned long long k) {
 // order-preserving keys computes exactly in any order; holes (< 0) then
 // take min(nearest non-hole left, right), 0.0 if neither is below 1e8.
 __global__ void __launch_bounds__(ROW_THREADS) k_left_to_right_row(const double* __restrict__ dl,
-                                                                   int W, int H,
+                                                                   Geom g,
                                                                    double* __restrict__ dr) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     unsigned long long* key = reinterpret_cast<unsigned long long*>(smem_raw);
-    int* lidx = reinterpret_cast<int*>(key + W);
-    int* ridx = lidx + W;
-    int* sc = ridx + W;
+    int* lidx = reinterpret_cast<int*>(key + g.W);
+    int* ridx = lidx + g.W;
+    int* sc = ridx + g.W;
     uint8_t* flag = reinterpret_cast<uint8_t*>(sc + 2 * ROW_THREADS);
     const int y = blockIdx.x, b = blockIdx.y;
-    const long long r = ((long long)b * H + y) * W;
+    const int W = g.w(b);
+    if (y >= g.h(b)) return;  // whole block below this image
+    const long long r = (long long)b * g.slot + (long long)y * W;
     const unsigned long long kinit = dkey(-1.0);
     for (int x = threadIdx.x; x < W; x += ROW_THREADS) key[x] = kinit;
     __syncthreads();
@@ -1215,15 +1244,17 @@ __global__ void __launch_bounds__(ROW_THREADS) k_left_to_right_row(const double*
 // lr_refine's check and fills, stereonet.py:5358-5390, one block per row.
 __global__ void __launch_bounds__(ROW_THREADS) k_lr_check_fill_row(
     const double* __restrict__ dl, const double* __restrict__ dr, const double* __restrict__ disp0,
-    int W, int H, double tol, double* __restrict__ out, uint8_t* __restrict__ status) {
+    Geom g, double tol, double* __restrict__ out, uint8_t* __restrict__ status) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     double* o = reinterpret_cast<double*>(smem_raw);
-    int* lidx = reinterpret_cast<int*>(o + W);
-    int* ridx = lidx + W;
-    int* sc = ridx + W;
+    int* lidx = reinterpret_cast<int*>(o + g.W);
+    int* ridx = lidx + g.W;
+    int* sc = ridx + g.W;
     uint8_t* flag = reinterpret_cast<uint8_t*>(sc + 2 * ROW_THREADS);
     const int y = blockIdx.x, b = blockIdx.y;
-    const long long r = ((long long)b * H + y) * W;
+    const int W = g.w(b);
+    if (y >= g.h(b)) return;  // whole block below this image
+    const long long r = (long long)b * g.slot + (long long)y * W;
     for (int x = threadIdx.x; x < W; x += ROW_THREADS) {
         const double d = dl[r + x];
         const int xr = warp_col(x, d, W);
@@ -1256,22 +1287,22 @@ inline size_t row_smem(int W) { return (size_t)W * (8 + 4 + 4 + 1) + 2 * ROW_THR
 // rows wider than this use the one-thread-per-row kernels above
 constexpr int ROW_PAR_MAX_W = 3500;
 
-void launch_l2r(const double* dl, int B, int W, int H, double* dr, hipStream_t st) {
-    if (W <= ROW_PAR_MAX_W)
-        hipLaunchKernelGGL(k_left_to_right_row, dim3(H, B), dim3(ROW_THREADS), row_smem(W), st, dl,
-                           W, H, dr);
+void launch_l2r(const double* dl, const Geom& g, double* dr, hipStream_t st) {
+    if (g.W <= ROW_PAR_MAX_W)
+        hipLaunchKernelGGL(k_left_to_right_row, dim3(g.H, g.B), dim3(ROW_THREADS), row_smem(g.W), st,
+                           dl, g, dr);
     else
-        hipLaunchKernelGGL(k_left_to_right, dim3((H + 63) / 64, B), dim3(64), 0, st, dl, B, W, H, dr);
+        hipLaunchKernelGGL(k_left_to_right, dim3((g.H + 63) / 64, g.B), dim3(64), 0, st, dl, g, dr);
 }
 
-void launch_lr_check(const double* dl, const double* dr, const double* d0, int B, int W, int H,
+void launch_lr_check(const double* dl, const double* dr, const double* d0, const Geom& g,
                      double tol, double* out, uint8_t* status, hipStream_t st) {
-    if (W <= ROW_PAR_MAX_W)
-        hipLaunchKernelGGL(k_lr_check_fill_row, dim3(H, B), dim3(ROW_THREADS), row_smem(W), st, dl,
-                           dr, d0, W, H, tol, out, status);
+    if (g.W <= ROW_PAR_MAX_W)
+        hipLaunchKernelGGL(k_lr_check_fill_row, dim3(g.H, g.B), dim3(ROW_THREADS), row_smem(g.W), st,
+                           dl, dr, d0, g, tol, out, status);
     else
-        hipLaunchKernelGGL(k_lr_check_fill, dim3((H + 63) / 64, B), dim3(64), 0, st, dl, dr, d0, B,
-                           W, H, tol, out, status);
+        hipLaunchKernelGGL(k_lr_check_fill, dim3((g.H + 63) / 64, g.B), dim3(64), 0, st, dl, dr, d0,
+                           g, tol, out, status);
 }
 
 // ------------------------------------------------------------- median
@@ -1279,13 +1310,14 @@ void launch_lr_check(const double* dl, const double* dr, const double* d0, int B
 // order, element n//2.  The window is padded with +inf to a fixed size and
 // sorted by an odd-even transposition network in registers.
 template <int RAD>
-__global__ void __launch_bounds__(64) k_median(const double* __restrict__ in, int W, int H,
+__global__ void __launch_bounds__(64) k_median(const double* __restrict__ in, Geom g,
                                                double* __restrict__ out) {
     constexpr int N = (2 * RAD + 1) * (2 * RAD + 1);
     const int x = blockIdx.x * 64 + threadIdx.x;
     const int y = blockIdx.y;
-    if (x >= W) return;
-    const long long plane = (long long)W * H;
+    const int W = g.w(blockIdx.z), H = g.h(blockIdx.z);
+    if (x >= W || y >= H) return;
+    const long long plane = g.slot;
     const double* im = in + (long long)blockIdx.z * plane;
     const int y0 = max(y - RAD, 0), y1 = min(y + RAD, H - 1);
     const int x0 = max(x - RAD, 0), x1 = min(x + RAD, W - 1);
@@ -1442,9 +1474,9 @@ struct RefineBufs {
     size_t bytes;
 };
 
-RefineBufs carve(void* ws, const Plan& pl, int V, int B, int W, int H, bool lr, bool median) {
+RefineBufs carve(void* ws, const Plan& pl, int V, int B, long long slot, bool lr, bool median) {
     sv::Carver cv(ws);
-    const size_t hw = (size_t)W * H;
+    const size_t hw = (size_t)slot;
     RefineBufs r;
     r.cen = cv.take<uint64_t>(2 * (size_t)B * hw);
     r.base = cv.take<int>((size_t)V * B * hw);
@@ -1459,24 +1491,24 @@ RefineBufs carve(void* ws, const Plan& pl, int V, int B, int W, int H, bool lr, 
 }
 
 template <typename T>
-int launch_census(const T* img, int n, int W, int H, int rx, int ry, uint64_t* out,
+int launch_census(const T* img, int n, const Geom& g, int rx, int ry, uint64_t* out,
                   hipStream_t st) {
-    dim3 grid((W + 63) / 64, (H + 3) / 4, n);
+    dim3 grid((g.W + 63) / 64, (g.H + 3) / 4, n);
     const size_t shm = (size_t)(64 + 2 * rx) * (4 + 2 * ry) * sizeof(T);
     if (rx == 4 && ry == 3)
-        hipLaunchKernelGGL((k_census<T, 4, 3>), grid, dim3(64, 4), shm, st, img, W, H, rx, ry, out);
+        hipLaunchKernelGGL((k_census<T, 4, 3>), grid, dim3(64, 4), shm, st, img, g, rx, ry, out);
     else
-        hipLaunchKernelGGL((k_census<T, -1, -1>), grid, dim3(64, 4), shm, st, img, W, H, rx, ry, out);
+        hipLaunchKernelGGL((k_census<T, -1, -1>), grid, dim3(64, 4), shm, st, img, g, rx, ry, out);
     SV_CHECK_LAUNCH();
     return SV_OK;
 }
 
-int census_any(const void* img, int dtype, int n, int W, int H, int rx, int ry, uint64_t* out,
+int census_any(const void* img, int dtype, int n, const Geom& g, int rx, int ry, uint64_t* out,
                hipStream_t st) {
     switch (dtype) {
-        case SV_U8: return launch_census((const uint8_t*)img, n, W, H, rx, ry, out, st);
-        case SV_F32: return launch_census((const float*)img, n, W, H, rx, ry, out, st);
-        case SV_F64: return launch_census((const double*)img, n, W, H, rx, ry, out, st);
+        case SV_U8: return launch_census((const uint8_t*)img, n, g, rx, ry, out, st);
+        case SV_F32: return launch_census((const float*)img, n, g, rx, ry, out, st);
+        case SV_F64: return launch_census((const double*)img, n, g, rx, ry, out, st);
     }
     return sv::set_error(SV_EINVAL, "unknown image dtype %d", dtype);
 }
@@ -1489,74 +1521,73 @@ int launch_base(const double* d0, long long n, int* base, hipStream_t st) {
 
 template <typename T>
 int costs_f64_t(bool right, const uint64_t* cA, const uint64_t* cB, const void* gA,
-                const void* gB, const int* base, const double* prior, int B, int W, int H, int R,
+                const void* gB, const int* base, const double* prior, int B, const Geom& g, int R,
                 int K, int Kp, double aw, double at, double inv, double pw, double pt, double* C,
                 hipStream_t st) {
-    dim3 grid((W + 63) / 64, H, B);
+    dim3 grid((g.W + 63) / 64, g.H, B);
     if (right)
         hipLaunchKernelGGL((k_costs_f64<true, T>), grid, dim3(64), 0, st, cA, cB, (const T*)gA,
-                           (const T*)gB, base, prior, W, H, R, K, Kp, aw, at, inv, pw, pt, C);
+                           (const T*)gB, base, prior, g, R, K, Kp, aw, at, inv, pw, pt, C);
     else
         hipLaunchKernelGGL((k_costs_f64<false, T>), grid, dim3(64), 0, st, cA, cB, (const T*)gA,
-                           (const T*)gB, base, prior, W, H, R, K, Kp, aw, at, inv, pw, pt, C);
+                           (const T*)gB, base, prior, g, R, K, Kp, aw, at, inv, pw, pt, C);
     SV_CHECK_LAUNCH();
     return SV_OK;
 }
 
 int launch_costs(const Plan& pl, bool right, const uint64_t* cA, const uint64_t* cB,
                  const void* gA, const void* gB, int dtype, const int* base, const double* prior,
-                 int B, int W, int H, int R, double aw, double at, double pw, double pt, void* C,
+                 int B, const Geom& g, int R, double aw, double at, double pw, double pt, void* C,
                  hipStream_t st) {
     if (pl.integer) {
-        dim3 grid((W + 63) / 64, H, B);
+        dim3 grid((g.W + 63) / 64, g.H, B);
         const size_t shm = (size_t)64 * pl.Kp;
         if (right)
-            hipLaunchKernelGGL(k_costs_int<true>, grid, dim3(64), shm, st, cA, cB, base, W, H, R,
+            hipLaunchKernelGGL(k_costs_int<true>, grid, dim3(64), shm, st, cA, cB, base, g, R,
                                pl.K, pl.Kp, (int)pl.invalid, (uint8_t*)C);
         else
-            hipLaunchKernelGGL(k_costs_int<false>, grid, dim3(64), shm, st, cA, cB, base, W, H, R,
+            hipLaunchKernelGGL(k_costs_int<false>, grid, dim3(64), shm, st, cA, cB, base, g, R,
                                pl.K, pl.Kp, (int)pl.invalid, (uint8_t*)C);
         SV_CHECK_LAUNCH();
         return SV_OK;
     }
     switch (dtype) {
         case SV_U8:
-            return costs_f64_t<uint8_t>(right, cA, cB, gA, gB, base, prior, B, W, H, R, pl.K,
+            return costs_f64_t<uint8_t>(right, cA, cB, gA, gB, base, prior, B, g, R, pl.K,
                                         pl.Kp, aw, at, pl.invalid, pw, pt, (double*)C, st);
         case SV_F32:
-            return costs_f64_t<float>(right, cA, cB, gA, gB, base, prior, B, W, H, R, pl.K, pl.Kp,
+            return costs_f64_t<float>(right, cA, cB, gA, gB, base, prior, B, g, R, pl.K, pl.Kp,
                                       aw, at, pl.invalid, pw, pt, (double*)C, st);
         case SV_F64:
-            return costs_f64_t<double>(right, cA, cB, gA, gB, base, prior, B, W, H, R, pl.K,
+            return costs_f64_t<double>(right, cA, cB, gA, gB, base, prior, B, g, R, pl.K,
                                        pl.Kp, aw, at, pl.invalid, pw, pt, (double*)C, st);
     }
     return sv::set_error(SV_EINVAL, "unknown image dtype %d", dtype);
 }
 
 template <typename CT, typename LT, typename AT, int KC>
-int sgm_t(const Plan& pl, const void* C, const int* base, void* L, int BV, int W, int H,
+int sgm_t(const Plan& pl, const void* C, const int* base, void* L, int BV, const Geom& g,
           double P1, double P2, int dir0, int ndir, hipStream_t st) {
     const int lpw = 64 / pl.G;
-    const int nl = W > H ? W : H;
+    const int nl = g.W > g.H ? g.W : g.H;
     dim3 grid((nl + lpw - 1) / lpw, ndir, BV);
     const size_t shm = (size_t)lpw * (pl.Kp + 5) * sizeof(AT);
-    const long long dirStride = (long long)BV * W * H * pl.Kp;
+    const long long dirStride = (long long)BV * g.slot * pl.Kp;
     hipLaunchKernelGGL((k_sgm<CT, LT, AT, KC>), grid, dim3(64), shm, st, (const CT*)C, base,
-                       (LT*)L, W, H, pl.K, pl.G, (AT)P1, (AT)P2, dir0, dirStride);
+                       (LT*)L, g, pl.K, pl.G, (AT)P1, (AT)P2, dir0, dirStride);
     SV_CHECK_LAUNCH();
     return SV_OK;
 }
 
 template <typename LT, typename AT, int KC>
 int wta_t(const Plan& pl, const void* L, int BV, int mirrorFrom, const int* base,
-          const double* d0a, const double* d0b, int B, int W, int H, int R, double uniq,
+          const double* d0a, const double* d0b, const Geom& g, int R, double uniq,
           double* out, double* conf, hipStream_t st) {
     const int lpw = 64 / pl.G;
-    const int HW = W * H;
-    dim3 grid((HW + lpw - 1) / lpw, BV);
-    const long long dirStride = (long long)BV * HW * pl.Kp;
+    dim3 grid((unsigned)((g.slot + lpw - 1) / lpw), BV);
+    const long long dirStride = (long long)BV * g.slot * pl.Kp;
     hipLaunchKernelGGL((k_wta<LT, AT, KC>), grid, dim3(64), 0, st, (const LT*)L, dirStride,
-                       pl.ndir, mirrorFrom, base, d0a, d0b, B, HW, pl.K, pl.G, R, uniq, out, conf);
+                       pl.ndir, mirrorFrom, base, d0a, d0b, g, pl.K, pl.G, R, uniq, out, conf);
     SV_CHECK_LAUNCH();
     return SV_OK;
 }
@@ -1571,34 +1602,34 @@ int wta_t(const Plan& pl, const void* L, int BV, int mirrorFrom, const int* base
     return sv::set_error(SV_EUNSUPPORTED, "band chunk %d", KCV);
 
 template <int K, typename LT>
-int sgm_k_t(const Plan& pl, const void* C, const int* base, void* L, int BV, int W, int H,
+int sgm_k_t(const Plan& pl, const void* C, const int* base, void* L, int BV, const Geom& g,
             double P1, double P2, int dir0, int ndir, hipStream_t st) {
-    const int nl = W > H ? W : H;
+    const int nl = g.W > g.H ? g.W : g.H;
     dim3 grid((nl + 63) / 64, ndir, BV);
-    const long long dirStride = (long long)BV * W * H * pl.Kp;
+    const long long dirStride = (long long)BV * g.slot * pl.Kp;
     // packed u16 kernels when every value stays below the u16 sentinel:
     // the horizontal directions (0, 1) through the LDS-staged kernel, the
     // others through the per-lane one
     if (sizeof(LT) == 1 && pl.invalid + P2 + P1 < 65000.0 && pl.Kp >= 8 * ((K + 1 + 7) / 8)) {
         const int hEnd = min(dir0 + ndir, 2);
         if (hEnd > dir0) {
-            dim3 hgrid((H + 63) / 64, hEnd - dir0, BV);
+            dim3 hgrid((g.H + 63) / 64, hEnd - dir0, BV);
             hipLaunchKernelGGL((k_sgm_ph<K>), hgrid, dim3(64), 0, st, (const uint8_t*)C, base,
-                               (uint8_t*)L, W, H, pl.Kp, (int)P1, (int)P2, dir0, dirStride);
+                               (uint8_t*)L, g, pl.Kp, (int)P1, (int)P2, dir0, dirStride);
             SV_CHECK_LAUNCH();
         }
         const int vBeg = max(dir0, 2), vEnd = dir0 + ndir;
         if (vEnd > vBeg) {
-            dim3 vgrid((W + 63) / 64, vEnd - vBeg, BV);
+            dim3 vgrid((g.W + 63) / 64, vEnd - vBeg, BV);
             hipLaunchKernelGGL((k_sgm_p<K>), vgrid, dim3(64), 0, st, (const uint8_t*)C, base,
-                               (uint8_t*)L + (long long)(vBeg - dir0) * dirStride, W, H, pl.Kp,
+                               (uint8_t*)L + (long long)(vBeg - dir0) * dirStride, g, pl.Kp,
                                (int)P1, (int)P2, vBeg, dirStride);
             SV_CHECK_LAUNCH();
         }
         return SV_OK;
     }
-    hipLaunchKernelGGL((k_sgm_k<K, LT>), grid, dim3(64), 0, st, (const uint8_t*)C, base, (LT*)L, W,
-                       H, pl.Kp, (int)P1, (int)P2, dir0, dirStride);
+    hipLaunchKernelGGL((k_sgm_k<K, LT>), grid, dim3(64), 0, st, (const uint8_t*)C, base, (LT*)L, g,
+                       pl.Kp, (int)P1, (int)P2, dir0, dirStride);
     SV_CHECK_LAUNCH();
     return SV_OK;
 }
@@ -1608,94 +1639,103 @@ inline bool has_k_specialisation(const Plan& pl) {
     return pl.integer && pl.G == 1 && (pl.K == 21 || pl.K == 7) && pl.Kp >= ((pl.K + 7) / 8) * 8;
 }
 
-int launch_sgm(const Plan& pl, const void* C, const int* base, void* L, int BV, int W, int H,
+int launch_sgm(const Plan& pl, const void* C, const int* base, void* L, int BV, const Geom& g,
                double P1, double P2, int dir0, int ndir, hipStream_t st) {
     if (has_k_specialisation(pl)) {
         if (pl.K == 21)
-            return pl.lt == 1 ? sgm_k_t<21, uint8_t>(pl, C, base, L, BV, W, H, P1, P2, dir0, ndir, st)
-                              : sgm_k_t<21, uint16_t>(pl, C, base, L, BV, W, H, P1, P2, dir0, ndir, st);
-        return pl.lt == 1 ? sgm_k_t<7, uint8_t>(pl, C, base, L, BV, W, H, P1, P2, dir0, ndir, st)
-                          : sgm_k_t<7, uint16_t>(pl, C, base, L, BV, W, H, P1, P2, dir0, ndir, st);
+            return pl.lt == 1 ? sgm_k_t<21, uint8_t>(pl, C, base, L, BV, g, P1, P2, dir0, ndir, st)
+                              : sgm_k_t<21, uint16_t>(pl, C, base, L, BV, g, P1, P2, dir0, ndir, st);
+        return pl.lt == 1 ? sgm_k_t<7, uint8_t>(pl, C, base, L, BV, g, P1, P2, dir0, ndir, st)
+                          : sgm_k_t<7, uint16_t>(pl, C, base, L, BV, g, P1, P2, dir0, ndir, st);
     }
     if (!pl.integer) {
         switch (pl.KC) {
-            case 8: return sgm_t<double, double, double, 8>(pl, C, base, L, BV, W, H, P1, P2, dir0, ndir, st);
-            case 16: return sgm_t<double, double, double, 16>(pl, C, base, L, BV, W, H, P1, P2, dir0, ndir, st);
+            case 8: return sgm_t<double, double, double, 8>(pl, C, base, L, BV, g, P1, P2, dir0, ndir, st);
+            case 16: return sgm_t<double, double, double, 16>(pl, C, base, L, BV, g, P1, P2, dir0, ndir, st);
         }
         return sv::set_error(SV_EUNSUPPORTED, "band chunk %d", pl.KC);
     }
     if (pl.lt == 1) {
-        SV_DISPATCH_KC(pl.KC, (sgm_t<uint8_t, uint8_t, int, KC_>(pl, C, base, L, BV, W, H, P1, P2, dir0, ndir, st)));
+        SV_DISPATCH_KC(pl.KC, (sgm_t<uint8_t, uint8_t, int, KC_>(pl, C, base, L, BV, g, P1, P2, dir0, ndir, st)));
     }
-    SV_DISPATCH_KC(pl.KC, (sgm_t<uint8_t, uint16_t, int, KC_>(pl, C, base, L, BV, W, H, P1, P2, dir0, ndir, st)));
+    SV_DISPATCH_KC(pl.KC, (sgm_t<uint8_t, uint16_t, int, KC_>(pl, C, base, L, BV, g, P1, P2, dir0, ndir, st)));
 }
 
 template <int K, typename LT>
 int wta_k_t(const Plan& pl, const void* L, int BV, int mirrorFrom, const int* base,
-            const double* d0a, const double* d0b, int B, int W, int H, int R, double uniq,
+            const double* d0a, const double* d0b, const Geom& g, int R, double uniq,
             double* out, double* conf, hipStream_t st) {
-    const int HW = W * H;
-    dim3 grid((HW + 63) / 64, BV);
-    const long long dirStride = (long long)BV * HW * pl.Kp;
+    dim3 grid((unsigned)((g.slot + 63) / 64), BV);
+    const long long dirStride = (long long)BV * g.slot * pl.Kp;
     if (pl.ndir == 8)
         hipLaunchKernelGGL((k_wta_k<K, LT, 8>), grid, dim3(64), 0, st, (const LT*)L, dirStride,
-                           mirrorFrom, base, d0a, d0b, B, HW, pl.Kp, R, uniq, out, conf);
+                           mirrorFrom, base, d0a, d0b, g, pl.Kp, R, uniq, out, conf);
     else
         hipLaunchKernelGGL((k_wta_k<K, LT, 4>), grid, dim3(64), 0, st, (const LT*)L, dirStride,
-                           mirrorFrom, base, d0a, d0b, B, HW, pl.Kp, R, uniq, out, conf);
+                           mirrorFrom, base, d0a, d0b, g, pl.Kp, R, uniq, out, conf);
     SV_CHECK_LAUNCH();
     return SV_OK;
 }
 
 int launch_wta(const Plan& pl, const void* L, int BV, int mirrorFrom, const int* base,
-               const double* d0a, const double* d0b, int B, int W, int H, int R, double uniq,
+               const double* d0a, const double* d0b, const Geom& g, int R, double uniq,
                double* out, double* conf, hipStream_t st) {
     if (has_k_specialisation(pl)) {
         if (pl.K == 21)
-            return pl.lt == 1 ? wta_k_t<21, uint8_t>(pl, L, BV, mirrorFrom, base, d0a, d0b, B, W, H, R, uniq, out, conf, st)
-                              : wta_k_t<21, uint16_t>(pl, L, BV, mirrorFrom, base, d0a, d0b, B, W, H, R, uniq, out, conf, st);
-        return pl.lt == 1 ? wta_k_t<7, uint8_t>(pl, L, BV, mirrorFrom, base, d0a, d0b, B, W, H, R, uniq, out, conf, st)
-                          : wta_k_t<7, uint16_t>(pl, L, BV, mirrorFrom, base, d0a, d0b, B, W, H, R, uniq, out, conf, st);
+            return pl.lt == 1 ? wta_k_t<21, uint8_t>(pl, L, BV, mirrorFrom, base, d0a, d0b, g, R, uniq, out, conf, st)
+                              : wta_k_t<21, uint16_t>(pl, L, BV, mirrorFrom, base, d0a, d0b, g, R, uniq, out, conf, st);
+        return pl.lt == 1 ? wta_k_t<7, uint8_t>(pl, L, BV, mirrorFrom, base, d0a, d0b, g, R, uniq, out, conf, st)
+                          : wta_k_t<7, uint16_t>(pl, L, BV, mirrorFrom, base, d0a, d0b, g, R, uniq, out, conf, st);
     }
     if (!pl.integer) {
         switch (pl.KC) {
-            case 8: return wta_t<double, double, 8>(pl, L, BV, mirrorFrom, base, d0a, d0b, B, W, H, R, uniq, out, conf, st);
-            case 16: return wta_t<double, double, 16>(pl, L, BV, mirrorFrom, base, d0a, d0b, B, W, H, R, uniq, out, conf, st);
+            case 8: return wta_t<double, double, 8>(pl, L, BV, mirrorFrom, base, d0a, d0b, g, R, uniq, out, conf, st);
+            case 16: return wta_t<double, double, 16>(pl, L, BV, mirrorFrom, base, d0a, d0b, g, R, uniq, out, conf, st);
         }
         return sv::set_error(SV_EUNSUPPORTED, "band chunk %d", pl.KC);
     }
     if (pl.lt == 1) {
-        SV_DISPATCH_KC(pl.KC, (wta_t<uint8_t, int, KC_>(pl, L, BV, mirrorFrom, base, d0a, d0b, B, W, H, R, uniq, out, conf, st)));
+        SV_DISPATCH_KC(pl.KC, (wta_t<uint8_t, int, KC_>(pl, L, BV, mirrorFrom, base, d0a, d0b, g, R, uniq, out, conf, st)));
     }
-    SV_DISPATCH_KC(pl.KC, (wta_t<uint16_t, int, KC_>(pl, L, BV, mirrorFrom, base, d0a, d0b, B, W, H, R, uniq, out, conf, st)));
+    SV_DISPATCH_KC(pl.KC, (wta_t<uint16_t, int, KC_>(pl, L, BV, mirrorFrom, base, d0a, d0b, g, R, uniq, out, conf, st)));
 }
 
-int launch_median(const double* in, int B, int W, int H, int radius, int passes, double* out,
+int launch_median(const double* in, const Geom& g, int radius, int passes, double* out,
                   double* tmp, hipStream_t st) {
     SV_REQUIRE(passes >= 0, "passes must be >= 0");
     if (passes == 0) {
-        hipError_t e = hipMemcpyAsync(out, in, sizeof(double) * B * W * H, hipMemcpyDeviceToDevice, st);
+        hipError_t e = hipMemcpyAsync(out, in, sizeof(double) * g.B * g.slot, hipMemcpyDeviceToDevice, st);
         if (e != hipSuccess) return sv::set_error(SV_EHIP, "median copy: %s", hipGetErrorString(e));
         return SV_OK;
     }
     if (radius < 0 || radius > 2)
         return sv::set_error(SV_EUNSUPPORTED, "median radius %d (the GPU path supports 0..2)", radius);
     SV_REQUIRE(passes == 1 || tmp, "multi-pass median needs a workspace");
-    dim3 grid((W + 63) / 64, H, B);
+    dim3 grid((g.W + 63) / 64, g.H, g.B);
     const double* src = in;
     for (int it = 0; it < passes; ++it) {
         // the last pass lands in out; earlier passes alternate so that the
         // final one does
         double* dst = ((passes - 1 - it) % 2 == 0) ? out : tmp;
         switch (radius) {
-            case 0: hipLaunchKernelGGL(k_median<0>, grid, dim3(64), 0, st, src, W, H, dst); break;
-            case 1: hipLaunchKernelGGL(k_median<1>, grid, dim3(64), 0, st, src, W, H, dst); break;
-            case 2: hipLaunchKernelGGL(k_median<2>, grid, dim3(64), 0, st, src, W, H, dst); break;
+            case 0: hipLaunchKernelGGL(k_median<0>, grid, dim3(64), 0, st, src, g, dst); break;
+            case 1: hipLaunchKernelGGL(k_median<1>, grid, dim3(64), 0, st, src, g, dst); break;
+            case 2: hipLaunchKernelGGL(k_median<2>, grid, dim3(64), 0, st, src, g, dst); break;
         }
         SV_CHECK_LAUNCH();
         src = dst;
     }
     return SV_OK;
+}
+
+Geom uniform_geom(int B, int W, int H) {
+    Geom g;
+    g.dims = nullptr;
+    g.W = W;
+    g.H = H;
+    g.B = B;
+    g.slot = (long long)W * H;
+    return g;
 }
 
 int check_ws(size_t need, void* ws, size_t have) {
@@ -1708,17 +1748,18 @@ int check_ws(size_t need, void* ws, size_t have) {
 // right with centre disp0; view 1 (lr only) matches right against left with
 // centre left_to_right(disp0).  Results land in outs[v*B*HW ...].
 int refine_views(const Plan& pl, const RefineBufs& bf, const void* left, const void* right,
-                 int dtype, const double* disp0, const double* d0r, int V, int B, int W, int H,
+                 int dtype, const double* disp0, const double* d0r, int V, const Geom& g,
                  int R, double P1, double P2, double aw, double at, double uniq, int rx, int ry,
                  double pw, double pt, double* outs, double* conf, hipStream_t st) {
-    const size_t hw = (size_t)W * H;
+    const int B = g.B;
+    const size_t hw = (size_t)g.slot;
     int rc;
     uint64_t* cL = bf.cen;
     uint64_t* cR = bf.cen + (size_t)B * hw;
     {
         StageTimer t(ST_CENSUS, st);
-        if ((rc = census_any(left, dtype, B, W, H, rx, ry, cL, st))) return rc;
-        if ((rc = census_any(right, dtype, B, W, H, rx, ry, cR, st))) return rc;
+        if ((rc = census_any(left, dtype, B, g, rx, ry, cL, st))) return rc;
+        if ((rc = census_any(right, dtype, B, g, rx, ry, cR, st))) return rc;
     }
     const size_t csz = pl.integer ? 1 : 8;
     StageTimer tc(ST_COSTS, st);
@@ -1728,18 +1769,18 @@ int refine_views(const Plan& pl, const RefineBufs& bf, const void* left, const v
         if ((rc = launch_base(d0, (long long)B * hw, base, st))) return rc;
         void* C = (char*)bf.C + (size_t)v * B * hw * pl.Kp * csz;
         if (v == 0)
-            rc = launch_costs(pl, false, cL, cR, left, right, dtype, base, d0, B, W, H, R, aw, at, pw, pt, C, st);
+            rc = launch_costs(pl, false, cL, cR, left, right, dtype, base, d0, B, g, R, aw, at, pw, pt, C, st);
         else
-            rc = launch_costs(pl, true, cR, cL, right, left, dtype, base, d0, B, W, H, R, aw, at, pw, pt, C, st);
+            rc = launch_costs(pl, true, cR, cL, right, left, dtype, base, d0, B, g, R, aw, at, pw, pt, C, st);
         if (rc) return rc;
     }
     tc.stop();
     {
         StageTimer t(ST_SGM, st);
-        if ((rc = launch_sgm(pl, bf.C, bf.base, bf.L, V * B, W, H, P1, P2, 0, pl.ndir, st))) return rc;
+        if ((rc = launch_sgm(pl, bf.C, bf.base, bf.L, V * B, g, P1, P2, 0, pl.ndir, st))) return rc;
     }
     StageTimer t(ST_WTA, st);
-    return launch_wta(pl, bf.L, V * B, V == 2 ? B : V * B, bf.base, disp0, d0r, B, W, H, R, uniq,
+    return launch_wta(pl, bf.L, V * B, V == 2 ? B : V * B, bf.base, disp0, d0r, g, R, uniq,
                       outs, conf, st);
 }
 
@@ -1754,7 +1795,8 @@ int sv_census(const void* img, int dtype, int n_images, int W, int H, int rx, in
     const int nbits = (2 * rx + 1) * (2 * ry + 1) - 1;
     if (rx < 0 || ry < 0 || nbits > 64)
         return sv::set_error(SV_EUNSUPPORTED, "census window of %d bits exceeds 64", nbits);
-    return census_any(img, dtype, n_images, W, H, rx, ry, out, (hipStream_t)stream);
+    return census_any(img, dtype, n_images, uniform_geom(n_images, W, H), rx, ry, out,
+                      (hipStream_t)stream);
 }
 
 int sv_band_costs(const uint64_t* cl, const uint64_t* cr, const void* gl, const void* gr,
@@ -1767,17 +1809,18 @@ int sv_band_costs(const uint64_t* cl, const uint64_t* cr, const void* gl, const 
     const int K = 2 * R + 1;
     hipStream_t st = (hipStream_t)stream;
     // costs are written unpadded ([p][K])
+    const Geom ug = uniform_geom(B, W, H);
     const void* ga = gl;
     const void* gb = gr;
     switch (dtype) {
         case SV_U8:
-            return costs_f64_t<uint8_t>(false, cl, cr, ga, gb, base, prior, B, W, H, R, K, K,
+            return costs_f64_t<uint8_t>(false, cl, cr, ga, gb, base, prior, B, ug, R, K, K,
                                         ad_weight, ad_trunc, invalid, prior_w, prior_t, C, st);
         case SV_F32:
-            return costs_f64_t<float>(false, cl, cr, ga, gb, base, prior, B, W, H, R, K, K,
+            return costs_f64_t<float>(false, cl, cr, ga, gb, base, prior, B, ug, R, K, K,
                                       ad_weight, ad_trunc, invalid, prior_w, prior_t, C, st);
         case SV_F64:
-            return costs_f64_t<double>(false, cl, cr, ga, gb, base, prior, B, W, H, R, K, K,
+            return costs_f64_t<double>(false, cl, cr, ga, gb, base, prior, B, ug, R, K, K,
                                        ad_weight, ad_trunc, invalid, prior_w, prior_t, C, st);
     }
     return sv::set_error(SV_EINVAL, "unknown image dtype %d", dtype);
@@ -1821,7 +1864,7 @@ int sv_aggregate_path(const double* C, const int* base, int B, int W, int H, int
     double* L = (double*)((char*)ws + need / 2);
     hipLaunchKernelGGL(k_pad_costs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, C, pl.Kp, K, n, Cp);
     SV_CHECK_LAUNCH();
-    if ((rc = launch_sgm(pl, Cp, base, L, B, W, H, P1, P2, dir, 1, st))) return rc;
+    if ((rc = launch_sgm(pl, Cp, base, L, B, uniform_geom(B, W, H), P1, P2, dir, 1, st))) return rc;
     const long long m = (long long)B * W * H * K;
     hipLaunchKernelGGL(k_add_into, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, L, pl.Kp, K, m, S);
     SV_CHECK_LAUNCH();
@@ -1838,7 +1881,7 @@ int sv_refine_workspace_bytes(int mode, int B, int W, int H, int R, double P1, d
     if (rc) return rc;
     pl.Kp = pl.G * pl.KC;
     const int V = mode == 0 ? 1 : 2;
-    RefineBufs r = carve(nullptr, pl, V, B, W, H, mode != 0, mode == 2);
+    RefineBufs r = carve(nullptr, pl, V, B, (long long)W * H, mode != 0, mode == 2);
     *bytes = r.bytes;
     return SV_OK;
 }
@@ -1853,50 +1896,51 @@ int sv_refine_fullres(const void* left, const void* right, int dtype, const doub
     int rc = make_plan(W, H, R, P1, P2, ad_weight, prior_w, ad_trunc, prior_t, paths, rx, ry, &pl);
     if (rc) return rc;
     pl.Kp = pl.G * pl.KC;
-    RefineBufs bf = carve(ws, pl, 1, B, W, H, false, false);
+    RefineBufs bf = carve(ws, pl, 1, B, (long long)W * H, false, false);
     if ((rc = check_ws(bf.bytes, ws, ws_bytes))) return rc;
-    return refine_views(pl, bf, left, right, dtype, disp0, nullptr, 1, B, W, H, R, P1, P2,
-                        ad_weight, ad_trunc, uniq, rx, ry, prior_w, prior_t, out, conf,
+    return refine_views(pl, bf, left, right, dtype, disp0, nullptr, 1, uniform_geom(B, W, H), R,
+                        P1, P2, ad_weight, ad_trunc, uniq, rx, ry, prior_w, prior_t, out, conf,
                         (hipStream_t)stream);
 }
 
 int sv_left_to_right(const double* dl, int B, int W, int H, double* dr, void* stream) {
     SV_REQUIRE(dl && dr && B > 0 && W > 0 && H > 0, "sv_left_to_right: bad arguments");
-    launch_l2r(dl, B, W, H, dr, (hipStream_t)stream);
+    launch_l2r(dl, uniform_geom(B, W, H), dr, (hipStream_t)stream);
     SV_CHECK_LAUNCH();
     return SV_OK;
 }
 
-static int lr_core(const void* left, const void* right, int dtype, const double* disp0, int B,
-                   int W, int H, double tol, int R, double P1, double P2, double ad_weight,
+static int lr_core(const void* left, const void* right, int dtype, const double* disp0,
+                   const Geom& g, double tol, int R, double P1, double P2, double ad_weight,
                    double ad_trunc, int paths, double uniq, int rx, int ry, double prior_w,
                    double prior_t, double* out, uint8_t* status, void* ws, size_t ws_bytes,
                    bool median, hipStream_t st) {
-    SV_REQUIRE(left && right && disp0 && out && status && B > 0, "lr_refine: bad arguments");
+    SV_REQUIRE(left && right && disp0 && out && status && g.B > 0, "lr_refine: bad arguments");
     Plan pl{};
-    int rc = make_plan(W, H, R, P1, P2, ad_weight, prior_w, ad_trunc, prior_t, paths, rx, ry, &pl);
+    int rc = make_plan(g.W, g.H, R, P1, P2, ad_weight, prior_w, ad_trunc, prior_t, paths, rx, ry, &pl);
     if (rc) return rc;
     pl.Kp = pl.G * pl.KC;
-    RefineBufs bf = carve(ws, pl, 2, B, W, H, true, median);
+    const int B = g.B;
+    RefineBufs bf = carve(ws, pl, 2, B, g.slot, true, median);
     if ((rc = check_ws(bf.bytes, ws, ws_bytes))) return rc;
     {
         StageTimer t(ST_LR, st);
-        launch_l2r(disp0, B, W, H, bf.d0r, st);
+        launch_l2r(disp0, g, bf.d0r, st);
         SV_CHECK_LAUNCH();
     }
-    rc = refine_views(pl, bf, left, right, dtype, disp0, bf.d0r, 2, B, W, H, R, P1, P2, ad_weight,
+    rc = refine_views(pl, bf, left, right, dtype, disp0, bf.d0r, 2, g, R, P1, P2, ad_weight,
                       ad_trunc, uniq, rx, ry, prior_w, prior_t, bf.disp, nullptr, st);
     if (rc) return rc;
-    const size_t hw = (size_t)W * H;
+    const size_t hw = (size_t)g.slot;
     double* lrout = median ? bf.tmp : out;
     {
         StageTimer t(ST_LR, st);
-        launch_lr_check(bf.disp, bf.disp + (size_t)B * hw, disp0, B, W, H, tol, lrout, status, st);
+        launch_lr_check(bf.disp, bf.disp + (size_t)B * hw, disp0, g, tol, lrout, status, st);
         SV_CHECK_LAUNCH();
     }
     if (median) {
         StageTimer t(ST_MEDIAN, st);
-        return launch_median(lrout, B, W, H, 1, 1, out, nullptr, st);
+        return launch_median(lrout, g, 1, 1, out, nullptr, st);
     }
     return SV_OK;
 }
@@ -1906,17 +1950,68 @@ int sv_lr_refine(const void* left, const void* right, int dtype, const double* d
                  double ad_trunc, int paths, double uniq, int rx, int ry, double prior_w,
                  double prior_t, double* out, uint8_t* status, void* ws, size_t ws_bytes,
                  void* stream) {
-    return lr_core(left, right, dtype, disp0, B, W, H, tol, R, P1, P2, ad_weight, ad_trunc, paths,
-                   uniq, rx, ry, prior_w, prior_t, out, status, ws, ws_bytes, false,
-                   (hipStream_t)stream);
+    SV_REQUIRE(B > 0 && W > 0 && H > 0, "sv_lr_refine: bad sizes");
+    return lr_core(left, right, dtype, disp0, uniform_geom(B, W, H), tol, R, P1, P2, ad_weight,
+                   ad_trunc, paths, uniq, rx, ry, prior_w, prior_t, out, status, ws, ws_bytes,
+                   false, (hipStream_t)stream);
 }
 
 int sv_refine_disparity(const void* left, const void* right, int dtype, const double* disp0,
                         int B, int W, int H, int R, double P1, double P2, int rx, int ry,
                         double tol, double* out, uint8_t* status, void* ws, size_t ws_bytes,
                         void* stream) {
-    return lr_core(left, right, dtype, disp0, B, W, H, tol, R, P1, P2, 0.0, 20.0, 8, 0.0, rx, ry,
-                   0.0, 4.0, out, status, ws, ws_bytes, true, (hipStream_t)stream);
+    SV_REQUIRE(B > 0 && W > 0 && H > 0, "sv_refine_disparity: bad sizes");
+    return lr_core(left, right, dtype, disp0, uniform_geom(B, W, H), tol, R, P1, P2, 0.0, 20.0, 8,
+                   0.0, rx, ry, 0.0, 4.0, out, status, ws, ws_bytes, true, (hipStream_t)stream);
+}
+
+// Ragged batches: pair n is dims[n] = (h, w) (device int32 pairs, every h <=
+// maxH, w <= maxW, h * w <= slot) and sits at n * slot pixels of every
+// per-pixel input and output, rows packed at its own width.  mode 0:
+// refine_fullres (8 paths, uniq 0, status unused), 1: lr_refine, 2:
+// refine_disparity with planes=False.  Integer pipeline only (census
+// costs, integral P1/P2).
+int sv_refine_ragged_workspace_bytes(int mode, int B, long long slot, int maxW, int maxH, int R,
+                                     double P1, double P2, int rx, int ry, size_t* bytes) {
+    SV_REQUIRE(bytes && B > 0 && slot > 0 && maxW > 0 && maxH > 0 && mode >= 0 && mode <= 2,
+               "sv_refine_ragged_workspace_bytes: bad arguments");
+    Plan pl{};
+    int rc = make_plan(maxW, maxH, R, P1, P2, 0.0, 0.0, 20.0, 4.0, 8, rx, ry, &pl);
+    if (rc) return rc;
+    SV_REQUIRE(pl.integer, "ragged batches need integral P1, P2");
+    pl.Kp = pl.G * pl.KC;
+    *bytes = carve(nullptr, pl, mode == 0 ? 1 : 2, B, slot, mode != 0, mode == 2).bytes;
+    return SV_OK;
+}
+
+int sv_refine_ragged(int mode, const void* left, const void* right, int dtype, const double* disp0,
+                     int B, const int* dims, int maxW, int maxH, long long slot, int R, double P1,
+                     double P2, int rx, int ry, double tol, double* out, uint8_t* status, void* ws,
+                     size_t ws_bytes, void* stream) {
+    SV_REQUIRE(left && right && disp0 && out && dims && B > 0 && slot > 0 && maxW > 0 && maxH > 0,
+               "sv_refine_ragged: bad arguments");
+    SV_REQUIRE(mode >= 0 && mode <= 2, "sv_refine_ragged: mode %d is not 0, 1 or 2", mode);
+    Geom g;
+    g.dims = dims;
+    g.W = maxW;
+    g.H = maxH;
+    g.B = B;
+    g.slot = slot;
+    hipStream_t st = (hipStream_t)stream;
+    if (mode != 0) {
+        SV_REQUIRE(status, "sv_refine_ragged: lr modes need a status buffer");
+        return lr_core(left, right, dtype, disp0, g, tol, R, P1, P2, 0.0, 20.0, 8, 0.0, rx, ry, 0.0,
+                       4.0, out, status, ws, ws_bytes, mode == 2, st);
+    }
+    Plan pl{};
+    int rc = make_plan(maxW, maxH, R, P1, P2, 0.0, 0.0, 20.0, 4.0, 8, rx, ry, &pl);
+    if (rc) return rc;
+    SV_REQUIRE(pl.integer, "ragged batches need integral P1, P2");
+    pl.Kp = pl.G * pl.KC;
+    RefineBufs bf = carve(ws, pl, 1, B, slot, false, false);
+    if ((rc = check_ws(bf.bytes, ws, ws_bytes))) return rc;
+    return refine_views(pl, bf, left, right, dtype, disp0, nullptr, 1, g, R, P1, P2, 0.0, 20.0, 0.0,
+                        rx, ry, 0.0, 4.0, out, nullptr, st);
 }
 
 // Stage timing (see StageTimer): on != 0 clears and starts collecting.
@@ -1958,7 +2053,7 @@ int sv_profile_read(const char* stage, double* total_ms, int* launches) {
 int sv_median_filter(const double* disp, int B, int W, int H, int radius, int passes,
                      double* out, double* tmp, void* stream) {
     SV_REQUIRE(disp && out && B > 0 && W > 0 && H > 0, "sv_median_filter: bad arguments");
-    return launch_median(disp, B, W, H, radius, passes, out, tmp, (hipStream_t)stream);
+    return launch_median(disp, uniform_geom(B, W, H), radius, passes, out, tmp, (hipStream_t)stream);
 }
 
 }  // extern "C"
