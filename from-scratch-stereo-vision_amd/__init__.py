@@ -14,6 +14,7 @@ from .net import (StereoNet, Adam, LEVEL_WEIGHTS, CE_WEIGHT, train_grads, train_
                   train_step, flat_params, set_flat_params, flat_grads, set_flat_grads,
                   save_weights, read_checkpoint_config, load_weights, load_model, predict,
                   predict_batch, predict_at_scale, predict_auto, predict_auto_batch, widen)
+from .ragged import RaggedBatch, refine_ragged, net_band_ragged, CapturedPipeline
 from . import scenes, synth
 
 __all__ = ["census", "band_costs", "_aggregate_path", "refine_fullres", "_mirror1",
@@ -21,4 +22,5 @@ __all__ = ["census", "band_costs", "_aggregate_path", "refine_fullres", "_mirror
            "LEVEL_WEIGHTS", "CE_WEIGHT", "train_grads", "train_grads_batch", "train_step",
            "flat_params", "set_flat_params", "flat_grads", "set_flat_grads", "save_weights",
            "read_checkpoint_config", "load_weights", "load_model", "predict", "predict_batch",
-           "predict_at_scale", "predict_auto", "predict_auto_batch", "widen", "scenes", "synth"]
+           "predict_at_scale", "predict_auto", "predict_auto_batch", "widen", "RaggedBatch",
+           "refine_ragged", "net_band_ragged", "CapturedPipeline", "scenes", "synth"]

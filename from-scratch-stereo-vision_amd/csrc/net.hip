@@ -868,25 +868,27 @@ __global__ void k_widen(const float* __restrict__ x, const int* __restrict__ dim
 
 // predict_auto's width rule on the device (stereonet.py:3897-3926 with
 // _peak :3866-3869 and predict_at_scale's working size :3943-3947), in the
-// reference's float64 arithmetic.  One thread per pair.  pass 0 takes the
-// probe's peak (times peak_scale = W / probe width) and activates every
-// pair; a later pass keeps a pair active only if its width moved by more
-// than 8%.  Writes the width used in this pass to trace[n][pass] (-1 when
+// reference's float64 arithmetic.  One thread per pair; pair n is
+// src_dims[n] = (H, W) and was probed at probe_dims[n].  pass 0 takes the
+// probe's peak (times W / probe width) and activates every pair; a later
+// pass keeps a pair active only if its width moved by more than 8%.  Writes the width used in this pass to trace[n][pass] (-1 when
 // the pair is done) and the per-level dims tables of the pass (zero dims
 // for a finished pair, which every kernel of the pass skips): table block
 // [full N][half N][f4 N][full2 2N][f1_2 2N][f4_2 2N] of (h, w) pairs.
 __device__ __forceinline__ int co_s2(int n) { return (n - 1) / 2 + 1; }
 
-__global__ void k_auto_widths(const float* __restrict__ peaks, int N, int W, int H, double tp,
-                              int max_width, double peak_scale, int pass, int passes,
+__global__ void k_auto_widths(const float* __restrict__ peaks, int N,
+                              const int* __restrict__ src_dims, const int* __restrict__ probe_dims,
+                              double tp, int max_width, int pass, int passes,
                               int* __restrict__ works, int* __restrict__ active,
                               int* __restrict__ trace, int* __restrict__ tables,
                               float* __restrict__ inv) {
     const int n = blockIdx.x * 64 + threadIdx.x;
     if (n >= N) return;
+    const int H = dh(src_dims, n), W = dw(src_dims, n);
     double v = (double)peaks[n];
     v = v > 0.5 ? v : 0.5;
-    const double pk = v * peak_scale;
+    const double pk = pass == 0 ? v * ((double)W / (double)dw(probe_dims, n)) : v;
     const double den = pk > 1.0e-6 ? pk : 1.0e-6;
     int w = (int)((double)W * tp / den + 0.5);
     if (w > W) w = W;
@@ -1323,14 +1325,14 @@ int sv_widen(const float* x, const int* dims, long long xstride, double* y, long
 }
 
 // predict_auto's per-pair width decision and dims tables (k_auto_widths).
-int sv_auto_widths(const float* peaks, int N, int W, int H, double target_peak, int max_width,
-                   double peak_scale, int pass, int passes, int* works, int* active, int* trace,
-                   int* tables, float* inv, void* stream) {
-    SV_REQUIRE(peaks && works && active && trace && tables && inv && N > 0 && W > 0 && H > 0,
+int sv_auto_widths(const float* peaks, int N, const int* src_dims, const int* probe_dims,
+                   double target_peak, int max_width, int pass, int passes, int* works, int* active,
+                   int* trace, int* tables, float* inv, void* stream) {
+    SV_REQUIRE(peaks && src_dims && probe_dims && works && active && trace && tables && inv && N > 0,
                "sv_auto_widths: bad arguments");
     SV_REQUIRE(pass >= 0 && pass < passes, "sv_auto_widths: pass %d outside [0, %d)", pass, passes);
     hipLaunchKernelGGL(k_auto_widths, dim3((N + 63) / 64), dim3(64), 0, (hipStream_t)stream, peaks,
-                       N, W, H, target_peak, max_width, peak_scale, pass, passes, works, active,
+                       N, src_dims, probe_dims, target_peak, max_width, pass, passes, works, active,
                        trace, tables, inv);
     SV_CHECK_LAUNCH();
     return SV_OK;

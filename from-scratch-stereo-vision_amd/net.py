@@ -49,7 +49,7 @@ L.extra_signatures({
     "sv_pool_valid": (_I, [_P, _P, _LL, _P, _P, _LL, _I, _I, _I, _P]),
     "sv_axpby": (_I, [_P, _P, _LL, _F, _F, _P]),
     "sv_widen": (_I, [_P, _P, _LL, _P, _LL, _P, _I, _I, _P]),
-    "sv_auto_widths": (_I, [_P, _I, _I, _I, _D, _I, _D, _I, _I, _P, _P, _P, _P, _P, _P]),
+    "sv_auto_widths": (_I, [_P, _I, _P, _P, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P]),
     "sv_conv_mfma_pack_bytes": (_I, [_I, _I, _I, ctypes.POINTER(_SZ)]),
     "sv_conv_fwd_mfma": (_I, [_P, _P, _P, _LL, _I, _I, _I, _P, _P, _I, _I, _I, _I, _P, _P, _LL, _I,
                               _LL, _I, _F, _P, _LL, _P]),
@@ -222,13 +222,14 @@ class StereoNet:
         self._cache = pl
         return pl
 
-    def forward_resized(self, left, right, W, H, dims):
+    def forward_resized(self, left, right, W, H, dims, src=None):
         """Forward over N pairs given at one source size (left/right:
         [N][H][W] uint8 or float32 on the GPU), each bilinearly resized
         (resize_gray, stereonet.py:3843-3845) straight into the plan's input
-        buffer at its own working size dims[n] = (h, w)."""
+        buffer at its own working size dims[n] = (h, w).  src: a _Level of
+        per-pair source sizes for inputs packed [N][src.slot] instead."""
         pl = self._plan(dims)
-        pl.load_resized(left, right, W, H)
+        pl.load_resized(left, right, W, H, src)
         pl.forward()
         self._cache = pl
         return pl
@@ -393,16 +394,17 @@ class _Plan:
         # the left image itself -> channel 1 of the level-2 input
         self._guides()
 
-    def load_resized(self, left, right, W, H):
+    def load_resized(self, left, right, W, H, src=None):
         N = self.N
-        src = _Level([(H, W)] * N, self.dev)
+        if src is None:
+            src = _Level([(H, W)] * N, self.dev)
         for side, imgs in ((0, left), (1, right)):
             s = imgs.reshape(N, -1)
             if s.dtype not in (torch.uint8, torch.float32):
                 s = s.float()
             s = s.contiguous()
             dt = L.SV_U8 if s.dtype == torch.uint8 else L.SV_F32
-            _call("sv_bilinear_fwd", _p(s), dt, _p(src.table), H * W, 1,
+            _call("sv_bilinear_fwd", _p(s), dt, _p(src.table), src.slot, 1,
                   _p(self.img[side * N * self.full.slot:]), _p(self.full.table), self.full.slot, 0,
                   1.0, None, N, self.full.maxh, self.full.maxw, L.stream())
         self._guides()
@@ -895,65 +897,80 @@ class AutoResult:
 
 
 class _AutoState:
-    """Device state of predict_auto_batch for one (N, W, H, settings): the
-    plan sized for the largest working size, its dims tables, and the
-    per-pair widths, flags, trace and 1/scale."""
+    """Device state of predict_auto_batch for one batch layout and setting:
+    the plan sized for each pair's largest working size, its dims tables,
+    and the per-pair widths, flags and 1/scale."""
 
-    def __init__(self, net, N, W, H, max_width, passes):
+    def __init__(self, net, sizes, max_width):
         dev = L.device()
-        cw = max(48, min(W, max_width))
-        Wc, Hc, _ = _work_size(W, H, cw)
+        N = len(sizes)
+        cap = []
+        for W, H in sizes:
+            Wc, Hc, _ = _work_size(W, H, max(48, min(W, max_width)))
+            cap.append((Hc, Wc))
         self.tables = torch.zeros(18 * N, dtype=torch.int32, device=dev)
-        self.plan = _Plan(net, [(Hc, Wc)] * N, tables=self.tables)
+        self.plan = _Plan(net, cap, tables=self.tables)
         self.works = torch.zeros(N, dtype=torch.int32, device=dev)
         self.active = torch.zeros(N, dtype=torch.int32, device=dev)
         self.inv = torch.zeros(N, dtype=torch.float32, device=dev)
-        self.full_lv = _Level([(H, W)] * N, dev)
 
 
 def predict_auto_batch(net, left, right, W, H, target_peak=None, probe_width=64,
-                       max_width=320, passes=3):
-    """predict_auto (stereonet.py:3872-3929) over N pairs of the same size
-    (left/right: [N][H][W] uint8 or float32 on the GPU): probe at
+                       max_width=320, passes=3, batch=None):
+    """predict_auto (stereonet.py:3872-3929) over N pairs: probe at
     probe_width, choose each pair's working width from its 95th-percentile
     disparity, rerun the pairs whose width moved by more than 8%.
+    left/right: [N][H][W] uint8 or float32 on the GPU, or, with batch (a
+    RaggedBatch of pairs of their own sizes; W and H unused), packed
+    [N][batch.slot].
 
     Nothing returns to the host: peaks are selected on the GPU, the width
     rule runs there in float64 (sv_auto_widths), and every pass is launched
-    for all N pairs at the largest possible working size, pairs that are
-    done carrying zero dims so that each kernel skips them.  The sequence
-    is therefore fixed for given (N, W, H, settings) and can be captured in
-    a graph."""
+    for all N pairs at each pair's largest possible working size, pairs that
+    are done carrying zero dims so that each kernel skips them.  The
+    sequence is therefore fixed for given sizes and settings and can be
+    captured in a graph.  Returns AutoResult with full [N][H][W] (or
+    [N][slot] for a ragged batch)."""
     N = left.shape[0]
     tp = float(getattr(net, "target_peak", 20)) if target_peak is None else float(target_peak)
-    pw = probe_width if probe_width < W else W
-    ph = int(H * pw / float(W) + 0.5)
-    if ph < 8:
-        ph = 8
-    key = ("auto", N, W, H, max_width, passes)
+    if batch is None:
+        sizes = [(W, H)] * N
+        src = _Level([(H, W)] * N, left.device)
+    else:
+        sizes = batch.sizes
+        src = batch.level
+    probe = []
+    for w, h in sizes:
+        pw = probe_width if probe_width < w else w
+        ph = int(h * pw / float(w) + 0.5)
+        probe.append((ph if ph >= 8 else 8, pw))
+    key = ("auto", tuple(sizes), max_width)
     stt = net._plans.get(key)
     if stt is None:
-        stt = net._plans[key] = _AutoState(net, N, W, H, max_width, passes)
+        stt = net._plans[key] = _AutoState(net, sizes, max_width)
     trace = torch.empty(N * passes, dtype=torch.int32, device=left.device)
-    full = torch.empty(N, H * W, dtype=torch.float32, device=left.device)
-    pl = net.forward_resized(left, right, W, H, [(ph, pw)] * N)
-    peaks = _select(pl.d2, pl.full, N, [_peak_index(pw * ph)] * N)
+    full = torch.empty(N, src.slot, dtype=torch.float32, device=left.device)
+    pl = net.forward_resized(left, right, W, H, probe, src=src)
+    peaks = _select(pl.d2, pl.full, N, [_peak_index(h * w) for h, w in probe])
     plw = stt.plan
     for it in range(passes):
-        _call("sv_auto_widths", _p(peaks), N, W, H, tp, max_width,
-              (W / float(pw)) if it == 0 else 1.0, it, passes, _p(stt.works), _p(stt.active),
-              _p(trace), _p(stt.tables), _p(stt.inv), L.stream())
-        plw.load_resized(left, right, W, H)
+        _call("sv_auto_widths", _p(peaks), N, _p(src.table), _p(pl.full.table), tp, max_width, it,
+              passes, _p(stt.works), _p(stt.active), _p(trace), _p(stt.tables), _p(stt.inv),
+              L.stream())
+        plw.load_resized(left, right, W, H, src)
         plw.forward()
         net._cache = plw
         # back to full resolution, values scaled by 1/scale (:3957-3959);
         # pairs that are done keep their previous result
         _call("sv_bilinear_fwd", _p(plw.d2), L.SV_F32, _p(plw.full.table), plw.full.slot, 1,
-              _p(full), _p(stt.full_lv.table), H * W, 0, 1.0, _p(stt.inv), N, H, W, L.stream())
+              _p(full), _p(src.table), src.slot, 0, 1.0, _p(stt.inv), N, src.maxh, src.maxw,
+              L.stream())
         if it == passes - 1:
             break
-        peaks = _select(full, stt.full_lv, N, [_peak_index(W * H)] * N)
-    return AutoResult(full.reshape(N, H, W), trace, passes)
+        peaks = _select(full, src, N, [_peak_index(w * h) for w, h in sizes])
+    if batch is None:
+        full = full.reshape(N, H, W)
+    return AutoResult(full, trace, passes)
 
 
 def predict_at_scale(net, left, right, W, H, work_width=96, verbose=True):

@@ -111,6 +111,21 @@ int sv_refine_disparity(const void* left, const void* right, int dtype, const do
                         double tol, double* out, uint8_t* status, void* ws, size_t ws_bytes,
                         void* stream);
 
+/* The three refinement drivers over a ragged batch (pairs of different
+ * sizes in one launch per stage): refine_fullres (mode 0, paths=8,
+ * uniq=0), lr_refine (mode 1) and refine_disparity planes=False (mode 2),
+ * stereonet.py:5244-5305, :5343-5391, :5865-5884, integer pipeline
+ * (ad_weight = prior_w = 0, integral P1 and P2).  Pair n is dims[n] = (h, w)
+ * (device int32 pairs; h <= maxH, w <= maxW, h*w <= slot) and occupies
+ * pixels [n*slot, n*slot + h*w) of every per-pixel input and output, rows
+ * packed at its own width.  status may be null in mode 0. */
+int sv_refine_ragged_workspace_bytes(int mode, int B, long long slot, int maxW, int maxH, int R,
+                                     double P1, double P2, int rx, int ry, size_t* bytes);
+int sv_refine_ragged(int mode, const void* left, const void* right, int dtype, const double* disp0,
+                     int B, const int* dims, int maxW, int maxH, long long slot, int R, double P1,
+                     double P2, int rx, int ry, double tol, double* out, uint8_t* status, void* ws,
+                     size_t ws_bytes, void* stream);
+
 /* Opt-in stage timing of the refinement entry points: sv_profile_enable(1)
  * clears and starts recording HIP events around each stage on the caller's
  * stream; sv_profile_read sums a stage's ("census", "costs", "sgm", "wta",
@@ -220,14 +235,15 @@ int sv_select_kth(const float* x, const int* dims, long long stride, const long 
 
 /* predict_auto's scale decision (:3897-3926, widths by width_for, working
  * size by predict_at_scale :3943-3947) for N pairs without a host round
- * trip: from the selected peaks (pass 0: probe peaks scaled by
- * peak_scale = W / probe width) writes works[n], active[n], trace[n*passes
- * + pass] (-1 once a pair is done), inv[n] = 1/scale, and the dims tables
- * of the pass: int block of 18N = [full N][half N][f4 N][full2 2N][f1_2 2N]
+ * trip.  Pair n is src_dims[n] = (H, W) and was probed at probe_dims[n]
+ * (device int32 pairs).  From the selected peaks (pass 0: the probe's,
+ * scaled by W / probe width) writes works[n], active[n], trace[n*passes +
+ * pass] (-1 once a pair is done), inv[n] = 1/scale, and the dims tables of
+ * the pass: int block of 18N = [full N][half N][f4 N][full2 2N][f1_2 2N]
  * [f4_2 2N] (h, w) pairs, all zero for a finished pair. */
-int sv_auto_widths(const float* peaks, int N, int W, int H, double target_peak, int max_width,
-                   double peak_scale, int pass, int passes, int* works, int* active, int* trace,
-                   int* tables, float* inv, void* stream);
+int sv_auto_widths(const float* peaks, int N, const int* src_dims, const int* probe_dims,
+                   double target_peak, int max_width, int pass, int passes, int* works, int* active,
+                   int* trace, int* tables, float* inv, void* stream);
 
 /* Elementwise helpers of the backward pass and of the network-to-
  * refinement hand-off (no reference counterpart: list arithmetic there). */
