@@ -57,11 +57,16 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the run, copy 256 MiB device-to-device once: a kernel of known "
                          "traffic to scale rocprofv3 FETCH_SIZE/WRITE_SIZE readings")
+    ap.add_argument("--workload", default="pipeline", choices=["pipeline", "ragged64"],
+                    help="pipeline: 460x370 pairs (the headline); ragged64: 64 pairs of uneven "
+                         "size in one captured graph")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(pairs are independent; see DESIGN.md for how it would shard)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "ragged64":
+        return ragged_main(args)
 
     import numpy as np
     import torch
@@ -222,6 +227,88 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         log("copy probe: 268435456 bytes read + 268435456 bytes written")
+
+
+def ragged_main(args):
+    """64 synthetic pairs of uneven size (200x150 up to 1390x1110, maximum
+    disparity 16-240 px) through net + band SGM as one ragged batch: the
+    whole pipeline (predict_auto, widening, refine_disparity planes=False)
+    is captured once in a HIP graph, and a step is one replay."""
+    import numpy as np
+    from multiprocessing import Pool
+
+    S = importlib.import_module("from-scratch-stereo-vision_amd.synth")
+    spec = S.ragged_spec(64)
+    log("generating 64 uneven pairs")
+    with Pool(min(16, os.cpu_count() or 1)) as pool:  # before this process touches the GPU
+        pairs = pool.map(S.ragged_pair, [(w, h, d, 1000 + i) for i, (w, h, d) in enumerate(spec)])
+    import torch
+
+    sv = importlib.import_module("from-scratch-stereo-vision_amd")
+    L = importlib.import_module("from-scratch-stereo-vision_amd._lib")
+    N = importlib.import_module("from-scratch-stereo-vision_amd.net")
+    RG = importlib.import_module("from-scratch-stereo-vision_amd.ragged")
+    b = RG.RaggedBatch([(w, h) for w, h, _ in spec])
+    l = b.pack([p["left"] for p in pairs])
+    r = b.pack([p["right"] for p in pairs])
+    net = N.load_model(os.path.join(ROOT, "tests", "golden", "stereonet_weights_v5.txt"))
+    # eager reference run (stage profile of the refinement) and capture
+    L.lib().sv_profile_enable(1)
+    eout, est, eres = RG.net_band_ragged(net, b, l, r)
+    torch.cuda.synchronize()
+    import ctypes
+    stages = {}
+    for name in ("census", "costs", "sgm", "wta", "lr", "median"):
+        ms, cnt = ctypes.c_double(0), ctypes.c_int(0)
+        L.check(L.lib().sv_profile_read(name.encode(), ctypes.byref(ms), ctypes.byref(cnt)))
+        stages[name] = round(ms.value, 4)
+    L.lib().sv_profile_enable(0)
+    t0 = time.perf_counter()
+    for _ in range(2):
+        RG.net_band_ragged(net, b, l, r)
+    torch.cuda.synchronize()
+    eager_s = (time.perf_counter() - t0) / 2
+    cap = RG.CapturedPipeline(lambda a, c: RG.net_band_ragged(net, b, a, c)[:2], l, r,
+                              warmup=args.warmup)
+    for _ in range(args.warmup):
+        cap.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out, st = cap.replay()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    same = all(torch.equal(u, v) for u, v in zip(b.unpack(out), b.unpack(eout)))
+    outs = [o.cpu().numpy() for o in b.unpack(out)]
+    bad = [float((np.abs(o - p["disp"]) > 2.0)[p["valid"]].mean() * 100.0) for o, p in zip(outs, pairs)]
+    px = sum(w * h for w, h, _ in spec)
+    K = 2 * R_BAND + 1
+    canon = sum(2 * w * h * K * (1 + 8 * 5 + 2) + 2 * w * h * 10 + w * h for w, h, _ in spec)
+    sgm_bytes = sum(sgm_bytes_per_pair(2, K, w * h) for w, h, _ in spec)
+    rec = {
+        "metric": "pairs/s, 64 uneven pairs (200x150..1390x1110), net+band-SGM in one graph replay",
+        "value": round(64 / dt, 2), "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 network / int SGM / f64 epilogue",
+        "data": "synthetic",
+        "config": {"workload": "64 uneven pairs, W 200-1390, H = 0.75-0.80 W, max disparity "
+                               "16-240 px (<= W/3), predict_auto v5 + refine_disparity(planes=False) "
+                               "as one ragged batch, HIP graph replay",
+                   "pairs_per_step": 64, "megapixels": round(px / 1e6, 3)},
+        "eager_ms_per_step": round(eager_s * 1e3, 3),
+        "graph_equals_eager": bool(same),
+        "accuracy": {"bad2_pct_mean": round(float(np.mean(bad)), 3),
+                     "bad2_pct_max": round(float(np.max(bad)), 3)},
+        "stage_ms_eager": stages,
+        "roofline": {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+                     "pipeline_canonical_bytes": canon,
+                     "pipeline_achieved": round(canon / dt / 1e9, 1),
+                     "pipeline_frac": round(canon / dt / 1e9 / HBM_PEAK_GBS, 4),
+                     "sgm_algorithmic_bytes": sgm_bytes,
+                     "sgm_achieved": round(sgm_bytes / (stages["sgm"] / 1e3) / 1e9, 1)
+                     if stages["sgm"] > 0 else None},
+    }
+    print(json.dumps(rec), flush=True)
 
 
 if __name__ == "__main__":

@@ -5,7 +5,8 @@ The generator follows the scene model of the reference's own generators
 make_dataset.py:201-209 rendered scenes) without reusing their code:
 
 * the left view is band-limited noise texture (uniform noise, 3x3 box blur
-  twice, contrast stretch, rounded to uint8);
+  twice, contrast stretch, rounded to uint8), or optionally a multiscale
+  texture (value-noise octaves plus that fine noise);
 * the disparity field is piecewise planar: 3-6 slanted planes over convex
   (Voronoi) regions, plus optional thin fronto-parallel foreground boxes;
 * the right view is a z-buffered forward warp of a wider virtual left view
@@ -37,6 +38,28 @@ def _box_blur3(a):
 def _texture(rng, H, W):
     t = rng.uniform(0.0, 1.0, size=(H, W))
     t = _box_blur3(_box_blur3(t))
+    lo, hi = np.percentile(t, 1.0), np.percentile(t, 99.0)
+    t = (t - lo) / max(hi - lo, 1e-9)
+    return np.clip(t * 235.0 + 10.0, 0.0, 255.0)
+
+
+def _multiscale_texture(rng, H, W):
+    """Texture with structure at every scale: octaves of value noise from 4
+    cells across up to cells of ~4 px (Catmull-Rom resampled, amplitude
+    x0.7 per octave) plus the band-limited fine noise, stretched to uint8
+    range.  Unlike the fine noise alone it survives the 10-20x downscaling
+    the network's working resolution applies to large pairs."""
+    from .scenes import _cubic_matrix
+    acc = np.zeros((H, W))
+    amp, cells = 1.0, 4
+    while cells <= max(4, W // 4):
+        cy = max(2, int(round(cells * H / float(W))))
+        lat = rng.random((cy + 1, cells + 1))
+        acc += amp * (_cubic_matrix(cy + 1, H) @ lat @ _cubic_matrix(cells + 1, W).T)
+        amp *= 0.7
+        cells *= 2
+    fine = _box_blur3(_box_blur3(rng.uniform(0.0, 1.0, size=(H, W))))
+    t = acc / max(acc.std(), 1e-9) + 0.6 * fine / max(fine.std(), 1e-9)
     lo, hi = np.percentile(t, 1.0), np.percentile(t, 99.0)
     t = (t - lo) / max(hi - lo, 1e-9)
     return np.clip(t * 235.0 + 10.0, 0.0, 255.0)
@@ -108,16 +131,17 @@ def _warp_right(left, disp):
 
 
 def make_pair(W, H, seed, dmin=4.0, dmax=30.0, n_planes=None, n_boxes=0,
-              noise_sigma=2.0, coarse_noise=6):
+              noise_sigma=2.0, coarse_noise=6, texture="band"):
     """One synthetic pair. Returns a dict with uint8 ``left``/``right``
     (H, W), float64 ``disp`` (ground truth in the left view), bool ``valid``
-    and float64 ``disp0`` (float32-representable coarse estimate)."""
+    and float64 ``disp0`` (float32-representable coarse estimate).
+    texture: "band" (band-limited fine noise) or "multiscale"."""
     rng = np.random.default_rng(seed)
     if n_planes is None:
         n_planes = int(rng.integers(3, 7))
     pad = int(np.ceil(dmax)) + 2
     Wv = W + pad
-    tex = _texture(rng, H, Wv)
+    tex = _texture(rng, H, Wv) if texture == "band" else _multiscale_texture(rng, H, Wv)
     disp_v = _plane_field(rng, H, Wv, dmin, dmax, n_planes)
     if n_boxes:
         disp_v = _add_boxes(rng, disp_v, n_boxes, dmax)
@@ -158,3 +182,26 @@ def gray(u8):
     """uint8 image -> float64 gray in [0, 1] the way the reference's PGM
     reader builds it (value times 1/maxval, stereonet.py:284,300)."""
     return u8.astype(np.float64) * (1.0 / 255.0)
+
+
+def ragged_spec(n=64, seed=0):
+    """Sizes and disparity ranges of an uneven batch: W ~ U{200..1390},
+    H = round(u * W) with u ~ U[0.75, 0.80] clipped to [150, 1110], maximum
+    disparity ~ U[16, 240] but at most W / 3 (a larger one would leave no
+    overlap between the views of the narrow pairs).  Returns [(W, H, dmax)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        W = int(rng.integers(200, 1391))
+        H = int(min(max(round(rng.uniform(0.75, 0.80) * W), 150), 1110))
+        dmax = float(min(rng.uniform(16.0, 240.0), W / 3.0))
+        out.append((W, H, dmax))
+    return out
+
+
+def ragged_pair(args):
+    """One pair of an uneven batch: (W, H, dmax, seed) -> make_pair with
+    slanted planes over [dmax / 4, dmax], two thin boxes and multiscale
+    texture (large pairs are matched by the network 10-20x downscaled)."""
+    W, H, dmax, seed = args
+    return make_pair(W, H, seed, dmin=0.25 * dmax, dmax=dmax, n_boxes=2, texture="multiscale")
