@@ -57,6 +57,8 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the run, copy 256 MiB device-to-device once: a kernel of known "
                          "traffic to scale rocprofv3 FETCH_SIZE/WRITE_SIZE readings")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the library's built-in kernel variant heuristics")
     ap.add_argument("--workload", default="pipeline", choices=["pipeline", "ragged64"],
                     help="pipeline: 460x370 pairs (the headline); ragged64: 64 pairs of uneven "
                          "size in one captured graph")
@@ -99,6 +101,10 @@ def main():
         out, st = sv.refine_disparity(L8, R8, None, d0, W, H, planes=False)
         return out, st, res, (e0, e1)
 
+    tuned = None
+    if not args.no_autotune:
+        A = importlib.import_module("from-scratch-stereo-vision_amd.autotune")
+        tuned = A.tune(lambda: step()[:2], reps=2, log=log)
     log("warmup %d" % args.warmup)
     for _ in range(args.warmup):
         step()
@@ -213,6 +219,7 @@ def main():
                      "network_only_bad2_pct": round(net_bad2, 3)},
         "fullrange": fullrange,
         "speedup_vs_fullrange": (round(pairs_s / fullrange["pairs_per_s"], 3) if fullrange else None),
+        "autotuned": ({k: v for k, v in tuned.items() if not k.startswith("_")} if tuned else None),
         "stage_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in stages.items()},
         "network_ms_per_step": round(net_ms / args.steps, 4),
         "roofline": roofline,
@@ -252,6 +259,13 @@ def ragged_main(args):
     l = b.pack([p["left"] for p in pairs])
     r = b.pack([p["right"] for p in pairs])
     net = N.load_model(os.path.join(ROOT, "tests", "golden", "stereonet_weights_v5.txt"))
+    A = importlib.import_module("from-scratch-stereo-vision_amd.autotune")
+
+    def run():
+        o, s_, _ = RG.net_band_ragged(net, b, l, r)
+        return b.unpack(o) + b.unpack(s_)
+
+    tuned = A.tune(run, reps=2, log=log) if not args.no_autotune else None
     # eager reference run (stage profile of the refinement) and capture
     L.lib().sv_profile_enable(1)
     eout, est, eres = RG.net_band_ragged(net, b, l, r)
@@ -296,6 +310,7 @@ def ragged_main(args):
                                "as one ragged batch, HIP graph replay",
                    "pairs_per_step": 64, "megapixels": round(px / 1e6, 3)},
         "eager_ms_per_step": round(eager_s * 1e3, 3),
+        "autotuned": ({k: v for k, v in tuned.items() if not k.startswith("_")} if tuned else None),
         "graph_equals_eager": bool(same),
         "accuracy": {"bad2_pct_mean": round(float(np.mean(bad)), 3),
                      "bad2_pct_max": round(float(np.max(bad)), 3)},

@@ -39,6 +39,8 @@ SIGNATURES = {
                           _D, _D, _P, _P, _P, _SZ, _P]),
     "sv_median_filter": (_I, [_P, _I, _I, _I, _I, _I, _P, _P, _P]),
     "sv_profile_enable": (_I, [_I]),
+    "sv_tune_set": (_I, [_I, _I]),
+    "sv_tune_get": (_I, [_I, ctypes.POINTER(_I)]),
     "sv_profile_read": (_I, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I)]),
     "sv_refine_disparity": (_I, [_P, _P, _I, _P, _I, _I, _I, _I, _D, _D, _I, _I, _D, _P, _P, _P,
                                  _SZ, _P]),

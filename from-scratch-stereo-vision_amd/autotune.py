@@ -1,0 +1,123 @@
+"""Autotuner for the library's kernel variants.
+
+Some kernels come in variants that compute identical results at different
+speeds: the prefetch depth of the vertical/diagonal SGM scan, the columns
+per LDS tile of the horizontal SGM scan, and the output rows per thread of
+the tiled convolution.  The library reads the choice from process-wide
+tuning knobs (sv_tune_set); this module picks them for a workload by timing
+it on the GPU.
+
+tune() does coordinate descent: for each knob in turn it times the workload
+under every allowed value, the other knobs held at their best so far, and
+keeps the fastest.  Every variant's output is compared with the first run's
+and must be identical (a knob that changed results would be a bug, and tune
+raises).  Choices can be cached in a JSON file keyed by device name and a
+workload key, so a later process applies them without re-timing.
+"""
+
+import ctypes
+import json
+import os
+
+import torch
+
+from . import _lib as L
+
+KNOBS = {"sgm_prefetch": (0, (2, 4, 8)),
+         "sgm_htile": (1, (8, 16)),
+         "conv_rows": (2, (1, 4))}
+
+
+def set_knob(name, value):
+    L.check(L.lib().sv_tune_set(KNOBS[name][0], int(value)))
+
+
+def get_knob(name):
+    v = ctypes.c_int(0)
+    L.check(L.lib().sv_tune_get(KNOBS[name][0], ctypes.byref(v)))
+    return v.value
+
+
+def apply(settings):
+    for k, v in settings.items():
+        set_knob(k, v)
+
+
+def reset():
+    for k in KNOBS:
+        set_knob(k, 0)
+
+
+def _flat(out):
+    if isinstance(out, torch.Tensor):
+        return [out]
+    res = []
+    for o in out:
+        res.extend(_flat(o))
+    return res
+
+
+def _time(run, reps):
+    run()  # warm-up (first use of a variant)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    best, out = None, None
+    for _ in range(reps):
+        e0.record()
+        out = run()
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1)
+        best = ms if best is None else min(best, ms)
+    return best, out
+
+
+def _cache_key(key):
+    return "%s|%s" % (torch.cuda.get_device_name(L.device()), key)
+
+
+def tune(run, key=None, knobs=None, reps=3, cache_path=None, log=None):
+    """Pick the knob values that make `run` fastest.  run(): queues one
+    workload on the current stream and returns its result tensor(s).
+    Returns {knob: value} (also applied), plus "_ms" (best time) and
+    "_trials" (every timing, for reporting)."""
+    knobs = list(knobs or KNOBS)
+    if cache_path and key and os.path.exists(cache_path):
+        with open(cache_path) as f:
+            cached = json.load(f).get(_cache_key(key))
+        if cached:
+            apply({k: v for k, v in cached.items() if k in KNOBS})
+            return cached
+    best = {k: KNOBS[k][1][0] for k in knobs}
+    apply(best)
+    ref_ms, ref = _time(run, reps)
+    ref = [t.clone() for t in _flat(ref)]
+    trials = {}
+    best_ms = ref_ms
+    for k in knobs:
+        for v in KNOBS[k][1]:
+            trial = dict(best)
+            trial[k] = v
+            apply(trial)
+            ms, out = _time(run, reps)
+            outs = _flat(out)
+            if len(outs) != len(ref) or not all(torch.equal(a, b) for a, b in zip(outs, ref)):
+                reset()
+                raise RuntimeError("tuning knob %s=%d changed the results" % (k, v))
+            trials[",".join("%s=%d" % kv for kv in sorted(trial.items()))] = ms
+            if log:
+                log("autotune %s: %.3f ms" % (trial, ms))
+            if ms < best_ms:
+                best_ms, best = ms, trial
+    apply(best)
+    res = dict(best, _ms=best_ms, _trials=trials)
+    if cache_path and key:
+        data = {}
+        if os.path.exists(cache_path):
+            with open(cache_path) as f:
+                data = json.load(f)
+        data[_cache_key(key)] = res
+        os.makedirs(os.path.dirname(os.path.abspath(cache_path)), exist_ok=True)
+        with open(cache_path, "w") as f:
+            json.dump(data, f, indent=1)
+    return res

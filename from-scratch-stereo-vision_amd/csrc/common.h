@@ -19,9 +19,19 @@
 #define SV_F32 1
 #define SV_F64 2
 
+// Tuning knobs: kernel variants that compute identical results at
+// different speeds, chosen at run time (0 = the library's own heuristic).
+#define SV_TUNE_SGM_PREFETCH 0  // steps of cost prefetch in the vertical/diagonal SGM kernel: 2, 4, 8
+#define SV_TUNE_SGM_HTILE 1     // columns per LDS tile in the horizontal SGM kernel: 8, 16
+#define SV_TUNE_CONV_ROWS 2     // output rows per thread in the tiled convolution: 1, 4
+#define SV_TUNE_COUNT 3
+
 namespace sv {
 
 int set_error(int code, const char* fmt, ...);
+
+// current value of a tuning knob (0: heuristic)
+int tune(int knob);
 
 inline size_t align_up(size_t n, size_t a = 256) { return (n + a - 1) / a * a; }
 

@@ -1,4 +1,5 @@
-// Error reporting and version query of the C ABI.
+// Error reporting, version query and tuning knobs of the C ABI.
+#include <atomic>
 #include <cstdio>
 
 #include "common.h"
@@ -6,6 +7,9 @@
 namespace sv {
 
 static thread_local char g_err[1024] = "";
+static std::atomic<int> g_tune[SV_TUNE_COUNT];
+
+int tune(int knob) { return knob >= 0 && knob < SV_TUNE_COUNT ? g_tune[knob].load() : 0; }
 
 int set_error(int code, const char* fmt, ...) {
     va_list ap;
@@ -22,5 +26,24 @@ extern "C" {
 const char* sv_last_error(void) { return sv::g_err; }
 
 int sv_version(void) { return 1; }
+
+// Allowed values per knob (0 always allowed: back to the heuristic).
+int sv_tune_set(int knob, int value) {
+    static const int allowed[SV_TUNE_COUNT][4] = {{2, 4, 8, -1}, {8, 16, -1, -1}, {1, 4, -1, -1}};
+    if (knob < 0 || knob >= SV_TUNE_COUNT)
+        return sv::set_error(SV_EINVAL, "tuning knob %d does not exist", knob);
+    bool ok = value == 0;
+    for (int i = 0; i < 4; ++i) ok = ok || allowed[knob][i] == value;
+    if (!ok) return sv::set_error(SV_EINVAL, "value %d is not allowed for tuning knob %d", value, knob);
+    sv::g_tune[knob].store(value);
+    return SV_OK;
+}
+
+int sv_tune_get(int knob, int* value) {
+    if (knob < 0 || knob >= SV_TUNE_COUNT || !value)
+        return sv::set_error(SV_EINVAL, "tuning knob %d does not exist", knob);
+    *value = sv::g_tune[knob].load();
+    return SV_OK;
+}
 
 }  // extern "C"

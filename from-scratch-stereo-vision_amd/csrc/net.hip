@@ -991,7 +991,8 @@ int sv_conv_fwd(const float* x, const float* xr, const int* idims, long long xst
         // double-buffered.  Four rows per thread unless that leaves the GPU
         // with fewer than ~2 blocks per CU, then one.
         const long long blocks4 = (long long)((max_wo + TT_W - 1) / TT_W) * ((max_ho + 15) / 16) * D * N;
-        const int RP = blocks4 >= 512 ? 4 : 1;
+        const int tuned = sv::tune(SV_TUNE_CONV_ROWS);
+        const int RP = tuned ? tuned : (blocks4 >= 512 ? 4 : 1);
         const int TT_H = 4 * RP;
         const int IHt = (TT_H - 1) * s + 2 * dil + 1, IWt = (TT_W - 1) * s + 2 * dil + 1;
         const int ocp = OC == 1 ? 4 : (OC <= 4 ? 4 : (OC <= 8 ? 8 : (OC <= 12 ? 12 : 16)));

@@ -675,14 +675,13 @@ __device__ __forceinline__ void packed_step(u16x2 (&L)[PackedBand<K>::NP], uint3
 // one lane per scanline, 64 lines per wave.  Neighbouring lanes hold
 // neighbouring columns of one row, so cost reads and path-cost writes are
 // coalesced; the costs of the next PF steps are prefetched into registers.
-template <int K>
+template <int K, int PF>
 __global__ void __launch_bounds__(64) k_sgm_p(const uint8_t* __restrict__ C,
                                               const int* __restrict__ base,
                                               uint8_t* __restrict__ Lout, Geom g, int Kp,
                                               int P1, int P2, int dir0, long long dirStride) {
     using PB = PackedBand<K>;
     constexpr int NP = PB::NP, NB = PB::NB, SD = PB::SD;
-    constexpr int PF = 4;  // steps of cost prefetch in flight
     __shared__ uint32_t lds[64 * SD];
     const int W = g.w(blockIdx.z), H = g.h(blockIdx.z);
     const int dir = dir0 + (int)blockIdx.y;
@@ -760,25 +759,23 @@ __global__ void __launch_bounds__(64) k_sgm_p(const uint8_t* __restrict__ C,
 // instruction.  Instead the wave stages T columns of its 64 rows through LDS:
 // each row's T*Kp cost bytes are one contiguous run, loaded and later stored
 // by the whole wave; every lane then steps through its row in LDS, writing
-// the path costs over the costs it has consumed.
-template <int K>
+// the path costs over the costs it has consumed.  TC columns per tile.
+template <int K, int TC>
 struct HorizTile {
     static constexpr int NB = PackedBand<K>::NB;
-    // columns per tile: one load instruction per row carries the row's
-    // T*NB cost units and its T band centres
-    static constexpr int T = 64 / (NB + 1) > 32 ? 32 : 64 / (NB + 1);
+    static constexpr int T = TC;
     static constexpr int ROWD = T * NB * 2;                 // dwords of one row's run
     static constexpr int RS = ROWD + ((34 - ROWD % 32) % 32);  // row stride, = 2 mod 32
     static constexpr int BS = T + 1;                        // band-centre row stride
 };
 
-template <int K>
+template <int K, int TC>
 __global__ void __launch_bounds__(64) k_sgm_ph(const uint8_t* __restrict__ C,
                                                const int* __restrict__ base,
                                                uint8_t* __restrict__ Lout, Geom g, int Kp,
                                                int P1, int P2, int dir0, long long dirStride) {
     using PB = PackedBand<K>;
-    using HT = HorizTile<K>;
+    using HT = HorizTile<K, TC>;
     const int W = g.w(blockIdx.z), H = g.h(blockIdx.z);
     if ((int)blockIdx.x * 64 >= H) return;  // whole block below this image
     constexpr int NP = PB::NP, NB = PB::NB, SD = PB::SD;
@@ -1614,16 +1611,31 @@ int sgm_k_t(const Plan& pl, const void* C, const int* base, void* L, int BV, con
         const int hEnd = min(dir0 + ndir, 2);
         if (hEnd > dir0) {
             dim3 hgrid((g.H + 63) / 64, hEnd - dir0, BV);
-            hipLaunchKernelGGL((k_sgm_ph<K>), hgrid, dim3(64), 0, st, (const uint8_t*)C, base,
-                               (uint8_t*)L, g, pl.Kp, (int)P1, (int)P2, dir0, dirStride);
+            if (sv::tune(SV_TUNE_SGM_HTILE) == 8)
+                hipLaunchKernelGGL((k_sgm_ph<K, 8>), hgrid, dim3(64), 0, st, (const uint8_t*)C,
+                                   base, (uint8_t*)L, g, pl.Kp, (int)P1, (int)P2, dir0, dirStride);
+            else
+                hipLaunchKernelGGL((k_sgm_ph<K, 16>), hgrid, dim3(64), 0, st, (const uint8_t*)C,
+                                   base, (uint8_t*)L, g, pl.Kp, (int)P1, (int)P2, dir0, dirStride);
             SV_CHECK_LAUNCH();
         }
         const int vBeg = max(dir0, 2), vEnd = dir0 + ndir;
         if (vEnd > vBeg) {
             dim3 vgrid((g.W + 63) / 64, vEnd - vBeg, BV);
-            hipLaunchKernelGGL((k_sgm_p<K>), vgrid, dim3(64), 0, st, (const uint8_t*)C, base,
-                               (uint8_t*)L + (long long)(vBeg - dir0) * dirStride, g, pl.Kp,
-                               (int)P1, (int)P2, vBeg, dirStride);
+            uint8_t* Lv = (uint8_t*)L + (long long)(vBeg - dir0) * dirStride;
+            switch (sv::tune(SV_TUNE_SGM_PREFETCH)) {
+                case 2:
+                    hipLaunchKernelGGL((k_sgm_p<K, 2>), vgrid, dim3(64), 0, st, (const uint8_t*)C,
+                                       base, Lv, g, pl.Kp, (int)P1, (int)P2, vBeg, dirStride);
+                    break;
+                case 8:
+                    hipLaunchKernelGGL((k_sgm_p<K, 8>), vgrid, dim3(64), 0, st, (const uint8_t*)C,
+                                       base, Lv, g, pl.Kp, (int)P1, (int)P2, vBeg, dirStride);
+                    break;
+                default:
+                    hipLaunchKernelGGL((k_sgm_p<K, 4>), vgrid, dim3(64), 0, st, (const uint8_t*)C,
+                                       base, Lv, g, pl.Kp, (int)P1, (int)P2, vBeg, dirStride);
+            }
             SV_CHECK_LAUNCH();
         }
         return SV_OK;

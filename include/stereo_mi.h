@@ -126,6 +126,16 @@ int sv_refine_ragged(int mode, const void* left, const void* right, int dtype, c
                      double P2, int rx, int ry, double tol, double* out, uint8_t* status, void* ws,
                      size_t ws_bytes, void* stream);
 
+/* Tuning knobs (no reference counterpart): kernel variants with identical
+ * results and different speed, set by the autotuner.  knob 0: prefetch
+ * depth of the vertical/diagonal SGM kernel (2, 4, 8); 1: columns per LDS
+ * tile of the horizontal SGM kernel (8, 16); 2: output rows per thread of
+ * the tiled convolution (1, 4).  Value 0 restores the built-in heuristic.
+ * Process-wide; set between launches, not while kernels are being queued
+ * from other threads. */
+int sv_tune_set(int knob, int value);
+int sv_tune_get(int knob, int* value);
+
 /* Opt-in stage timing of the refinement entry points: sv_profile_enable(1)
  * clears and starts recording HIP events around each stage on the caller's
  * stream; sv_profile_read sums a stage's ("census", "costs", "sgm", "wta",

@@ -103,3 +103,27 @@ def test_captured_pipeline_replays_match_eager():
     torch.cuda.synchronize()
     assert _same(b, out, ref_out)
     assert _same(b, st, ref_st)
+
+
+def test_autotuner_knobs_keep_results_identical():
+    R, N = _mods()
+    A = importlib.import_module("from-scratch-stereo-vision_amd.autotune")
+    ps = _pairs()
+    net = N.load_model(package().__path__[0] + "/weights/stereonet_c5.txt")
+    b = R.RaggedBatch(SIZES)
+    l = b.pack([p["left"] for p in ps])
+    r = b.pack([p["right"] for p in ps])
+    try:
+        # tune() raises if any variant changes a single output value
+        def run():
+            out, st, _ = R.net_band_ragged(net, b, l, r)
+            return b.unpack(out) + b.unpack(st)  # padding past each pair carries no result
+
+        res = A.tune(run, reps=1)
+        for k, (_, values) in A.KNOBS.items():
+            assert res[k] in values
+            assert A.get_knob(k) == res[k]
+        assert len(res["_trials"]) >= 3
+    finally:
+        A.reset()
+    assert all(A.get_knob(k) == 0 for k in A.KNOBS)
