@@ -2,8 +2,8 @@
 
 Some kernels come in variants that compute identical results at different
 speeds: the prefetch depth of the vertical/diagonal SGM scan, the columns
-per LDS tile of the horizontal SGM scan, and the output rows per thread of
-the tiled convolution.  The library reads the choice from process-wide
+per LDS tile of the horizontal SGM scan, and the output rows per thread and
+columns per tile of the tiled convolution.  The library reads the choice from process-wide
 tuning knobs (sv_tune_set); this module picks them for a workload by timing
 it on the GPU.
 
@@ -25,7 +25,8 @@ from . import _lib as L
 
 KNOBS = {"sgm_prefetch": (0, (2, 4, 8)),
          "sgm_htile": (1, (8, 16)),
-         "conv_rows": (2, (1, 4))}
+         "conv_rows": (2, (1, 4)),
+         "conv_tile_width": (3, (64, 32, 16))}
 
 
 def set_knob(name, value):

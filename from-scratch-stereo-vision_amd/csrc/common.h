@@ -24,7 +24,8 @@
 #define SV_TUNE_SGM_PREFETCH 0  // steps of cost prefetch in the vertical/diagonal SGM kernel: 2, 4, 8
 #define SV_TUNE_SGM_HTILE 1     // columns per LDS tile in the horizontal SGM kernel: 8, 16
 #define SV_TUNE_CONV_ROWS 2     // output rows per thread in the tiled convolution: 1, 4
-#define SV_TUNE_COUNT 3
+#define SV_TUNE_CONV_TW 3       // output columns per tile of the tiled convolution: 16, 32, 64
+#define SV_TUNE_COUNT 4
 
 namespace sv {
 

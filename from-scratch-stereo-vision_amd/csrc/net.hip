@@ -108,15 +108,17 @@ __global__ void __launch_bounds__(256) k_conv_fwd(
     }
 }
 
-// LDS-tiled form of k_conv_fwd.  A block makes a 16 x 64 tile of output
-// pixels at one output depth; for each input channel it stages the input
+// LDS-tiled form of k_conv_fwd.  A block of 256 threads (TW columns x
+// 256/TW rows of threads) makes a tile of TW output columns x RP*256/TW
+// rows at one output depth; for each input channel it stages the input
 // halo tile of all KD depth taps in LDS (zero outside the image, the cost
 // volume formed while staging when SRC == 1), double-buffered so the next
-// channel loads while this one is consumed.  Each thread accumulates four
-// output rows x OCB channels; weights come through the scalar cache.
-constexpr int TT_W = 64;
-
-template <int KD, int SRC, int OCB, int RP>
+// channel loads while this one is consumed.  Each thread accumulates RP
+// output rows x OCB channels; weights are staged in LDS per channel.  TW
+// (64, 32 or 16) only changes how the plane is cut into tiles: narrow maps
+// waste less of a narrower tile, and every output is summed in the same
+// order whatever TW is.
+template <int KD, int SRC, int OCB, int RP, int TW>
 __global__ void __launch_bounds__(256) k_conv_tile(
     const float* __restrict__ x, const float* __restrict__ xr, const int* __restrict__ idims,
     long long xstride, int IC, int D, const float* __restrict__ w, const float* __restrict__ b,
@@ -126,17 +128,18 @@ __global__ void __launch_bounds__(256) k_conv_tile(
     // OCB padded to a multiple of 4 for float4 weight reads
     constexpr int OCP = (OCB + 3) / 4 * 4;
     constexpr int WSZ = KD * 9 * OCP;
-    constexpr int TT_H = 4 * RP;  // rows per tile: 4 threads high, RP rows each
+    constexpr int TY = 256 / TW;   // thread rows
+    constexpr int TT_H = TY * RP;  // rows per tile: TY threads high, RP rows each
     extern __shared__ __align__(16) float smem_f[];
     const int n = blockIdx.z;
     const int od = blockIdx.y / tilesY;
     const int ty0 = (blockIdx.y - od * tilesY) * TT_H;
-    const int tx0 = blockIdx.x * TT_W;
+    const int tx0 = blockIdx.x * TW;
     const int hi = dh(idims, n), wi = dw(idims, n);
     const int ho = dh(odims, n), wo = dw(odims, n);
     if (tx0 >= wo || ty0 >= ho) return;  // uniform per block
-    const int tx = threadIdx.x, ty = threadIdx.y;  // 64 x 4
-    const int tid = ty * 64 + tx;
+    const int tx = threadIdx.x, ty = threadIdx.y;  // TW x TY
+    const int tid = ty * TW + tx;
     const int plane_t = IHt * IWt;
     const int tsz = KD * plane_t;
     float* wsm = smem_f;                   // [2][KD*9][OCP]
@@ -988,28 +991,47 @@ int sv_conv_fwd(const float* x, const float* xr, const int* idims, long long xst
     const int src = xr ? 1 : 0;
     {
         // LDS-tiled path: the input halo tile of one channel (all KD planes),
-        // double-buffered.  Four rows per thread unless that leaves the GPU
-        // with fewer than ~2 blocks per CU, then one.
-        const long long blocks4 = (long long)((max_wo + TT_W - 1) / TT_W) * ((max_ho + 15) / 16) * D * N;
+        // double-buffered.  Tile width: the widest of 64, 32, 16 columns
+        // that wastes at most 1/8 of the row on padding (or the tuning
+        // knob's); rows per thread: 4 unless that leaves the GPU with fewer
+        // than ~2 blocks per CU, then 1 (or the knob's).
+        int TWc = sv::tune(SV_TUNE_CONV_TW);
+        if (!TWc) {
+            TWc = 16;
+            for (int c : {64, 32}) {
+                const int waste = (max_wo + c - 1) / c * c - max_wo;
+                if (8 * waste <= max_wo) {
+                    TWc = c;
+                    break;
+                }
+            }
+        }
+        const int TYc = 256 / TWc;
+        const long long blocks4 = (long long)((max_wo + TWc - 1) / TWc) * ((max_ho + 4 * TYc - 1) / (4 * TYc)) * D * N;
         const int tuned = sv::tune(SV_TUNE_CONV_ROWS);
         const int RP = tuned ? tuned : (blocks4 >= 512 ? 4 : 1);
-        const int TT_H = 4 * RP;
-        const int IHt = (TT_H - 1) * s + 2 * dil + 1, IWt = (TT_W - 1) * s + 2 * dil + 1;
+        const int TT_H = TYc * RP;
+        const int IHt = (TT_H - 1) * s + 2 * dil + 1, IWt = (TWc - 1) * s + 2 * dil + 1;
         const int ocp = OC == 1 ? 4 : (OC <= 4 ? 4 : (OC <= 8 ? 8 : (OC <= 12 ? 12 : 16)));
         const size_t shm = 2 * ((size_t)KD * IHt * IWt + (size_t)KD * 9 * ocp) * sizeof(float);
         if (shm <= 64 * 1024) {
             const int tYt = (max_ho + TT_H - 1) / TT_H;
-            dim3 g((max_wo + TT_W - 1) / TT_W, tYt * D, N), bk(64, 4);
+            dim3 g((max_wo + TWc - 1) / TWc, tYt * D, N), bk(TWc, TYc);
+#define LAUNCH_TW(KD_, SRC_, OCB_, RP_, TW_)                                                      \
+    hipLaunchKernelGGL((k_conv_tile<KD_, SRC_, OCB_, RP_, TW_>), g, bk, shm, st, x, xr, idims,   \
+                       xstride, IC, D, w, b, OC, s, p, dil, y, odims, ystride, tYt, mode, slope,  \
+                       resid, rstride, IHt, IWt)
 #define LAUNCH_T(KD_, SRC_, OCB_)                                                                \
     do {                                                                                         \
-        if (RP == 4)                                                                             \
-            hipLaunchKernelGGL((k_conv_tile<KD_, SRC_, OCB_, 4>), g, bk, shm, st, x, xr, idims,    \
-                               xstride, IC, D, w, b, OC, s, p, dil, y, odims, ystride, tYt, mode,  \
-                               slope, resid, rstride, IHt, IWt);                                  \
-        else                                                                                     \
-            hipLaunchKernelGGL((k_conv_tile<KD_, SRC_, OCB_, 1>), g, bk, shm, st, x, xr, idims,    \
-                               xstride, IC, D, w, b, OC, s, p, dil, y, odims, ystride, tYt, mode,  \
-                               slope, resid, rstride, IHt, IWt);                                  \
+        if (RP == 4) {                                                                           \
+            if (TWc == 64) LAUNCH_TW(KD_, SRC_, OCB_, 4, 64);                                    \
+            else if (TWc == 32) LAUNCH_TW(KD_, SRC_, OCB_, 4, 32);                               \
+            else LAUNCH_TW(KD_, SRC_, OCB_, 4, 16);                                              \
+        } else {                                                                                 \
+            if (TWc == 64) LAUNCH_TW(KD_, SRC_, OCB_, 1, 64);                                    \
+            else if (TWc == 32) LAUNCH_TW(KD_, SRC_, OCB_, 1, 32);                               \
+            else LAUNCH_TW(KD_, SRC_, OCB_, 1, 16);                                              \
+        }                                                                                        \
     } while (0)
 #define LAUNCH_T_OC(KD_, SRC_)                                                                   \
     do {                                                                                         \
@@ -1025,6 +1047,7 @@ int sv_conv_fwd(const float* x, const float* xr, const int* idims, long long xst
             else LAUNCH_T_OC(1, 1);
 #undef LAUNCH_T_OC
 #undef LAUNCH_T
+#undef LAUNCH_TW
             SV_CHECK_LAUNCH();
             return SV_OK;
         }

@@ -130,7 +130,8 @@ int sv_refine_ragged(int mode, const void* left, const void* right, int dtype, c
  * results and different speed, set by the autotuner.  knob 0: prefetch
  * depth of the vertical/diagonal SGM kernel (2, 4, 8); 1: columns per LDS
  * tile of the horizontal SGM kernel (8, 16); 2: output rows per thread of
- * the tiled convolution (1, 4).  Value 0 restores the built-in heuristic.
+ * the tiled convolution (1, 4); 3: output columns per tile of the tiled
+ * convolution (16, 32, 64).  Value 0 restores the built-in heuristic.
  * Process-wide; set between launches, not while kernels are being queued
  * from other threads. */
 int sv_tune_set(int knob, int value);
