@@ -38,6 +38,13 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+# HBM bytes the band-SGM stage (k_sgm_ph + k_sgm_p) moves per 460x370 pair,
+# both views: rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 (the factors a
+# 256 MiB device copy reads under the same counters), 1,152.4 MB per 8-pair
+# launch pair.  Used for the roofline's "traffic" field.
+SGM_TRAFFIC_PER_PAIR = 1152.4e6 / 8
+
+
 def sgm_bytes_per_pair(views, K, HW):
     """Algorithmic HBM bytes of the eight-path aggregation per pair: per
     view, cell and path, read the u8 cost and read+write a u16 path sum
@@ -59,6 +66,9 @@ def main():
                          "traffic to scale rocprofv3 FETCH_SIZE/WRITE_SIZE readings")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's built-in kernel variant heuristics")
+    ap.add_argument("--tune", default=None,
+                    help="apply these kernel variants instead of autotuning, e.g. "
+                         "sgm_prefetch=8,sgm_htile=16,conv_rows=1,conv_tile_width=32")
     ap.add_argument("--workload", default="pipeline", choices=["pipeline", "ragged64"],
                     help="pipeline: 460x370 pairs (the headline); ragged64: 64 pairs of uneven "
                          "size in one captured graph")
@@ -102,8 +112,11 @@ def main():
         return out, st, res, (e0, e1)
 
     tuned = None
-    if not args.no_autotune:
-        A = importlib.import_module("from-scratch-stereo-vision_amd.autotune")
+    A = importlib.import_module("from-scratch-stereo-vision_amd.autotune")
+    if args.tune:
+        tuned = {k: int(v) for k, v in (kv.split("=") for kv in args.tune.split(","))}
+        A.apply(tuned)
+    elif not args.no_autotune:
         tuned = A.tune(lambda: step()[:2], reps=2, log=log)
     log("warmup %d" % args.warmup)
     for _ in range(args.warmup):
@@ -179,11 +192,14 @@ def main():
     sgm_ms, sgm_n = stages["sgm"]
     K = 2 * R_BAND + 1
     bytes_launch = sgm_bytes_per_pair(2, K, W * H) * B
-    avg_s = (sgm_ms / max(sgm_n, 1)) / 1e3
+    avg_s = (sgm_ms / max(args.steps, 1)) / 1e3  # the stage's two kernels, per step
     achieved = bytes_launch / avg_s / 1e9 if avg_s > 0 else 0.0
-    roofline = {"bound": "hbm", "kernel": "k_sgm (8 paths, both views, batch of %d pairs)" % B,
+    roofline = {"bound": "hbm",
+                "kernel": "band SGM aggregation: k_sgm_ph + k_sgm_p (8 paths, both views, %d pairs)" % B,
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": round(SGM_TRAFFIC_PER_PAIR * B),
+                "traffic_GBs": round(SGM_TRAFFIC_PER_PAIR * B / avg_s / 1e9, 1) if avg_s > 0 else None,
                 "avg_launch_ms": round(avg_s * 1e3, 4),
                 "algorithmic_bytes_per_launch": bytes_launch}
 
