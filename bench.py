@@ -69,7 +69,7 @@ def main():
     ap.add_argument("--tune", default=None,
                     help="apply these kernel variants instead of autotuning, e.g. "
                          "sgm_prefetch=8,sgm_htile=16,conv_rows=1,conv_tile_width=32")
-    ap.add_argument("--workload", default="pipeline", choices=["pipeline", "ragged64"],
+    ap.add_argument("--workload", default="pipeline", choices=["pipeline", "ragged64", "widened"],
                     help="pipeline: 460x370 pairs (the headline); ragged64: 64 pairs of uneven "
                          "size in one captured graph")
     args = ap.parse_args()
@@ -79,6 +79,8 @@ def main():
         sys.exit(2)
     if args.workload == "ragged64":
         return ragged_main(args)
+    if args.workload == "widened":
+        return widened_main(args)
 
     import numpy as np
     import torch
@@ -250,6 +252,84 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         log("copy probe: 268435456 bytes read + 268435456 bytes written")
+
+
+def widened_main(args):
+    """The precision variant: the v5 network widened 4x in channels (48/24/32,
+    widen_net.py:18-41) on the same 460x370 pairs, once with every
+    convolution in FP32 and once with the convolutions of 4k input and 8+
+    output channels in BF16 on the matrix cores (FP32 accumulation), each
+    followed by band SGM.  Reports both throughputs and the BF16 network's
+    per-pixel error and EPE shift against the FP32 one."""
+    import numpy as np
+    import torch
+
+    sv = importlib.import_module("from-scratch-stereo-vision_amd")
+    L = importlib.import_module("from-scratch-stereo-vision_amd._lib")
+    N = importlib.import_module("from-scratch-stereo-vision_amd.net")
+    dev = L.device()
+    B = args.batch
+    pairs = [sv.synth.pipeline_pair(s) for s in range(B)]
+    L8 = torch.from_numpy(np.stack([p["left"] for p in pairs])).to(dev)
+    R8 = torch.from_numpy(np.stack([p["right"] for p in pairs])).to(dev)
+    gt = np.stack([p["disp"] for p in pairs]).reshape(B, -1)
+    valid = np.stack([p["valid"] for p in pairs]).reshape(B, -1)
+    v5 = N.load_model(os.path.join(ROOT, "tests", "golden", "stereonet_weights_v5.txt"))
+    net = N.widen(v5, 48, 24, 32)
+    lv = N._Level([(H, W)] * B, dev)
+    d0 = torch.empty(B, H * W, dtype=torch.float64, device=dev)
+
+    def step():
+        res = N.predict_auto_batch(net, L8, R8, W, H)
+        N._call("sv_widen", N._p(res.full), N._p(lv.table), H * W, N._p(d0), H * W, None, B,
+                H * W, L.stream())
+        out, _ = sv.refine_disparity(L8, R8, None, d0, W, H, planes=False)
+        return res.full, out, res
+
+    rec = {}
+    outs = {}
+    traces = {}
+    for mode in ("fp32", "bf16"):
+        net.use_bf16(mode == "bf16")
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            full, out, res = step()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        traces[mode] = res.trace
+        f = full.reshape(B, -1).double().cpu().numpy()
+        o = out.reshape(B, -1).cpu().numpy()
+        outs[mode] = f
+        rec[mode] = {"pairs_per_s": round(B / dt, 2), "ms_per_step": round(dt * 1e3, 3),
+                     "network_bad2_pct": round(float((np.abs(f - gt) > 2)[valid].mean() * 100), 3),
+                     "network_epe_px": round(float(np.abs(f - gt)[valid].mean()), 4),
+                     "refined_bad2_pct": round(float((np.abs(o - gt) > 2)[valid].mean() * 100), 3),
+                     "refined_epe_px": round(float(np.abs(o - gt)[valid].mean()), 4)}
+        log("%s: %.1f pairs/s" % (mode, B / dt))
+    net.use_bf16(False)
+    e = np.abs(outs["bf16"] - outs["fp32"])
+    same = [a == b for a, b in zip(traces["bf16"], traces["fp32"])]
+    es = e[np.array(same)] if any(same) else e[:0]
+    line = {
+        "metric": "pairs/s @460x370, 4x-widened StereoNet (48/24/32), BF16 MFMA vs FP32 network + band SGM",
+        "value": rec["bf16"]["pairs_per_s"], "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": rec["bf16"]["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16 convolutions (fp32 accumulate)",
+        "data": "synthetic",
+        "config": {"workload": "widened net 48/24/32 (widen_net rule from v5), predict_auto + "
+                               "refine_disparity(planes=False), 460x370", "pairs_per_step": B},
+        "fp32": rec["fp32"], "bf16": rec["bf16"],
+        "bf16_vs_fp32_network_px": {"max": round(float(e.max()), 4), "mean": round(float(e.mean()), 5),
+                                    "epe_shift": round(rec["bf16"]["network_epe_px"] -
+                                                       rec["fp32"]["network_epe_px"], 4),
+                                    "same_width_decisions": "%d/%d" % (sum(same), len(same)),
+                                    "max_where_same_widths": round(float(es.max()), 4) if es.size else None,
+                                    "mean_where_same_widths": round(float(es.mean()), 5) if es.size else None},
+    }
+    print(json.dumps(line), flush=True)
 
 
 def ragged_main(args):
