@@ -122,7 +122,12 @@ def main():
         tuned = {k: int(v) for k, v in (kv.split("=") for kv in args.tune.split(","))}
         A.apply(tuned)
     elif not args.no_autotune:
-        tuned = A.tune(lambda: step()[:2], reps=2, log=log)
+        # per-layer convolution variants first, then the global knobs of the
+        # rest (every variant leaves the results unchanged)
+        layers = A.tune_layers(net, lambda: step()[:2], reps=2)
+        tuned = A.tune(lambda: step()[:2], knobs=["sgm_prefetch", "sgm_htile", "sgm_overlap"],
+                       reps=2, log=log)
+        tuned["conv_layers_tuned"] = len(layers)
     log("warmup %d" % args.warmup)
     for _ in range(args.warmup):
         step()
@@ -419,7 +424,11 @@ def ragged_main(args):
         o, s_, _ = RG.net_band_ragged(net, b, l, r)
         return b.unpack(o) + b.unpack(s_)
 
-    tuned = A.tune(run, reps=2, log=log) if not args.no_autotune else None
+    tuned = None
+    if not args.no_autotune:
+        layers = A.tune_layers(net, run, reps=2)
+        tuned = A.tune(run, knobs=["sgm_prefetch", "sgm_htile", "sgm_overlap"], reps=2, log=log)
+        tuned["conv_layers_tuned"] = len(layers)
     # eager reference run (stage profile of the refinement) and capture
     L.lib().sv_profile_enable(1)
     eout, est, eres = RG.net_band_ragged(net, b, l, r)

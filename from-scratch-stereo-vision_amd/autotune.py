@@ -2,8 +2,9 @@
 
 Some kernels come in variants that compute identical results at different
 speeds: the prefetch depth of the vertical/diagonal SGM scan, the columns
-per LDS tile of the horizontal SGM scan, and the output rows per thread and
-columns per tile of the tiled convolution.  The library reads the choice from process-wide
+per LDS tile of the horizontal SGM scan, whether the horizontal scan runs
+beside the others on a second stream, and the output rows per thread,
+columns per tile and weight source of the tiled convolution.  The library reads the choice from process-wide
 tuning knobs (sv_tune_set); this module picks them for a workload by timing
 it on the GPU.
 
@@ -26,7 +27,9 @@ from . import _lib as L
 KNOBS = {"sgm_prefetch": (0, (2, 4, 8)),
          "sgm_htile": (1, (8, 16)),
          "conv_rows": (2, (1, 4)),
-         "conv_tile_width": (3, (64, 32, 16))}
+         "conv_tile_width": (3, (64, 32, 16)),
+         "conv_weights": (4, (1, 2)),
+         "sgm_overlap": (5, (2, 1))}
 
 
 def set_knob(name, value):
@@ -122,3 +125,55 @@ def tune(run, key=None, knobs=None, reps=3, cache_path=None, log=None):
         with open(cache_path, "w") as f:
             json.dump(data, f, indent=1)
     return res
+
+
+
+CONV_KNOBS = ("conv_rows", "conv_tile_width", "conv_weights")
+
+
+def tune_layers(net, run, reps=3, log=None):
+    """Per-layer variants of the tiled convolution.  Records every
+    convolution launch of one run() (a workload on `net`), then for each
+    distinct launch shape times every combination of rows per thread, tile
+    width and weight source on that launch alone, checks that the layer's
+    output is identical under each, and keeps the fastest for that shape
+    (net._conv_tune, applied at launch).  Returns {shape: (variant, ms)}."""
+    from . import net as N
+
+    calls = {}
+
+    def hook(key, fn, args, out):
+        if fn == "sv_conv_fwd" and key not in calls:
+            calls[key] = (args, out)
+
+    net._conv_hook = hook
+    try:
+        run()
+        torch.cuda.synchronize()
+    finally:
+        net._conv_hook = None
+    combos = [(r, t, w) for r in KNOBS["conv_rows"][1] for t in KNOBS["conv_tile_width"][1]
+              for w in KNOBS["conv_weights"][1]]
+    chosen = {}
+    try:
+        for key, (args, out) in calls.items():
+            best, ref = None, None
+            for combo in combos:
+                variant = tuple((KNOBS[k][0], v) for k, v in zip(CONV_KNOBS, combo))
+                for knob, v in variant:
+                    L.check(L.lib().sv_tune_set(knob, v))
+                ms, _ = _time(lambda: N._call("sv_conv_fwd", *args), reps)
+                if ref is None:
+                    ref = out.clone()
+                elif not torch.equal(out, ref):
+                    raise RuntimeError("conv variant %s changed the output of layer %s"
+                                       % (combo, key))
+                if log:
+                    log("autotune layer %s %s: %.4f ms" % (key, combo, ms))
+                if best is None or ms < best[1]:
+                    best = (variant, ms)
+            chosen[key] = best
+            net._conv_tune[key] = best[0]
+    finally:
+        reset()
+    return chosen

@@ -25,7 +25,9 @@
 #define SV_TUNE_SGM_HTILE 1     // columns per LDS tile in the horizontal SGM kernel: 8, 16
 #define SV_TUNE_CONV_ROWS 2     // output rows per thread in the tiled convolution: 1, 4
 #define SV_TUNE_CONV_TW 3       // output columns per tile of the tiled convolution: 16, 32, 64
-#define SV_TUNE_COUNT 4
+#define SV_TUNE_CONV_WSRC 4     // weights of the tiled convolution: 1 LDS, 2 scalar cache
+#define SV_TUNE_SGM_OVERLAP 5   // horizontal SGM paths: 1 after the others, 2 beside them on a second stream
+#define SV_TUNE_COUNT 6
 
 namespace sv {
 

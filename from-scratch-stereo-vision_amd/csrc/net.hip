@@ -114,11 +114,14 @@ __global__ void __launch_bounds__(256) k_conv_fwd(
 // halo tile of all KD depth taps in LDS (zero outside the image, the cost
 // volume formed while staging when SRC == 1), double-buffered so the next
 // channel loads while this one is consumed.  Each thread accumulates RP
-// output rows x OCB channels; weights are staged in LDS per channel.  TW
-// (64, 32 or 16) only changes how the plane is cut into tiles: narrow maps
-// waste less of a narrower tile, and every output is summed in the same
-// order whatever TW is.
-template <int KD, int SRC, int OCB, int RP, int TW>
+// output rows x OCB channels.  WS selects where the weights come from: 1,
+// staged in LDS per channel (read back as float4 broadcasts); 2, straight
+// from memory through the scalar cache into SGPR operands (the weights are
+// uniform across the block), which leaves the LDS return path to the
+// activations.  TW (64, 32 or 16) only changes how the plane is cut into
+// tiles: narrow maps waste less of a narrower tile.  Every output is summed
+// in the same order whatever TW, RP and WS are.
+template <int KD, int SRC, int OCB, int RP, int TW, int WS>
 __global__ void __launch_bounds__(256) k_conv_tile(
     const float* __restrict__ x, const float* __restrict__ xr, const int* __restrict__ idims,
     long long xstride, int IC, int D, const float* __restrict__ w, const float* __restrict__ b,
@@ -172,10 +175,12 @@ __global__ void __launch_bounds__(256) k_conv_tile(
             }
             buf[e] = v;
         }
-        float* wb = wsm + slotb * WSZ;
-        for (int e = tid; e < WSZ; e += 256) {
-            const int t = e / OCP, j = e - t * OCP;
-            wb[e] = (j < OCB && oc0 + j < OC) ? w[((long long)(oc0 + j) * IC + ic) * KD * 9 + t] : 0.f;
+        if (WS == 1) {
+            float* wb = wsm + slotb * WSZ;
+            for (int e = tid; e < WSZ; e += 256) {
+                const int t = e / OCP, j = e - t * OCP;
+                wb[e] = (j < OCB && oc0 + j < OC) ? w[((long long)(oc0 + j) * IC + ic) * KD * 9 + t] : 0.f;
+            }
         }
     };
     for (int oc0 = 0; oc0 < OC; oc0 += OCB) {
@@ -200,15 +205,26 @@ __global__ void __launch_bounds__(256) k_conv_tile(
                         for (int i = 0; i < RP; ++i)
                             v[i] = cur[kd * plane_t + ((ty * RP + i) * s + ky * dil) * IWt + tx * s + kx * dil];
                         const int wt = (kd * 3 + ky) * 3 + kx;
+                        if (WS == 1) {
 #pragma unroll
-                        for (int q = 0; q < OCP / 4; ++q) {
-                            const float4 wv = wc[wt * (OCP / 4) + q];
+                            for (int q = 0; q < OCP / 4; ++q) {
+                                const float4 wv = wc[wt * (OCP / 4) + q];
 #pragma unroll
-                            for (int i = 0; i < RP; ++i) {
-                                acc[i][4 * q + 0] += v[i] * wv.x;
-                                acc[i][4 * q + 1] += v[i] * wv.y;
-                                acc[i][4 * q + 2] += v[i] * wv.z;
-                                acc[i][4 * q + 3] += v[i] * wv.w;
+                                for (int i = 0; i < RP; ++i) {
+                                    acc[i][4 * q + 0] += v[i] * wv.x;
+                                    acc[i][4 * q + 1] += v[i] * wv.y;
+                                    acc[i][4 * q + 2] += v[i] * wv.z;
+                                    acc[i][4 * q + 3] += v[i] * wv.w;
+                                }
+                            }
+                        } else {
+                            // weights of (oc0 + j, ic, tap wt): uniform addresses
+                            const float* wg = w + ((long long)oc0 * IC + ic) * KD * 9 + wt;
+#pragma unroll
+                            for (int j = 0; j < OCB; ++j) {
+                                const float wj = oc0 + j < OC ? wg[(long long)j * IC * KD * 9] : 0.f;
+#pragma unroll
+                                for (int i = 0; i < RP; ++i) acc[i][j] += v[i] * wj;
                             }
                         }
                     }
@@ -1010,6 +1026,7 @@ int sv_conv_fwd(const float* x, const float* xr, const int* idims, long long xst
         const long long blocks4 = (long long)((max_wo + TWc - 1) / TWc) * ((max_ho + 4 * TYc - 1) / (4 * TYc)) * D * N;
         const int tuned = sv::tune(SV_TUNE_CONV_ROWS);
         const int RP = tuned ? tuned : (blocks4 >= 512 ? 4 : 1);
+        const int WSc = sv::tune(SV_TUNE_CONV_WSRC) ? sv::tune(SV_TUNE_CONV_WSRC) : 1;
         const int TT_H = TYc * RP;
         const int IHt = (TT_H - 1) * s + 2 * dil + 1, IWt = (TWc - 1) * s + 2 * dil + 1;
         const int ocp = OC == 1 ? 4 : (OC <= 4 ? 4 : (OC <= 8 ? 8 : (OC <= 12 ? 12 : 16)));
@@ -1017,10 +1034,15 @@ int sv_conv_fwd(const float* x, const float* xr, const int* idims, long long xst
         if (shm <= 64 * 1024) {
             const int tYt = (max_ho + TT_H - 1) / TT_H;
             dim3 g((max_wo + TWc - 1) / TWc, tYt * D, N), bk(TWc, TYc);
+#define LAUNCH_TWS(KD_, SRC_, OCB_, RP_, TW_, WS_)                                               \
+    hipLaunchKernelGGL((k_conv_tile<KD_, SRC_, OCB_, RP_, TW_, WS_>), g, bk, shm, st, x, xr,       \
+                       idims, xstride, IC, D, w, b, OC, s, p, dil, y, odims, ystride, tYt, mode,  \
+                       slope, resid, rstride, IHt, IWt)
 #define LAUNCH_TW(KD_, SRC_, OCB_, RP_, TW_)                                                      \
-    hipLaunchKernelGGL((k_conv_tile<KD_, SRC_, OCB_, RP_, TW_>), g, bk, shm, st, x, xr, idims,   \
-                       xstride, IC, D, w, b, OC, s, p, dil, y, odims, ystride, tYt, mode, slope,  \
-                       resid, rstride, IHt, IWt)
+    do {                                                                                         \
+        if (WSc == 1) LAUNCH_TWS(KD_, SRC_, OCB_, RP_, TW_, 1);                                  \
+        else LAUNCH_TWS(KD_, SRC_, OCB_, RP_, TW_, 2);                                           \
+    } while (0)
 #define LAUNCH_T(KD_, SRC_, OCB_)                                                                \
     do {                                                                                         \
         if (RP == 4) {                                                                           \
@@ -1048,6 +1070,7 @@ int sv_conv_fwd(const float* x, const float* xr, const int* idims, long long xst
 #undef LAUNCH_T_OC
 #undef LAUNCH_T
 #undef LAUNCH_TW
+#undef LAUNCH_TWS
             SV_CHECK_LAUNCH();
             return SV_OK;
         }

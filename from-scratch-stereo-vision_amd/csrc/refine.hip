@@ -28,6 +28,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 
 #include "common.h"
 
@@ -1562,6 +1563,35 @@ int launch_costs(const Plan& pl, bool right, const uint64_t* cA, const uint64_t*
     return sv::set_error(SV_EINVAL, "unknown image dtype %d", dtype);
 }
 
+// A second stream per device for work that can overlap the caller's: the
+// horizontal SGM paths run there while the vertical and diagonal ones run
+// on the caller's stream (fork and join through events, so the caller's
+// stream order is kept and a graph capture of the caller's stream records
+// both branches).  Created on first use, kept until the process ends.
+struct SideStream {
+    hipStream_t s = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+
+std::mutex g_side_mu;
+SideStream g_side[64];
+
+int side_stream(SideStream** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= 64)
+        return sv::set_error(SV_EHIP, "side stream: no current device");
+    SideStream& ss = g_side[dev];
+    if (!ss.s) {
+        if ((e = hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking)) != hipSuccess ||
+            (e = hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming)) != hipSuccess ||
+            (e = hipEventCreateWithFlags(&ss.join, hipEventDisableTiming)) != hipSuccess)
+            return sv::set_error(SV_EHIP, "side stream: %s", hipGetErrorString(e));
+    }
+    *out = &ss;
+    return SV_OK;
+}
+
 template <typename CT, typename LT, typename AT, int KC>
 int sgm_t(const Plan& pl, const void* C, const int* base, void* L, int BV, const Geom& g,
           double P1, double P2, int dir0, int ndir, hipStream_t st) {
@@ -1609,18 +1639,32 @@ int sgm_k_t(const Plan& pl, const void* C, const int* base, void* L, int BV, con
     // others through the per-lane one
     if (sizeof(LT) == 1 && pl.invalid + P2 + P1 < 65000.0 && pl.Kp >= 8 * ((K + 1 + 7) / 8)) {
         const int hEnd = min(dir0 + ndir, 2);
+        const int vBeg = max(dir0, 2), vEnd = dir0 + ndir;
+        // both kinds of path in this call: horizontal ones on the side stream
+        const bool fork = hEnd > dir0 && vEnd > vBeg && sv::tune(SV_TUNE_SGM_OVERLAP) == 2;
+        std::unique_lock<std::mutex> lock(g_side_mu, std::defer_lock);
+        SideStream* ss = nullptr;
+        hipStream_t hst = st;
+        if (fork) {
+            lock.lock();
+            int rc = side_stream(&ss);
+            if (rc) return rc;
+            hipError_t e = hipEventRecord(ss->fork, st);
+            if (e == hipSuccess) e = hipStreamWaitEvent(ss->s, ss->fork, 0);
+            if (e != hipSuccess) return sv::set_error(SV_EHIP, "sgm fork: %s", hipGetErrorString(e));
+            hst = ss->s;
+        }
         if (hEnd > dir0) {
             dim3 hgrid((g.H + 63) / 64, hEnd - dir0, BV);
             if (sv::tune(SV_TUNE_SGM_HTILE) == 8)
-                hipLaunchKernelGGL((k_sgm_ph<K, 8>), hgrid, dim3(64), 0, st, (const uint8_t*)C,
+                hipLaunchKernelGGL((k_sgm_ph<K, 8>), hgrid, dim3(64), 0, hst, (const uint8_t*)C,
                                    base, (uint8_t*)L, g, pl.Kp, (int)P1, (int)P2, dir0, dirStride);
             else
-                hipLaunchKernelGGL((k_sgm_ph<K, 16>), hgrid, dim3(64), 0, st, (const uint8_t*)C,
+                hipLaunchKernelGGL((k_sgm_ph<K, 16>), hgrid, dim3(64), 0, hst, (const uint8_t*)C,
                                    base, (uint8_t*)L, g, pl.Kp, (int)P1, (int)P2, dir0, dirStride);
-            SV_CHECK_LAUNCH();
         }
-        const int vBeg = max(dir0, 2), vEnd = dir0 + ndir;
-        if (vEnd > vBeg) {
+        hipError_t lerr = hipGetLastError();
+        if (vEnd > vBeg && lerr == hipSuccess) {
             dim3 vgrid((g.W + 63) / 64, vEnd - vBeg, BV);
             uint8_t* Lv = (uint8_t*)L + (long long)(vBeg - dir0) * dirStride;
             switch (sv::tune(SV_TUNE_SGM_PREFETCH)) {
@@ -1636,8 +1680,14 @@ int sgm_k_t(const Plan& pl, const void* C, const int* base, void* L, int BV, con
                     hipLaunchKernelGGL((k_sgm_p<K, 4>), vgrid, dim3(64), 0, st, (const uint8_t*)C,
                                        base, Lv, g, pl.Kp, (int)P1, (int)P2, vBeg, dirStride);
             }
-            SV_CHECK_LAUNCH();
+            lerr = hipGetLastError();
         }
+        if (fork) {  // join before reporting anything, so the caller's stream never runs ahead
+            hipError_t e = hipEventRecord(ss->join, ss->s);
+            if (e == hipSuccess) e = hipStreamWaitEvent(st, ss->join, 0);
+            if (e != hipSuccess) return sv::set_error(SV_EHIP, "sgm join: %s", hipGetErrorString(e));
+        }
+        if (lerr != hipSuccess) return sv::set_error(SV_EHIP, "sgm launch: %s", hipGetErrorString(lerr));
         return SV_OK;
     }
     hipLaunchKernelGGL((k_sgm_k<K, LT>), grid, dim3(64), 0, st, (const uint8_t*)C, base, (LT*)L, g,

@@ -156,6 +156,10 @@ class StereoNet:
         self._cache = None
         self.bf16 = False
         self._packs = {}
+        # per-layer kernel variants chosen by the autotuner: launch shape ->
+        # ((knob, value), ...); and a recorder of conv launches it uses
+        self._conv_tune = {}
+        self._conv_hook = None
         # FP32 convolutions: "vector" runs every layer on the LDS-tiled
         # vector kernel; "mfma" runs the layers with MFMA_MIN_OC or more
         # output channels on the matrix cores (FP32 products and sums),
@@ -450,10 +454,29 @@ class _Plan:
         fn = "sv_conv_fwd"
         if self.net.conv_engine == "mfma" and MFMA_MIN_OC <= c.oc <= 48:
             fn = "sv_conv_fwd_mf32"
-        _call(fn, _p(x), _p(xr), _p(xlv.table), xst, c.ic, D, c.kd,
-              _p(P[c.w_off:]), _p(P[c.b_off:]), c.oc, c.s, c.p, c.dil, _p(y), _p(ylv.table),
-              c.oc * D * ylv.slot, N, ylv.maxh, ylv.maxw, mode, SLOPE, _p(resid), rstride,
-              L.stream())
+        args = (_p(x), _p(xr), _p(xlv.table), xst, c.ic, D, c.kd, _p(P[c.w_off:]), _p(P[c.b_off:]),
+                c.oc, c.s, c.p, c.dil, _p(y), _p(ylv.table), c.oc * D * ylv.slot, N, ylv.maxh,
+                ylv.maxw, mode, SLOPE, _p(resid), rstride, L.stream())
+        key = (c.w_off, D, N, ylv.maxh, ylv.maxw)
+        if self.net._conv_hook is not None:
+            self.net._conv_hook(key, fn, args, y[:c.oc * D * ylv.slot * N])
+        variant = self.net._conv_tune.get(key)
+        if variant is None:
+            _call(fn, *args)
+            return
+        # this layer's own kernel variant (identical results), then back
+        lib = L.lib()
+        old = []
+        for knob, v in variant:
+            cur = ctypes.c_int(0)
+            L.check(lib.sv_tune_get(knob, ctypes.byref(cur)))
+            old.append((knob, cur.value))
+            L.check(lib.sv_tune_set(knob, v))
+        try:
+            _call(fn, *args)
+        finally:
+            for knob, v in old:
+                lib.sv_tune_set(knob, v)
 
     def forward(self):
         net, N = self.net, self.N

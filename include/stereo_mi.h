@@ -131,7 +131,10 @@ int sv_refine_ragged(int mode, const void* left, const void* right, int dtype, c
  * depth of the vertical/diagonal SGM kernel (2, 4, 8); 1: columns per LDS
  * tile of the horizontal SGM kernel (8, 16); 2: output rows per thread of
  * the tiled convolution (1, 4); 3: output columns per tile of the tiled
- * convolution (16, 32, 64).  Value 0 restores the built-in heuristic.
+ * convolution (16, 32, 64); 4: weight source of the tiled convolution (1
+ * LDS, 2 scalar cache); 5: horizontal SGM paths after the others (1) or on a
+ * second, library-owned stream beside them (2).  Value 0 restores the
+ * built-in heuristic (LDS weights; horizontal paths after the others).
  * Process-wide; set between launches, not while kernels are being queued
  * from other threads. */
 int sv_tune_set(int knob, int value);

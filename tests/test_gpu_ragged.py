@@ -127,3 +127,21 @@ def test_autotuner_knobs_keep_results_identical():
     finally:
         A.reset()
     assert all(A.get_knob(k) == 0 for k in A.KNOBS)
+
+
+def test_per_layer_conv_variants_keep_results_identical():
+    R, N = _mods()
+    A = importlib.import_module("from-scratch-stereo-vision_amd.autotune")
+    ps = _pairs()
+    net = N.load_model(package().__path__[0] + "/weights/stereonet_c5.txt")
+    b = R.RaggedBatch(SIZES)
+    l = b.pack([p["left"] for p in ps])
+    r = b.pack([p["right"] for p in ps])
+    res0 = N.predict_auto_batch(net, l, r, None, None, batch=b)
+    ref = [t.clone() for t in b.unpack(res0.full)]
+    # tune_layers raises if any variant changes a layer's output
+    chosen = A.tune_layers(net, lambda: N.predict_auto_batch(net, l, r, None, None, batch=b), reps=1)
+    assert len(chosen) >= 15 and len(net._conv_tune) == len(chosen)
+    res1 = N.predict_auto_batch(net, l, r, None, None, batch=b)
+    assert all(torch.equal(u, v) for u, v in zip(b.unpack(res1.full), ref))
+    assert all(A.get_knob(k) == 0 for k in A.KNOBS)
