@@ -162,6 +162,9 @@ def tune_layers(net, run, reps=3, log=None):
                 variant = tuple((KNOBS[k][0], v) for k, v in zip(CONV_KNOBS, combo))
                 for knob, v in variant:
                     L.check(L.lib().sv_tune_set(knob, v))
+                # the slots' padding past each sample is never written: clear
+                # it so that stale contents (NaN patterns included) compare
+                out.zero_()
                 ms, _ = _time(lambda: N._call("sv_conv_fwd", *args), reps)
                 if ref is None:
                     ref = out.clone()

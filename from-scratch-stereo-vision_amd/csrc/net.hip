@@ -120,7 +120,8 @@ __global__ void __launch_bounds__(256) k_conv_fwd(
 // uniform across the block), which leaves the LDS return path to the
 // activations.  TW (64, 32 or 16) only changes how the plane is cut into
 // tiles: narrow maps waste less of a narrower tile.  Every output is summed
-// in the same order whatever TW, RP and WS are.
+// in the same order, one fused multiply-add per term, whatever TW, RP and WS
+// are, so the variants agree bit for bit.
 template <int KD, int SRC, int OCB, int RP, int TW, int WS>
 __global__ void __launch_bounds__(256) k_conv_tile(
     const float* __restrict__ x, const float* __restrict__ xr, const int* __restrict__ idims,
@@ -211,10 +212,10 @@ __global__ void __launch_bounds__(256) k_conv_tile(
                                 const float4 wv = wc[wt * (OCP / 4) + q];
 #pragma unroll
                                 for (int i = 0; i < RP; ++i) {
-                                    acc[i][4 * q + 0] += v[i] * wv.x;
-                                    acc[i][4 * q + 1] += v[i] * wv.y;
-                                    acc[i][4 * q + 2] += v[i] * wv.z;
-                                    acc[i][4 * q + 3] += v[i] * wv.w;
+                                    acc[i][4 * q + 0] = __builtin_fmaf(v[i], wv.x, acc[i][4 * q + 0]);
+                                    acc[i][4 * q + 1] = __builtin_fmaf(v[i], wv.y, acc[i][4 * q + 1]);
+                                    acc[i][4 * q + 2] = __builtin_fmaf(v[i], wv.z, acc[i][4 * q + 2]);
+                                    acc[i][4 * q + 3] = __builtin_fmaf(v[i], wv.w, acc[i][4 * q + 3]);
                                 }
                             }
                         } else {
@@ -224,7 +225,7 @@ __global__ void __launch_bounds__(256) k_conv_tile(
                             for (int j = 0; j < OCB; ++j) {
                                 const float wj = oc0 + j < OC ? wg[(long long)j * IC * KD * 9] : 0.f;
 #pragma unroll
-                                for (int i = 0; i < RP; ++i) acc[i][j] += v[i] * wj;
+                                for (int i = 0; i < RP; ++i) acc[i][j] = __builtin_fmaf(v[i], wj, acc[i][j]);
                             }
                         }
                     }
