@@ -3,10 +3,11 @@
 Some kernels come in variants that compute identical results at different
 speeds: the prefetch depth of the vertical/diagonal SGM scan, the columns
 per LDS tile of the horizontal SGM scan, whether the horizontal scan runs
-beside the others on a second stream, and the output rows per thread,
-columns per tile and weight source of the tiled convolution.  The library reads the choice from process-wide
-tuning knobs (sv_tune_set); this module picks them for a workload by timing
-it on the GPU.
+beside the others on a second stream, and for the tiled convolution the
+engine (vector ALUs or matrix cores), the output rows per thread, columns
+per tile, weight source and pixel groups per wave.  The library reads the
+choice from process-wide tuning knobs (sv_tune_set); this module picks them
+for a workload by timing it on the GPU.
 
 tune() does coordinate descent: for each knob in turn it times the workload
 under every allowed value, the other knobs held at their best so far, and
@@ -29,7 +30,9 @@ KNOBS = {"sgm_prefetch": (0, (2, 4, 8)),
          "conv_rows": (2, (1, 4)),
          "conv_tile_width": (3, (64, 32, 16)),
          "conv_weights": (4, (1, 2)),
-         "sgm_overlap": (5, (2, 1))}
+         "sgm_overlap": (5, (2, 1)),
+         "conv_engine": (6, (1, 2, 3)),
+         "conv_mgroups": (7, (1, 2, 4))}
 
 
 def set_knob(name, value):
@@ -128,16 +131,34 @@ def tune(run, key=None, knobs=None, reps=3, cache_path=None, log=None):
 
 
 
-CONV_KNOBS = ("conv_rows", "conv_tile_width", "conv_weights")
+def _conv_variants():
+    """Every convolution variant as ((knob, value), ...): the vector kernel
+    under each rows x tile width x weight source, the matrix-core kernel
+    (3D layers sliding over depth, or one depth per tile) under each pixel
+    groups per wave x tile width."""
+    K = {k: v[0] for k, v in KNOBS.items()}
+    out = []
+    for r in KNOBS["conv_rows"][1]:
+        for t in KNOBS["conv_tile_width"][1]:
+            for w in KNOBS["conv_weights"][1]:
+                out.append(((K["conv_engine"], 1), (K["conv_rows"], r),
+                            (K["conv_tile_width"], t), (K["conv_weights"], w)))
+    for e in (2, 3):
+        for g in KNOBS["conv_mgroups"][1]:
+            for t in KNOBS["conv_tile_width"][1]:
+                out.append(((K["conv_engine"], e), (K["conv_mgroups"], g), (K["conv_tile_width"], t)))
+    return out
 
 
 def tune_layers(net, run, reps=3, log=None):
     """Per-layer variants of the tiled convolution.  Records every
     convolution launch of one run() (a workload on `net`), then for each
-    distinct launch shape times every combination of rows per thread, tile
-    width and weight source on that launch alone, checks that the layer's
-    output is identical under each, and keeps the fastest for that shape
-    (net._conv_tune, applied at launch).  Returns {shape: (variant, ms)}."""
+    distinct launch shape times every variant (the vector kernel's rows per
+    thread, tile width and weight source; the matrix-core kernel's pixel
+    groups per wave and tile width) on that launch alone, checks that the
+    layer's output is identical under each, and keeps the fastest for that
+    shape (net._conv_tune, applied at launch).  Returns {shape: (variant,
+    ms)}."""
     from . import net as N
 
     calls = {}
@@ -152,14 +173,13 @@ def tune_layers(net, run, reps=3, log=None):
         torch.cuda.synchronize()
     finally:
         net._conv_hook = None
-    combos = [(r, t, w) for r in KNOBS["conv_rows"][1] for t in KNOBS["conv_tile_width"][1]
-              for w in KNOBS["conv_weights"][1]]
+    variants = _conv_variants()
     chosen = {}
     try:
         for key, (args, out) in calls.items():
             best, ref = None, None
-            for combo in combos:
-                variant = tuple((KNOBS[k][0], v) for k, v in zip(CONV_KNOBS, combo))
+            for variant in variants:
+                reset()
                 for knob, v in variant:
                     L.check(L.lib().sv_tune_set(knob, v))
                 # the slots' padding past each sample is never written: clear
@@ -170,9 +190,9 @@ def tune_layers(net, run, reps=3, log=None):
                     ref = out.clone()
                 elif not torch.equal(out, ref):
                     raise RuntimeError("conv variant %s changed the output of layer %s"
-                                       % (combo, key))
+                                       % (variant, key))
                 if log:
-                    log("autotune layer %s %s: %.4f ms" % (key, combo, ms))
+                    log("autotune layer %s %s: %.4f ms" % (key, variant, ms))
                 if best is None or ms < best[1]:
                     best = (variant, ms)
             chosen[key] = best

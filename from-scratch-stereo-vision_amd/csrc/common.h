@@ -27,7 +27,9 @@
 #define SV_TUNE_CONV_TW 3       // output columns per tile of the tiled convolution: 16, 32, 64
 #define SV_TUNE_CONV_WSRC 4     // weights of the tiled convolution: 1 LDS, 2 scalar cache
 #define SV_TUNE_SGM_OVERLAP 5   // horizontal SGM paths: 1 after the others, 2 beside them on a second stream
-#define SV_TUNE_COUNT 6
+#define SV_TUNE_CONV_ENGINE 6   // tiled convolution: 1 vector ALUs, 2 matrix cores (FP32 4x4x1 MFMA; 3D layers slide over depth), 3 matrix cores, one depth per tile
+#define SV_TUNE_CONV_MGROUPS 7  // matrix-core convolution: 64-pixel groups per wave, 1, 2, 4
+#define SV_TUNE_COUNT 8
 
 namespace sv {
 
@@ -35,6 +37,9 @@ int set_error(int code, const char* fmt, ...);
 
 // current value of a tuning knob (0: heuristic)
 int tune(int knob);
+
+// compute units of the current device (cached per device)
+int cu_count();
 
 inline size_t align_up(size_t n, size_t a = 256) { return (n + a - 1) / a * a; }
 

@@ -11,6 +11,21 @@ static std::atomic<int> g_tune[SV_TUNE_COUNT];
 
 int tune(int knob) { return knob >= 0 && knob < SV_TUNE_COUNT ? g_tune[knob].load() : 0; }
 
+int cu_count() {
+    static std::atomic<int> cached[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) return 256;
+    if (dev < 64) {
+        const int c = cached[dev].load();
+        if (c > 0) return c;
+    }
+    int c = 0;
+    if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0)
+        c = 256;
+    if (dev < 64) cached[dev].store(c);
+    return c;
+}
+
 int set_error(int code, const char* fmt, ...) {
     va_list ap;
     va_start(ap, fmt);
@@ -31,7 +46,7 @@ int sv_version(void) { return 1; }
 int sv_tune_set(int knob, int value) {
     static const int allowed[SV_TUNE_COUNT][4] = {
         {2, 4, 8, -1}, {8, 16, -1, -1}, {1, 4, -1, -1}, {16, 32, 64, -1}, {1, 2, -1, -1},
-        {1, 2, -1, -1}};
+        {1, 2, -1, -1}, {1, 2, 3, -1}, {1, 2, 4, -1}};
     if (knob < 0 || knob >= SV_TUNE_COUNT)
         return sv::set_error(SV_EINVAL, "tuning knob %d does not exist", knob);
     bool ok = value == 0;

@@ -17,11 +17,9 @@
 #include <cmath>
 
 #include "common.h"
+#include "conv_tiles.h"
 
 namespace {
-
-__device__ __forceinline__ int dh(const int* d, int n) { return d[2 * n]; }
-__device__ __forceinline__ int dw(const int* d, int n) { return d[2 * n + 1]; }
 
 // ---------------------------------------------------------------- conv fwd
 // Conv2d.forward (stereonet.py:489-526) and Conv3d.forward (:596-641) as one
@@ -108,149 +106,131 @@ __global__ void __launch_bounds__(256) k_conv_fwd(
     }
 }
 
-// LDS-tiled form of k_conv_fwd.  A block of 256 threads (TW columns x
-// 256/TW rows of threads) makes a tile of TW output columns x RP*256/TW
-// rows at one output depth; for each input channel it stages the input
-// halo tile of all KD depth taps in LDS (zero outside the image, the cost
-// volume formed while staging when SRC == 1), double-buffered so the next
-// channel loads while this one is consumed.  Each thread accumulates RP
-// output rows x OCB channels.  WS selects where the weights come from: 1,
-// staged in LDS per channel (read back as float4 broadcasts); 2, straight
-// from memory through the scalar cache into SGPR operands (the weights are
-// uniform across the block), which leaves the LDS return path to the
-// activations.  TW (64, 32 or 16) only changes how the plane is cut into
-// tiles: narrow maps waste less of a narrower tile.  Every output is summed
-// in the same order, one fused multiply-add per term, whatever TW, RP and WS
-// are, so the variants agree bit for bit.
+// LDS-tiled form of k_conv_fwd on the vector ALUs.  A block of 256 threads
+// (TW columns x 256/TW rows of threads) makes output tiles of TW columns x
+// RP*256/TW rows at one output depth, each thread RP rows x OCB channels,
+// walking the launch's tile list; per tile it stages the input of ICG
+// channels at a time (all of them when they fit, then once for every
+// output-channel group).  Tile list and staging: conv_tiles.h.
+//
+// WS selects where the weights come from: 1, staged in LDS with the input
+// (read back as float4 broadcasts); 2, straight from memory through the
+// scalar cache into SGPR operands (uniform across the block).  TW (64, 32 or
+// 16) only changes how the plane is cut into tiles.  Every output is summed
+// in the same order (input channel, then depth, row, column tap; one fused
+// multiply-add per term; bias last) whatever TW, RP, WS and ICG are, so the
+// variants agree bit for bit.
 template <int KD, int SRC, int OCB, int RP, int TW, int WS>
 __global__ void __launch_bounds__(256) k_conv_tile(
     const float* __restrict__ x, const float* __restrict__ xr, const int* __restrict__ idims,
     long long xstride, int IC, int D, const float* __restrict__ w, const float* __restrict__ b,
     int OC, int s, int p, int dil, float* __restrict__ y, const int* __restrict__ odims,
-    long long ystride, int tilesY, int mode, float slope, const float* __restrict__ resid,
-    long long rstride, int IHt, int IWt) {
+    long long ystride, int N, int mode, float slope, const float* __restrict__ resid,
+    long long rstride, sv::ConvTiles ct) {
     // OCB padded to a multiple of 4 for float4 weight reads
     constexpr int OCP = (OCB + 3) / 4 * 4;
-    constexpr int WSZ = KD * 9 * OCP;
+    constexpr int TAPS = KD * 9;
     constexpr int TY = 256 / TW;   // thread rows
     constexpr int TT_H = TY * RP;  // rows per tile: TY threads high, RP rows each
     extern __shared__ __align__(16) float smem_f[];
-    const int n = blockIdx.z;
-    const int od = blockIdx.y / tilesY;
-    const int ty0 = (blockIdx.y - od * tilesY) * TT_H;
-    const int tx0 = blockIdx.x * TW;
-    const int hi = dh(idims, n), wi = dw(idims, n);
-    const int ho = dh(odims, n), wo = dw(odims, n);
-    if (tx0 >= wo || ty0 >= ho) return;  // uniform per block
+    const int IWt = ct.IWt, ICG = ct.ICG;
+    const int plane_t = ct.IHt * IWt;
+    const int tsz = KD * plane_t;
+    float* tile = smem_f;                                     // [ICG][KD][IHt][IWt]
+    float* wsm = smem_f + ICG * tsz;                          // [ICG][TAPS][OCP] (WS == 1)
+    int* pre = reinterpret_cast<int*>(wsm + (WS == 1 ? ICG * TAPS * OCP : 0));  // [N + 1]
+    __shared__ int wsum[4];
     const int tx = threadIdx.x, ty = threadIdx.y;  // TW x TY
     const int tid = ty * TW + tx;
-    const int plane_t = IHt * IWt;
-    const int tsz = KD * plane_t;
-    float* wsm = smem_f;                   // [2][KD*9][OCP]
-    float* tile = smem_f + 2 * WSZ;        // [2][KD][IHt][IWt]
-    const int iy0 = ty0 * s - p, ix0 = tx0 * s - p;
-    const long long iplane = (long long)hi * wi;
-    const float* xs = x + (long long)n * xstride;
-    const float* xrs = SRC == 1 ? xr + (long long)n * xstride : nullptr;
-    constexpr int PD = KD / 2;
+    const int total = conv_tile_prefix<TW, TT_H>(odims, N, D, pre, wsum, tid);
     float acc[RP][OCP];
-    auto stage = [&](int ic, int oc0, int slotb) {
-        float* buf = tile + slotb * tsz;
-        for (int e = tid; e < tsz; e += 256) {
-            const int kd = e / plane_t;
-            const int r = e - kd * plane_t;
-            const int yy = r / IWt, xx = r - yy * IWt;
-            const int dd = od + kd - PD, iy = iy0 + yy, ix = ix0 + xx;
-            float v = 0.f;
-            if (dd >= 0 && dd < D && iy >= 0 && iy < hi && ix >= 0 && ix < wi) {
-                if (SRC == 1) {
-                    if (ix < dd) {
-                        v = 2.0f;
-                    } else {
-                        const long long o = (long long)ic * iplane + (long long)iy * wi;
-                        v = fabsf(xs[o + ix] - xrs[o + ix - dd]);
+    for (int t = conv_first_tile(); t < total; t += gridDim.x) {
+        const TileAt at = conv_tile_at<TW, TT_H>(t, pre, N, D, odims);
+        const int n = at.n, od = at.od, ty0 = at.ty0, tx0 = at.tx0;
+        const int hin = dh(idims, n), win = dw(idims, n);
+        const int ho = dh(odims, n), wo = dw(odims, n);
+        const float* xs = x + (long long)n * xstride;
+        const float* xrs = SRC == 1 ? xr + (long long)n * xstride : nullptr;
+        for (int oc0 = 0; oc0 < OC; oc0 += OCB) {
+#pragma unroll
+            for (int i = 0; i < RP; ++i)
+#pragma unroll
+                for (int j = 0; j < OCP; ++j) acc[i][j] = 0.f;
+            for (int ic0 = 0; ic0 < IC; ic0 += ICG) {
+                const int icn = min(ICG, IC - ic0);
+                if (oc0 == 0 || ICG < IC)
+                    conv_stage_inputs<KD, SRC>(tile, xs, xrs, ic0, icn, D, od, ty0 * s - p,
+                                               tx0 * s - p, hin, win, ct, tid);
+                if (WS == 1) {
+                    const int WE = icn * TAPS * OCP;
+                    for (int e = tid; e < WE; e += 256) {
+                        const int c = e / (TAPS * OCP);
+                        const int r = e - c * (TAPS * OCP);
+                        const int tp = r / OCP, j = r - tp * OCP;
+                        wsm[e] = (j < OCB && oc0 + j < OC)
+                                     ? w[((long long)(oc0 + j) * IC + ic0 + c) * TAPS + tp] : 0.f;
                     }
-                } else {
-                    v = xs[((long long)ic * D + dd) * iplane + (long long)iy * wi + ix];
                 }
-            }
-            buf[e] = v;
-        }
-        if (WS == 1) {
-            float* wb = wsm + slotb * WSZ;
-            for (int e = tid; e < WSZ; e += 256) {
-                const int t = e / OCP, j = e - t * OCP;
-                wb[e] = (j < OCB && oc0 + j < OC) ? w[((long long)(oc0 + j) * IC + ic) * KD * 9 + t] : 0.f;
-            }
-        }
-    };
-    for (int oc0 = 0; oc0 < OC; oc0 += OCB) {
+                __syncthreads();
+                for (int c = 0; c < icn; ++c) {
+                    const float* cur = tile + c * tsz;
+                    const float4* wc = reinterpret_cast<const float4*>(wsm + c * TAPS * OCP);
 #pragma unroll
-        for (int i = 0; i < RP; ++i)
+                    for (int kd = 0; kd < KD; ++kd)
 #pragma unroll
-            for (int j = 0; j < OCP; ++j) acc[i][j] = 0.f;
-        stage(0, oc0, 0);
-        __syncthreads();
-        for (int ic = 0; ic < IC; ++ic) {
-            const float* cur = tile + (ic & 1) * tsz;
-            const float4* wc = reinterpret_cast<const float4*>(wsm + (ic & 1) * WSZ);
-            if (ic + 1 < IC) stage(ic + 1, oc0, (ic + 1) & 1);
+                        for (int ky = 0; ky < 3; ++ky)
 #pragma unroll
-            for (int kd = 0; kd < KD; ++kd)
+                            for (int kx = 0; kx < 3; ++kx) {
+                                float v[RP];
 #pragma unroll
-                for (int ky = 0; ky < 3; ++ky)
+                                for (int i = 0; i < RP; ++i)
+                                    v[i] = cur[kd * plane_t + ((ty * RP + i) * s + ky * dil) * IWt +
+                                               tx * s + kx * dil];
+                                const int wt = (kd * 3 + ky) * 3 + kx;
+                                if (WS == 1) {
 #pragma unroll
-                    for (int kx = 0; kx < 3; ++kx) {
-                        float v[RP];
+                                    for (int q = 0; q < OCP / 4; ++q) {
+                                        const float4 wv = wc[wt * (OCP / 4) + q];
 #pragma unroll
-                        for (int i = 0; i < RP; ++i)
-                            v[i] = cur[kd * plane_t + ((ty * RP + i) * s + ky * dil) * IWt + tx * s + kx * dil];
-                        const int wt = (kd * 3 + ky) * 3 + kx;
-                        if (WS == 1) {
+                                        for (int i = 0; i < RP; ++i) {
+                                            acc[i][4 * q + 0] = __builtin_fmaf(v[i], wv.x, acc[i][4 * q + 0]);
+                                            acc[i][4 * q + 1] = __builtin_fmaf(v[i], wv.y, acc[i][4 * q + 1]);
+                                            acc[i][4 * q + 2] = __builtin_fmaf(v[i], wv.z, acc[i][4 * q + 2]);
+                                            acc[i][4 * q + 3] = __builtin_fmaf(v[i], wv.w, acc[i][4 * q + 3]);
+                                        }
+                                    }
+                                } else {
+                                    // weights of (oc0 + j, ic0 + c, tap wt): uniform
+                                    // addresses, channels past OC clamped (their
+                                    // sums are never stored)
+                                    const float* wg = w + (long long)(ic0 + c) * TAPS + wt;
 #pragma unroll
-                            for (int q = 0; q < OCP / 4; ++q) {
-                                const float4 wv = wc[wt * (OCP / 4) + q];
+                                    for (int j = 0; j < OCB; ++j) {
+                                        const float wj = wg[(long long)min(oc0 + j, OC - 1) * IC * TAPS];
 #pragma unroll
-                                for (int i = 0; i < RP; ++i) {
-                                    acc[i][4 * q + 0] = __builtin_fmaf(v[i], wv.x, acc[i][4 * q + 0]);
-                                    acc[i][4 * q + 1] = __builtin_fmaf(v[i], wv.y, acc[i][4 * q + 1]);
-                                    acc[i][4 * q + 2] = __builtin_fmaf(v[i], wv.z, acc[i][4 * q + 2]);
-                                    acc[i][4 * q + 3] = __builtin_fmaf(v[i], wv.w, acc[i][4 * q + 3]);
+                                        for (int i = 0; i < RP; ++i)
+                                            acc[i][j] = __builtin_fmaf(v[i], wj, acc[i][j]);
+                                    }
                                 }
                             }
-                        } else {
-                            // weights of (oc0 + j, ic, tap wt): uniform addresses
-                            const float* wg = w + ((long long)oc0 * IC + ic) * KD * 9 + wt;
-#pragma unroll
-                            for (int j = 0; j < OCB; ++j) {
-                                const float wj = oc0 + j < OC ? wg[(long long)j * IC * KD * 9] : 0.f;
-#pragma unroll
-                                for (int i = 0; i < RP; ++i) acc[i][j] = __builtin_fmaf(v[i], wj, acc[i][j]);
-                            }
-                        }
-                    }
-            __syncthreads();
-        }
-        const int ox = tx0 + tx;
-        const long long oplane = (long long)ho * wo;
-        float* ys = y + (long long)n * ystride + (long long)od * oplane;
-#pragma unroll
-        for (int i = 0; i < RP; ++i) {
-            const int oy = ty0 + ty * RP + i;
-            if (ox >= wo || oy >= ho) continue;
-#pragma unroll
-            for (int j = 0; j < OCB; ++j) {
-                const int oc = oc0 + j;
-                if (oc >= OC) break;
-                float v = acc[i][j] + b[oc];
-                if (mode == 1) {
-                    v = v > 0.f ? v : v * slope;
-                } else if (mode == 2) {
-                    v = resid[(long long)n * rstride + (long long)oc * D * oplane +
-                              (long long)od * oplane + (long long)oy * wo + ox] + v;
-                    v = v > 0.f ? v : 0.f;
                 }
-                ys[(long long)oc * D * oplane + (long long)oy * wo + ox] = v;
+                __syncthreads();
+            }
+            const int ox = tx0 + tx;
+            const long long oplane = (long long)ho * wo;
+            float* ys = y + (long long)n * ystride + (long long)od * oplane;
+#pragma unroll
+            for (int i = 0; i < RP; ++i) {
+                const int oy = ty0 + ty * RP + i;
+                if (ox >= wo || oy >= ho) continue;
+#pragma unroll
+                for (int j = 0; j < OCB; ++j) {
+                    const int oc = oc0 + j;
+                    if (oc >= OC) break;
+                    const long long o = (long long)oc * D * oplane + (long long)oy * wo + ox;
+                    ys[o] = conv_finish(acc[i][j], b[oc], mode, slope, resid,
+                                        (long long)n * rstride + (long long)od * oplane + o);
+                }
             }
         }
     }
@@ -1006,12 +986,28 @@ int sv_conv_fwd(const float* x, const float* xr, const int* idims, long long xst
     SV_REQUIRE(mode != 2 || resid, "sv_conv_fwd: residual mode needs resid");
     hipStream_t st = (hipStream_t)stream;
     const int src = xr ? 1 : 0;
+    SV_REQUIRE(xstride < (1LL << 31), "sv_conv_fwd: a sample's input must hold fewer than 2^31 values");
+    // engine: the matrix-core kernel when chosen (tuning knob) or, by
+    // default, for 4 to 16 output channels; the vector kernel otherwise or
+    // when the shape does not fit the matrix-core kernel.  Both sum every
+    // output in the same order with single-rounding multiply-adds, so they
+    // agree bit for bit.
     {
-        // LDS-tiled path: the input halo tile of one channel (all KD planes),
-        // double-buffered.  Tile width: the widest of 64, 32, 16 columns
-        // that wastes at most 1/8 of the row on padding (or the tuning
-        // knob's); rows per thread: 4 unless that leaves the GPU with fewer
-        // than ~2 blocks per CU, then 1 (or the knob's).
+        const int eng = sv::tune(SV_TUNE_CONV_ENGINE);
+        if (eng >= 2 || (eng == 0 && OC >= 4 && OC <= 16)) {
+            const int rc = sv::conv_fwd_m4(x, xr, idims, xstride, IC, D, KD, w, b, OC, s, p, dil, y,
+                                           odims, ystride, N, max_ho, max_wo, mode, slope, resid,
+                                           rstride, st);
+            if (rc != SV_EUNSUPPORTED) return rc;
+        }
+    }
+    {
+        // LDS-tiled vector path.  Tile width: the widest of 64, 32, 16
+        // columns that wastes at most 1/8 of the row on padding (or the
+        // tuning knob's); rows per thread: 4 unless that leaves the GPU with
+        // fewer than ~2 tiles per CU, then 1 (or the knob's); weights from
+        // LDS (or the knob's source).  Input channels per staging pass: as
+        // many as fit a 40 KB LDS budget.
         int TWc = sv::tune(SV_TUNE_CONV_TW);
         if (!TWc) {
             TWc = 16;
@@ -1024,21 +1020,22 @@ int sv_conv_fwd(const float* x, const float* xr, const int* idims, long long xst
             }
         }
         const int TYc = 256 / TWc;
-        const long long blocks4 = (long long)((max_wo + TWc - 1) / TWc) * ((max_ho + 4 * TYc - 1) / (4 * TYc)) * D * N;
+        const long long ub4 = (long long)((max_wo + TWc - 1) / TWc) * ((max_ho + 4 * TYc - 1) / (4 * TYc)) * D * N;
         const int tuned = sv::tune(SV_TUNE_CONV_ROWS);
-        const int RP = tuned ? tuned : (blocks4 >= 512 ? 4 : 1);
+        const int RP = tuned ? tuned : (ub4 >= 2LL * sv::cu_count() ? 4 : 1);
         const int WSc = sv::tune(SV_TUNE_CONV_WSRC) ? sv::tune(SV_TUNE_CONV_WSRC) : 1;
         const int TT_H = TYc * RP;
-        const int IHt = (TT_H - 1) * s + 2 * dil + 1, IWt = (TWc - 1) * s + 2 * dil + 1;
         const int ocp = OC == 1 ? 4 : (OC <= 4 ? 4 : (OC <= 8 ? 8 : (OC <= 12 ? 12 : 16)));
-        const size_t shm = 2 * ((size_t)KD * IHt * IWt + (size_t)KD * 9 * ocp) * sizeof(float);
-        if (shm <= 64 * 1024) {
-            const int tYt = (max_ho + TT_H - 1) / TT_H;
-            dim3 g((max_wo + TWc - 1) / TWc, tYt * D, N), bk(TWc, TYc);
-#define LAUNCH_TWS(KD_, SRC_, OCB_, RP_, TW_, WS_)                                               \
-    hipLaunchKernelGGL((k_conv_tile<KD_, SRC_, OCB_, RP_, TW_, WS_>), g, bk, shm, st, x, xr,       \
-                       idims, xstride, IC, D, w, b, OC, s, p, dil, y, odims, ystride, tYt, mode,  \
-                       slope, resid, rstride, IHt, IWt)
+        sv::ConvTiles ct;
+        const long long shm = sv::conv_tiles(KD, IC, TWc, TT_H, s, dil, N,
+                                             WSc == 1 ? (long long)KD * 9 * ocp : 0, 40 * 1024, ct);
+        if (shm > 0) {
+            const long long ub = (long long)((max_wo + TWc - 1) / TWc) * ((max_ho + TT_H - 1) / TT_H) * D * N;
+            dim3 g(sv::conv_grid(ub, shm)), bk(TWc, TYc);
+#define LAUNCH_TWS(KD_, SRC_, OCB_, RP_, TW_, WS_)                                              \
+    hipLaunchKernelGGL((k_conv_tile<KD_, SRC_, OCB_, RP_, TW_, WS_>), g, bk, (size_t)shm, st, x, \
+                       xr, idims, xstride, IC, D, w, b, OC, s, p, dil, y, odims, ystride, N,    \
+                       mode, slope, resid, rstride, ct)
 #define LAUNCH_TW(KD_, SRC_, OCB_, RP_, TW_)                                                      \
     do {                                                                                         \
         if (WSc == 1) LAUNCH_TWS(KD_, SRC_, OCB_, RP_, TW_, 1);                                  \

@@ -298,3 +298,99 @@ def test_fp32_matrix_core_engine_matches_vector():
         net.conv_engine = "vector"
     for a, b in zip(outs_v, outs_m):
         assert float((a - b).abs().max()) < 1e-3
+
+
+_CONV_CASES = [
+    # IC, OC, KD, SRC, stride, dilation, depth, mode
+    (1, 12, 1, 0, 2, 1, 1, 1),
+    (12, 12, 1, 0, 2, 1, 1, 1),
+    (12, 12, 1, 0, 1, 1, 1, 0),
+    (2, 8, 1, 0, 1, 1, 1, 1),
+    (8, 8, 1, 0, 1, 4, 1, 1),
+    (8, 1, 1, 0, 1, 1, 1, 2),
+    (12, 6, 3, 1, 1, 1, 5, 1),
+    (6, 6, 3, 0, 1, 1, 5, 1),
+    (6, 1, 3, 0, 1, 1, 5, 0),
+    (5, 16, 1, 0, 1, 2, 1, 1),
+    (3, 13, 3, 0, 1, 1, 4, 0),
+]
+
+
+@pytest.mark.parametrize("case", _CONV_CASES, ids=lambda c: "ic%d_oc%d_kd%d_src%d_s%d_d%d" % c[:6])
+def test_conv_variants_bit_identical_and_match_float64(sv, N, case):
+    """sv_conv_fwd on a ragged batch of 3 samples under every kernel variant
+    the autotuner may pick (vector ALUs: rows per thread x tile width x
+    weight source; matrix cores: pixel groups per wave x tile width): every
+    variant's output is identical, and it equals the float64 convolution
+    (torch conv2d/conv3d; cost volume |fl(x) - fr(x-d)|, 2.0 where x < d)
+    to 2e-4 on unit-scale data."""
+    import importlib
+    import torch.nn.functional as F
+    A = importlib.import_module("from-scratch-stereo-vision_amd.autotune")
+    L = importlib.import_module("from-scratch-stereo-vision_amd._lib")
+    IC, OC, KD, SRC, s, dil, D, mode = case
+    dev = torch.device("cuda")
+    g = torch.Generator().manual_seed(sum(case))
+    ins = [(37, 50), (20, 71), (9, 13)]
+    outs = [((h - 1) // s + 1, (w - 1) // s + 1) for h, w in ins]
+    Nn = len(ins)
+    islot, oslot = max(h * w for h, w in ins), max(h * w for h, w in outs)
+    xd = 1 if SRC else D
+    xstride = IC * xd * islot
+    x = torch.rand(Nn, xstride, generator=g) - 0.5
+    xr = torch.rand(Nn, xstride, generator=g) - 0.5 if SRC else None
+    w = (torch.rand(OC * IC * KD * 9, generator=g) - 0.5) * 0.4
+    b = torch.rand(OC, generator=g) - 0.5
+    rstride = 2 * oslot
+    resid = torch.rand(Nn, rstride, generator=g) - 0.5 if mode == 2 else None
+    want = []
+    for n, ((h, wd), (ho, wo)) in enumerate(zip(ins, outs)):
+        xn = x[n, :IC * xd * h * wd].double().view(IC, xd, h, wd)
+        if SRC:
+            fr = xr[n, :IC * h * wd].double().view(IC, h, wd)
+            cv = torch.full((IC, D, h, wd), 2.0, dtype=torch.float64)
+            for d in range(D):
+                cv[:, d, :, d:] = (xn[:, 0, :, d:] - fr[:, :, :wd - d]).abs()
+            xn = cv
+        wt = w.double()
+        if KD == 1:
+            o = F.conv2d(xn[:, 0][None], wt.view(OC, IC, 3, 3), b.double(), stride=s, padding=dil,
+                         dilation=dil)[0][:, None]
+        else:
+            o = F.conv3d(xn[None], wt.view(OC, IC, 3, 3, 3), b.double(), stride=(1, s, s),
+                         padding=(1, dil, dil), dilation=(1, dil, dil))[0]
+        if mode == 1:
+            o = torch.where(o > 0, o, o * N.SLOPE)
+        elif mode == 2:
+            o = (o + resid[n, :ho * wo].double().view(1, 1, ho, wo)).clamp_min(0.0)
+        want.append(o)
+    xg, bg, wg = x.to(dev), b.to(dev), w.to(dev)
+    xrg = xr.to(dev) if SRC else None
+    rg = resid.to(dev) if mode == 2 else None
+    idims = torch.tensor([v for d in ins for v in d], dtype=torch.int32, device=dev)
+    odims = torch.tensor([v for d in outs for v in d], dtype=torch.int32, device=dev)
+    ystride = OC * D * oslot
+    y = torch.zeros(Nn, ystride, device=dev)
+    first = None
+    try:
+        for variant in A._conv_variants():
+            A.reset()
+            for knob, v in variant:
+                L.check(L.lib().sv_tune_set(knob, v))
+            y.zero_()
+            N._call("sv_conv_fwd", N._p(xg), N._p(xrg), N._p(idims), xstride, IC, D, KD, N._p(wg),
+                    N._p(bg), OC, s, dil, dil, N._p(y), N._p(odims), ystride, Nn,
+                    max(h for h, _ in outs), max(w_ for _, w_ in outs), mode, N.SLOPE, N._p(rg),
+                    rstride, L.stream())
+            torch.cuda.synchronize()
+            got = [y[n, :OC * D * ho * wo].view(OC, D, ho, wo).cpu() for n, (ho, wo) in enumerate(outs)]
+            if first is None:
+                first = got
+                err = max(float((a.double() - e).abs().max()) for a, e in zip(got, want))
+                print("max |fp32 - fp64| = %.2e" % err)
+                assert err <= 2e-4
+            else:
+                for a, c in zip(got, first):
+                    assert torch.equal(a, c), "variant %s differs" % (variant,)
+    finally:
+        A.reset()
