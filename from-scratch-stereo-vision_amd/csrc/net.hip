@@ -8,8 +8,9 @@
 // Ragged batches: every kernel takes a per-sample dims table (int h, w per
 // sample) and a per-sample slot stride; sample n of a tensor lives at
 // ptr + n*stride, laid out densely as [C][D][h_n][w_n].  Grids are sized for
-// the largest sample and blocks outside a sample's extent exit, so pairs run
-// at different working resolutions share one launch per layer.
+// the largest sample and blocks outside a sample's extent exit (the tiled
+// forward convolutions instead walk a list of the samples' own tiles), so
+// pairs run at different working resolutions share one launch per layer.
 //
 // Determinism: backward passes are gathers or fixed-order two-stage
 // reductions; no floating-point atomics.

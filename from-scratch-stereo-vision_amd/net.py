@@ -53,8 +53,6 @@ L.extra_signatures({
     "sv_conv_mfma_pack_bytes": (_I, [_I, _I, _I, ctypes.POINTER(_SZ)]),
     "sv_conv_fwd_mfma": (_I, [_P, _P, _P, _LL, _I, _I, _I, _P, _P, _I, _I, _I, _I, _P, _P, _LL, _I,
                               _LL, _I, _F, _P, _LL, _P]),
-    "sv_conv_fwd_mf32": (_I, [_P, _P, _P, _LL, _I, _I, _I, _P, _P, _I, _I, _I, _I, _P, _P, _LL, _I, _I,
-                              _I, _I, _F, _P, _LL, _P]),
 })
 
 
@@ -83,7 +81,6 @@ LEVEL_WEIGHTS = [0.5, 0.7, 1.0]   # stereonet.py:1615
 CE_WEIGHT = 2.0                   # stereonet.py:1616
 SLOPE = 0.2                       # LeakyReLU slope, stereonet.py:701
 INVALID_PENALTY = 8.0             # CostVolume.invalid_mask, stereonet.py:827
-MFMA_MIN_OC = 12                  # narrower layers pad the 16-wide MFMA N tile too much
 
 
 def _co(n, k=3, s=1, p=1, d=1):
@@ -160,11 +157,6 @@ class StereoNet:
         # ((knob, value), ...); and a recorder of conv launches it uses
         self._conv_tune = {}
         self._conv_hook = None
-        # FP32 convolutions: "vector" runs every layer on the LDS-tiled
-        # vector kernel; "mfma" runs the layers with MFMA_MIN_OC or more
-        # output channels on the matrix cores (FP32 products and sums),
-        # measured slower on this network's 6-12-channel layers
-        self.conv_engine = "vector"
 
     # -- precision -------------------------------------------------------
     def use_bf16(self, on=True):
@@ -452,8 +444,6 @@ class _Plan:
                   L.stream())
             return
         fn = "sv_conv_fwd"
-        if self.net.conv_engine == "mfma" and MFMA_MIN_OC <= c.oc <= 48:
-            fn = "sv_conv_fwd_mf32"
         args = (_p(x), _p(xr), _p(xlv.table), xst, c.ic, D, c.kd, _p(P[c.w_off:]), _p(P[c.b_off:]),
                 c.oc, c.s, c.p, c.dil, _p(y), _p(ylv.table), c.oc * D * ylv.slot, N, ylv.maxh,
                 ylv.maxw, mode, SLOPE, _p(resid), rstride, L.stream())

@@ -271,35 +271,6 @@ def test_select_kth_matches_sort(sv, N):
     assert np.array_equal(got, want)
 
 
-def test_fp32_matrix_core_engine_matches_vector():
-    """The FP32 MFMA convolution engine (v_mfma_f32_16x16x4_f32) against the
-    vector engine on the v5 forward: same FP32 arithmetic up to summation
-    order, so the full-resolution disparity agrees to 1e-3 px."""
-    import importlib
-    N = importlib.import_module("from-scratch-stereo-vision_amd.net")
-    net = N.load_model(GOLDEN + "/stereonet_weights_v5.txt")
-    g = torch.Generator().manual_seed(3)
-    left = torch.rand(3, 96, 128, generator=g).cuda()
-    right = left.roll(5, 2)
-    dims = [(96, 128), (80, 120), (64, 96)]
-    ls = [left[i, :h, :w].contiguous() for i, (h, w) in enumerate(dims)]
-    rs = [right[i, :h, :w].contiguous() for i, (h, w) in enumerate(dims)]
-    net.conv_engine = "vector"
-    pv = net.forward_batch(ls, rs, dims)
-    outs_v = [pv.out(l, n).clone() for l in range(3) for n in range(3)]
-    net.conv_engine = "mfma"
-    old = N.MFMA_MIN_OC
-    N.MFMA_MIN_OC = 1
-    try:
-        pm = net.forward_batch(ls, rs, dims)
-        outs_m = [pm.out(l, n).clone() for l in range(3) for n in range(3)]
-    finally:
-        N.MFMA_MIN_OC = old
-        net.conv_engine = "vector"
-    for a, b in zip(outs_v, outs_m):
-        assert float((a - b).abs().max()) < 1e-3
-
-
 _CONV_CASES = [
     # IC, OC, KD, SRC, stride, dilation, depth, mode
     (1, 12, 1, 0, 2, 1, 1, 1),
