@@ -64,6 +64,13 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the run, copy 256 MiB device-to-device once: a kernel of known "
                          "traffic to scale rocprofv3 FETCH_SIZE/WRITE_SIZE readings")
+    ap.add_argument("--tune-file", default=None,
+                    help="JSON of kernel variant choices: loaded when it exists (no autotuning), "
+                         "else written after autotuning")
+    ap.add_argument("--eager", action="store_true",
+                    help="time the step as individual launches only (default: also capture it "
+                         "in one HIP graph, check the replay equals the eager output, and report "
+                         "the replay rate)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's built-in kernel variant heuristics")
     ap.add_argument("--tune", default=None,
@@ -105,29 +112,37 @@ def main():
 
     t_net = [0.0]
 
-    def step():
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record()
+    def step(events=True):
+        if events:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
         res = N.predict_auto_batch(net, L8, R8, W, H)
         N._call("sv_widen", N._p(res.full), N._p(lv.table), H * W, N._p(d0), H * W, None, B,
                 H * W, L.stream())
-        e1.record()
+        if events:
+            e1.record()
         out, st = sv.refine_disparity(L8, R8, None, d0, W, H, planes=False)
-        return out, st, res, (e0, e1)
+        return out, st, res, ((e0, e1) if events else None)
 
     tuned = None
     A = importlib.import_module("from-scratch-stereo-vision_amd.autotune")
     if args.tune:
         tuned = {k: int(v) for k, v in (kv.split("=") for kv in args.tune.split(","))}
         A.apply(tuned)
+    elif args.tune_file and os.path.exists(args.tune_file):
+        tuned = A.load_choices(net, args.tune_file)
+        log("kernel variants from %s" % args.tune_file)
     elif not args.no_autotune:
         # per-layer convolution variants first, then the global knobs of the
         # rest (every variant leaves the results unchanged)
         layers = A.tune_layers(net, lambda: step()[:2], reps=2)
-        tuned = A.tune(lambda: step()[:2], knobs=["sgm_prefetch", "sgm_htile", "sgm_overlap"],
-                       reps=2, log=log)
+        tuned = A.tune(lambda: step()[:2],
+                       knobs=["sgm_split", "sgm_prefetch", "sgm_htile", "sgm_overlap"], reps=2,
+                       log=log)
         tuned["conv_layers_tuned"] = len(layers)
+        if args.tune_file:
+            A.save_choices(net, tuned, args.tune_file)
     log("warmup %d" % args.warmup)
     for _ in range(args.warmup):
         step()
@@ -150,7 +165,28 @@ def main():
         stages[name] = (ms.value, cnt.value)
     L.lib().sv_profile_enable(0)
     net_ms = sum(a.elapsed_time(b) for a, b in net_events)
-    pairs_s = B * args.steps / dt
+    eager_pairs_s = B * args.steps / dt
+    pairs_s, execution = eager_pairs_s, "eager"
+    if not args.eager:
+        # the same step replayed as one HIP graph (every launch recorded
+        # once, the kernel variants chosen above baked in); its output must
+        # equal the eager step's
+        R = importlib.import_module("from-scratch-stereo-vision_amd.ragged")
+        g = R.CapturedPipeline(lambda a, b: step(events=False)[:2], L8, R8, warmup=1)
+        gout, gst = g.replay()
+        torch.cuda.synchronize()
+        if not (torch.equal(gout, out) and torch.equal(gst, st)):
+            raise RuntimeError("graph replay differs from the eager step")
+        for _ in range(args.warmup):
+            g.replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            g.replay()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        pairs_s, execution = B * args.steps / dt, "hip graph"
+        log("graph replay: %.1f pairs/s (eager %.1f)" % (pairs_s, eager_pairs_s))
 
     # accuracy of net + band refinement
     pred = out.reshape(B, -1).cpu().numpy()
@@ -210,6 +246,8 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": round(SGM_TRAFFIC_PER_PAIR * B),
                 "traffic_GBs": round(SGM_TRAFFIC_PER_PAIR * B / avg_s / 1e9, 1) if avg_s > 0 else None,
+                "traffic_frac": (round(SGM_TRAFFIC_PER_PAIR * B / avg_s / 1e9 / HBM_PEAK_GBS, 4)
+                                 if avg_s > 0 else None),
                 "avg_launch_ms": round(avg_s * 1e3, 4),
                 "algorithmic_bytes_per_launch": bytes_launch}
 
@@ -240,7 +278,8 @@ def main():
         "dtype": "f32 network / int SGM / f64 epilogue", "data": "synthetic",
         "config": {"workload": "net+band-SGM 460x370: predict_auto (v5 StereoNet, FP32) + "
                                "refine_disparity(planes=False), R=10 census 9x7",
-                   "pairs_per_step": B, "parallelism": "single GPU"},
+                   "pairs_per_step": B, "parallelism": "single GPU", "execution": execution},
+        "eager_pairs_per_s": round(eager_pairs_s, 2),
         "accuracy": {"bad2_pct": round(bad2, 3), "epe_px": round(epe, 4),
                      "network_only_bad2_pct": round(net_bad2, 3)},
         "fullrange": fullrange,
@@ -427,7 +466,7 @@ def ragged_main(args):
     tuned = None
     if not args.no_autotune:
         layers = A.tune_layers(net, run, reps=2)
-        tuned = A.tune(run, knobs=["sgm_prefetch", "sgm_htile", "sgm_overlap"], reps=2, log=log)
+        tuned = A.tune(run, knobs=["sgm_split", "sgm_prefetch", "sgm_htile", "sgm_overlap"], reps=2, log=log)
         tuned["conv_layers_tuned"] = len(layers)
     # eager reference run (stage profile of the refinement) and capture
     L.lib().sv_profile_enable(1)

@@ -32,7 +32,8 @@ KNOBS = {"sgm_prefetch": (0, (2, 4, 8)),
          "conv_weights": (4, (1, 2)),
          "sgm_overlap": (5, (2, 1)),
          "conv_engine": (6, (1, 2, 3)),
-         "conv_mgroups": (7, (1, 2, 4))}
+         "conv_mgroups": (7, (1, 2, 4)),
+         "sgm_split": (8, (1, 2))}
 
 
 def set_knob(name, value):
@@ -200,3 +201,25 @@ def tune_layers(net, run, reps=3, log=None):
     finally:
         reset()
     return chosen
+
+
+def save_choices(net, knobs, path):
+    """Write the process-wide knob choices and net's per-layer convolution
+    variants (tune_layers) to a JSON file."""
+    data = {"knobs": {k: int(v) for k, v in knobs.items() if k in KNOBS},
+            "layers": [[list(key), [list(kv) for kv in variant]]
+                       for key, variant in net._conv_tune.items()]}
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    with open(path, "w") as f:
+        json.dump(data, f, indent=1)
+
+
+def load_choices(net, path):
+    """Apply choices written by save_choices; returns the knob settings
+    plus the number of layers with a variant."""
+    with open(path) as f:
+        data = json.load(f)
+    apply(data["knobs"])
+    net._conv_tune = {tuple(key): tuple(tuple(kv) for kv in variant)
+                      for key, variant in data["layers"]}
+    return dict(data["knobs"], conv_layers_tuned=len(net._conv_tune))

@@ -875,6 +875,285 @@ __global__ void __launch_bounds__(64) k_sgm_ph(const uint8_t* __restrict__ C,
     }
 }
 
+// ------------------------------------------------ two lanes per scanline
+// The same recurrence with each scanline's band split over a lane pair
+// (lanes 2l, 2l+1): lane h holds u16 pairs [h*NPH, (h+1)*NPH) of the band
+// (entries >= K are pads held at the sentinel), reads and writes its NW
+// words of each pixel's costs and path costs, and the pair shares one LDS
+// slot for the neighbour minima M.  Per step the pair swaps the pairs at
+// the split (one DPP lane swap) for the M of the entries beside it and the
+// two half-band minima (another swap); the re-centring read M[k+s] spans
+// both halves of the shared slot, written by the pair in the previous step
+// (one wave's LDS accesses complete in issue order).  Pads have both
+// neighbours at the sentinel, so their M uses a saturating +P1 and stays at
+// the sentinel; every other value is far below it, so the results equal
+// packed_step's.  A wave covers 32 scanlines, so a launch has twice the
+// waves of the one-lane kernels and each lane half the work per step.
+template <int K>
+struct PackedBand2 {
+    static constexpr int NP = (K + 1) / 2;
+    static constexpr int NPH = (((NP + 1) / 2) + 1) & ~1;  // pairs per lane, even
+    static constexpr int NW = NPH / 2;                      // cost words per lane
+    static constexpr int OFF = K + 3;                       // u16 index of M[0]
+    static constexpr int SLOT16 = OFF + (K + 2) + 4 * NPH + 2;
+    static constexpr int SD = ((SLOT16 + 1) / 2) | 1;       // slot dwords per pair, odd
+    // sentinel bits of lane 1's pair j (entries 2*NPH + 2j, +1 at or past K)
+    static constexpr uint32_t pad1(int j) {
+        return (2 * NPH + 2 * j >= K ? 0x0000FFFFu : 0u) | (2 * NPH + 2 * j + 1 >= K ? 0xFFFF0000u : 0u);
+    }
+    static_assert(2 * NPH <= K, "lane 0 holds no pads");
+};
+
+__device__ __forceinline__ uint32_t swap_pair(uint32_t v) {
+    // quad_perm [1, 0, 3, 2]: exchange with the other lane of the pair
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);
+}
+
+template <int K>
+__device__ __forceinline__ void packed_step2(u16x2 (&L)[PackedBand2<K>::NPH], uint32_t& mprev,
+                                             int& bprev, u32_alias* sl32, u16_alias* s16,
+                                             const uint32_t (&cw)[PackedBand2<K>::NW], int b,
+                                             bool border, int P1, int P2, int h,
+                                             uint32_t (&out)[PackedBand2<K>::NW]) {
+    using PB = PackedBand2<K>;
+    constexpr int NPH = PB::NPH, NW = PB::NW, OFF = PB::OFF;
+    const u16x2 p1v = {(unsigned short)P1, (unsigned short)P1};
+    u16x2 c[NPH];
+#pragma unroll
+    for (int j = 0; j < NPH; ++j)
+        c[j] = as_u16x2(__builtin_amdgcn_perm(0u, cw[j >> 1], (j & 1) ? 0x0c030c02u : 0x0c010c00u));
+    if (border) {
+#pragma unroll
+        for (int j = 0; j < NPH; ++j) L[j] = c[j];
+    } else {
+        const int sh = min(max(b - bprev, -(K + 2)), K + 2);
+        const uint32_t cap = mprev + (uint32_t)P2;
+        const u16x2 capv = {(unsigned short)cap, (unsigned short)cap};
+        const u16x2 mpv = {(unsigned short)mprev, (unsigned short)mprev};
+        const u16_alias* gm = s16 + OFF + sh + 2 * NPH * h;
+#pragma unroll
+        for (int j = 0; j < NPH; ++j) {
+            u16x2 m2;
+            m2.x = gm[2 * j];
+            m2.y = gm[2 * j + 1];
+            const u16x2 best = __builtin_elementwise_min(m2, capv);
+            L[j] = c[j] + best - mpv;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NPH; ++j)
+        if (PB::pad1(j)) L[j] = as_u16x2(as_u32(L[j]) | (h ? PB::pad1(j) : 0u));
+    u16x2 mm = L[0];
+#pragma unroll
+    for (int j = 1; j < NPH; ++j) mm = __builtin_elementwise_min(mm, L[j]);
+    uint32_t m = min((uint32_t)mm.x, (uint32_t)mm.y);
+    m = min(m, swap_pair(m));
+    if (h == 0) s16[OFF - 1] = (uint16_t)(L[0].x + P1);
+    const uint32_t recv = swap_pair(as_u32(h ? L[0] : L[NPH - 1]));
+#pragma unroll
+    for (int j = 0; j < NPH; ++j) {
+        const uint32_t lo = j > 0 ? as_u32(L[j - 1]) : (h ? recv : 0xFFFFFFFFu);
+        const uint32_t hi = j + 1 < NPH ? as_u32(L[j + 1]) : (h ? 0xFFFFFFFFu : recv);
+        const u16x2 left = as_u16x2(__builtin_amdgcn_alignbit(as_u32(L[j]), lo, 16));
+        const u16x2 right = as_u16x2(__builtin_amdgcn_alignbit(hi, as_u32(L[j]), 16));
+        const u16x2 nb = __builtin_elementwise_add_sat(__builtin_elementwise_min(left, right), p1v);
+        sl32[OFF / 2 + NPH * h + j] = as_u32(__builtin_elementwise_min(L[j], nb));
+    }
+    // path costs: low bytes of the pairs, pads 0
+#pragma unroll
+    for (int q = 0; q < NW; ++q) {
+        uint32_t lo = as_u32(L[2 * q]), hi = as_u32(L[2 * q + 1]);
+        if (PB::pad1(2 * q)) lo &= h ? ~PB::pad1(2 * q) : 0xFFFFFFFFu;
+        if (PB::pad1(2 * q + 1)) hi &= h ? ~PB::pad1(2 * q + 1) : 0xFFFFFFFFu;
+        out[q] = __builtin_amdgcn_perm(hi, lo, 0x06040200u);
+    }
+    mprev = m;
+    int bc = b;
+    asm volatile("" : "+v"(bc));
+    bprev = bc;
+}
+
+// k_sgm_p with two lanes per scanline (32 lines per wave).
+template <int K, int PF>
+__global__ void __launch_bounds__(64) k_sgm_p2(const uint8_t* __restrict__ C,
+                                               const int* __restrict__ base,
+                                               uint8_t* __restrict__ Lout, Geom g, int Kp,
+                                               int P1, int P2, int dir0, long long dirStride) {
+    using PB = PackedBand2<K>;
+    constexpr int NPH = PB::NPH, NW = PB::NW, SD = PB::SD;
+    __shared__ uint32_t lds[32 * SD];
+    const int W = g.w(blockIdx.z), H = g.h(blockIdx.z);
+    const int dir = dir0 + (int)blockIdx.y;
+    LinePos lp;
+    lp.dx = c_dx[dir];
+    lp.dy = c_dy[dir];
+    lp.W = W;
+    lp.H = H;
+    lp.horiz = lp.dy == 0;
+    const int lane = threadIdx.x, h = lane & 1, ln = lane >> 1;
+    const int nLines = lp.horiz ? H : W;
+    const int len = lp.horiz ? W : H;
+    lp.line = blockIdx.x * 32 + ln;
+    if (lp.line >= nLines) return;  // both lanes of a pair leave together
+    u32_alias* sl32 = reinterpret_cast<u32_alias*>(lds + ln * SD);
+    u16_alias* s16 = reinterpret_cast<u16_alias*>(sl32);
+    for (int i = h; i < SD; i += 2) sl32[i] = 0xFFFFFFFFu;
+    const long long plane = g.slot;
+    const uint8_t* Cb = C + (long long)blockIdx.z * plane * Kp + 4 * NW * h;
+    const int* Bb = base + (long long)blockIdx.z * plane;
+    uint8_t* Lb = Lout + (long long)blockIdx.y * dirStride + (long long)blockIdx.z * plane * Kp + 4 * NW * h;
+    uint32_t ring[PF][NW];
+    int rb[PF];
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+        bool bd;
+        const long long pq = lp.at(u < len ? u : 0, bd);
+#pragma unroll
+        for (int q = 0; q < NW; ++q) ring[u][q] = *reinterpret_cast<const uint32_t*>(Cb + pq * Kp + 4 * q);
+        rb[u] = Bb[pq];
+    }
+    u16x2 L[NPH];
+    uint32_t mprev = 0;
+    int bprev = 0;
+    auto step = [&](int i, const uint32_t (&cr)[NW], int b) {
+        bool border;
+        const long long pix = lp.at(i, border);
+        uint32_t cw[NW];
+#pragma unroll
+        for (int q = 0; q < NW; ++q) cw[q] = cr[q];
+        uint32_t o[NW];
+        packed_step2<K>(L, mprev, bprev, sl32, s16, cw, b, border, P1, P2, h, o);
+#pragma unroll
+        for (int q = 0; q < NW; ++q) *reinterpret_cast<uint32_t*>(Lb + pix * Kp + 4 * q) = o[q];
+    };
+    auto refill = [&](int u, int inext) {
+        bool bd;
+        const long long pq = lp.at(inext, bd);
+#pragma unroll
+        for (int q = 0; q < NW; ++q) ring[u][q] = *reinterpret_cast<const uint32_t*>(Cb + pq * Kp + 4 * q);
+        rb[u] = Bb[pq];
+    };
+    int i0 = 0;
+    for (; i0 + PF <= len; i0 += PF) {
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            step(i0 + u, ring[u], rb[u]);
+            refill(u, min(i0 + u + PF, len - 1));
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < PF; ++u)
+        if (i0 + u < len) step(i0 + u, ring[u], rb[u]);
+}
+
+// k_sgm_ph with two lanes per row: 32 rows per wave, the row tiles staged
+// through LDS as in k_sgm_ph.
+template <int K, int TC>
+__global__ void __launch_bounds__(64) k_sgm_ph2(const uint8_t* __restrict__ C,
+                                                const int* __restrict__ base,
+                                                uint8_t* __restrict__ Lout, Geom g, int Kp,
+                                                int P1, int P2, int dir0, long long dirStride) {
+    using PB = PackedBand2<K>;
+    using HT = HorizTile<K, TC>;
+    constexpr int ROWS = 32;
+    const int W = g.w(blockIdx.z), H = g.h(blockIdx.z);
+    if ((int)blockIdx.x * ROWS >= H) return;  // whole block below this image
+    constexpr int NPH = PB::NPH, NW = PB::NW, SD = PB::SD;
+    constexpr int NB = PackedBand<K>::NB;
+    constexpr int T = HT::T, RS = HT::RS, BS = HT::BS;
+    static_assert(RS % 2 == 0, "8-byte aligned rows");
+    static_assert(4 * NW * 2 <= 8 * NB, "a lane pair's words fit the pixel's chunks");
+    __shared__ __align__(16) uint32_t tile[ROWS * RS];
+    __shared__ int btile[ROWS * BS];
+    __shared__ uint32_t lds[ROWS * SD];
+    const int dx = c_dx[dir0 + (int)blockIdx.y];
+    const int lane = threadIdx.x, h = lane & 1, ln = lane >> 1;
+    const int y0 = blockIdx.x * ROWS;
+    const int rows = min(ROWS, H - y0);
+    u32_alias* sl32 = reinterpret_cast<u32_alias*>(lds + ln * SD);
+    u16_alias* s16 = reinterpret_cast<u16_alias*>(sl32);
+    for (int i = h; i < SD; i += 2) sl32[i] = 0xFFFFFFFFu;
+    const long long plane = g.slot;
+    const uint8_t* Cb = C + (long long)blockIdx.z * plane * Kp + (long long)y0 * W * Kp;
+    const int* Bb = base + (long long)blockIdx.z * plane + (long long)y0 * W;
+    uint8_t* Lb = Lout + (long long)blockIdx.y * dirStride + (long long)blockIdx.z * plane * Kp +
+                  (long long)y0 * W * Kp;
+    const long long rowBytes = (long long)W * Kp;
+    u16x2 L[NPH];
+    uint32_t mprev = 0;
+    int bprev = 0;
+    // this lane's words of a pixel: u32 offset 2*NB*col + NW*h in its row
+    const uint32_t* mine = tile + ln * RS + NW * h;
+    uint32_t* mine_w = tile + ln * RS + NW * h;
+    const int* bmine = btile + ln * BS;
+    constexpr int CU = T * NB;          // 8-byte cost units of a full row run
+    constexpr int TOT = ROWS * CU;      // units of a full tile
+    constexpr int TB = ROWS * T;        // band centres of a full tile
+    for (int c0 = 0; c0 < W; c0 += T) {
+        const int n = min(T, W - c0);
+        const int xlo = dx > 0 ? c0 : W - c0 - n;
+        const int units = n * NB;
+        const uint8_t* Cx = Cb + (long long)xlo * Kp;
+        const int* Bx = Bb + xlo;
+        // staging over the (row, unit) grid, clamped onto the last valid
+        // element (a duplicate load and an identical LDS write)
+#pragma unroll 1
+        for (int it = 0; it < TOT; it += 64 * 8) {
+            uint2 v[8];
+            int off[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int idx = it + u * 64 + lane;
+                const int r = min(idx / CU, rows - 1);
+                const int q = min(idx % CU, units - 1);
+                off[u] = r * RS + 2 * q;
+                v[u] = *reinterpret_cast<const uint2*>(Cx + r * rowBytes + 8 * q);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) *reinterpret_cast<uint2*>(tile + off[u]) = v[u];
+        }
+#pragma unroll
+        for (int it = 0; it < TB; it += 64 * 4) {
+            int v[4], off[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int idx = it + u * 64 + lane;
+                const int r = min(idx / T, rows - 1);
+                const int c = min(idx % T, n - 1);
+                off[u] = r * BS + c;
+                v[u] = Bx[(long long)r * W + c];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) btile[off[u]] = v[u];
+        }
+        __syncthreads();
+        for (int j = 0; j < n; ++j) {
+            const int col = dx > 0 ? j : n - 1 - j;
+            uint32_t cw[NW];
+#pragma unroll
+            for (int q = 0; q < NW; ++q) cw[q] = mine[col * 2 * NB + q];
+            uint32_t o[NW];
+            packed_step2<K>(L, mprev, bprev, sl32, s16, cw, bmine[col], c0 + j == 0, P1, P2, h, o);
+#pragma unroll
+            for (int q = 0; q < NW; ++q) mine_w[col * 2 * NB + q] = o[q];
+        }
+        __syncthreads();
+        uint8_t* Lx = Lb + (long long)xlo * Kp;
+#pragma unroll 1
+        for (int it = 0; it < TOT; it += 64 * 8) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int idx = it + u * 64 + lane;
+                const int r = min(idx / CU, rows - 1);
+                const int q = min(idx % CU, units - 1);
+                *reinterpret_cast<uint2*>(Lx + r * rowBytes + 8 * q) =
+                    *reinterpret_cast<const uint2*>(tile + r * RS + 2 * q);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------------- WTA
 // refine_fullres epilogue, stereonet.py:5271-5302: path sums S, first
 // minimum, band-edge passthrough, second best outside bk+-1, confidence,
@@ -1654,7 +1933,16 @@ int sgm_k_t(const Plan& pl, const void* C, const int* base, void* L, int BV, con
             if (e != hipSuccess) return sv::set_error(SV_EHIP, "sgm fork: %s", hipGetErrorString(e));
             hst = ss->s;
         }
-        if (hEnd > dir0) {
+        const bool split = sv::tune(SV_TUNE_SGM_SPLIT) == 2;
+        if (hEnd > dir0 && split) {
+            dim3 hgrid((g.H + 31) / 32, hEnd - dir0, BV);
+            if (sv::tune(SV_TUNE_SGM_HTILE) == 8)
+                hipLaunchKernelGGL((k_sgm_ph2<K, 8>), hgrid, dim3(64), 0, hst, (const uint8_t*)C,
+                                   base, (uint8_t*)L, g, pl.Kp, (int)P1, (int)P2, dir0, dirStride);
+            else
+                hipLaunchKernelGGL((k_sgm_ph2<K, 16>), hgrid, dim3(64), 0, hst, (const uint8_t*)C,
+                                   base, (uint8_t*)L, g, pl.Kp, (int)P1, (int)P2, dir0, dirStride);
+        } else if (hEnd > dir0) {
             dim3 hgrid((g.H + 63) / 64, hEnd - dir0, BV);
             if (sv::tune(SV_TUNE_SGM_HTILE) == 8)
                 hipLaunchKernelGGL((k_sgm_ph<K, 8>), hgrid, dim3(64), 0, hst, (const uint8_t*)C,
@@ -1664,7 +1952,24 @@ int sgm_k_t(const Plan& pl, const void* C, const int* base, void* L, int BV, con
                                    base, (uint8_t*)L, g, pl.Kp, (int)P1, (int)P2, dir0, dirStride);
         }
         hipError_t lerr = hipGetLastError();
-        if (vEnd > vBeg && lerr == hipSuccess) {
+        if (vEnd > vBeg && lerr == hipSuccess && split) {
+            dim3 vgrid((g.W + 31) / 32, vEnd - vBeg, BV);
+            uint8_t* Lv = (uint8_t*)L + (long long)(vBeg - dir0) * dirStride;
+            switch (sv::tune(SV_TUNE_SGM_PREFETCH)) {
+                case 2:
+                    hipLaunchKernelGGL((k_sgm_p2<K, 2>), vgrid, dim3(64), 0, st, (const uint8_t*)C,
+                                       base, Lv, g, pl.Kp, (int)P1, (int)P2, vBeg, dirStride);
+                    break;
+                case 8:
+                    hipLaunchKernelGGL((k_sgm_p2<K, 8>), vgrid, dim3(64), 0, st, (const uint8_t*)C,
+                                       base, Lv, g, pl.Kp, (int)P1, (int)P2, vBeg, dirStride);
+                    break;
+                default:
+                    hipLaunchKernelGGL((k_sgm_p2<K, 4>), vgrid, dim3(64), 0, st, (const uint8_t*)C,
+                                       base, Lv, g, pl.Kp, (int)P1, (int)P2, vBeg, dirStride);
+            }
+            lerr = hipGetLastError();
+        } else if (vEnd > vBeg && lerr == hipSuccess) {
             dim3 vgrid((g.W + 63) / 64, vEnd - vBeg, BV);
             uint8_t* Lv = (uint8_t*)L + (long long)(vBeg - dir0) * dirStride;
             switch (sv::tune(SV_TUNE_SGM_PREFETCH)) {

@@ -136,7 +136,8 @@ int sv_refine_ragged(int mode, const void* left, const void* right, int dtype, c
  * second, library-owned stream beside them (2); 6: convolution engine (1
  * vector ALUs, 2 matrix cores with 3D layers sliding over depth, 3 matrix
  * cores with one depth per tile); 7: 64-pixel groups per wave of the
- * matrix-core convolution (1, 2, 4).  Value 0 restores the built-in
+ * matrix-core convolution (1, 2, 4); 8: lanes per scanline of the packed
+ * SGM kernels (1, 2).  Value 0 restores the built-in
  * heuristic (LDS weights; horizontal paths after the others; matrix cores
  * for 4 to 16 output channels, sliding over depth when a 3D layer has at
  * least two spatial tiles per CU).
