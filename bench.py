@@ -38,11 +38,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-# HBM bytes the band-SGM stage (k_sgm_ph + k_sgm_p) moves per 460x370 pair,
-# both views: rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 (the factors a
-# 256 MiB device copy reads under the same counters), 1,152.4 MB per 8-pair
-# launch pair.  Used for the roofline's "traffic" field.
-SGM_TRAFFIC_PER_PAIR = 1152.4e6 / 8
+# HBM bytes the band-SGM stage (k_sgm_ph2 + k_sgm_p2, two lanes per
+# scanline) moves per 460x370 pair, both views: rocprofv3 FETCH_SIZE x2.0 +
+# WRITE_SIZE x1.0 (the factors a 256 MiB device copy reads under the same
+# counters), 5,215.6 MB per 32-pair launch pair.  Used for the roofline's
+# "traffic" field.
+SGM_TRAFFIC_PER_PAIR = 5215.6e6 / 32
 
 
 def sgm_bytes_per_pair(views, K, HW):
