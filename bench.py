@@ -38,12 +38,13 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-# HBM bytes the band-SGM stage (k_sgm_ph2 + k_sgm_p2, two lanes per
-# scanline) moves per 460x370 pair, both views: rocprofv3 FETCH_SIZE x2.0 +
-# WRITE_SIZE x1.0 (the factors a 256 MiB device copy reads under the same
-# counters), 5,215.6 MB per 32-pair launch pair.  Used for the roofline's
-# "traffic" field.
-SGM_TRAFFIC_PER_PAIR = 5215.6e6 / 32
+# HBM bytes the band-SGM stage moves per 460x370 pair, both views, by lanes
+# per scanline of its kernels: rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0
+# (the factors a 256 MiB device copy reads under the same counters).  Two
+# lanes (k_sgm_ph2 + k_sgm_p2): 5,215.6 MB per 32-pair launch pair; one lane
+# (k_sgm_ph + k_sgm_p): 1,152.4 MB per 8-pair launch pair.  Used for the
+# roofline's "traffic" field.
+SGM_TRAFFIC_PER_PAIR = {2: 5215.6e6 / 32, 1: 1152.4e6 / 8}
 
 
 def sgm_bytes_per_pair(views, K, HW):
@@ -241,14 +242,15 @@ def main():
     bytes_launch = sgm_bytes_per_pair(2, K, W * H) * B
     avg_s = (sgm_ms / max(args.steps, 1)) / 1e3  # the stage's two kernels, per step
     achieved = bytes_launch / avg_s / 1e9 if avg_s > 0 else 0.0
+    A_ = importlib.import_module("from-scratch-stereo-vision_amd.autotune")
+    traffic = SGM_TRAFFIC_PER_PAIR[2 if A_.get_knob("sgm_split") == 2 else 1] * B
     roofline = {"bound": "hbm",
                 "kernel": "band SGM aggregation: k_sgm_ph + k_sgm_p (8 paths, both views, %d pairs)" % B,
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": round(SGM_TRAFFIC_PER_PAIR * B),
-                "traffic_GBs": round(SGM_TRAFFIC_PER_PAIR * B / avg_s / 1e9, 1) if avg_s > 0 else None,
-                "traffic_frac": (round(SGM_TRAFFIC_PER_PAIR * B / avg_s / 1e9 / HBM_PEAK_GBS, 4)
-                                 if avg_s > 0 else None),
+                "traffic": round(traffic),
+                "traffic_GBs": round(traffic / avg_s / 1e9, 1) if avg_s > 0 else None,
+                "traffic_frac": round(traffic / avg_s / 1e9 / HBM_PEAK_GBS, 4) if avg_s > 0 else None,
                 "avg_launch_ms": round(avg_s * 1e3, 4),
                 "algorithmic_bytes_per_launch": bytes_launch}
 
