@@ -31,7 +31,7 @@ KNOBS = {"sgm_prefetch": (0, (2, 4, 8)),
          "conv_tile_width": (3, (64, 32, 16)),
          "conv_weights": (4, (1, 2)),
          "sgm_overlap": (5, (2, 1)),
-         "conv_engine": (6, (1, 2, 3)),
+         "conv_engine": (6, (1, 2, 3, 4)),
          "conv_mgroups": (7, (1, 2, 4)),
          "sgm_split": (8, (1, 2))}
 
@@ -134,7 +134,8 @@ def tune(run, key=None, knobs=None, reps=3, cache_path=None, log=None):
 
 def _conv_variants():
     """Every convolution variant as ((knob, value), ...): the vector kernel
-    under each rows x tile width x weight source, the matrix-core kernel
+    under each rows x tile width x weight source, its per-channel form under
+    each rows x tile width, the matrix-core kernel
     (3D layers sliding over depth, or one depth per tile) under each pixel
     groups per wave x tile width."""
     K = {k: v[0] for k, v in KNOBS.items()}
@@ -144,6 +145,9 @@ def _conv_variants():
             for w in KNOBS["conv_weights"][1]:
                 out.append(((K["conv_engine"], 1), (K["conv_rows"], r),
                             (K["conv_tile_width"], t), (K["conv_weights"], w)))
+    for r in KNOBS["conv_rows"][1]:
+        for t in KNOBS["conv_tile_width"][1]:
+            out.append(((K["conv_engine"], 4), (K["conv_rows"], r), (K["conv_tile_width"], t)))
     for e in (2, 3):
         for g in KNOBS["conv_mgroups"][1]:
             for t in KNOBS["conv_tile_width"][1]:

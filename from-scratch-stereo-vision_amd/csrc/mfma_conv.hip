@@ -260,13 +260,15 @@ __global__ void __launch_bounds__(256) k_conv_m4(
 }
 
 // Depth-sliding form of k_conv_m4 for 3D convolutions (KD = 3): a block
-// owns a spatial tile of a sample for every output depth.  The input planes
-// of all IC channels live in a ring of three depth slots in LDS; output
-// depth od reads planes od-1, od, od+1, and moving to od+1 stages just
-// plane od+2 (the cost volume |fl - fr| of that plane when SRC == 1) in
-// place of plane od-1, so every input plane is staged once per tile instead
-// of three times.  The weights of every channel are staged once per block.
-// Sums, operand layout and epilogue are those of k_conv_m4.
+// owns a spatial tile of a sample for a chunk of DC consecutive output
+// depths.  The input planes of all IC channels live in a ring of three
+// depth slots in LDS; output depth od reads planes od-1, od, od+1, and
+// moving to od+1 stages just plane od+2 (the cost volume |fl - fr| of that
+// plane when SRC == 1) in place of plane od-1, so a chunk stages DC + 2
+// planes instead of 3*DC.  Tiles are numbered (sample, spatial tile, chunk),
+// chunk fastest; the chunk length trades staging against the number of
+// tiles.  The weights of every channel are staged once per block.  Sums,
+// operand layout and epilogue are those of k_conv_m4.
 template <int SRC>
 __device__ __forceinline__ void stage_plane(float* ring, const float* __restrict__ xs,
                                             const float* __restrict__ xrs, int IC, int D, int dd,
@@ -311,7 +313,7 @@ __global__ void __launch_bounds__(256) k_conv_m4d(
     long long xstride, int IC, int D, const float* __restrict__ w, const float* __restrict__ b,
     int OC, int s, int p, int dil, float* __restrict__ y, const int* __restrict__ odims,
     long long ystride, int N, int mode, float slope, const float* __restrict__ resid,
-    long long rstride, sv::ConvTiles ct) {
+    long long rstride, sv::ConvTiles ct, int DC) {
     constexpr int TAPS = 27;
     constexpr int NJ = 2;
     constexpr int GR = 64 / TW;
@@ -336,11 +338,13 @@ __global__ void __launch_bounds__(256) k_conv_m4d(
         const int tp = 16 * j + (l >> 2), oc = 4 * g + (l & 3);
         wl[e] = (tp < TAPS && oc < OC) ? w[((long long)oc * IC + c) * TAPS + tp] : 0.f;
     }
-    const int total = conv_tile_prefix<TW, TT_H>(odims, N, 1, pre, wsum, tid);
+    const int nch = (D + DC - 1) / DC;
+    const int total = conv_tile_prefix<TW, TT_H>(odims, N, nch, pre, wsum, tid);
     v4f32 acc[P][OCG];
     for (int t = conv_first_tile(); t < total; t += gridDim.x) {
-        const TileAt at = conv_tile_at<TW, TT_H>(t, pre, N, 1, odims);
+        const TileAt at = conv_tile_at<TW, TT_H>(t, pre, N, nch, odims);
         const int n = at.n, ty0 = at.ty0, tx0 = at.tx0;
+        const int od0 = at.od * DC, od1 = min(D, od0 + DC);
         const int hin = dh(idims, n), win = dw(idims, n);
         const int ho = dh(odims, n), wo = dw(odims, n);
         const int iy0 = ty0 * s - p, ix0 = tx0 * s - p;
@@ -348,13 +352,13 @@ __global__ void __launch_bounds__(256) k_conv_m4d(
         const float* xrs = SRC == 1 ? xr + (long long)n * xstride : nullptr;
         const long long oplane = (long long)ho * wo;
         float* ys = y + (long long)n * ystride;
-        // planes -1, 0, 1 in slots 2, 0, 1
-        stage_plane<SRC>(ring, xs, xrs, IC, D, -1, 2, iy0, ix0, hin, win, ct, tid);
-        stage_plane<SRC>(ring, xs, xrs, IC, D, 0, 0, iy0, ix0, hin, win, ct, tid);
-        stage_plane<SRC>(ring, xs, xrs, IC, D, 1, 1, iy0, ix0, hin, win, ct, tid);
+        // planes od0-1, od0, od0+1; plane dd lives in slot (dd + 3) % 3
+        stage_plane<SRC>(ring, xs, xrs, IC, D, od0 - 1, (od0 + 2) % 3, iy0, ix0, hin, win, ct, tid);
+        stage_plane<SRC>(ring, xs, xrs, IC, D, od0, od0 % 3, iy0, ix0, hin, win, ct, tid);
+        stage_plane<SRC>(ring, xs, xrs, IC, D, od0 + 1, (od0 + 1) % 3, iy0, ix0, hin, win, ct, tid);
         __syncthreads();
-        for (int od = 0; od < D; ++od) {
-            if (od > 0) {
+        for (int od = od0; od < od1; ++od) {
+            if (od > od0) {
                 stage_plane<SRC>(ring, xs, xrs, IC, D, od + 1, (od + 1) % 3, iy0, ix0, hin, win,
                                  ct, tid);
                 __syncthreads();
@@ -499,18 +503,25 @@ int conv_fwd_m4(const float* x, const float* xr, const int* idims, long long xst
     ConvTiles ct;
     const int eng = tune(SV_TUNE_CONV_ENGINE);
     const long long ub3 = (long long)((max_wo + TWc - 1) / TWc) * ((max_ho + TT_H - 1) / TT_H) * N;
-    if (KD == 3 && (eng == 2 || (eng == 0 && ub3 >= 2LL * cu_count()))) {
-        // depth-sliding kernel (by default when there are at least two
-        // spatial tiles per CU) when every channel's three-plane ring and
+    if (KD == 3 && (eng == 2 || eng == 0)) {
+        // depth-sliding kernel when every channel's three-plane ring and
         // weights fit 64 KB
         const long long shm3 = conv_tiles(3, IC, TWc, TT_H, s, dil, N, (long long)OCG * NJ * 64,
                                           64 * 1024, ct);
         if (shm3 > 0 && ct.ICG == IC) {
-            dim3 g3(conv_grid(ub3, shm3)), bk3(256);
+            // chunk of output depths per tile: all of them when that leaves
+            // at least two spatial tiles per CU, else short enough to reach
+            // two tiles (or one chunk per depth pair) per CU
+            const long long want = 2LL * cu_count();
+            int nch = (int)std::min<long long>(D, std::max<long long>(1, (want + ub3 - 1) / ub3));
+            int DC = (D + nch - 1) / nch;
+            if (DC < 2 && D >= 2) DC = 2;
+            nch = (D + DC - 1) / DC;
+            dim3 g3(conv_grid(ub3 * nch, shm3)), bk3(256);
 #define LAUNCH_M4D(SRC_, OCG_, TW_, P_)                                                         \
     hipLaunchKernelGGL((k_conv_m4d<SRC_, OCG_, TW_, P_>), g3, bk3, (size_t)shm3, st, x, xr,     \
                        idims, xstride, IC, D, w, b, OC, s, p, dil, y, odims, ystride, N, mode,  \
-                       slope, resid, rstride, ct)
+                       slope, resid, rstride, ct, DC)
 #define LAUNCH_M4D_P(SRC_, OCG_, TW_)                                                           \
     do {                                                                                        \
         if (P == 1) LAUNCH_M4D(SRC_, OCG_, TW_, 1);                                             \

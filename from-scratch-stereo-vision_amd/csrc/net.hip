@@ -107,6 +107,157 @@ __global__ void __launch_bounds__(256) k_conv_fwd(
     }
 }
 
+// Per-channel form of the LDS-tiled convolution, for layers whose input
+// channels do not fit one staging pass of k_conv_tile (the widened
+// network).  A block of 256 threads (TW columns x 256/TW rows of threads)
+// makes one tile of TW output columns x RP*256/TW rows at one output depth
+// (grid over the largest sample; blocks outside a sample exit); for each
+// input channel it stages the input halo tile of all KD depth taps in LDS
+// (zero outside the image, the cost volume formed while staging when
+// SRC == 1), double-buffered so the next channel loads while this one is
+// consumed.  Each thread accumulates RP
+// output rows x OCB channels.  WS selects where the weights come from: 1,
+// staged in LDS per channel (read back as float4 broadcasts); 2, straight
+// from memory through the scalar cache into SGPR operands (the weights are
+// uniform across the block), which leaves the LDS return path to the
+// activations.  TW (64, 32 or 16) only changes how the plane is cut into
+// tiles: narrow maps waste less of a narrower tile.  Every output is summed
+// in the same order, one fused multiply-add per term, whatever TW, RP and WS
+// are, so the variants agree bit for bit.
+template <int KD, int SRC, int OCB, int RP, int TW, int WS>
+__global__ void __launch_bounds__(256) k_conv_chan(
+    const float* __restrict__ x, const float* __restrict__ xr, const int* __restrict__ idims,
+    long long xstride, int IC, int D, const float* __restrict__ w, const float* __restrict__ b,
+    int OC, int s, int p, int dil, float* __restrict__ y, const int* __restrict__ odims,
+    long long ystride, int tilesY, int mode, float slope, const float* __restrict__ resid,
+    long long rstride, int IHt, int IWt) {
+    // OCB padded to a multiple of 4 for float4 weight reads
+    constexpr int OCP = (OCB + 3) / 4 * 4;
+    constexpr int WSZ = KD * 9 * OCP;
+    constexpr int TY = 256 / TW;   // thread rows
+    constexpr int TT_H = TY * RP;  // rows per tile: TY threads high, RP rows each
+    extern __shared__ __align__(16) float smem_f[];
+    const int n = blockIdx.z;
+    const int od = blockIdx.y / tilesY;
+    const int ty0 = (blockIdx.y - od * tilesY) * TT_H;
+    const int tx0 = blockIdx.x * TW;
+    const int hi = dh(idims, n), wi = dw(idims, n);
+    const int ho = dh(odims, n), wo = dw(odims, n);
+    if (tx0 >= wo || ty0 >= ho) return;  // uniform per block
+    const int tx = threadIdx.x, ty = threadIdx.y;  // TW x TY
+    const int tid = ty * TW + tx;
+    const int plane_t = IHt * IWt;
+    const int tsz = KD * plane_t;
+    float* wsm = smem_f;                   // [2][KD*9][OCP]
+    float* tile = smem_f + 2 * WSZ;        // [2][KD][IHt][IWt]
+    const int iy0 = ty0 * s - p, ix0 = tx0 * s - p;
+    const long long iplane = (long long)hi * wi;
+    const float* xs = x + (long long)n * xstride;
+    const float* xrs = SRC == 1 ? xr + (long long)n * xstride : nullptr;
+    constexpr int PD = KD / 2;
+    float acc[RP][OCP];
+    auto stage = [&](int ic, int oc0, int slotb) {
+        float* buf = tile + slotb * tsz;
+        for (int e = tid; e < tsz; e += 256) {
+            const int kd = e / plane_t;
+            const int r = e - kd * plane_t;
+            const int yy = r / IWt, xx = r - yy * IWt;
+            const int dd = od + kd - PD, iy = iy0 + yy, ix = ix0 + xx;
+            float v = 0.f;
+            if (dd >= 0 && dd < D && iy >= 0 && iy < hi && ix >= 0 && ix < wi) {
+                if (SRC == 1) {
+                    if (ix < dd) {
+                        v = 2.0f;
+                    } else {
+                        const long long o = (long long)ic * iplane + (long long)iy * wi;
+                        v = fabsf(xs[o + ix] - xrs[o + ix - dd]);
+                    }
+                } else {
+                    v = xs[((long long)ic * D + dd) * iplane + (long long)iy * wi + ix];
+                }
+            }
+            buf[e] = v;
+        }
+        if (WS == 1) {
+            float* wb = wsm + slotb * WSZ;
+            for (int e = tid; e < WSZ; e += 256) {
+                const int t = e / OCP, j = e - t * OCP;
+                wb[e] = (j < OCB && oc0 + j < OC) ? w[((long long)(oc0 + j) * IC + ic) * KD * 9 + t] : 0.f;
+            }
+        }
+    };
+    for (int oc0 = 0; oc0 < OC; oc0 += OCB) {
+#pragma unroll
+        for (int i = 0; i < RP; ++i)
+#pragma unroll
+            for (int j = 0; j < OCP; ++j) acc[i][j] = 0.f;
+        stage(0, oc0, 0);
+        __syncthreads();
+        for (int ic = 0; ic < IC; ++ic) {
+            const float* cur = tile + (ic & 1) * tsz;
+            const float4* wc = reinterpret_cast<const float4*>(wsm + (ic & 1) * WSZ);
+            if (ic + 1 < IC) stage(ic + 1, oc0, (ic + 1) & 1);
+#pragma unroll
+            for (int kd = 0; kd < KD; ++kd)
+#pragma unroll
+                for (int ky = 0; ky < 3; ++ky)
+#pragma unroll
+                    for (int kx = 0; kx < 3; ++kx) {
+                        float v[RP];
+#pragma unroll
+                        for (int i = 0; i < RP; ++i)
+                            v[i] = cur[kd * plane_t + ((ty * RP + i) * s + ky * dil) * IWt + tx * s + kx * dil];
+                        const int wt = (kd * 3 + ky) * 3 + kx;
+                        if (WS == 1) {
+#pragma unroll
+                            for (int q = 0; q < OCP / 4; ++q) {
+                                const float4 wv = wc[wt * (OCP / 4) + q];
+#pragma unroll
+                                for (int i = 0; i < RP; ++i) {
+                                    acc[i][4 * q + 0] = __builtin_fmaf(v[i], wv.x, acc[i][4 * q + 0]);
+                                    acc[i][4 * q + 1] = __builtin_fmaf(v[i], wv.y, acc[i][4 * q + 1]);
+                                    acc[i][4 * q + 2] = __builtin_fmaf(v[i], wv.z, acc[i][4 * q + 2]);
+                                    acc[i][4 * q + 3] = __builtin_fmaf(v[i], wv.w, acc[i][4 * q + 3]);
+                                }
+                            }
+                        } else {
+                            // weights of (oc0 + j, ic, tap wt): uniform addresses
+                            const float* wg = w + ((long long)oc0 * IC + ic) * KD * 9 + wt;
+#pragma unroll
+                            for (int j = 0; j < OCB; ++j) {
+                                const float wj = oc0 + j < OC ? wg[(long long)j * IC * KD * 9] : 0.f;
+#pragma unroll
+                                for (int i = 0; i < RP; ++i) acc[i][j] = __builtin_fmaf(v[i], wj, acc[i][j]);
+                            }
+                        }
+                    }
+            __syncthreads();
+        }
+        const int ox = tx0 + tx;
+        const long long oplane = (long long)ho * wo;
+        float* ys = y + (long long)n * ystride + (long long)od * oplane;
+#pragma unroll
+        for (int i = 0; i < RP; ++i) {
+            const int oy = ty0 + ty * RP + i;
+            if (ox >= wo || oy >= ho) continue;
+#pragma unroll
+            for (int j = 0; j < OCB; ++j) {
+                const int oc = oc0 + j;
+                if (oc >= OC) break;
+                float v = acc[i][j] + b[oc];
+                if (mode == 1) {
+                    v = v > 0.f ? v : v * slope;
+                } else if (mode == 2) {
+                    v = resid[(long long)n * rstride + (long long)oc * D * oplane +
+                              (long long)od * oplane + (long long)oy * wo + ox] + v;
+                    v = v > 0.f ? v : 0.f;
+                }
+                ys[(long long)oc * D * oplane + (long long)oy * wo + ox] = v;
+            }
+        }
+    }
+}
+
 // LDS-tiled form of k_conv_fwd on the vector ALUs.  A block of 256 threads
 // (TW columns x 256/TW rows of threads) makes output tiles of TW columns x
 // RP*256/TW rows at one output depth, each thread RP rows x OCB channels,
@@ -1030,6 +1181,48 @@ int sv_conv_fwd(const float* x, const float* xr, const int* idims, long long xst
         sv::ConvTiles ct;
         const long long shm = sv::conv_tiles(KD, IC, TWc, TT_H, s, dil, N,
                                              WSc == 1 ? (long long)KD * 9 * ocp : 0, 40 * 1024, ct);
+        const int eng = sv::tune(SV_TUNE_CONV_ENGINE);
+        const int IHc = (TT_H - 1) * s + 2 * dil + 1, IWc = (TWc - 1) * s + 2 * dil + 1;
+        const size_t shc = 2 * ((size_t)KD * IHc * IWc + (size_t)KD * 9 * ocp) * sizeof(float);
+        if ((eng == 4 || (eng == 0 && (shm < 0 || ct.ICG < IC))) && shc <= 64 * 1024) {
+            // per-channel double-buffered kernel: the channels do not fit
+            // one staging pass (or the knob asks for it)
+            const int tYt = (max_ho + TT_H - 1) / TT_H;
+            dim3 gc((max_wo + TWc - 1) / TWc, tYt * D, N), bkc(TWc, TYc);
+#define LAUNCH_C(KD_, SRC_, OCB_, RP_, TW_)                                                       \
+    hipLaunchKernelGGL((k_conv_chan<KD_, SRC_, OCB_, RP_, TW_, 1>), gc, bkc, shc, st, x, xr,     \
+                       idims, xstride, IC, D, w, b, OC, s, p, dil, y, odims, ystride, tYt, mode, \
+                       slope, resid, rstride, IHc, IWc)
+#define LAUNCH_CT(KD_, SRC_, OCB_)                                                               \
+    do {                                                                                         \
+        if (RP == 4) {                                                                           \
+            if (TWc == 64) LAUNCH_C(KD_, SRC_, OCB_, 4, 64);                                     \
+            else if (TWc == 32) LAUNCH_C(KD_, SRC_, OCB_, 4, 32);                                \
+            else LAUNCH_C(KD_, SRC_, OCB_, 4, 16);                                               \
+        } else {                                                                                 \
+            if (TWc == 64) LAUNCH_C(KD_, SRC_, OCB_, 1, 64);                                     \
+            else if (TWc == 32) LAUNCH_C(KD_, SRC_, OCB_, 1, 32);                                \
+            else LAUNCH_C(KD_, SRC_, OCB_, 1, 16);                                               \
+        }                                                                                        \
+    } while (0)
+#define LAUNCH_CT_OC(KD_, SRC_)                                                                  \
+    do {                                                                                         \
+        if (OC == 1) LAUNCH_CT(KD_, SRC_, 1);                                                    \
+        else if (OC <= 4) LAUNCH_CT(KD_, SRC_, 4);                                               \
+        else if (OC <= 8) LAUNCH_CT(KD_, SRC_, 8);                                               \
+        else if (OC <= 12) LAUNCH_CT(KD_, SRC_, 12);                                             \
+        else LAUNCH_CT(KD_, SRC_, 16);                                                           \
+    } while (0)
+            if (KD == 1 && !src) LAUNCH_CT_OC(1, 0);
+            else if (KD == 3 && !src) LAUNCH_CT_OC(3, 0);
+            else if (KD == 3) LAUNCH_CT_OC(3, 1);
+            else LAUNCH_CT_OC(1, 1);
+#undef LAUNCH_CT_OC
+#undef LAUNCH_CT
+#undef LAUNCH_C
+            SV_CHECK_LAUNCH();
+            return SV_OK;
+        }
         if (shm > 0) {
             const long long ub = (long long)((max_wo + TWc - 1) / TWc) * ((max_ho + TT_H - 1) / TT_H) * D * N;
             dim3 g(sv::conv_grid(ub, shm)), bk(TWc, TYc);

@@ -135,12 +135,14 @@ int sv_refine_ragged(int mode, const void* left, const void* right, int dtype, c
  * LDS, 2 scalar cache); 5: horizontal SGM paths after the others (1) or on a
  * second, library-owned stream beside them (2); 6: convolution engine (1
  * vector ALUs, 2 matrix cores with 3D layers sliding over depth, 3 matrix
- * cores with one depth per tile); 7: 64-pixel groups per wave of the
+ * cores with one depth per tile, 4 vector ALUs staging one input channel at
+ * a time); 7: 64-pixel groups per wave of the
  * matrix-core convolution (1, 2, 4); 8: lanes per scanline of the packed
  * SGM kernels (1, 2).  Value 0 restores the built-in
  * heuristic (LDS weights; horizontal paths after the others; matrix cores
- * for 4 to 16 output channels, sliding over depth when a 3D layer has at
- * least two spatial tiles per CU).
+ * for 4 to 16 output channels, 3D layers sliding over depth chunks; the
+ * per-channel vector kernel when the input channels do not fit one staging
+ * pass).
  * Process-wide; set between launches, not while kernels are being queued
  * from other threads. */
 int sv_tune_set(int knob, int value);

@@ -284,6 +284,10 @@ _CONV_CASES = [
     (6, 1, 3, 0, 1, 1, 5, 0),
     (5, 16, 1, 0, 1, 2, 1, 1),
     (3, 13, 3, 0, 1, 1, 4, 0),
+    # channels beyond one staging pass (the widened network's shapes)
+    (48, 48, 1, 0, 2, 1, 1, 1),
+    (48, 24, 3, 1, 1, 1, 4, 1),
+    (32, 1, 1, 0, 1, 1, 1, 2),
 ]
 
 
