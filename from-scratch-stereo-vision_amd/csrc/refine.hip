@@ -85,20 +85,22 @@ __device__ __forceinline__ double big<double>() {
 }
 
 // ------------------------------------------------------------------ census
-// census, stereonet.py:5115-5142.  A block makes 64 x 4 words: it stages
-// the window-padded tile of its pixels in LDS with border-clamped
-// coordinates (the reference clamps every read the same way), then each
-// thread compares its window from LDS.  Bit order: MSB = first offset, dy
-// outer, dx inner.  RX / RY >= 0 fix the window at compile time; -1 takes
-// the runtime radii.
+// census, stereonet.py:5115-5142.  A block makes 64 x CR words (CR = 16 rows;
+// each of its 64 x 4 threads makes the rows threadIdx.y + 4r): it stages the
+// window-padded tile of its pixels in LDS with border-clamped coordinates
+// (the reference clamps every read the same way), then each thread compares
+// its windows from LDS.  Bit order: MSB = first offset, dy outer, dx inner.
+// RX / RY >= 0 fix the window at compile time; -1 takes the runtime radii.
+constexpr int CENSUS_ROWS = 16;
+
 template <typename T, int RX, int RY>
 __global__ void __launch_bounds__(256) k_census(const T* __restrict__ img, Geom g, int rx_,
                                                 int ry_, uint64_t* __restrict__ out) {
     extern __shared__ __align__(16) unsigned char smem_c[];
     T* tile = reinterpret_cast<T*>(smem_c);
     const int rx = RX >= 0 ? RX : rx_, ry = RY >= 0 ? RY : ry_;
-    const int TW = 64 + 2 * rx, TH = 4 + 2 * ry;
-    const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 4;
+    const int TW = 64 + 2 * rx, TH = CENSUS_ROWS + 2 * ry;
+    const int x0 = blockIdx.x * 64, y0 = blockIdx.y * CENSUS_ROWS;
     const int W = g.w(blockIdx.z), H = g.h(blockIdx.z);
     if (x0 >= W || y0 >= H) return;  // whole block outside this image
     const long long plane = g.slot;
@@ -111,38 +113,44 @@ __global__ void __launch_bounds__(256) k_census(const T* __restrict__ img, Geom 
         tile[e] = im[(long long)yy * W + xx];
     }
     __syncthreads();
-    const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
-    if (x >= W || y >= H) return;
-    const T* t0 = tile + threadIdx.y * TW + threadIdx.x;  // window corner (-rx, -ry)
-    const T c = t0[ry * TW + rx];
-    uint64_t s;
-    if constexpr (RX >= 0 && RY >= 0) {
-        constexpr int NBITS = (2 * RX + 1) * (2 * RY + 1) - 1;
-        uint32_t hi = 0, lo = 0;
-        int i = 0;
+    const int x = x0 + threadIdx.x;
+    if (x >= W) return;
+#pragma unroll 1
+    for (int r = 0; r < CENSUS_ROWS / 4; ++r) {
+        const int ly = threadIdx.y + 4 * r;
+        const int y = y0 + ly;
+        if (y >= H) break;
+        const T* t0 = tile + ly * TW + threadIdx.x;  // window corner (-rx, -ry)
+        const T c = t0[ry * TW + rx];
+        uint64_t s;
+        if constexpr (RX >= 0 && RY >= 0) {
+            constexpr int NBITS = (2 * RX + 1) * (2 * RY + 1) - 1;
+            uint32_t hi = 0, lo = 0;
+            int i = 0;
 #pragma unroll
-        for (int dy = 0; dy <= 2 * RY; ++dy)
+            for (int dy = 0; dy <= 2 * RY; ++dy)
 #pragma unroll
-            for (int dx = 0; dx <= 2 * RX; ++dx) {
-                if (dy == RY && dx == RX) continue;
-                const int pos = NBITS - 1 - i;  // bit of this offset
-                const uint32_t bit = t0[dy * (64 + 2 * RX) + dx] < c ? 1u : 0u;
-                if (pos >= 32)
-                    hi |= bit << (pos - 32);
-                else
-                    lo |= bit << pos;
-                ++i;
-            }
-        s = ((uint64_t)hi << 32) | lo;
-    } else {
-        s = 0;
-        for (int dy = 0; dy <= 2 * ry; ++dy)
-            for (int dx = 0; dx <= 2 * rx; ++dx) {
-                if (dy == ry && dx == rx) continue;
-                s = (s << 1) | (uint64_t)(t0[dy * TW + dx] < c);
-            }
+                for (int dx = 0; dx <= 2 * RX; ++dx) {
+                    if (dy == RY && dx == RX) continue;
+                    const int pos = NBITS - 1 - i;  // bit of this offset
+                    const uint32_t bit = t0[dy * (64 + 2 * RX) + dx] < c ? 1u : 0u;
+                    if (pos >= 32)
+                        hi |= bit << (pos - 32);
+                    else
+                        lo |= bit << pos;
+                    ++i;
+                }
+            s = ((uint64_t)hi << 32) | lo;
+        } else {
+            s = 0;
+            for (int dy = 0; dy <= 2 * ry; ++dy)
+                for (int dx = 0; dx <= 2 * rx; ++dx) {
+                    if (dy == ry && dx == rx) continue;
+                    s = (s << 1) | (uint64_t)(t0[dy * TW + dx] < c);
+                }
+        }
+        out[(long long)blockIdx.z * plane + (long long)y * W + x] = s;
     }
-    out[(long long)blockIdx.z * plane + (long long)y * W + x] = s;
 }
 
 // band centre, stereonet.py:5254-5257: max(0, int(disp0 + 0.5)), int()
@@ -167,20 +175,24 @@ __global__ void k_base(const double* __restrict__ d0, long long n, int* __restri
 // `invalid` where the candidate leaves the image.  RIGHT selects the right
 // view matched in left coordinates: partner column x + d instead of x - d.
 template <bool RIGHT>
-__global__ void __launch_bounds__(64) k_costs_int(const uint64_t* __restrict__ cA,
-                                                  const uint64_t* __restrict__ cB,
-                                                  const int* __restrict__ base, Geom g, int R,
-                                                  int K, int Kp, int invalid,
-                                                  uint8_t* __restrict__ C) {
-    // the block's 64 pixels x Kp bytes, written back as contiguous 8-byte
-    // units (a lane's own Kp bytes are Kp apart from its neighbour's)
-    extern __shared__ __align__(16) uint32_t cstage[];
+__global__ void __launch_bounds__(256) k_costs_int(const uint64_t* __restrict__ cA,
+                                                   const uint64_t* __restrict__ cB,
+                                                   const int* __restrict__ base, Geom g, int R,
+                                                   int K, int Kp, int invalid,
+                                                   uint8_t* __restrict__ C) {
+    // each wave: 64 pixels of one row (blockDim.y rows per block) x Kp bytes,
+    // staged in the wave's own LDS region and written back as contiguous
+    // 8-byte units (a lane's own Kp bytes are Kp apart from its neighbour's).
+    // A wave reads back only what it wrote, in issue order, so no block
+    // barrier is needed and a wave past the image leaves at once.
+    extern __shared__ __align__(16) uint32_t cstage_all[];
     const int lane = threadIdx.x;
+    uint32_t* cstage = cstage_all + threadIdx.y * 64 * (Kp / 4);
     const int x0 = blockIdx.x * 64;
     const int x = x0 + lane;
-    const int y = blockIdx.y;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
     const int W = g.w(blockIdx.z);
-    if (x0 >= W || y >= g.h(blockIdx.z)) return;  // whole block outside this image
+    if (x0 >= W || y >= g.h(blockIdx.z)) return;  // whole wave outside this image
     const long long plane = g.slot;
     const long long row0 = (long long)blockIdx.z * plane + (long long)y * W;
     if (x < W) {
@@ -205,7 +217,7 @@ __global__ void __launch_bounds__(64) k_costs_int(const uint64_t* __restrict__ c
             cstage[lane * (Kp / 4) + k0 / 4] = packed;
         }
     }
-    __syncthreads();
+    __builtin_amdgcn_wave_barrier();
     const int units = (min(64, W - x0) * Kp) / 8;
     const uint2* src = reinterpret_cast<const uint2*>(cstage);
     uint2* dst = reinterpret_cast<uint2*>(C + (row0 + x0) * Kp);
@@ -1770,8 +1782,8 @@ RefineBufs carve(void* ws, const Plan& pl, int V, int B, long long slot, bool lr
 template <typename T>
 int launch_census(const T* img, int n, const Geom& g, int rx, int ry, uint64_t* out,
                   hipStream_t st) {
-    dim3 grid((g.W + 63) / 64, (g.H + 3) / 4, n);
-    const size_t shm = (size_t)(64 + 2 * rx) * (4 + 2 * ry) * sizeof(T);
+    dim3 grid((g.W + 63) / 64, (g.H + CENSUS_ROWS - 1) / CENSUS_ROWS, n);
+    const size_t shm = (size_t)(64 + 2 * rx) * (CENSUS_ROWS + 2 * ry) * sizeof(T);
     if (rx == 4 && ry == 3)
         hipLaunchKernelGGL((k_census<T, 4, 3>), grid, dim3(64, 4), shm, st, img, g, rx, ry, out);
     else
@@ -1817,13 +1829,15 @@ int launch_costs(const Plan& pl, bool right, const uint64_t* cA, const uint64_t*
                  int B, const Geom& g, int R, double aw, double at, double pw, double pt, void* C,
                  hipStream_t st) {
     if (pl.integer) {
-        dim3 grid((g.W + 63) / 64, g.H, B);
-        const size_t shm = (size_t)64 * pl.Kp;
+        // four rows per block while their staging fits 64 KB of LDS
+        const int rb = 4 * 64 * pl.Kp <= 64 * 1024 ? 4 : 1;
+        dim3 grid((g.W + 63) / 64, (g.H + rb - 1) / rb, B);
+        const size_t shm = (size_t)rb * 64 * pl.Kp;
         if (right)
-            hipLaunchKernelGGL(k_costs_int<true>, grid, dim3(64), shm, st, cA, cB, base, g, R,
+            hipLaunchKernelGGL(k_costs_int<true>, grid, dim3(64, rb), shm, st, cA, cB, base, g, R,
                                pl.K, pl.Kp, (int)pl.invalid, (uint8_t*)C);
         else
-            hipLaunchKernelGGL(k_costs_int<false>, grid, dim3(64), shm, st, cA, cB, base, g, R,
+            hipLaunchKernelGGL(k_costs_int<false>, grid, dim3(64, rb), shm, st, cA, cB, base, g, R,
                                pl.K, pl.Kp, (int)pl.invalid, (uint8_t*)C);
         SV_CHECK_LAUNCH();
         return SV_OK;
