@@ -245,7 +245,9 @@ def main():
     A_ = importlib.import_module("from-scratch-stereo-vision_amd.autotune")
     traffic = SGM_TRAFFIC_PER_PAIR[2 if A_.get_knob("sgm_split") == 2 else 1] * B
     roofline = {"bound": "hbm",
-                "kernel": "band SGM aggregation: k_sgm_ph + k_sgm_p (8 paths, both views, %d pairs)" % B,
+                "kernel": "band SGM aggregation: %s (8 paths, both views, %d pairs)"
+                          % ("k_sgm_ph2 + k_sgm_p2" if A_.get_knob("sgm_split") == 2
+                             else "k_sgm_ph + k_sgm_p", B),
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": round(traffic),

@@ -917,7 +917,10 @@ struct PackedBand2 {
 };
 
 __device__ __forceinline__ uint32_t swap_pair(uint32_t v) {
-    // quad_perm [1, 0, 3, 2]: exchange with the other lane of the pair
+    // quad_perm [1, 0, 3, 2]: exchange with the other lane of the pair.  It
+    // reads the partner lane, so both lanes must execute it: callers apply it
+    // to a lane-selected operand, never inside a branch on the lane's half
+    // (a DPP read from a lane the branch disabled returns garbage)
     return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);
 }
 
