@@ -73,6 +73,10 @@ def main():
                     help="time the step as individual launches only (default: also capture it "
                          "in one HIP graph, check the replay equals the eager output, and report "
                          "the replay rate)")
+    ap.add_argument("--streams", type=int, default=2,
+                    help="graph replay: cut the batch into this many equal parts, each on its "
+                         "own stream with its own network buffers, captured together so that one "
+                         "part's kernels fill the other's gaps (1: the eager step as it is)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's built-in kernel variant heuristics")
     ap.add_argument("--tune", default=None,
@@ -108,9 +112,15 @@ def main():
     R8 = torch.from_numpy(np.stack([p["right"] for p in pairs])).to(dev)
     gt = np.stack([p["disp"] for p in pairs]).reshape(B, -1)
     valid = np.stack([p["valid"] for p in pairs]).reshape(B, -1)
-    net = N.load_model(os.path.join(ROOT, "tests", "golden", "stereonet_weights_v5.txt"))
+    wpath = os.path.join(ROOT, "tests", "golden", "stereonet_weights_v5.txt")
+    net = N.load_model(wpath)
     d0 = torch.empty(B, H * W, dtype=torch.float64, device=dev)
     lv = N._Level([(H, W)] * B, dev)
+    S = 1 if args.eager else args.streams
+    if S < 1 or B % S:
+        print("--streams must divide the batch", file=sys.stderr)
+        sys.exit(2)
+    hb = B // S
 
     t_net = [0.0]
 
@@ -126,6 +136,30 @@ def main():
             e1.record()
         out, st = sv.refine_disparity(L8, R8, None, d0, W, H, planes=False)
         return out, st, res, ((e0, e1) if events else None)
+
+    # the parted step: part i (pairs i*hb ...) on streams[i] with nets[i]
+    # (activation buffers are per network object, scratch per stream)
+    nets = [N.load_model(wpath) for _ in range(S)] if S > 1 else []
+    streams = [torch.cuda.Stream() for _ in range(S)] if S > 1 else []
+    lvh = N._Level([(H, W)] * hb, dev)
+
+    def part(i):
+        sl = slice(i * hb, (i + 1) * hb)
+        res = N.predict_auto_batch(nets[i], L8[sl], R8[sl], W, H)
+        N._call("sv_widen", N._p(res.full), N._p(lvh.table), H * W, N._p(d0[sl]), H * W, None,
+                hb, H * W, L.stream())
+        return sv.refine_disparity(L8[sl], R8[sl], None, d0[sl], W, H, planes=False)
+
+    def parted_step():
+        cur = torch.cuda.current_stream()
+        outs = []
+        for i, s in enumerate(streams):
+            s.wait_stream(cur)
+            with torch.cuda.stream(s):
+                outs.append(part(i))
+        for s in streams:
+            cur.wait_stream(s)
+        return outs
 
     tuned = None
     A = importlib.import_module("from-scratch-stereo-vision_amd.autotune")
@@ -143,8 +177,23 @@ def main():
                        knobs=["sgm_split", "sgm_prefetch", "sgm_htile", "sgm_overlap"], reps=2,
                        log=log)
         tuned["conv_layers_tuned"] = len(layers)
-        if args.tune_file:
-            A.save_choices(net, tuned, args.tune_file)
+    # the parts share the per-layer choices (keyed by launch shape, batch
+    # included); shapes of the part batch get their own when missing
+    for nt in nets:
+        nt._conv_tune = net._conv_tune
+    if nets and not args.no_autotune and not args.tune \
+            and not any(k[2] == hb for k in net._conv_tune):
+        more = A.tune_layers(nets[0], lambda: part(0), reps=2)
+        # (tune_layers leaves every knob at its default)
+        A.apply({k: v for k, v in (tuned or {}).items() if k in A.KNOBS})
+        if tuned is not None:
+            tuned["conv_layers_tuned"] = tuned.get("conv_layers_tuned", 0) + len(more)
+        if args.tune_file and os.path.exists(args.tune_file):
+            A.save_choices(net, {k: v for k, v in tuned.items() if k in A.KNOBS},
+                           args.tune_file)
+    if args.tune_file and not os.path.exists(args.tune_file) and tuned is not None \
+            and not args.tune:
+        A.save_choices(net, tuned, args.tune_file)
     log("warmup %d" % args.warmup)
     for _ in range(args.warmup):
         step()
@@ -174,9 +223,16 @@ def main():
         # once, the kernel variants chosen above baked in); its output must
         # equal the eager step's
         R = importlib.import_module("from-scratch-stereo-vision_amd.ragged")
-        g = R.CapturedPipeline(lambda a, b: step(events=False)[:2], L8, R8, warmup=1)
-        gout, gst = g.replay()
-        torch.cuda.synchronize()
+        if S > 1:
+            g = R.CapturedPipeline(lambda a, b: parted_step(), L8, R8, warmup=1)
+            parts = g.replay()
+            torch.cuda.synchronize()
+            gout = torch.cat([o for o, _ in parts])
+            gst = torch.cat([t for _, t in parts])
+        else:
+            g = R.CapturedPipeline(lambda a, b: step(events=False)[:2], L8, R8, warmup=1)
+            gout, gst = g.replay()
+            torch.cuda.synchronize()
         if not (torch.equal(gout, out) and torch.equal(gst, st)):
             raise RuntimeError("graph replay differs from the eager step")
         for _ in range(args.warmup):
@@ -187,7 +243,8 @@ def main():
             g.replay()
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        pairs_s, execution = B * args.steps / dt, "hip graph"
+        pairs_s = B * args.steps / dt
+        execution = "hip graph" if S == 1 else "hip graph, %d streams x %d pairs" % (S, hb)
         log("graph replay: %.1f pairs/s (eager %.1f)" % (pairs_s, eager_pairs_s))
 
     # accuracy of net + band refinement

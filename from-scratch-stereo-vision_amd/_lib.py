@@ -100,18 +100,20 @@ def ptr(t):
 
 
 class _Workspace:
-    """One growing scratch buffer per device; kernels queued on the current
-    stream use it in order, so reuse across calls on one stream is safe."""
+    """One growing scratch buffer per device and stream; kernels queued on a
+    stream use its buffer in order, so reuse across calls on one stream is
+    safe, and calls on different streams may run concurrently."""
 
     def __init__(self):
         self.bufs = {}
 
     def get(self, nbytes):
         dev = device()
-        buf = self.bufs.get(dev.index)
+        key = (dev.index, torch.cuda.current_stream(dev).cuda_stream)
+        buf = self.bufs.get(key)
         if buf is None or buf.numel() < nbytes:
             buf = torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=dev)
-            self.bufs[dev.index] = buf
+            self.bufs[key] = buf
         return buf
 
 

@@ -84,6 +84,16 @@ __device__ __forceinline__ double big<double>() {
     return HUGE_VAL;
 }
 
+// two u16 lanes in one register (packed integer ops: v_pk_*_u16)
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+// LDS slots written as u32 pairs and read as single u16 entries: both
+// pointer types must be allowed to alias
+typedef uint32_t __attribute__((may_alias)) u32_alias;
+typedef uint16_t __attribute__((may_alias)) u16_alias;
+
+__device__ __forceinline__ u16x2 as_u16x2(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
+__device__ __forceinline__ uint32_t as_u32(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+
 // ------------------------------------------------------------------ census
 // census, stereonet.py:5115-5142.  A block makes 64 x CR words (CR = 16 rows;
 // each of its 64 x 4 threads makes the rows threadIdx.y + 4r): it stages the
@@ -106,11 +116,29 @@ __global__ void __launch_bounds__(256) k_census(const T* __restrict__ img, Geom 
     const long long plane = g.slot;
     const T* im = img + (long long)blockIdx.z * plane;
     const int tid = threadIdx.y * 64 + threadIdx.x;
-    for (int e = tid; e < TW * TH; e += 256) {
-        const int ty = e / TW, tx = e - ty * TW;
-        const int yy = min(max(y0 - ry + ty, 0), H - 1);
-        const int xx = min(max(x0 - rx + tx, 0), W - 1);
-        tile[e] = im[(long long)yy * W + xx];
+    if constexpr (RX >= 0 && RY >= 0) {
+        // fixed tile: every load of the thread in flight before its stores
+        constexpr int TWc = 64 + 2 * RX, NE = TWc * (CENSUS_ROWS + 2 * RY);
+        constexpr int PER = (NE + 255) / 256;
+        T v[PER];
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int e = min(tid + 256 * u, NE - 1);
+            const int ty = e / TWc, tx = e - ty * TWc;
+            const int yy = min(max(y0 - RY + ty, 0), H - 1);
+            const int xx = min(max(x0 - RX + tx, 0), W - 1);
+            v[u] = im[(long long)yy * W + xx];
+        }
+#pragma unroll
+        for (int u = 0; u < PER; ++u)
+            if (tid + 256 * u < NE) tile[tid + 256 * u] = v[u];
+    } else {
+        for (int e = tid; e < TW * TH; e += 256) {
+            const int ty = e / TW, tx = e - ty * TW;
+            const int yy = min(max(y0 - ry + ty, 0), H - 1);
+            const int xx = min(max(x0 - rx + tx, 0), W - 1);
+            tile[e] = im[(long long)yy * W + xx];
+        }
     }
     __syncthreads();
     const int x = x0 + threadIdx.x;
@@ -153,6 +181,107 @@ __global__ void __launch_bounds__(256) k_census(const T* __restrict__ img, Geom 
     }
 }
 
+// census with the 9 x 7 window on 8-bit images (the refinement default).
+// Each thread makes 4 neighbouring pixels of C8_RPT rows.  The LDS tile
+// holds the pixels widened to u16, so a thread's 12-pixel run of a window
+// row is three aligned 8-byte words, and one packed u16 subtract serves two
+// pixels: (n - c) mod 2^16 has bit 15 set exactly when n < c (both in
+// 0..255).  That bit is shifted into a per-pixel u16 accumulator, 16
+// offsets per accumulator; the four accumulators of a pixel hold its bits
+// 61..46, 45..30, 29..14 and 13..0 (first offset most significant, dy
+// outer, dx inner, as k_census).  Reads are border-clamped like k_census.
+constexpr int C8_RPT = 2;
+constexpr int C8_ROWS = 16 * C8_RPT;
+
+__global__ void __launch_bounds__(256) k_census8(const uint8_t* __restrict__ img, Geom g,
+                                                 uint64_t* __restrict__ out) {
+    constexpr int RX = 4, RY = 3;
+    constexpr int TW = 64 + 2 * RX, TH = C8_ROWS + 2 * RY, NE = TW * TH;
+    constexpr int WR = C8_RPT + 2 * RY;  // window rows of a thread
+    __shared__ __align__(16) uint16_t tile[NE];
+    const int x0 = blockIdx.x * 64, y0 = blockIdx.y * C8_ROWS;
+    const int W = g.w(blockIdx.z), H = g.h(blockIdx.z);
+    if (x0 >= W || y0 >= H) return;  // whole block outside this image
+    const long long plane = g.slot;
+    const uint8_t* im = img + (long long)blockIdx.z * plane;
+    const int tid = threadIdx.x;
+    {
+        constexpr int PER = (NE + 255) / 256;
+        uint32_t v[PER];
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int e = min(tid + 256 * u, NE - 1);
+            const int ty = e / TW, tx = e - ty * TW;
+            const int yy = min(max(y0 - RY + ty, 0), H - 1);
+            const int xx = min(max(x0 - RX + tx, 0), W - 1);
+            v[u] = im[(long long)yy * W + xx];
+        }
+#pragma unroll
+        for (int u = 0; u < PER; ++u)
+            if (tid + 256 * u < NE) tile[tid + 256 * u] = (uint16_t)v[u];
+    }
+    __syncthreads();
+    const int tx = tid & 15, ty = tid >> 4;
+    const int xg = x0 + 4 * tx;  // this thread's first pixel
+    if (xg >= W) return;
+    const int ly0 = ty * C8_RPT;
+    // w[r]: tile columns 4tx .. 4tx+11 of window row r as u16 pairs
+    uint32_t w[WR][6];
+#pragma unroll
+    for (int r = 0; r < WR; ++r) {
+        const uint2* src = reinterpret_cast<const uint2*>(tile + (ly0 + r) * TW + 4 * tx);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const uint2 t = src[q];
+            w[r][2 * q] = t.x;
+            w[r][2 * q + 1] = t.y;
+        }
+    }
+    // columns k, k+1 of window row r as one u16 pair
+    auto pair_at = [&](int r, int k) -> uint32_t {
+        return (k & 1) ? __builtin_amdgcn_alignbit(w[r][(k + 1) >> 1], w[r][(k - 1) >> 1], 16)
+                       : w[r][k >> 1];
+    };
+#pragma unroll
+    for (int r = 0; r < C8_RPT; ++r) {
+        const int y = y0 + ly0 + r;
+        if (y >= H) break;
+        const u16x2 cA = as_u16x2(w[r + RY][2]), cB = as_u16x2(w[r + RY][3]);
+        // offsets last to first: each bit enters at the top of its half
+        // and moves down one place per later offset of its group, so the
+        // group's first offset ends most significant (the 14-offset last
+        // group ends two places up)
+        uint32_t accA[4] = {0u, 0u, 0u, 0u}, accB[4] = {0u, 0u, 0u, 0u};
+        int i = (2 * RX + 1) * (2 * RY + 1) - 2;
+#pragma unroll
+        for (int dy = 2 * RY; dy >= 0; --dy)
+#pragma unroll
+            for (int dx = 2 * RX; dx >= 0; --dx) {
+                if (dy == RY && dx == RX) continue;
+                const uint32_t dA = as_u32(as_u16x2(pair_at(r + dy, dx)) - cA);
+                const uint32_t dB = as_u32(as_u16x2(pair_at(r + dy, dx + 2)) - cB);
+                accA[i >> 4] = (accA[i >> 4] >> 1) | (dA & 0x80008000u);
+                accB[i >> 4] = (accB[i >> 4] >> 1) | (dB & 0x80008000u);
+                // keep one shift and one and-or per step (unfenced, the
+                // compiler re-associates the chain into more operations)
+                asm volatile("" : "+v"(accA[i >> 4]), "+v"(accB[i >> 4]));
+                --i;
+            }
+        uint64_t* o = out + (long long)blockIdx.z * plane + (long long)y * W + xg;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (xg + j >= W) break;
+            const uint32_t* acc = j < 2 ? accA : accB;
+            const int sh = 16 * (j & 1);
+            const uint32_t g0 = (acc[0] >> sh) & 0xFFFFu, g1 = (acc[1] >> sh) & 0xFFFFu;
+            const uint32_t g2 = (acc[2] >> sh) & 0xFFFFu, g3 = ((acc[3] >> sh) & 0xFFFFu) >> 2;
+            const uint32_t hi = (g0 << 14) | (g1 >> 2);
+            const uint32_t lo = (g1 << 30) | (g2 << 14) | g3;
+            o[j] = ((uint64_t)hi << 32) | lo;
+        }
+    }
+}
+
 // band centre, stereonet.py:5254-5257: max(0, int(disp0 + 0.5)), int()
 // truncating toward zero.  Values beyond 2^30 (and NaN) put the whole band
 // outside the image, which is what the reference's huge centre does.
@@ -174,7 +303,10 @@ __global__ void k_base(const double* __restrict__ d0, long long n, int* __restri
 // band_costs, stereonet.py:5156-5189, integer path: C = popcount(a ^ b) or
 // `invalid` where the candidate leaves the image.  RIGHT selects the right
 // view matched in left coordinates: partner column x + d instead of x - d.
-template <bool RIGHT>
+// KT > 0 fixes the band length (Kp = KT rounded up to 8): the partner words
+// are read from clamped columns, all KT loads in flight, and the candidates
+// outside the image take `invalid` afterwards.
+template <bool RIGHT, int KT>
 __global__ void __launch_bounds__(256) k_costs_int(const uint64_t* __restrict__ cA,
                                                    const uint64_t* __restrict__ cB,
                                                    const int* __restrict__ base, Geom g, int R,
@@ -200,21 +332,48 @@ __global__ void __launch_bounds__(256) k_costs_int(const uint64_t* __restrict__ 
         const uint64_t a = cA[pix];
         const int b0 = base[pix] - R;
         const uint64_t* rowB = cB + row0;
-        for (int k0 = 0; k0 < Kp; k0 += 4) {
-            uint32_t packed = 0;
+        if constexpr (KT > 0) {
+            constexpr int KPT = (KT + 7) / 8 * 8;
+            uint64_t w[KT];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int k = k0 + j;
-                uint32_t v = 0;
-                if (k < K) {
-                    const int d = b0 + k;
-                    const int xr = RIGHT ? x + d : x - d;
-                    const bool ok = d >= 0 && xr >= 0 && xr < W;
-                    v = ok ? (uint32_t)__popcll(a ^ rowB[ok ? xr : 0]) : (uint32_t)invalid;
-                }
-                packed |= v << (8 * j);
+            for (int k = 0; k < KT; ++k) {
+                const int d = b0 + k;
+                const int xr = RIGHT ? x + d : x - d;
+                w[k] = rowB[min(max(xr, 0), W - 1)];
             }
-            cstage[lane * (Kp / 4) + k0 / 4] = packed;
+#pragma unroll
+            for (int k0 = 0; k0 < KPT; k0 += 4) {
+                uint32_t packed = 0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int k = k0 + j;
+                    if (k < KT) {
+                        const int d = b0 + k;
+                        const int xr = RIGHT ? x + d : x - d;
+                        const bool ok = d >= 0 && xr >= 0 && xr < W;
+                        const uint32_t v = ok ? (uint32_t)__popcll(a ^ w[k]) : (uint32_t)invalid;
+                        packed |= v << (8 * j);
+                    }
+                }
+                cstage[lane * (KPT / 4) + k0 / 4] = packed;
+            }
+        } else {
+            for (int k0 = 0; k0 < Kp; k0 += 4) {
+                uint32_t packed = 0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int k = k0 + j;
+                    uint32_t v = 0;
+                    if (k < K) {
+                        const int d = b0 + k;
+                        const int xr = RIGHT ? x + d : x - d;
+                        const bool ok = d >= 0 && xr >= 0 && xr < W;
+                        v = ok ? (uint32_t)__popcll(a ^ rowB[ok ? xr : 0]) : (uint32_t)invalid;
+                    }
+                    packed |= v << (8 * j);
+                }
+                cstage[lane * (Kp / 4) + k0 / 4] = packed;
+            }
         }
     }
     __builtin_amdgcn_wave_barrier();
@@ -570,14 +729,6 @@ __global__ void __launch_bounds__(64) k_sgm_k(const uint8_t* __restrict__ C,
 // as in the reference).  Neighbour pairs (L[2j-1], L[2j]) and
 // (L[2j+1], L[2j+2]) come from v_alignbit.  The pad entry (index K) is
 // kept at INF, which makes the computed M[K] equal L[K-1] + P1.
-typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
-// the slot is written as u32 pairs and read as single u16 entries: both
-// pointer types must be allowed to alias
-typedef uint32_t __attribute__((may_alias)) u32_alias;
-typedef uint16_t __attribute__((may_alias)) u16_alias;
-
-__device__ __forceinline__ u16x2 as_u16x2(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
-__device__ __forceinline__ uint32_t as_u32(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
 
 // Position of step i of a scanline (see k_sgm for the line geometry).
 struct LinePos {
@@ -1787,7 +1938,10 @@ int launch_census(const T* img, int n, const Geom& g, int rx, int ry, uint64_t* 
                   hipStream_t st) {
     dim3 grid((g.W + 63) / 64, (g.H + CENSUS_ROWS - 1) / CENSUS_ROWS, n);
     const size_t shm = (size_t)(64 + 2 * rx) * (CENSUS_ROWS + 2 * ry) * sizeof(T);
-    if (rx == 4 && ry == 3)
+    if (std::is_same<T, uint8_t>::value && rx == 4 && ry == 3) {
+        dim3 g8((g.W + 63) / 64, (g.H + C8_ROWS - 1) / C8_ROWS, n);
+        hipLaunchKernelGGL(k_census8, g8, dim3(256), 0, st, (const uint8_t*)img, g, out);
+    } else if (rx == 4 && ry == 3)
         hipLaunchKernelGGL((k_census<T, 4, 3>), grid, dim3(64, 4), shm, st, img, g, rx, ry, out);
     else
         hipLaunchKernelGGL((k_census<T, -1, -1>), grid, dim3(64, 4), shm, st, img, g, rx, ry, out);
@@ -1836,12 +1990,19 @@ int launch_costs(const Plan& pl, bool right, const uint64_t* cA, const uint64_t*
         const int rb = 4 * 64 * pl.Kp <= 64 * 1024 ? 4 : 1;
         dim3 grid((g.W + 63) / 64, (g.H + rb - 1) / rb, B);
         const size_t shm = (size_t)rb * 64 * pl.Kp;
-        if (right)
-            hipLaunchKernelGGL(k_costs_int<true>, grid, dim3(64, rb), shm, st, cA, cB, base, g, R,
-                               pl.K, pl.Kp, (int)pl.invalid, (uint8_t*)C);
+        const bool k21 = pl.K == 21 && pl.Kp == 24;
+        if (right && k21)
+            hipLaunchKernelGGL((k_costs_int<true, 21>), grid, dim3(64, rb), shm, st, cA, cB, base,
+                               g, R, pl.K, pl.Kp, (int)pl.invalid, (uint8_t*)C);
+        else if (k21)
+            hipLaunchKernelGGL((k_costs_int<false, 21>), grid, dim3(64, rb), shm, st, cA, cB, base,
+                               g, R, pl.K, pl.Kp, (int)pl.invalid, (uint8_t*)C);
+        else if (right)
+            hipLaunchKernelGGL((k_costs_int<true, 0>), grid, dim3(64, rb), shm, st, cA, cB, base,
+                               g, R, pl.K, pl.Kp, (int)pl.invalid, (uint8_t*)C);
         else
-            hipLaunchKernelGGL(k_costs_int<false>, grid, dim3(64, rb), shm, st, cA, cB, base, g, R,
-                               pl.K, pl.Kp, (int)pl.invalid, (uint8_t*)C);
+            hipLaunchKernelGGL((k_costs_int<false, 0>), grid, dim3(64, rb), shm, st, cA, cB, base,
+                               g, R, pl.K, pl.Kp, (int)pl.invalid, (uint8_t*)C);
         SV_CHECK_LAUNCH();
         return SV_OK;
     }

@@ -195,6 +195,24 @@ def test_u16_path_large_p2(sv):
     assert np.array_equal(out.cpu().numpy().ravel(), oracle.refine_fullres(gl, gr, p["disp0"], W, H, **kw))
 
 
+@pytest.mark.parametrize("W,H", [(460, 370), (203, 77), (64, 32), (65, 33), (7, 5), (9, 1),
+                                 (1, 6), (4, 40), (130, 2)])
+def test_census_9x7_uint8_matches_oracle(sv, W, H):
+    """The 9x7 census of 8-bit images (four pixels per thread, packed
+    compares) against the oracle on the gray image: partial pixel groups,
+    tiles and rows, images narrower than the window, ties (few gray
+    levels), three images per launch."""
+    print("census 9x7", W, H, flush=True)
+    rng = np.random.default_rng(W * 1000 + H)
+    imgs = [rng.integers(0, 256, size=(H, W)), rng.integers(0, 4, size=(H, W)),
+            np.full((H, W), 255)]
+    batch = torch.as_tensor(np.stack(imgs).astype(np.uint8)).cuda()
+    got = sv.census(batch, W, H, 4, 3).cpu().numpy().view(np.uint64).reshape(3, -1)
+    for n, img in enumerate(imgs):
+        ref = oracle.census(img.astype(np.float64).ravel() / 255.0, W, H, 4, 3)
+        assert np.array_equal(got[n], np.asarray(ref, dtype=np.uint64).ravel()), n
+
+
 def test_errors(sv):
     img = torch.zeros(20 * 10, dtype=torch.uint8, device="cuda")
     with pytest.raises(NotImplementedError):
