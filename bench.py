@@ -19,6 +19,7 @@ bounded sample of the same workload).
 """
 
 import argparse
+import gc
 import importlib
 import json
 import os
@@ -77,6 +78,9 @@ def main():
                     help="graph replay: cut the batch into this many equal parts, each on its "
                          "own stream with its own network buffers, captured together so that one "
                          "part's kernels fill the other's gaps (1: the eager step as it is)")
+    ap.add_argument("--tune-only", action="store_true",
+                    help="autotune, write the choices to --tune-file and exit (the default "
+                         "run tunes this way in a child process, then loads the file)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's built-in kernel variant heuristics")
     ap.add_argument("--tune", default=None,
@@ -97,6 +101,24 @@ def main():
         return widened_main(args)
     if args.workload == "band":
         return band_main(args)
+    if args.tune_only and not args.tune_file:
+        print("--tune-only needs --tune-file", file=sys.stderr)
+        sys.exit(2)
+    if not (args.tune_only or args.tune or args.no_autotune
+            or (args.tune_file and os.path.exists(args.tune_file))):
+        # tune in a child process and time in this one with the saved
+        # choices: a graph replayed in the process that just ran the tuner
+        # measured 2-3% slower than the same choices loaded in a fresh one
+        import subprocess
+        import tempfile
+        path = args.tune_file or os.path.join(tempfile.mkdtemp(prefix="svtune"), "tune.json")
+        cmd = [sys.executable, os.path.abspath(__file__), "--tune-only", "--tune-file", path,
+               "--batch", str(args.batch), "--streams", str(args.streams)]
+        if args.eager:
+            cmd.append("--eager")
+        log("autotuning in a child process: %s" % " ".join(cmd[1:]))
+        subprocess.run(cmd, check=True, stdout=sys.stderr)
+        args.tune_file = path
 
     import numpy as np
     import torch
@@ -139,13 +161,13 @@ def main():
 
     # the parted step: part i (pairs i*hb ...) on streams[i] with nets[i]
     # (activation buffers are per network object, scratch per stream)
-    nets = [N.load_model(wpath) for _ in range(S)] if S > 1 else []
+    nets = []  # made after tuning, so their buffers sit in a settled allocator
     streams = [torch.cuda.Stream() for _ in range(S)] if S > 1 else []
     lvh = N._Level([(H, W)] * hb, dev)
 
-    def part(i):
+    def part(i, nt=None):
         sl = slice(i * hb, (i + 1) * hb)
-        res = N.predict_auto_batch(nets[i], L8[sl], R8[sl], W, H)
+        res = N.predict_auto_batch(nt or nets[i], L8[sl], R8[sl], W, H)
         N._call("sv_widen", N._p(res.full), N._p(lvh.table), H * W, N._p(d0[sl]), H * W, None,
                 hb, H * W, L.stream())
         return sv.refine_disparity(L8[sl], R8[sl], None, d0[sl], W, H, planes=False)
@@ -178,12 +200,14 @@ def main():
                        log=log)
         tuned["conv_layers_tuned"] = len(layers)
     # the parts share the per-layer choices (keyed by launch shape, batch
-    # included); shapes of the part batch get their own when missing
-    for nt in nets:
-        nt._conv_tune = net._conv_tune
-    if nets and not args.no_autotune and not args.tune \
+    # included); shapes of the part batch get their own when missing,
+    # tuned on a network object of their own
+    if S > 1 and not args.no_autotune and not args.tune \
             and not any(k[2] == hb for k in net._conv_tune):
-        more = A.tune_layers(nets[0], lambda: part(0), reps=2)
+        tnet = N.load_model(wpath)
+        tnet._conv_tune = net._conv_tune
+        more = A.tune_layers(tnet, lambda: part(0, tnet), reps=2)
+        del tnet
         # (tune_layers leaves every knob at its default)
         A.apply({k: v for k, v in (tuned or {}).items() if k in A.KNOBS})
         if tuned is not None:
@@ -194,6 +218,17 @@ def main():
     if args.tune_file and not os.path.exists(args.tune_file) and tuned is not None \
             and not args.tune:
         A.save_choices(net, tuned, args.tune_file)
+    if args.tune_only:
+        log("kernel variants written to %s" % args.tune_file)
+        return
+    # return what tuning left cached before the parts' buffers are made
+    gc.collect()
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    for _ in range(S if S > 1 else 0):
+        nt = N.load_model(wpath)
+        nt._conv_tune = net._conv_tune
+        nets.append(nt)
     log("warmup %d" % args.warmup)
     for _ in range(args.warmup):
         step()

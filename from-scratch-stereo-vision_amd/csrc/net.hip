@@ -926,25 +926,74 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 }
 
 // ------------------------------------------------------------- selection
-// k-th smallest value per sample (sorted(values)[k], _peak :3866-3869) by
-// a 4-pass 8-bit radix select over order-preserving keys.
+// k-th smallest value per sample (sorted(values)[k], _peak :3866-3869) by a
+// radix select over order-preserving keys: three passes of 11, 11 and 10
+// bits, most significant first.  Pass q histograms the keys that match the
+// prefix chosen so far; the next launch moves the prefix to the bucket that
+// holds the k-th key (every block of a sample computes the same bucket from
+// the same counts, and block 0 records it for the launch after).
 __device__ __forceinline__ uint32_t fkey(float f) {
     const uint32_t u = __float_as_uint(f);
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
-// Per 8-bit pass, every block of a sample adds its LDS
-// histogram of the keys matching the prefix into the sample's global
-// histogram (integer atomics, so the counts are exact); k_sel_pick then
-// moves the prefix to the bucket holding the k-th key.
+constexpr int SEL_PASSES = 3;
+constexpr int SEL_BINS = 2048;
+__device__ __forceinline__ int sel_shift(int q) { return q == 0 ? 21 : (q == 1 ? 10 : 0); }
+__device__ __forceinline__ int sel_bins(int q) { return q == 2 ? 1024 : 2048; }
+
 struct SelState {
     uint32_t prefix, k;
 };
 
+// (prefix, k) moved to the bucket of pass q's histogram that holds the k-th
+// key: the bucket with excl <= k < incl, or the last bucket when k is past
+// the end (as a serial scan that stops at the last bucket would).  Called by
+// all 256 threads; each gets the result.
+__device__ SelState sel_pick(const uint32_t* __restrict__ hist, int q, SelState s,
+                             uint32_t* wsum, SelState* res) {
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int nb = sel_bins(q), per = nb / 256;
+    uint32_t c[SEL_BINS / 256];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < SEL_BINS / 256; ++j) {
+        c[j] = j < per ? hist[t * per + j] : 0u;
+        sum += c[j];
+    }
+    uint32_t v = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+    }
+    if (lane == 63) wsum[wave] = v;
+    __syncthreads();
+    uint32_t run = v - sum;
+    for (int w = 0; w < wave; ++w) run += wsum[w];
+#pragma unroll
+    for (int j = 0; j < SEL_BINS / 256; ++j) {
+        if (j < per) {
+            const uint32_t excl = run, incl = run + c[j];
+            const int b = t * per + j;
+            if ((excl <= s.k && s.k < incl) || (b == nb - 1 && incl <= s.k)) {
+                res->prefix = s.prefix | ((uint32_t)b << sel_shift(q));
+                res->k = s.k - excl;
+            }
+            run = incl;
+        }
+    }
+    __syncthreads();
+    const SelState r = *res;
+    __syncthreads();
+    return r;
+}
+
+// ghist: [SEL_PASSES][N][SEL_BINS] counts, st: [SEL_PASSES][N] states
 __global__ void k_sel_init(const long long* __restrict__ kth, SelState* __restrict__ st,
                            uint32_t* __restrict__ ghist, int N) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < N * 256) ghist[i] = 0;
+    const long long i = blockIdx.x * 256LL + threadIdx.x;
+    if (i < (long long)SEL_PASSES * N * SEL_BINS) ghist[i] = 0;
     if (i < N) {
         st[i].prefix = 0;
         st[i].k = (uint32_t)kth[i];
@@ -953,57 +1002,73 @@ __global__ void k_sel_init(const long long* __restrict__ kth, SelState* __restri
 
 __global__ void __launch_bounds__(256) k_sel_hist(const float* __restrict__ x,
                                                   const int* __restrict__ dims, long long stride,
-                                                  const SelState* __restrict__ st, int shift,
-                                                  uint32_t* __restrict__ ghist) {
-    __shared__ uint32_t h[256];
+                                                  SelState* __restrict__ st,
+                                                  uint32_t* __restrict__ ghist, int q, int N) {
+    __shared__ uint32_t h[SEL_BINS];
+    __shared__ uint32_t wsum[4];
+    __shared__ SelState res;
     const int n = blockIdx.y;
-    h[threadIdx.x] = 0;
-    __syncthreads();
+    const int t = threadIdx.x;
+    for (int i = t; i < SEL_BINS; i += 256) h[i] = 0;
     const long long m = (long long)dh(dims, n) * dw(dims, n);
     const float* xs = x + (long long)n * stride;
-    const uint32_t prefix = st[n].prefix;
-    const uint32_t mask = shift == 24 ? 0u : (0xFFFFFFFFu << (shift + 8));
-    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < m; i += (long long)gridDim.x * 256) {
-        const uint32_t k = fkey(xs[i]);
-        if ((k & mask) == (prefix & mask)) atomicAdd(&h[(k >> shift) & 0xFFu], 1u);
+    // this block's first chunk of 256 x U values is requested before the
+    // bucket choice (which waits on the previous pass's counts)
+    constexpr int U = 16;
+    constexpr long long CH = 256LL * U;
+    long long b0 = (long long)blockIdx.x * CH;
+    float v[U];
+    auto load = [&]() {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long i = b0 + t + 256LL * u;
+            v[u] = i < m ? xs[i] : 0.f;
+        }
+    };
+    load();
+    SelState s = st[(q == 0 ? 0 : q - 1) * N + n];
+    if (q > 0) {
+        s = sel_pick(ghist + ((long long)(q - 1) * N + n) * SEL_BINS, q - 1, s, wsum, &res);
+        if (blockIdx.x == 0 && t == 0) st[(long long)q * N + n] = s;
     }
     __syncthreads();
-    const uint32_t c = h[threadIdx.x];
-    if (c) atomicAdd(&ghist[n * 256 + threadIdx.x], c);
+    const int shift = sel_shift(q);
+    const uint32_t bmask = (uint32_t)sel_bins(q) - 1u;
+    const uint32_t mask = q == 0 ? 0u : (0xFFFFFFFFu << (shift + (q == 2 ? 10 : 11)));
+    const uint32_t want = s.prefix & mask;
+    while (b0 < m) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t k = fkey(v[u]);
+            const uint32_t bin = (k >> shift) & bmask;
+            const bool act = b0 + t + 256LL * u < m && (k & mask) == want;
+            if (act) atomicAdd(&h[bin], 1u);
+        }
+        b0 += (long long)gridDim.x * CH;
+        if (b0 < m) load();
+    }
+    __syncthreads();
+    uint32_t* gh = ghist + ((long long)q * N + n) * SEL_BINS;
+    for (int i = t; i <= (int)bmask; i += 256) {
+        const uint32_t c = h[i];
+        if (c) atomicAdd(&gh[i], c);
+    }
 }
 
-// One block of 256 threads per sample: an exclusive scan of the 256 bucket
-// counts finds the bucket holding the k-th key (the last bucket when k is
-// past the end, as a serial scan that stops at 255 would).
-__global__ void __launch_bounds__(256) k_sel_pick(SelState* __restrict__ st,
-                                                  uint32_t* __restrict__ ghist, int shift,
-                                                  float* __restrict__ out, int N) {
-    __shared__ uint32_t sc[256];
+// the last pass's pick; one block of 256 threads per sample
+__global__ void __launch_bounds__(256) k_sel_final(const SelState* __restrict__ st,
+                                                   const uint32_t* __restrict__ ghist,
+                                                   float* __restrict__ out, int N) {
+    __shared__ uint32_t wsum[4];
+    __shared__ SelState res;
     const int n = blockIdx.x;
-    const int t = threadIdx.x;
-    uint32_t* h = ghist + (long long)n * 256;
-    const uint32_t kk = st[n].k, pre0 = st[n].prefix;  // read before any thread updates them
-    const uint32_t c = h[t];
-    sc[t] = c;
-    __syncthreads();
-    // Hillis-Steele inclusive scan
-    for (int o = 1; o < 256; o <<= 1) {
-        const uint32_t v = t >= o ? sc[t - o] : 0u;
-        __syncthreads();
-        sc[t] += v;
-        __syncthreads();
-    }
-    const uint32_t incl = sc[t], excl = incl - c;
-    const bool mine = (excl <= kk && kk < incl) || (t == 255 && incl <= kk);
-    h[t] = 0;  // ready for the next pass
-    if (mine) {
-        const uint32_t pre = pre0 | ((uint32_t)t << shift);
-        st[n].k = kk - excl;
-        st[n].prefix = pre;
-        if (shift == 0) {
-            const uint32_t u = (pre & 0x80000000u) ? (pre & 0x7FFFFFFFu) : ~pre;
-            out[n] = __uint_as_float(u);
-        }
+    constexpr int q = SEL_PASSES - 1;
+    const SelState s = sel_pick(ghist + ((long long)q * N + n) * SEL_BINS, q,
+                                st[(long long)q * N + n], wsum, &res);
+    if (threadIdx.x == 0) {
+        const uint32_t pre = s.prefix;
+        const uint32_t u = (pre & 0x80000000u) ? (pre & 0x7FFFFFFFu) : ~pre;
+        out[n] = __uint_as_float(u);
     }
 }
 
@@ -1507,7 +1572,7 @@ int sv_adam_step(float* p, const float* g, float* m, float* v, long long n, floa
 
 int sv_select_kth_workspace_bytes(int N, size_t* bytes) {
     SV_REQUIRE(bytes && N > 0, "bad arguments");
-    *bytes = (size_t)N * (sizeof(SelState) + 256 * sizeof(uint32_t));
+    *bytes = (size_t)N * SEL_PASSES * (sizeof(SelState) + SEL_BINS * sizeof(uint32_t));
     return SV_OK;
 }
 
@@ -1522,16 +1587,20 @@ int sv_select_kth(const float* x, const int* dims, long long stride, const long 
     if (!ws || ws_bytes < need) return sv::set_error(SV_EWORKSPACE, "sv_select_kth needs %zu bytes", need);
     hipStream_t st = (hipStream_t)stream;
     uint32_t* ghist = (uint32_t*)ws;
-    SelState* state = (SelState*)(ghist + (size_t)N * 256);
-    hipLaunchKernelGGL(k_sel_init, dim3((N * 256 + 255) / 256), dim3(256), 0, st, kth, state, ghist, N);
+    SelState* state = (SelState*)(ghist + (size_t)SEL_PASSES * N * SEL_BINS);
+    const long long ninit = (long long)SEL_PASSES * N * SEL_BINS;
+    hipLaunchKernelGGL(k_sel_init, dim3((unsigned)((ninit + 255) / 256)), dim3(256), 0, st, kth,
+                       state, ghist, N);
     SV_CHECK_LAUNCH();
-    const int nblk = (int)std::min<long long>(64, std::max<long long>(1, max_count / 4096));
-    for (int shift = 24; shift >= 0; shift -= 8) {
-        hipLaunchKernelGGL(k_sel_hist, dim3(nblk, N), dim3(256), 0, st, x, dims, stride, state, shift, ghist);
-        SV_CHECK_LAUNCH();
-        hipLaunchKernelGGL(k_sel_pick, dim3(N), dim3(256), 0, st, state, ghist, shift, out, N);
+    // one chunk of 4096 values per block up to 64 blocks per sample
+    const int nblk = (int)std::min<long long>(64, std::max<long long>(1, (max_count + 4095) / 4096));
+    for (int q = 0; q < SEL_PASSES; ++q) {
+        hipLaunchKernelGGL(k_sel_hist, dim3(nblk, N), dim3(256), 0, st, x, dims, stride, state,
+                           ghist, q, N);
         SV_CHECK_LAUNCH();
     }
+    hipLaunchKernelGGL(k_sel_final, dim3(N), dim3(256), 0, st, state, ghist, out, N);
+    SV_CHECK_LAUNCH();
     return SV_OK;
 }
 

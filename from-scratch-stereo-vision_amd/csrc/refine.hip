@@ -2097,8 +2097,16 @@ int sgm_k_t(const Plan& pl, const void* C, const int* base, void* L, int BV, con
     if (sizeof(LT) == 1 && pl.invalid + P2 + P1 < 65000.0 && pl.Kp >= 8 * ((K + 1 + 7) / 8)) {
         const int hEnd = min(dir0 + ndir, 2);
         const int vBeg = max(dir0, 2), vEnd = dir0 + ndir;
-        // both kinds of path in this call: horizontal ones on the side stream
-        const bool fork = hEnd > dir0 && vEnd > vBeg && sv::tune(SV_TUNE_SGM_OVERLAP) == 2;
+        // both kinds of path in this call: horizontal ones on the side stream,
+        // except while the caller's stream is being captured into a graph
+        // (the one side stream per device would then join every capturing
+        // stream that reaches here; such captures run both kinds in order)
+        bool fork = hEnd > dir0 && vEnd > vBeg && sv::tune(SV_TUNE_SGM_OVERLAP) == 2;
+        if (fork) {
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+                fork = false;
+        }
         std::unique_lock<std::mutex> lock(g_side_mu, std::defer_lock);
         SideStream* ss = nullptr;
         hipStream_t hst = st;

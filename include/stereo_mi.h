@@ -133,7 +133,8 @@ int sv_refine_ragged(int mode, const void* left, const void* right, int dtype, c
  * the tiled convolution (1, 4); 3: output columns per tile of the tiled
  * convolution (16, 32, 64); 4: weight source of the tiled convolution (1
  * LDS, 2 scalar cache); 5: horizontal SGM paths after the others (1) or on a
- * second, library-owned stream beside them (2); 6: convolution engine (1
+ * second, library-owned stream beside them (2; not while the caller's
+ * stream is being captured into a graph: then after the others); 6: convolution engine (1
  * vector ALUs, 2 matrix cores with 3D layers sliding over depth, 3 matrix
  * cores with one depth per tile, 4 vector ALUs staging one input channel at
  * a time); 7: 64-pixel groups per wave of the

@@ -105,11 +105,25 @@ def test_captured_pipeline_replays_match_eager():
     assert _same(b, st, ref_st)
 
 
-def test_parts_on_two_streams_captured_match_eager():
+@pytest.mark.parametrize("overlap", [1, 2])
+def test_parts_on_two_streams_captured_match_eager(overlap):
     """The batch cut into two parts, each with its own network object on its
     own stream (scratch buffers are per stream), both captured in one graph:
-    every replay equals the single-stream eager run, pair by pair."""
+    every replay equals the single-stream eager run, pair by pair.  With
+    the horizontal SGM paths on the library's side stream (overlap 2) the
+    eager runs fork onto it and the captured ones keep to their own
+    streams."""
+    print("parts on two streams, sgm_overlap", overlap, flush=True)
     R, N = _mods()
+    A = importlib.import_module("from-scratch-stereo-vision_amd.autotune")
+    A.set_knob("sgm_overlap", overlap)
+    try:
+        _parts_on_two_streams(R, N)
+    finally:
+        A.reset()
+
+
+def _parts_on_two_streams(R, N):
     ps = _pairs()
     path = package().__path__[0] + "/weights/stereonet_c5.txt"
     b = R.RaggedBatch(SIZES)
