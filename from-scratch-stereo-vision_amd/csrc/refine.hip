@@ -327,20 +327,54 @@ __global__ void __launch_bounds__(256) k_costs_int(const uint64_t* __restrict__ 
     if (x0 >= W || y >= g.h(blockIdx.z)) return;  // whole wave outside this image
     const long long plane = g.slot;
     const long long row0 = (long long)blockIdx.z * plane + (long long)y * W;
-    if (x < W) {
-        const long long pix = row0 + x;
+    if constexpr (KT > 0) {
+        // The wave's partner words (every lane's KT columns, clamped to the
+        // row) usually span a short run of the row: it is read once with
+        // coalesced loads into the wave's LDS window and each lane takes its
+        // KT words from there; a longer run is read per lane from memory.
+        // Every lane takes part (a lane past the row stands in for the last
+        // pixel and writes nothing).
+        constexpr int KPT = (KT + 7) / 8 * 8;
+        constexpr int WIN = 256;  // words per wave window
+        uint64_t* win = reinterpret_cast<uint64_t*>(cstage_all + blockDim.y * 64 * (KPT / 4)) +
+                        threadIdx.y * WIN;
+        const bool live = x < W;
+        const int xc = live ? x : W - 1;
+        const long long pix = row0 + xc;
         const uint64_t a = cA[pix];
         const int b0 = base[pix] - R;
         const uint64_t* rowB = cB + row0;
-        if constexpr (KT > 0) {
-            constexpr int KPT = (KT + 7) / 8 * 8;
-            uint64_t w[KT];
+        const int e0 = RIGHT ? xc + b0 : xc - b0 - (KT - 1);
+        int wlo = min(max(e0, 0), W - 1), whi = min(max(e0 + KT - 1, 0), W - 1);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            wlo = min(wlo, __shfl_xor(wlo, o, 64));
+            whi = max(whi, __shfl_xor(whi, o, 64));
+        }
+        wlo = __builtin_amdgcn_readfirstlane(wlo);
+        whi = __builtin_amdgcn_readfirstlane(whi);
+        uint64_t w[KT];
+        if (whi - wlo < WIN) {
+            for (int i = 0; wlo + 64 * i <= whi; ++i) {
+                const int e = wlo + 64 * i + lane;
+                if (e <= whi) win[64 * i + lane] = rowB[e];
+            }
+            __builtin_amdgcn_wave_barrier();
 #pragma unroll
             for (int k = 0; k < KT; ++k) {
                 const int d = b0 + k;
-                const int xr = RIGHT ? x + d : x - d;
+                const int xr = RIGHT ? xc + d : xc - d;
+                w[k] = win[min(max(xr, 0), W - 1) - wlo];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < KT; ++k) {
+                const int d = b0 + k;
+                const int xr = RIGHT ? xc + d : xc - d;
                 w[k] = rowB[min(max(xr, 0), W - 1)];
             }
+        }
+        if (live) {
 #pragma unroll
             for (int k0 = 0; k0 < KPT; k0 += 4) {
                 uint32_t packed = 0;
@@ -357,23 +391,27 @@ __global__ void __launch_bounds__(256) k_costs_int(const uint64_t* __restrict__ 
                 }
                 cstage[lane * (KPT / 4) + k0 / 4] = packed;
             }
-        } else {
-            for (int k0 = 0; k0 < Kp; k0 += 4) {
-                uint32_t packed = 0;
+        }
+    } else if (x < W) {
+        const long long pix = row0 + x;
+        const uint64_t a = cA[pix];
+        const int b0 = base[pix] - R;
+        const uint64_t* rowB = cB + row0;
+        for (int k0 = 0; k0 < Kp; k0 += 4) {
+            uint32_t packed = 0;
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int k = k0 + j;
-                    uint32_t v = 0;
-                    if (k < K) {
-                        const int d = b0 + k;
-                        const int xr = RIGHT ? x + d : x - d;
-                        const bool ok = d >= 0 && xr >= 0 && xr < W;
-                        v = ok ? (uint32_t)__popcll(a ^ rowB[ok ? xr : 0]) : (uint32_t)invalid;
-                    }
-                    packed |= v << (8 * j);
+            for (int j = 0; j < 4; ++j) {
+                const int k = k0 + j;
+                uint32_t v = 0;
+                if (k < K) {
+                    const int d = b0 + k;
+                    const int xr = RIGHT ? x + d : x - d;
+                    const bool ok = d >= 0 && xr >= 0 && xr < W;
+                    v = ok ? (uint32_t)__popcll(a ^ rowB[ok ? xr : 0]) : (uint32_t)invalid;
                 }
-                cstage[lane * (Kp / 4) + k0 / 4] = packed;
+                packed |= v << (8 * j);
             }
+            cstage[lane * (Kp / 4) + k0 / 4] = packed;
         }
     }
     __builtin_amdgcn_wave_barrier();
@@ -1989,8 +2027,9 @@ int launch_costs(const Plan& pl, bool right, const uint64_t* cA, const uint64_t*
         // four rows per block while their staging fits 64 KB of LDS
         const int rb = 4 * 64 * pl.Kp <= 64 * 1024 ? 4 : 1;
         dim3 grid((g.W + 63) / 64, (g.H + rb - 1) / rb, B);
-        const size_t shm = (size_t)rb * 64 * pl.Kp;
         const bool k21 = pl.K == 21 && pl.Kp == 24;
+        // the fixed-length kernel adds a 256-word partner window per wave
+        const size_t shm = (size_t)rb * 64 * pl.Kp + (k21 ? (size_t)rb * 256 * 8 : 0);
         if (right && k21)
             hipLaunchKernelGGL((k_costs_int<true, 21>), grid, dim3(64, rb), shm, st, cA, cB, base,
                                g, R, pl.K, pl.Kp, (int)pl.invalid, (uint8_t*)C);
