@@ -15,6 +15,8 @@ from .net import (StereoNet, Adam, LEVEL_WEIGHTS, CE_WEIGHT, train_grads, train_
                   save_weights, read_checkpoint_config, load_weights, load_model, predict,
                   predict_batch, predict_at_scale, predict_auto, predict_auto_batch, widen)
 from .ragged import RaggedBatch, refine_ragged, net_band_ragged, CapturedPipeline
+from .consistency import (mirror_h, lr_consistency, fill_invalid, predict_consistent,
+                          predict_consistent_batch)
 from . import autotune, scenes, synth
 
 __all__ = ["census", "band_costs", "_aggregate_path", "refine_fullres", "_mirror1",
@@ -23,4 +25,6 @@ __all__ = ["census", "band_costs", "_aggregate_path", "refine_fullres", "_mirror
            "flat_params", "set_flat_params", "flat_grads", "set_flat_grads", "save_weights",
            "read_checkpoint_config", "load_weights", "load_model", "predict", "predict_batch",
            "predict_at_scale", "predict_auto", "predict_auto_batch", "widen", "RaggedBatch",
-           "refine_ragged", "net_band_ragged", "CapturedPipeline", "autotune", "scenes", "synth"]
+           "refine_ragged", "net_band_ragged", "CapturedPipeline", "mirror_h", "lr_consistency",
+           "fill_invalid", "predict_consistent", "predict_consistent_batch", "autotune", "scenes",
+           "synth"]

@@ -2379,6 +2379,137 @@ int refine_views(const Plan& pl, const RefineBufs& bf, const void* left, const v
 }  // namespace
 
 // =============================================================== C ABI
+// -------------------------------------- network left-right consistency
+// mirror_h, stereonet.py:4974-4983: `rows` rows of W elements, each reversed
+// (every channel of every image of a dense planar buffer).
+template <typename T>
+__global__ void k_mirror_h(const T* __restrict__ in, long long rows, int W, T* __restrict__ out) {
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= W) return;
+    for (long long r = blockIdx.y; r < rows; r += gridDim.y)
+        out[r * W + x] = in[r * W + (W - 1 - x)];
+}
+
+__global__ void k_zero_int(int* __restrict__ p, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// lr_consistency, stereonet.py:4986-5008, in float64 from T maps: valid = 0
+// where x - d leaves [0, W-1] (or is NaN, where the reference's int()
+// raises) or |d - dr(x - d)| > tol, dr interpolated linearly between
+// columns int(x - d) and the next (clamped to the row).  counts[b] (when
+// given) receives the image's agreeing pixels: one atomic per wave.
+template <typename T>
+__global__ void k_lr_consistency(const T* __restrict__ dl, const T* __restrict__ dr, int W, int H,
+                                 double tol, uint8_t* __restrict__ valid, int* __restrict__ counts) {
+    const int b = blockIdx.z, y = blockIdx.y;
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    bool ok = false;
+    if (x < W) {
+        const long long row = ((long long)b * H + y) * W;
+        const double d = (double)dl[row + x];
+        const double xr = (double)x - d;
+        if (!(xr < 0.0 || xr > (double)(W - 1) || xr != xr)) {
+            const int x0 = (int)xr;
+            const int x1 = x0 + 1 < W ? x0 + 1 : x0;
+            const double f = xr - (double)x0;
+            const double v = (double)dr[row + x0] * (1.0 - f) + (double)dr[row + x1] * f;
+            double diff = d - v;
+            if (diff < 0.0) diff = -diff;
+            ok = !(diff > tol);
+        }
+        valid[row + x] = ok ? 1 : 0;
+    }
+    if (counts) {
+        const uint64_t m = __ballot(ok);
+        if ((threadIdx.x & 63) == 0 && m) atomicAdd(&counts[b], (int)__popcll(m));
+    }
+}
+
+// fill_invalid, stereonet.py:5011-5046: one wave per row, 64 columns per
+// step.  A max-scan of (valid ? x : -1) gives each pixel's nearest valid
+// column at or left of it (left to right, each step's carry kept in LDS),
+// a min-scan of (valid ? x : W) from the right the nearest at or right of
+// it; an invalid pixel takes the smaller of the two neighbours' values (the
+// one there is when only one side has one), and a row without a valid pixel
+// is copied.
+constexpr int FILL_STEPS = 128;  // rows up to 8192 columns keep every carry
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_fill_invalid(const T* __restrict__ disp,
+                                                      const uint8_t* __restrict__ valid, int W,
+                                                      int rows, T* __restrict__ out) {
+    __shared__ int lcarry[4][FILL_STEPS];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = blockIdx.x * 4 + wave;
+    if (r >= rows) return;  // whole wave
+    const long long row = (long long)r * W;
+    // left to right: copy every pixel, record the carry before each step
+    int carry = -1;
+    for (int c0 = 0; c0 < W; c0 += 64) {
+        const int x = c0 + lane;
+        if (lane == 0 && c0 / 64 < FILL_STEPS) lcarry[wave][c0 / 64] = carry;
+        int v = (x < W && valid[row + x]) ? x : -1;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(v, o, 64);
+            if (lane >= o) v = max(v, t);
+        }
+        carry = max(__shfl(v, 63, 64), carry);
+        if (x < W) out[row + x] = disp[row + x];
+    }
+    __builtin_amdgcn_wave_barrier();
+    // right to left: nearest valid on each side, then the fill
+    carry = W;
+    for (int c0 = (W - 1) / 64 * 64; c0 >= 0; c0 -= 64) {
+        const int x = c0 + lane;
+        const bool in = x < W;
+        const bool vx = in && valid[row + x];
+        int rt = vx ? x : W, lt = vx ? x : -1;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int a = __shfl_down(rt, o, 64);
+            if (lane + o < 64) rt = min(rt, a);
+            const int bq = __shfl_up(lt, o, 64);
+            if (lane >= o) lt = max(lt, bq);
+        }
+        rt = min(rt, carry);
+        carry = __shfl(rt, 0, 64);
+        if (in && !vx) {
+            if (lt < 0) {
+                if (c0 / 64 < FILL_STEPS) {
+                    lt = lcarry[wave][c0 / 64];
+                } else {
+                    for (int xx = c0 - 1; xx >= 0; --xx)
+                        if (valid[row + xx]) {
+                            lt = xx;
+                            break;
+                        }
+                }
+            }
+            const bool hl = lt >= 0, hr = rt < W;
+            if (hl || hr) {
+                const T a = hl ? disp[row + lt] : T(0), bb = hr ? disp[row + rt] : T(0);
+                out[row + x] = !hl ? bb : (!hr ? a : (a < bb ? a : bb));
+            }
+        }
+    }
+}
+
+template <typename T>
+static int consistency_t(const void* dl, const void* dr, int B, int W, int H, double tol,
+                         uint8_t* valid, int* counts, hipStream_t st) {
+    if (counts) {
+        hipLaunchKernelGGL(k_zero_int, dim3((B + 255) / 256), dim3(256), 0, st, counts, B);
+        SV_CHECK_LAUNCH();
+    }
+    hipLaunchKernelGGL((k_lr_consistency<T>), dim3((W + 255) / 256, H, B), dim3(256), 0, st,
+                       (const T*)dl, (const T*)dr, W, H, tol, valid, counts);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
 extern "C" {
 
 int sv_census(const void* img, int dtype, int n_images, int W, int H, int rx, int ry,
@@ -2646,6 +2777,61 @@ int sv_median_filter(const double* disp, int B, int W, int H, int radius, int pa
                      double* out, double* tmp, void* stream) {
     SV_REQUIRE(disp && out && B > 0 && W > 0 && H > 0, "sv_median_filter: bad arguments");
     return launch_median(disp, uniform_geom(B, W, H), radius, passes, out, tmp, (hipStream_t)stream);
+}
+
+int sv_mirror_h(const void* in, int dtype, long long rows, int W, void* out, void* stream) {
+    SV_REQUIRE(in && out && rows > 0 && W > 0, "sv_mirror_h: bad arguments");
+    SV_REQUIRE(in != out, "sv_mirror_h: out must not alias in");
+    hipStream_t st = (hipStream_t)stream;
+    dim3 grid((W + 255) / 256, (unsigned)std::min<long long>(rows, 65535));
+    switch (dtype) {
+        case SV_U8:
+            hipLaunchKernelGGL((k_mirror_h<uint8_t>), grid, dim3(256), 0, st, (const uint8_t*)in,
+                               rows, W, (uint8_t*)out);
+            break;
+        case SV_F32:
+            hipLaunchKernelGGL((k_mirror_h<float>), grid, dim3(256), 0, st, (const float*)in, rows,
+                               W, (float*)out);
+            break;
+        case SV_F64:
+            hipLaunchKernelGGL((k_mirror_h<double>), grid, dim3(256), 0, st, (const double*)in,
+                               rows, W, (double*)out);
+            break;
+        default:
+            return sv::set_error(SV_EINVAL, "sv_mirror_h: unknown dtype %d", dtype);
+    }
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+int sv_lr_consistency(const void* dl, const void* dr, int dtype, int B, int W, int H, double tol,
+                      uint8_t* valid, int* counts, void* stream) {
+    SV_REQUIRE(dl && dr && valid && B > 0 && W > 0 && H > 0, "sv_lr_consistency: bad arguments");
+    SV_REQUIRE(H <= 65535 && B <= 65535, "sv_lr_consistency: %d images of height %d", B, H);
+    hipStream_t st = (hipStream_t)stream;
+    if (dtype == SV_F32) return consistency_t<float>(dl, dr, B, W, H, tol, valid, counts, st);
+    if (dtype == SV_F64) return consistency_t<double>(dl, dr, B, W, H, tol, valid, counts, st);
+    return sv::set_error(SV_EINVAL, "sv_lr_consistency: maps must be f32 or f64 (dtype %d)", dtype);
+}
+
+int sv_fill_invalid(const void* disp, const uint8_t* valid, int dtype, int B, int W, int H,
+                    void* out, void* stream) {
+    SV_REQUIRE(disp && valid && out && B > 0 && W > 0 && H > 0, "sv_fill_invalid: bad arguments");
+    SV_REQUIRE(disp != out, "sv_fill_invalid: out must not alias disp");
+    const long long rows = (long long)B * H;
+    SV_REQUIRE(rows <= (long long)INT_MAX, "sv_fill_invalid: too many rows");
+    hipStream_t st = (hipStream_t)stream;
+    dim3 grid((unsigned)((rows + 3) / 4));
+    if (dtype == SV_F32)
+        hipLaunchKernelGGL((k_fill_invalid<float>), grid, dim3(256), 0, st, (const float*)disp,
+                           valid, W, (int)rows, (float*)out);
+    else if (dtype == SV_F64)
+        hipLaunchKernelGGL((k_fill_invalid<double>), grid, dim3(256), 0, st, (const double*)disp,
+                           valid, W, (int)rows, (double*)out);
+    else
+        return sv::set_error(SV_EINVAL, "sv_fill_invalid: maps must be f32 or f64 (dtype %d)", dtype);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
 }
 
 }  // extern "C"

@@ -103,6 +103,24 @@ int sv_lr_refine(const void* left, const void* right, int dtype, const double* d
 int sv_median_filter(const double* disp, int B, int W, int H, int radius, int passes,
                      double* out, double* tmp, void* stream);
 
+/* mirror_h(img, C, W, H), stereonet.py:4974-4983: each of `rows` rows of W
+ * elements reversed (rows = images x channels x H of a planar buffer).
+ * dtype SV_U8, SV_F32 or SV_F64; out must not alias in. */
+int sv_mirror_h(const void* in, int dtype, long long rows, int W, void* out, void* stream);
+
+/* lr_consistency(disp_l, disp_r, W, H, tol=1.5), stereonet.py:4986-5008,
+ * for B maps of W x H (dtype SV_F32 or SV_F64, evaluated in float64).
+ * valid: [B][H][W] u8, 1 where the maps agree.  counts (B ints, may be
+ * NULL) receives each map's number of agreeing pixels.  A NaN disparity,
+ * on which the reference raises, is marked invalid. */
+int sv_lr_consistency(const void* dl, const void* dr, int dtype, int B, int W, int H, double tol,
+                      uint8_t* valid, int* counts, void* stream);
+
+/* fill_invalid(disp, valid, W, H), stereonet.py:5011-5046, for B maps
+ * (dtype SV_F32 or SV_F64); out must not alias disp. */
+int sv_fill_invalid(const void* disp, const uint8_t* valid, int dtype, int B, int W, int H,
+                    void* out, void* stream);
+
 /* refine_disparity(left, right, rgb, disp0, W, H, R=10, P1=12.0, P2=150.0,
  * rx=4, ry=3, tol=2.0, planes=False), stereonet.py:5865-5884: lr_refine then
  * one 3x3 median pass. */

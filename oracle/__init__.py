@@ -81,6 +81,9 @@ def _declare(l):
         "or_peak": (D_, [P, L_, D_]),
         "or_predict_at_scale": (None, [I, I, I, I, I, P, P, P, P, I, I, I, P, P, P]),
         "or_predict_auto": (None, [I, I, I, I, I, P, P, D_, P, P, I, I, I, I, I, P, P, P, P]),
+        "or_mirror_h": (None, [P, I, I, I, P]),
+        "or_lr_consistency": (None, [P, P, I, I, D_, P]),
+        "or_fill_invalid": (None, [P, P, I, I, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(l, name)
@@ -424,3 +427,45 @@ def load_weights_txt(path):
         cnt = int(parts[0])
         blocks.append(np.array([float(v) for v in parts[1:cnt + 1]]))
     return cfg, blocks
+
+
+# ------------------------------------------------ network LR consistency
+
+def mirror_h(img, C, W, H):
+    """mirror_h, stereonet.py:4974-4983."""
+    img = _f64(img).ravel()
+    out = np.zeros(C * W * H)
+    lib().or_mirror_h(_p(img), C, W, H, _p(out))
+    return out
+
+
+def lr_consistency(disp_l, disp_r, W, H, tol=1.5):
+    """lr_consistency, stereonet.py:4986-5008 (bool array)."""
+    dl, dr = _f64(disp_l).ravel(), _f64(disp_r).ravel()
+    v = np.zeros(W * H, dtype=np.uint8)
+    lib().or_lr_consistency(_p(dl), _p(dr), W, H, float(tol), _p(v))
+    return v.astype(bool)
+
+
+def fill_invalid(disp, valid, W, H):
+    """fill_invalid, stereonet.py:5011-5046."""
+    d = _f64(disp).ravel()
+    v = np.ascontiguousarray(np.asarray(valid, dtype=bool).ravel()).astype(np.uint8)
+    out = np.zeros(W * H)
+    lib().or_fill_invalid(_p(d), _p(v), W, H, _p(out))
+    return out
+
+
+def predict_consistent(arch, params, left, right, W, H, work_width=None, tol=1.5):
+    """predict_consistent, stereonet.py:5049-5076: (disp, valid fraction),
+    plus the validity mask for the tests."""
+    if work_width is None:
+        d_l, _, Ww, _, _ = predict_auto(arch, params, left, right, W, H)
+    else:
+        d_l, _, Ww, _ = predict_at_scale(arch, params, left, right, W, H, work_width)
+    ml, mr = mirror_h(left, 1, W, H), mirror_h(right, 1, W, H)
+    d_rm, _, _, _ = predict_at_scale(arch, params, mr, ml, W, H, Ww)
+    d_r = mirror_h(d_rm, 1, W, H)
+    valid = lr_consistency(d_l, d_r, W, H, tol)
+    frac = int(valid.sum()) / float(W * H)
+    return fill_invalid(d_l, valid, W, H), frac, valid

@@ -22,6 +22,10 @@
  *   network layers: torch float64 library routines (conv2d, conv3d,
  *   interpolate, avg_pool2d, softmax) + central differences;
  *   predict_auto: the reference demo page's embedded predictions;
+ *   mirror_h, lr_consistency, fill_invalid, predict_consistent: golden
+ *   vectors written from the reference (tests/golden/consistency.npz) +
+ *   closed forms (a shift-consistent pair of maps is valid everywhere; a
+ *   double flip is the identity);
  *   train_grads: a golden vector written from the reference;
  *   Adam: torch.optim.Adam.
  */
@@ -1248,4 +1252,70 @@ EXPORT void or_predict_auto(int feat_C, int cost_C, int D, int refine_C,
                             right, W, H, work, out_full, out_work, dims);
     }
     free(lp); free(rp); free(dp);
+}
+
+/* ------------------------------------------------------------------ */
+/* Section 24: network left-right consistency (stereonet.py:4974-5076) */
+/* ------------------------------------------------------------------ */
+
+/* mirror_h, stereonet.py:4974-4983: every row of every channel reversed. */
+EXPORT void or_mirror_h(const double *img, int C, int W, int H, double *out) {
+    for (long r = 0; r < (long)C * H; r++) {
+        const double *src = img + r * W;
+        double *dst = out + r * W;
+        for (int x = 0; x < W; x++) dst[x] = src[W - 1 - x];
+    }
+}
+
+/* lr_consistency, stereonet.py:4986-5008.  valid[p] = 0 where x - d leaves
+ * [0, W-1] or |d - dr(x - d)| > tol, dr linearly interpolated between
+ * columns int(x - d) and the next (clamped).  A NaN disparity, on which the
+ * reference's int() raises, is marked invalid here. */
+EXPORT void or_lr_consistency(const double *dl, const double *dr, int W, int H, double tol,
+                              uint8_t *valid) {
+    for (int y = 0; y < H; y++) {
+        const long row = (long)y * W;
+        for (int x = 0; x < W; x++) {
+            const double d = dl[row + x];
+            const double xr = x - d;
+            if (xr < 0.0 || xr > W - 1 || xr != xr) {
+                valid[row + x] = 0;
+                continue;
+            }
+            const int x0 = (int)xr;
+            const int x1 = x0 + 1 < W ? x0 + 1 : x0;
+            const double f = xr - x0;
+            const double v = dr[row + x0] * (1.0 - f) + dr[row + x1] * f;
+            double diff = d - v;
+            if (diff < 0.0) diff = -diff;
+            valid[row + x] = diff > tol ? 0 : 1;
+        }
+    }
+}
+
+/* fill_invalid, stereonet.py:5011-5046: each run of invalid pixels takes
+ * the smaller of its nearest valid neighbours on the row (the one there is
+ * when only one side has one); a row with no valid pixel is left as is. */
+EXPORT void or_fill_invalid(const double *disp, const uint8_t *valid, int W, int H,
+                            double *out) {
+    memcpy(out, disp, sizeof(double) * (size_t)W * H);
+    for (int y = 0; y < H; y++) {
+        const long row = (long)y * W;
+        int x = 0;
+        while (x < W) {
+            if (valid[row + x]) {
+                x++;
+                continue;
+            }
+            const int start = x;
+            while (x < W && !valid[row + x]) x++;
+            const int end = x;
+            const int has_l = start > 0, has_r = end < W;
+            if (!has_l && !has_r) continue;
+            const double lv = has_l ? disp[row + start - 1] : 0.0;
+            const double rv = has_r ? disp[row + end] : 0.0;
+            const double fv = !has_l ? rv : (!has_r ? lv : (lv < rv ? lv : rv));
+            for (int i = start; i < end; i++) out[row + i] = fv;
+        }
+    }
 }

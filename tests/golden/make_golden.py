@@ -17,7 +17,11 @@ functions that no library routine, closed form or golden file pins:
 * left_to_right, lr_refine status/fill and median_filter border windows
   (:5317-5391, :2231-2258),
 * the train_grads loss composition (level weights, CE weight, pooled targets;
-  :1572-1653).
+  :1572-1653),
+* the network left-right consistency: mirror_h, lr_consistency (clamped
+  linear interpolation of the right map, the tolerance test) and
+  fill_invalid run rules (:4974-5046), and predict_consistent's composition
+  of them with predict_auto / predict_at_scale on a small pair (:5049-5076).
 
 Inputs come from this package's own seeded generator.  Outputs are written as
 compressed .npz files (no pickles).  The network forward, layer gradients and
@@ -216,5 +220,49 @@ def main(ref_root):
     print("wrote train_grads_tiny (loss %.6f)" % loss)
 
 
+def consistency(SN):
+    """consistency.npz: the three helpers on crafted maps, then
+    predict_consistent with the v5 weights on a 96x64 pair (automatic and
+    fixed working width)."""
+    rng = np.random.default_rng(24)
+    W, H = 41, 11
+    x = np.arange(W, dtype=np.float64)
+    dl = rng.uniform(-4.0, 44.0, size=(H, W))
+    dl[0] = x - (W - 1)                    # lands exactly on the last column
+    dl[1] = x                              # lands exactly on column 0
+    dl[2] = x + 0.5                        # just outside on the left
+    dl[3] = -100.0                         # a whole row outside: left as is
+    dl[4] = np.round(rng.uniform(0.0, 20.0, size=W) * 4) / 4
+    dl[5] = 7.25                           # constant, consistent with row 5 of dr
+    dr = rng.uniform(-4.0, 44.0, size=(H, W))
+    dr[4] = np.round(dl[4] + rng.choice([-1.5, 1.5, -0.75, 0.0, 2.0], size=W), 2)
+    dr[5] = 7.25
+    dr[6, :] = dl[6, :]                    # nearly consistent where d is small
+    img = rng.uniform(0.0, 1.0, size=2 * W * H)
+    Ld, Rd = list(map(float, dl.ravel())), list(map(float, dr.ravel()))
+    out = dict(W=W, H=H, dl=dl.ravel(), dr=dr.ravel(), img=img,
+               mirror=np.array(SN["mirror_h"](list(map(float, img)), 2, W, H)))
+    for name, tol in (("t15", 1.5), ("t075", 0.75)):
+        v = SN["lr_consistency"](Ld, Rd, W, H, tol)
+        out["valid_" + name] = np.array(v, dtype=bool)
+        out["fill_" + name] = np.array(SN["fill_invalid"](Ld, v, W, H), dtype=np.float64)
+    net = SN["load_model"](os.path.join(HERE, "stereonet_weights_v5.txt"))
+    Wp, Hp = 96, 64
+    left, right, _, _ = small_pair(31, Wp, Hp, 3.0, 14.0)
+    Lp, Rp = list(map(float, left.ravel())), list(map(float, right.ravel()))
+    out.update(Wp=Wp, Hp=Hp, left=left.ravel(), right=right.ravel())
+    for name, ww in (("auto", None), ("w64", 64)):
+        d, frac = SN["predict_consistent"](net, Lp, Rp, Wp, Hp, work_width=ww, verbose=False)
+        out["pc_" + name] = np.array(d, dtype=np.float64)
+        out["pcfrac_" + name] = float(frac)
+    np.savez_compressed(os.path.join(HERE, "consistency.npz"), **out)
+    print("wrote consistency (valid fractions %.4f %.4f)" % (out["pcfrac_auto"], out["pcfrac_w64"]))
+
+
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
+    ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+    if len(sys.argv) > 2 and sys.argv[2] == "consistency":
+        consistency(load_reference(ref))
+    else:
+        main(ref)
+        consistency(load_reference(ref))

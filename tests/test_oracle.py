@@ -384,3 +384,57 @@ def test_predict_auto_matches_demo_page(key):
     full, _, _, _, _ = oracle.predict_auto(arch, P, gl, gr, W, H)
     q = np.clip(np.round(full * 64), 0, 65535).astype(np.uint16)
     assert np.array_equal(q, dp[key + "_raw"])
+
+
+def _consistency_golden():
+    return np.load(os.path.join(GOLDEN, "consistency.npz"))
+
+
+def test_consistency_helpers_golden():
+    """mirror_h, lr_consistency (two tolerances) and fill_invalid against
+    the reference's own outputs on crafted maps: columns landing exactly on
+    0 and W-1, just outside, a row entirely outside (left unchanged by the
+    fill), quarter-pixel values whose difference equals the tolerance."""
+    z = _consistency_golden()
+    W, H = int(z["W"]), int(z["H"])
+    assert np.array_equal(oracle.mirror_h(z["img"], 2, W, H), z["mirror"])
+    for name, tol in (("t15", 1.5), ("t075", 0.75)):
+        v = oracle.lr_consistency(z["dl"], z["dr"], W, H, tol)
+        assert np.array_equal(v, z["valid_" + name]), name
+        assert np.array_equal(oracle.fill_invalid(z["dl"], v, W, H), z["fill_" + name]), name
+
+
+def test_consistency_closed_forms():
+    """A double flip is the identity; two constant maps of the same value c
+    agree wherever x - c lands in [0, W-1] and nowhere else; filling an
+    all-valid map changes nothing; a run between valid values a < b takes a,
+    a run at the row start takes its right neighbour."""
+    rng = np.random.default_rng(5)
+    W, H = 23, 4
+    img = rng.uniform(size=3 * W * H)
+    assert np.array_equal(oracle.mirror_h(oracle.mirror_h(img, 3, W, H), 3, W, H), img)
+    c = 6.5
+    v = oracle.lr_consistency(np.full(W * H, c), np.full(W * H, c), W, H, 0.0).reshape(H, W)
+    x = np.arange(W)
+    assert np.array_equal(v, np.broadcast_to((x - c >= 0) & (x - c <= W - 1), (H, W)))
+    d = rng.uniform(0, 30, size=W * H)
+    assert np.array_equal(oracle.fill_invalid(d, np.ones(W * H, bool), W, H), d)
+    row = np.array([3.0, 9.0, 9.0, 2.0, 5.0, 1.0])
+    valid = np.array([False, True, False, True, False, False])
+    assert np.array_equal(oracle.fill_invalid(row, valid, 6, 1), [9.0, 9.0, 2.0, 2.0, 2.0, 2.0])
+
+
+@pytest.mark.parametrize("name,ww", [("auto", None), ("w64", 64)])
+def test_predict_consistent_golden(name, ww):
+    """predict_consistent (v5 weights, 96x64 pair) against the reference's
+    output: same valid fraction, disparities within 1e-12 px (the oracle's
+    libm exp differs from the reference's hand-made one by ~1e-16)."""
+    z = _consistency_golden()
+    cfg, blocks = oracle.load_weights_txt(os.path.join(GOLDEN, "stereonet_weights_v5.txt"))
+    arch = oracle.Arch(cfg["feat_C"], cfg["cost_C"], cfg["D"], cfg["refine_C"],
+                       cfg["refine_dil"], cfg["target_peak"])
+    Wp, Hp = int(z["Wp"]), int(z["Hp"])
+    d, frac, _ = oracle.predict_consistent(arch, np.concatenate(blocks), z["left"], z["right"],
+                                           Wp, Hp, ww)
+    assert frac == float(z["pcfrac_" + name])
+    assert np.abs(d - z["pc_" + name]).max() <= 1e-12
