@@ -87,7 +87,7 @@ def main():
                     help="apply these kernel variants instead of autotuning, e.g. "
                          "sgm_prefetch=8,sgm_htile=16,conv_rows=1,conv_tile_width=32")
     ap.add_argument("--workload", default="pipeline",
-                    choices=["pipeline", "band", "ragged64", "widened"],
+                    choices=["pipeline", "band", "ragged64", "widened", "consistent"],
                     help="pipeline: 460x370 pairs (the headline); ragged64: 64 pairs of uneven "
                          "size in one captured graph")
     args = ap.parse_args()
@@ -99,6 +99,8 @@ def main():
         return ragged_main(args)
     if args.workload == "widened":
         return widened_main(args)
+    if args.workload == "consistent":
+        return consistent_main(args)
     if args.workload == "band":
         return band_main(args)
     if args.tune_only and not args.tune_file:
@@ -528,6 +530,107 @@ def widened_main(args):
                                     "max_where_same_widths": round(float(es.max()), 4) if es.size else None,
                                     "mean_where_same_widths": round(float(es.mean()), 5) if es.size else None},
     }
+    print(json.dumps(line), flush=True)
+
+
+def consistent_main(args):
+    """predict_consistent (stereonet.py:5049-5076) on B synthetic 460x370
+    pairs per step: predict_auto for all pairs, the mirrored pass of every
+    pair at its own working width, the left-right check and the row fill
+    (predict_consistent_batch).  Reports pairs/s, the three consistency
+    kernels' HBM rate, accuracy against the ground truth, and the float64 C
+    oracle on a few of the same pairs as the CPU baseline and the check."""
+    import numpy as np
+    import torch
+
+    sv = importlib.import_module("from-scratch-stereo-vision_amd")
+    L = importlib.import_module("from-scratch-stereo-vision_amd._lib")
+    N = importlib.import_module("from-scratch-stereo-vision_amd.net")
+    C = importlib.import_module("from-scratch-stereo-vision_amd.consistency")
+    A = importlib.import_module("from-scratch-stereo-vision_amd.autotune")
+    dev = L.device()
+    B = args.batch
+    pairs = [sv.synth.pipeline_pair(s) for s in range(B)]
+    L8 = torch.from_numpy(np.stack([p["left"] for p in pairs])).to(dev)
+    R8 = torch.from_numpy(np.stack([p["right"] for p in pairs])).to(dev)
+    gt = np.stack([p["disp"] for p in pairs]).reshape(B, -1)
+    gvalid = np.stack([p["valid"] for p in pairs]).reshape(B, -1)
+    wpath = os.path.join(ROOT, "tests", "golden", "stereonet_weights_v5.txt")
+    net = N.load_model(wpath)
+    if args.tune_file and os.path.exists(args.tune_file):
+        A.load_choices(net, args.tune_file)
+    for _ in range(max(args.warmup, 1)):
+        out, valid, cnt = C.predict_consistent_batch(net, L8, R8, W, H)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out, valid, cnt = C.predict_consistent_batch(net, L8, R8, W, H)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    # the consistency kernels alone (mirror of both views and of the right
+    # map, the check, the fill), timed with HIP events on the stream
+    d_l = out.reshape(B, -1).contiguous()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 20
+    e0.record()
+    for _ in range(reps):
+        C._mirror_dev(L8, B * H, W)
+        C._mirror_dev(R8, B * H, W)
+        d_r = C._mirror_dev(d_l, B * H, W)
+        v, _ = C._consistency_dev(d_l, d_r, W, H, 1.5, counts=True)
+        C._fill_dev(d_l, v, W, H)
+    e1.record()
+    torch.cuda.synchronize()
+    kms = e0.elapsed_time(e1) / reps
+    # algorithmic bytes per pixel: mirrors 2 x (1 + 1) + (4 + 4), check
+    # 4 + 4 + 1, fill 4 + 1 + 4 (reads of the row neighbours not counted)
+    kbytes = B * W * H * (2 * 2 + 8 + 9 + 9)
+    o = out.reshape(B, -1).double().cpu().numpy()
+    frac = cnt.cpu().numpy() / float(W * H)
+    err = np.abs(o - gt)
+    log("predict_consistent: %.1f pairs/s, consistency kernels %.3f ms/step" % (B / dt, kms))
+    line = {
+        "metric": "pairs/s @460x370: predict_consistent (network LR check)",
+        "value": round(B / dt, 2), "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 network / f64 check",
+        "data": "synthetic",
+        "config": {"workload": "predict_consistent: predict_auto (v5) + mirrored pass at the "
+                               "working width + lr_consistency(tol=1.5) + fill_invalid, 460x370",
+                   "pairs_per_step": B, "execution": "eager, one stream"},
+        "valid_fraction": {"mean": round(float(frac.mean()), 4), "min": round(float(frac.min()), 4)},
+        "accuracy": {"bad2_pct": round(float((err > 2.0)[gvalid].mean() * 100), 3),
+                     "epe_px": round(float(err[gvalid].mean()), 4)},
+        "roofline": {"bound": "hbm", "kernel": "sv_mirror_h x3 + sv_lr_consistency + sv_fill_invalid",
+                     "achieved": round(kbytes / (kms * 1e-3) / 1e9, 1), "peak": 8000.0,
+                     "unit": "GB/s", "frac": round(kbytes / (kms * 1e-3) / 8e12, 4),
+                     "traffic": None, "avg_launch_ms": round(kms, 4),
+                     "algorithmic_bytes_per_launch": kbytes},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        import oracle
+        cfg, blocks = oracle.load_weights_txt(wpath)
+        arch = oracle.Arch(cfg["feat_C"], cfg["cost_C"], cfg["D"], cfg["refine_C"],
+                           cfg["refine_dil"], cfg["target_peak"])
+        P = np.concatenate(blocks)
+        n = min(max(args.cpu_pairs // 4, 1), B)
+        tc = time.perf_counter()
+        worst, dfrac = 0.0, 0.0
+        for i in range(n):
+            p = pairs[i]
+            rd, rfrac, _ = oracle.predict_consistent(arch, P, sv.synth.gray(p["left"]),
+                                                     sv.synth.gray(p["right"]), W, H)
+            worst = max(worst, float((np.abs(o[i] - rd) > 2e-3).mean()))
+            dfrac = max(dfrac, abs(float(frac[i]) - rfrac))
+        tcpu = time.perf_counter() - tc
+        line["cpu_baseline"] = {"value": round(n / tcpu, 3), "unit": "pairs/s", "cores": 1,
+                                "kind": "port",
+                                "sample": "%d of the same pairs through the float64 C oracle "
+                                          "(predict_consistent), %.1f s" % (n, tcpu)}
+        line["gpu_over_cpu"] = round((B / dt) / (n / tcpu), 1)
+        line["check_vs_oracle"] = {"pairs": n, "max_valid_fraction_diff": round(dfrac, 5),
+                                   "max_share_beyond_2e-3_px": round(worst, 5)}
     print(json.dumps(line), flush=True)
 
 

@@ -2398,33 +2398,57 @@ __global__ void k_zero_int(int* __restrict__ p, int n) {
 // lr_consistency, stereonet.py:4986-5008, in float64 from T maps: valid = 0
 // where x - d leaves [0, W-1] (or is NaN, where the reference's int()
 // raises) or |d - dr(x - d)| > tol, dr interpolated linearly between
-// columns int(x - d) and the next (clamped to the row).  counts[b] (when
-// given) receives the image's agreeing pixels: one atomic per wave.
+// columns int(x - d) and the next (clamped to the row).
 template <typename T>
 __global__ void k_lr_consistency(const T* __restrict__ dl, const T* __restrict__ dr, int W, int H,
-                                 double tol, uint8_t* __restrict__ valid, int* __restrict__ counts) {
+                                 double tol, uint8_t* __restrict__ valid) {
     const int b = blockIdx.z, y = blockIdx.y;
     const int x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= W) return;
+    const long long row = ((long long)b * H + y) * W;
+    const double d = (double)dl[row + x];
+    const double xr = (double)x - d;
     bool ok = false;
-    if (x < W) {
-        const long long row = ((long long)b * H + y) * W;
-        const double d = (double)dl[row + x];
-        const double xr = (double)x - d;
-        if (!(xr < 0.0 || xr > (double)(W - 1) || xr != xr)) {
-            const int x0 = (int)xr;
-            const int x1 = x0 + 1 < W ? x0 + 1 : x0;
-            const double f = xr - (double)x0;
-            const double v = (double)dr[row + x0] * (1.0 - f) + (double)dr[row + x1] * f;
-            double diff = d - v;
-            if (diff < 0.0) diff = -diff;
-            ok = !(diff > tol);
-        }
-        valid[row + x] = ok ? 1 : 0;
+    if (!(xr < 0.0 || xr > (double)(W - 1) || xr != xr)) {
+        const int x0 = (int)xr;
+        const int x1 = x0 + 1 < W ? x0 + 1 : x0;
+        const double f = xr - (double)x0;
+        const double v = (double)dr[row + x0] * (1.0 - f) + (double)dr[row + x1] * f;
+        double diff = d - v;
+        if (diff < 0.0) diff = -diff;
+        ok = !(diff > tol);
     }
-    if (counts) {
-        const uint64_t m = __ballot(ok);
-        if ((threadIdx.x & 63) == 0 && m) atomicAdd(&counts[b], (int)__popcll(m));
+    valid[row + x] = ok ? 1 : 0;
+}
+
+// counts[b] += the 0/1 flags of map b: CNT_BLOCKS blocks per map, each
+// summing its share 16 bytes at a time, one atomic per block.
+constexpr int CNT_BLOCKS = 8;
+
+__global__ void __launch_bounds__(256) k_count_flags(const uint8_t* __restrict__ flags,
+                                                     long long n, int* __restrict__ counts) {
+    __shared__ int ws[4];
+    const int b = blockIdx.y, tid = threadIdx.x;
+    const uint8_t* f = flags + (long long)b * n;
+    const long long share = (n + CNT_BLOCKS - 1) / CNT_BLOCKS;
+    const long long lo = min(n, (long long)blockIdx.x * share), hi = min(n, lo + share);
+    // [lo, a) bytes, [a, e) 16-byte words (flags are 0 or 1: one bit per
+    // byte), [e, hi) bytes
+    long long a = lo + (long long)((16 - ((uintptr_t)(f + lo) & 15)) & 15);
+    if (a > hi) a = hi;
+    const long long e = a + (hi - a) / 16 * 16;
+    int acc = 0;
+    for (long long i = lo + tid; i < a; i += 256) acc += f[i];
+    for (long long q = a + 16LL * tid; q < e; q += 16LL * 256) {
+        const uint4 v = *reinterpret_cast<const uint4*>(f + q);
+        acc += __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w);
     }
+    for (long long i = e + tid; i < hi; i += 256) acc += f[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(&counts[b], ws[0] + ws[1] + ws[2] + ws[3]);
 }
 
 // fill_invalid, stereonet.py:5011-5046: one wave per row, 64 columns per
@@ -2497,16 +2521,94 @@ __global__ void __launch_bounds__(256) k_fill_invalid(const T* __restrict__ disp
     }
 }
 
+// fill_invalid for rows of at most 64*NC columns: the row's flags and
+// values are loaded into registers at once (NC columns per lane), the two
+// scans run there, and a rejected pixel reads its two neighbours' values.
+template <typename T, int NC>
+__global__ void __launch_bounds__(256) k_fill_rows(const T* __restrict__ disp,
+                                                   const uint8_t* __restrict__ valid, int W,
+                                                   int rows, T* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= rows) return;  // whole wave
+    const long long row = (long long)r * W;
+    bool v[NC];
+    T d[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const int x = 64 * c + lane;
+        v[c] = x < W && valid[row + x];
+        d[c] = x < W ? disp[row + x] : T(0);
+    }
+    int lt[NC], rt[NC];
+    int carry = -1;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        int a = v[c] ? 64 * c + lane : -1;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(a, o, 64);
+            if (lane >= o) a = max(a, t);
+        }
+        lt[c] = max(a, carry);
+        carry = __shfl(lt[c], 63, 64);
+    }
+    carry = W;
+#pragma unroll
+    for (int c = NC - 1; c >= 0; --c) {
+        int a = v[c] ? 64 * c + lane : W;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_down(a, o, 64);
+            if (lane + o < 64) a = min(a, t);
+        }
+        rt[c] = min(a, carry);
+        carry = __shfl(rt[c], 0, 64);
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const int x = 64 * c + lane;
+        if (x >= W) continue;
+        T o = d[c];
+        if (!v[c]) {
+            const bool hl = lt[c] >= 0, hr = rt[c] < W;
+            if (hl || hr) {
+                const T a = hl ? disp[row + lt[c]] : T(0), bb = hr ? disp[row + rt[c]] : T(0);
+                o = !hl ? bb : (!hr ? a : (a < bb ? a : bb));
+            }
+        }
+        out[row + x] = o;
+    }
+}
+
+template <typename T>
+static void fill_t(const void* disp, const uint8_t* valid, int W, int rows, void* out,
+                   hipStream_t st) {
+    dim3 grid((unsigned)((rows + 3) / 4));
+    if (W <= 512)
+        hipLaunchKernelGGL((k_fill_rows<T, 8>), grid, dim3(256), 0, st, (const T*)disp, valid, W,
+                           rows, (T*)out);
+    else if (W <= 1024)
+        hipLaunchKernelGGL((k_fill_rows<T, 16>), grid, dim3(256), 0, st, (const T*)disp, valid, W,
+                           rows, (T*)out);
+    else
+        hipLaunchKernelGGL((k_fill_invalid<T>), grid, dim3(256), 0, st, (const T*)disp, valid, W,
+                           rows, (T*)out);
+}
+
 template <typename T>
 static int consistency_t(const void* dl, const void* dr, int B, int W, int H, double tol,
                          uint8_t* valid, int* counts, hipStream_t st) {
+    hipLaunchKernelGGL((k_lr_consistency<T>), dim3((W + 255) / 256, H, B), dim3(256), 0, st,
+                       (const T*)dl, (const T*)dr, W, H, tol, valid);
+    SV_CHECK_LAUNCH();
     if (counts) {
         hipLaunchKernelGGL(k_zero_int, dim3((B + 255) / 256), dim3(256), 0, st, counts, B);
         SV_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_count_flags, dim3(CNT_BLOCKS, B), dim3(256), 0, st, valid,
+                           (long long)W * H, counts);
+        SV_CHECK_LAUNCH();
     }
-    hipLaunchKernelGGL((k_lr_consistency<T>), dim3((W + 255) / 256, H, B), dim3(256), 0, st,
-                       (const T*)dl, (const T*)dr, W, H, tol, valid, counts);
-    SV_CHECK_LAUNCH();
     return SV_OK;
 }
 
@@ -2821,13 +2923,10 @@ int sv_fill_invalid(const void* disp, const uint8_t* valid, int dtype, int B, in
     const long long rows = (long long)B * H;
     SV_REQUIRE(rows <= (long long)INT_MAX, "sv_fill_invalid: too many rows");
     hipStream_t st = (hipStream_t)stream;
-    dim3 grid((unsigned)((rows + 3) / 4));
     if (dtype == SV_F32)
-        hipLaunchKernelGGL((k_fill_invalid<float>), grid, dim3(256), 0, st, (const float*)disp,
-                           valid, W, (int)rows, (float*)out);
+        fill_t<float>(disp, valid, W, (int)rows, out, st);
     else if (dtype == SV_F64)
-        hipLaunchKernelGGL((k_fill_invalid<double>), grid, dim3(256), 0, st, (const double*)disp,
-                           valid, W, (int)rows, (double*)out);
+        fill_t<double>(disp, valid, W, (int)rows, out, st);
     else
         return sv::set_error(SV_EINVAL, "sv_fill_invalid: maps must be f32 or f64 (dtype %d)", dtype);
     SV_CHECK_LAUNCH();

@@ -43,7 +43,8 @@ def test_helpers_golden(sv):
         assert isinstance(f, list) and np.array_equal(np.array(f), z["fill_" + name]), name
 
 
-@pytest.mark.parametrize("W,H", [(460, 37), (64, 3), (65, 5), (1, 4), (130, 2), (9000, 2)])
+@pytest.mark.parametrize("W,H", [(460, 37), (64, 3), (65, 5), (1, 4), (130, 2), (700, 3), (2000, 2),
+                                 (9000, 2)])
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 def test_helpers_random_match_oracle(sv, W, H, dtype):
     """Random maps, three per call, against the oracle bit for bit: rows
