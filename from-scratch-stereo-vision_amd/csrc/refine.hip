@@ -2381,18 +2381,26 @@ int refine_views(const Plan& pl, const RefineBufs& bf, const void* left, const v
 // =============================================================== C ABI
 // -------------------------------------- network left-right consistency
 // mirror_h, stereonet.py:4974-4983: `rows` rows of W elements, each reversed
-// (every channel of every image of a dense planar buffer).
-template <typename T>
-__global__ void k_mirror_h(const T* __restrict__ in, long long rows, int W, T* __restrict__ out) {
-    const int x = blockIdx.x * 256 + threadIdx.x;
-    if (x >= W) return;
-    for (long long r = blockIdx.y; r < rows; r += gridDim.y)
-        out[r * W + x] = in[r * W + (W - 1 - x)];
-}
+// (every channel of every image of a dense planar buffer).  A block takes
+// MIR_ROWS rows, every load of a thread in flight before its stores.
+constexpr int MIR_ROWS = 8;
 
-__global__ void k_zero_int(int* __restrict__ p, int n) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < n) p[i] = 0;
+template <typename T>
+__global__ void __launch_bounds__(256) k_mirror_h(const T* __restrict__ in, long long rows, int W,
+                                                  T* __restrict__ out) {
+    for (long long r0 = (long long)blockIdx.x * MIR_ROWS; r0 < rows;
+         r0 += (long long)gridDim.x * MIR_ROWS) {
+        for (int x0 = 0; x0 < W; x0 += 256) {
+            const int x = x0 + threadIdx.x;
+            T v[MIR_ROWS];
+#pragma unroll
+            for (int k = 0; k < MIR_ROWS; ++k)
+                if (x < W && r0 + k < rows) v[k] = in[(r0 + k) * W + (W - 1 - x)];
+#pragma unroll
+            for (int k = 0; k < MIR_ROWS; ++k)
+                if (x < W && r0 + k < rows) out[(r0 + k) * W + x] = v[k];
+        }
+    }
 }
 
 // lr_consistency, stereonet.py:4986-5008, in float64 from T maps: valid = 0
@@ -2421,34 +2429,33 @@ __global__ void k_lr_consistency(const T* __restrict__ dl, const T* __restrict__
     valid[row + x] = ok ? 1 : 0;
 }
 
-// counts[b] += the 0/1 flags of map b: CNT_BLOCKS blocks per map, each
-// summing its share 16 bytes at a time, one atomic per block.
-constexpr int CNT_BLOCKS = 8;
-
-__global__ void __launch_bounds__(256) k_count_flags(const uint8_t* __restrict__ flags,
-                                                     long long n, int* __restrict__ counts) {
-    __shared__ int ws[4];
-    const int b = blockIdx.y, tid = threadIdx.x;
+// counts[b] = the number of 0/1 flags set in map b: one block per map,
+// 16 bytes per load (one bit per flag byte), no atomics, nothing to clear.
+__global__ void __launch_bounds__(1024) k_count_flags(const uint8_t* __restrict__ flags,
+                                                      long long n, int* __restrict__ counts) {
+    __shared__ int ws[16];
+    const int b = blockIdx.x, tid = threadIdx.x;
     const uint8_t* f = flags + (long long)b * n;
-    const long long share = (n + CNT_BLOCKS - 1) / CNT_BLOCKS;
-    const long long lo = min(n, (long long)blockIdx.x * share), hi = min(n, lo + share);
-    // [lo, a) bytes, [a, e) 16-byte words (flags are 0 or 1: one bit per
-    // byte), [e, hi) bytes
-    long long a = lo + (long long)((16 - ((uintptr_t)(f + lo) & 15)) & 15);
-    if (a > hi) a = hi;
-    const long long e = a + (hi - a) / 16 * 16;
+    // [0, a) bytes, [a, e) 16-byte words, [e, n) bytes
+    long long a = (long long)((16 - ((uintptr_t)f & 15)) & 15);
+    if (a > n) a = n;
+    const long long e = a + (n - a) / 16 * 16;
     int acc = 0;
-    for (long long i = lo + tid; i < a; i += 256) acc += f[i];
-    for (long long q = a + 16LL * tid; q < e; q += 16LL * 256) {
+    for (long long i = tid; i < a; i += 1024) acc += f[i];
+    for (long long q = a + 16LL * tid; q < e; q += 16LL * 1024) {
         const uint4 v = *reinterpret_cast<const uint4*>(f + q);
         acc += __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w);
     }
-    for (long long i = e + tid; i < hi; i += 256) acc += f[i];
+    for (long long i = e + tid; i < n; i += 1024) acc += f[i];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = acc;
+    if ((tid & 63) == 0) ws[tid >> 6] = acc;
     __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(&counts[b], ws[0] + ws[1] + ws[2] + ws[3]);
+    if (tid == 0) {
+        int t = 0;
+        for (int w = 0; w < 16; ++w) t += ws[w];
+        counts[b] = t;
+    }
 }
 
 // fill_invalid, stereonet.py:5011-5046: one wave per row, 64 columns per
@@ -2603,10 +2610,8 @@ static int consistency_t(const void* dl, const void* dr, int B, int W, int H, do
                        (const T*)dl, (const T*)dr, W, H, tol, valid);
     SV_CHECK_LAUNCH();
     if (counts) {
-        hipLaunchKernelGGL(k_zero_int, dim3((B + 255) / 256), dim3(256), 0, st, counts, B);
-        SV_CHECK_LAUNCH();
-        hipLaunchKernelGGL(k_count_flags, dim3(CNT_BLOCKS, B), dim3(256), 0, st, valid,
-                           (long long)W * H, counts);
+        hipLaunchKernelGGL(k_count_flags, dim3(B), dim3(1024), 0, st, valid, (long long)W * H,
+                           counts);
         SV_CHECK_LAUNCH();
     }
     return SV_OK;
@@ -2885,7 +2890,7 @@ int sv_mirror_h(const void* in, int dtype, long long rows, int W, void* out, voi
     SV_REQUIRE(in && out && rows > 0 && W > 0, "sv_mirror_h: bad arguments");
     SV_REQUIRE(in != out, "sv_mirror_h: out must not alias in");
     hipStream_t st = (hipStream_t)stream;
-    dim3 grid((W + 255) / 256, (unsigned)std::min<long long>(rows, 65535));
+    dim3 grid((unsigned)std::min<long long>((rows + MIR_ROWS - 1) / MIR_ROWS, 1LL << 20));
     switch (dtype) {
         case SV_U8:
             hipLaunchKernelGGL((k_mirror_h<uint8_t>), grid, dim3(256), 0, st, (const uint8_t*)in,
