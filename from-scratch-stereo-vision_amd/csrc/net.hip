@@ -667,9 +667,24 @@ __device__ __forceinline__ void tap(int o, int in, int out, int& i0, int& i1, fl
     wt = (float)(f - a);
 }
 
+// tap() with the ratio in / out already formed (the same float64 steps)
+__device__ __forceinline__ void tap_sc(int o, int in, double sc, int& i0, int& i1, float& wt) {
+    double f = (o + 0.5) * sc - 0.5;
+    if (f < 0.0) f = 0.0;
+    int a = (int)f;
+    if (a > in - 1) a = in - 1;
+    i0 = a;
+    i1 = a + 1 < in ? a + 1 : a;
+    wt = (float)(f - a);
+}
+
 // BilinearUpsample.forward (:913-960) / resize_gray (:3843-3845): y =
 // mul * bilinear(x) into output channels [och, och + C).  T is the input
-// element type (uint8 images are read as value / 255).
+// element type (uint8 images are read as value / 255).  A thread makes
+// BL_X outputs of one row, 64 columns apart, so the row's tap and the two
+// float64 ratios are formed once for them.
+constexpr int BL_X = 4;
+
 template <typename T>
 __global__ void k_bilinear_fwd(const T* __restrict__ x, const int* __restrict__ idims,
                                long long xstride, int C, float* __restrict__ y,
@@ -678,15 +693,15 @@ __global__ void k_bilinear_fwd(const T* __restrict__ x, const int* __restrict__ 
     const int n = blockIdx.z;
     const int c = blockIdx.y / tilesY;
     const int oy = (blockIdx.y - c * tilesY) * 4 + threadIdx.y;
-    const int ox = blockIdx.x * 64 + threadIdx.x;
+    const int ox0 = blockIdx.x * 64 * BL_X + threadIdx.x;
     const int hi = dh(idims, n), wi = dw(idims, n);
     const int ho = dh(odims, n), wo = dw(odims, n);
-    if (ox >= wo || oy >= ho || hi <= 0 || wi <= 0) return;  // empty source: pair skipped
+    if (ox0 >= wo || oy >= ho || hi <= 0 || wi <= 0) return;  // empty source: pair skipped
     if (muln) mul = muln[n];
-    int y0, y1, x0, x1;
-    float wy, wx;
-    tap(oy, hi, ho, y0, y1, wy);
-    tap(ox, wi, wo, x0, x1, wx);
+    int y0, y1;
+    float wy;
+    tap_sc(oy, hi, (double)hi / (double)ho, y0, y1, wy);
+    const double scx = (double)wi / (double)wo;
     const T* xs = x + (long long)n * xstride + (long long)c * hi * wi;
     auto rd = [&](int yy, int xx) -> float {
         const T v = xs[(long long)yy * wi + xx];
@@ -695,10 +710,19 @@ __global__ void k_bilinear_fwd(const T* __restrict__ x, const int* __restrict__ 
         else
             return (float)v;
     };
-    const float w00 = (1.f - wy) * (1.f - wx), w01 = (1.f - wy) * wx;
-    const float w10 = wy * (1.f - wx), w11 = wy * wx;
-    const float v = rd(y0, x0) * w00 + rd(y0, x1) * w01 + rd(y1, x0) * w10 + rd(y1, x1) * w11;
-    y[(long long)n * ystride + (long long)(och + c) * ho * wo + (long long)oy * wo + ox] = v * mul;
+    float* yrow = y + (long long)n * ystride + (long long)(och + c) * ho * wo + (long long)oy * wo;
+#pragma unroll
+    for (int k = 0; k < BL_X; ++k) {
+        const int ox = ox0 + 64 * k;
+        if (ox >= wo) break;
+        int x0, x1;
+        float wx;
+        tap_sc(ox, wi, scx, x0, x1, wx);
+        const float w00 = (1.f - wy) * (1.f - wx), w01 = (1.f - wy) * wx;
+        const float w10 = wy * (1.f - wx), w11 = wy * wx;
+        const float v = rd(y0, x0) * w00 + rd(y0, x1) * w01 + rd(y1, x0) * w10 + rd(y1, x1) * w11;
+        yrow[ox] = v * mul;
+    }
 }
 
 // BilinearUpsample.backward (:962-973) as a gather: each input pixel sums
@@ -1459,7 +1483,7 @@ int sv_bilinear_fwd(const void* x, int dtype, const int* idims, long long xstrid
                     const float* muln, int N, int max_ho, int max_wo, void* stream) {
     SV_REQUIRE(x && idims && y && odims && N > 0 && C > 0, "sv_bilinear_fwd: bad arguments");
     const int tY = tiles4(max_ho);
-    dim3 grid((max_wo + 63) / 64, tY * C, N), blk(64, 4);
+    dim3 grid((max_wo + 64 * BL_X - 1) / (64 * BL_X), tY * C, N), blk(64, 4);
     hipStream_t st = (hipStream_t)stream;
     if (dtype == SV_U8)
         hipLaunchKernelGGL(k_bilinear_fwd<uint8_t>, grid, blk, 0, st, (const uint8_t*)x, idims,
