@@ -9,7 +9,7 @@ build.py); torch supplies device memory and streams only.
 """
 
 from .refine import (census, band_costs, _aggregate_path, refine_fullres, _mirror1,
-                     left_to_right, lr_refine, median_filter, refine_disparity)
+                     left_to_right, lr_refine, median_filter, refine_disparity, slic, plane_snap)
 from .net import (StereoNet, Adam, LEVEL_WEIGHTS, CE_WEIGHT, train_grads, train_grads_batch,
                   train_step, flat_params, set_flat_params, flat_grads, set_flat_grads,
                   save_weights, read_checkpoint_config, load_weights, load_model, predict,
@@ -20,7 +20,8 @@ from .consistency import (mirror_h, lr_consistency, fill_invalid, predict_consis
 from . import autotune, scenes, synth
 
 __all__ = ["census", "band_costs", "_aggregate_path", "refine_fullres", "_mirror1",
-           "left_to_right", "lr_refine", "median_filter", "refine_disparity", "StereoNet", "Adam",
+           "left_to_right", "lr_refine", "median_filter", "refine_disparity", "slic", "plane_snap",
+           "StereoNet", "Adam",
            "LEVEL_WEIGHTS", "CE_WEIGHT", "train_grads", "train_grads_batch", "train_step",
            "flat_params", "set_flat_params", "flat_grads", "set_flat_grads", "save_weights",
            "read_checkpoint_config", "load_weights", "load_model", "predict", "predict_batch",

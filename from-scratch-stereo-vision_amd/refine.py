@@ -21,6 +21,15 @@ from . import _lib as L
 
 _DTYPES = {torch.uint8: L.SV_U8, torch.float32: L.SV_F32, torch.float64: L.SV_F64}
 
+_CI, _CD, _CP, _CSZ = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_size_t
+L.extra_signatures({
+    "sv_slic": (_CI, [_CP, _CI, _CI, _CI, _CI, _CI, _CD, _CI, _CP, ctypes.POINTER(_CI), _CP, _CP,
+                      _CSZ, _CP]),
+    "sv_plane_snap_workspace_bytes": (_CI, [_CI, _CI, _CI, _CI, ctypes.POINTER(_CSZ)]),
+    "sv_plane_snap": (_CI, [_CP, _CP, _CI, _CP, _CI, _CI, _CI, _CI, _CI, _CD, _CD, _CI,
+                            ctypes.c_uint, _CP, _CP, _CP, _CSZ, _CP]),
+})
+
 
 def _kind(a):
     if isinstance(a, torch.Tensor):
@@ -238,15 +247,107 @@ def median_filter(disp, W, H, radius=1, passes=1):
     return _ret(out, kind, shape)
 
 
+_PS = ctypes.c_size_t
+
+
+def _rgb(rgb, W, H):
+    """RGB as [B][H][W][3] on the device: uint8 (read as value * (1/255)) or
+    float64 in [0, 1] (lists: the reference's interleaved floats)."""
+    t = _dev(rgb)
+    if t.dtype == torch.float32:
+        t = t.to(torch.float64)
+    n = t.numel()
+    if n == 0 or n % (3 * W * H):
+        raise ValueError("rgb of %d values does not hold W x H = %d x %d RGB images" % (n, W, H))
+    return t, _DTYPES[t.dtype], n // (3 * W * H)
+
+
+def _ps_ws(B, W, H, step):
+    nb = _PS(0)
+    L.check(L.lib().sv_plane_snap_workspace_bytes(B, W, H, step, ctypes.byref(nb)))
+    return L.workspace.get(nb.value), nb.value
+
+
+def _ps_check(flag):
+    if int(flag.item()) != 0:
+        raise RuntimeError("plane_snap: a segment had more than 256 pixels outside every SLIC "
+                           "window (outside what the GPU kernels support)")
+
+
+def slic(rgb, W, H, step=12, compactness=10.0, iters=5):
+    """slic, stereonet.py:5394-5448: (labels, number of centres); labels -1
+    where no centre ever reached."""
+    kind = _kind(rgb)
+    t, dt, B = _rgb(rgb, W, H)
+    lab = torch.empty(B * W * H, dtype=torch.int32, device=t.device)
+    flag = torch.zeros(1, dtype=torch.int32, device=t.device)
+    k = ctypes.c_int(0)
+    ws, nb = _ps_ws(B, W, H, step)
+    L.check(L.lib().sv_slic(L.ptr(t), dt, B, W, H, step, float(compactness), iters, L.ptr(lab),
+                            ctypes.byref(k), L.ptr(flag), L.ptr(ws), nb, L.stream()))
+    _ps_check(flag)
+    shape = None if kind == "list" else (tuple(rgb.shape[:-1]) if rgb.shape[-1] == 3 and rgb.ndim > 1
+                                         else None)
+    return _ret(lab, kind, shape), k.value
+
+
+def _snap_dev(d, t, dt, B, W, H, valid, vkind, step, thr, min_inl, trials, seed):
+    out = torch.empty_like(d)
+    flag = torch.zeros(1, dtype=torch.int32, device=d.device)
+    ws, nb = _ps_ws(B, W, H, step)
+    L.check(L.lib().sv_plane_snap(L.ptr(d), L.ptr(t), dt, L.ptr(valid), vkind, B, W, H, step,
+                                  float(thr), float(min_inl), int(trials), int(seed) & 0xFFFFFFFF,
+                                  L.ptr(out), L.ptr(flag), L.ptr(ws), nb, L.stream()))
+    _ps_check(flag)
+    return out
+
+
+def plane_snap(disp, rgb, W, H, valid=None, step=12, thr=1.0, min_inl=0.6, trials=40,
+               seed=12345):
+    """plane_snap, stereonet.py:5498-5563: a slanted plane per SLIC segment
+    (RANSAC from the reference's xorshift32 stream, least-squares refit on
+    the inliers); pixels on it, and the segment's invalid pixels, take the
+    plane's value (floored at 0)."""
+    kind, shape = _kind(disp), _shape_of(disp)
+    d = _dev(disp, torch.float64)
+    B = _batch(d, W, H)
+    t, dt, Bc = _rgb(rgb, W, H)
+    if Bc != B:
+        raise ValueError("%d disparity maps and %d RGB images" % (B, Bc))
+    v = None
+    if valid is not None:
+        v = valid if isinstance(valid, torch.Tensor) else torch.from_numpy(
+            np.ascontiguousarray(np.asarray(valid, dtype=bool)))
+        v = v.to(d.device).reshape(-1).to(torch.uint8).contiguous()
+        if v.numel() != d.numel():
+            raise ValueError("%d validity flags for %d disparities" % (v.numel(), d.numel()))
+    out = _snap_dev(d, t, dt, B, W, H, v, 0, step, thr, min_inl, trials, seed)
+    return _ret(out, kind, shape)
+
+
 def refine_disparity(left, right, rgb, disp0, W, H, R=10, P1=12.0, P2=150.0,
                      rx=4, ry=3, tol=2.0, planes=True, plane_step=12,
                      plane_thr=1.0, plane_min_inl=0.4, verbose=False):
     """refine_disparity, stereonet.py:5865-5884: lr_refine (both views), then
-    one 3x3 median.  Plane snapping (planes=True with an RGB image) is not
-    part of this library; pass planes=False or rgb=None."""
+    (planes=True with an RGB image) plane_snap on the pixels whose status is
+    0, then one 3x3 median."""
     if planes and rgb is not None:
-        raise NotImplementedError("plane_snap (SLIC + RANSAC planes) is not implemented on the "
-                                  "GPU path; call refine_disparity(..., planes=False)")
+        kind, shape = _kind(disp0), _shape_of(disp0)
+        d, st = lr_refine(left, right, _dev(disp0, torch.float64), W, H, tol=tol, R=R, P1=P1,
+                          P2=P2, rx=rx, ry=ry)
+        B = _batch(d, W, H)
+        if verbose:
+            n = float(W * H * B)
+            print("  full-res refinement: %.1f%% occluded, %.1f%% mismatched"
+                  % (100.0 * int((st == 1).sum()) / n, 100.0 * int((st == 2).sum()) / n))
+        t, dt, Bc = _rgb(rgb, W, H)
+        if Bc != B:
+            raise ValueError("%d disparity maps and %d RGB images" % (B, Bc))
+        sn = _snap_dev(d.reshape(-1).contiguous(), t, dt, B, W, H, st.reshape(-1).contiguous(), 1,
+                       plane_step, plane_thr, plane_min_inl, 40, 12345)
+        out = median_filter(sn, W, H, radius=1, passes=1)
+        stt = st if kind == "torch" else st.to(torch.int32)
+        return _ret(out, kind, shape), _ret(stt.reshape(-1), kind, shape)
     kind, shape = _kind(disp0), _shape_of(disp0)
     l, r, dt, d0, B = _pair(left, right, disp0, W, H)
     out = torch.empty(B * W * H, dtype=torch.float64, device=l.device)

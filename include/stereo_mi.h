@@ -121,6 +121,31 @@ int sv_lr_consistency(const void* dl, const void* dr, int dtype, int B, int W, i
 int sv_fill_invalid(const void* disp, const uint8_t* valid, int dtype, int B, int W, int H,
                     void* out, void* stream);
 
+/* slic(rgb, W, H, step=12, compactness=10.0, iters=5), stereonet.py:5394-5448,
+ * for B images: rgb [B][H][W][3] (SV_U8 read as value * (1/255), as the
+ * reference's image reader scales 8-bit samples, or SV_F64 in [0,1]);
+ * labels [B][H][W] int32 (-1 where no centre reached); *nseg (host, may be
+ * NULL) receives the number of centres per image.  step 2..16.  The
+ * workspace is sv_plane_snap_workspace_bytes(B, W, H, step).  status_out
+ * (device int, may be NULL) is set non-zero if a segment had more than 256
+ * pixels outside every window (results then incomplete). */
+int sv_slic(const void* rgb, int rgb_dtype, int B, int W, int H, int step, double compactness,
+            int iters, int* labels, int* nseg, int* status_out, void* ws, size_t ws_bytes,
+            void* stream);
+
+/* plane_snap(disp, rgb, W, H, valid=None, step=12, thr=1.0, min_inl=0.6,
+ * trials=40, seed=12345), stereonet.py:5498-5563, for B images (f64 maps):
+ * SLIC segments, a RANSAC plane per segment from the reference's xorshift32
+ * stream, the least-squares refit on the inliers and the snap.  valid may
+ * be NULL (all valid); valid_kind 0: non-zero bytes are valid, 1: zero
+ * bytes are valid (an lr_refine status array).  trials must be 40, step
+ * 2..16; out must not alias disp; status_out as for sv_slic. */
+int sv_plane_snap_workspace_bytes(int B, int W, int H, int step, size_t* bytes);
+int sv_plane_snap(const double* disp, const void* rgb, int rgb_dtype, const uint8_t* valid,
+                  int valid_kind, int B, int W, int H, int step, double thr, double min_inl,
+                  int trials, unsigned int seed, double* out, int* status_out, void* ws,
+                  size_t ws_bytes, void* stream);
+
 /* refine_disparity(left, right, rgb, disp0, W, H, R=10, P1=12.0, P2=150.0,
  * rx=4, ry=3, tol=2.0, planes=False), stereonet.py:5865-5884: lr_refine then
  * one 3x3 median pass. */

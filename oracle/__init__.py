@@ -84,6 +84,8 @@ def _declare(l):
         "or_mirror_h": (None, [P, I, I, I, P]),
         "or_lr_consistency": (None, [P, P, I, I, D_, P]),
         "or_fill_invalid": (None, [P, P, I, I, P]),
+        "or_slic": (I, [P, I, I, I, D_, I, P]),
+        "or_plane_snap": (L_, [P, P, I, I, P, I, D_, D_, I, ctypes.c_uint32, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(l, name)
@@ -469,3 +471,34 @@ def predict_consistent(arch, params, left, right, W, H, work_width=None, tol=1.5
     valid = lr_consistency(d_l, d_r, W, H, tol)
     frac = int(valid.sum()) / float(W * H)
     return fill_invalid(d_l, valid, W, H), frac, valid
+
+
+# ------------------------------------------------------------ planes
+
+def slic(rgb, W, H, step=12, compactness=10.0, iters=5):
+    """slic, stereonet.py:5394-5448: (labels int32, number of centres)."""
+    rgb = _f64(rgb).ravel()
+    lab = np.zeros(W * H, dtype=np.int32)
+    k = lib().or_slic(_p(rgb), W, H, step, float(compactness), iters, _p(lab))
+    return lab, int(k)
+
+
+def plane_snap(disp, rgb, W, H, valid=None, step=12, thr=1.0, min_inl=0.6, trials=40,
+               seed=12345):
+    """plane_snap, stereonet.py:5498-5563."""
+    d, rgb = _f64(disp).ravel(), _f64(rgb).ravel()
+    v = None if valid is None else np.ascontiguousarray(np.asarray(valid, dtype=bool).ravel()).astype(np.uint8)
+    out = np.zeros(W * H)
+    lib().or_plane_snap(_p(d), _p(rgb), W, H, _p(v), step, float(thr), float(min_inl), trials,
+                        seed, _p(out))
+    return out
+
+
+def refine_disparity_planes(left, right, rgb, disp0, W, H, R=10, P1=12.0, P2=150.0, rx=4, ry=3,
+                            tol=2.0, plane_step=12, plane_thr=1.0, plane_min_inl=0.4):
+    """refine_disparity with planes=True and an RGB image, stereonet.py:5865-5884:
+    lr_refine, plane_snap on the status-0 pixels, one 3x3 median."""
+    d, st = lr_refine(left, right, disp0, W, H, tol=tol, R=R, P1=P1, P2=P2, rx=rx, ry=ry)
+    d = plane_snap(d, rgb, W, H, valid=np.asarray(st) == 0, step=plane_step, thr=plane_thr,
+                   min_inl=plane_min_inl)
+    return median_filter(d, W, H, 1, 1), np.asarray(st)

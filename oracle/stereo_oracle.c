@@ -1319,3 +1319,218 @@ EXPORT void or_fill_invalid(const double *disp, const uint8_t *valid, int W, int
         }
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* Section 25 (planes): SLIC, plane fit, plane_snap (stereonet.py:5394-5563) */
+/* ------------------------------------------------------------------ */
+
+/* slic, stereonet.py:5394-5448.  rgb: 3 floats per pixel in [0,1],
+ * interleaved.  lab receives a label per pixel (-1 where no centre ever
+ * reached); returns the number of centres. */
+EXPORT int or_slic(const double *rgb, int W, int H, int step, double compactness, int iters,
+                   int *lab) {
+    int K = 0;
+    for (int y = step / 2; y < H; y += step)
+        for (int x = step / 2; x < W; x += step) K++;
+    double *cs = (double *)malloc(sizeof(double) * 5 * (K > 0 ? K : 1));
+    int k = 0;
+    for (int y = step / 2; y < H; y += step)
+        for (int x = step / 2; x < W; x += step) {
+            long i = (long)y * W + x;
+            cs[5 * k + 0] = rgb[3 * i] * 255.0;
+            cs[5 * k + 1] = rgb[3 * i + 1] * 255.0;
+            cs[5 * k + 2] = rgb[3 * i + 2] * 255.0;
+            cs[5 * k + 3] = (double)x;
+            cs[5 * k + 4] = (double)y;
+            k++;
+        }
+    const long N = (long)W * H;
+    for (long i = 0; i < N; i++) lab[i] = -1;
+    const double q = compactness / step;
+    const double m2 = q * q;
+    double *dist = (double *)malloc(sizeof(double) * N);
+    double *acc = (double *)malloc(sizeof(double) * 6 * (K > 0 ? K : 1));
+    for (int it = 0; it < iters; it++) {
+        for (long i = 0; i < N; i++) dist[i] = 1.0e30;
+        for (int ci = 0; ci < K; ci++) {
+            const double cr = cs[5 * ci], cg = cs[5 * ci + 1], cb = cs[5 * ci + 2];
+            const double cx = cs[5 * ci + 3], cy = cs[5 * ci + 4];
+            int x0 = (int)cx - step, x1 = (int)cx + step, y0 = (int)cy - step, y1 = (int)cy + step;
+            if (x0 < 0) x0 = 0;
+            if (y0 < 0) y0 = 0;
+            if (x1 >= W) x1 = W - 1;
+            if (y1 >= H) y1 = H - 1;
+            for (int yy = y0; yy <= y1; yy++) {
+                const double dy = yy - cy;
+                for (int xx = x0; xx <= x1; xx++) {
+                    const long i = (long)yy * W + xx;
+                    const double a = rgb[3 * i] * 255.0 - cr;
+                    const double b = rgb[3 * i + 1] * 255.0 - cg;
+                    const double c = rgb[3 * i + 2] * 255.0 - cb;
+                    const double dx = xx - cx;
+                    const double dd = a * a + b * b + c * c + m2 * (dx * dx + dy * dy);
+                    if (dd < dist[i]) {
+                        dist[i] = dd;
+                        lab[i] = ci;
+                    }
+                }
+            }
+        }
+        memset(acc, 0, sizeof(double) * 6 * (K > 0 ? K : 1));
+        for (long i = 0; i < N; i++) {
+            const int l = lab[i];
+            if (l < 0) continue;
+            double *a = acc + 6 * l;
+            a[0] += rgb[3 * i];
+            a[1] += rgb[3 * i + 1];
+            a[2] += rgb[3 * i + 2];
+            a[3] += (double)(i % W);
+            a[4] += (double)(i / W);
+            a[5] += 1.0;
+        }
+        for (int ci = 0; ci < K; ci++) {
+            const double *a = acc + 6 * ci;
+            if (a[5] != 0.0) {
+                const double n = a[5];
+                cs[5 * ci + 0] = a[0] / n * 255.0;
+                cs[5 * ci + 1] = a[1] / n * 255.0;
+                cs[5 * ci + 2] = a[2] / n * 255.0;
+                cs[5 * ci + 3] = a[3] / n;
+                cs[5 * ci + 4] = a[4] / n;
+            }
+        }
+    }
+    free(cs);
+    free(dist);
+    free(acc);
+    return K;
+}
+
+/* _solve3, stereonet.py:5451-5463 (Cramer's rule); 0 when |det| < 1e-9. */
+static int solve3(const double A[9], const double b[3], double out[3]) {
+    const double a = A[0], bb = A[1], c = A[2], d = A[3], e = A[4], f = A[5], g = A[6],
+                 h = A[7], k = A[8];
+    const double det = a * (e * k - f * h) - bb * (d * k - f * g) + c * (d * h - e * g);
+    if (fabs(det) < 1.0e-9) return 0;
+    out[0] = (b[0] * (e * k - f * h) - bb * (b[1] * k - f * b[2]) + c * (b[1] * h - e * b[2])) / det;
+    out[1] = (a * (b[1] * k - f * b[2]) - b[0] * (d * k - f * g) + c * (d * b[2] - b[1] * g)) / det;
+    out[2] = (a * (e * b[2] - b[1] * h) - bb * (d * b[2] - b[1] * g) + b[0] * (d * h - e * g)) / det;
+    return 1;
+}
+
+/* _lsq_plane, stereonet.py:5466-5495: least squares d = a x + b y + c over
+ * the points idx[0..n) (x = i % W, y = i / W), sums taken in list order. */
+static void lsq_plane(const long *idx, long n, int W, const double *disp, double out[3]) {
+    const double nn = (double)n;
+    double sx = 0.0, sy = 0.0, sd = 0.0;
+    for (long j = 0; j < n; j++) sx += (double)(idx[j] % W);
+    for (long j = 0; j < n; j++) sy += (double)(idx[j] / W);
+    for (long j = 0; j < n; j++) sd += disp[idx[j]];
+    const double mx = sx / nn, my = sy / nn, md = sd / nn;
+    double sxx = 0.0, sxy = 0.0, syy = 0.0, sxd = 0.0, syd = 0.0;
+    for (long j = 0; j < n; j++) {
+        const double u = (double)(idx[j] % W) - mx;
+        const double v = (double)(idx[j] / W) - my;
+        const double w = disp[idx[j]] - md;
+        sxx += u * u;
+        sxy += u * v;
+        syy += v * v;
+        sxd += u * w;
+        syd += v * w;
+    }
+    const double det = sxx * syy - sxy * sxy;
+    if (fabs(det) < 1.0e-9) {
+        out[0] = 0.0;
+        out[1] = 0.0;
+        out[2] = md;
+        return;
+    }
+    out[0] = (sxd * syy - syd * sxy) / det;
+    out[1] = (syd * sxx - sxd * sxy) / det;
+    out[2] = md - out[0] * mx - out[1] * my;
+}
+
+static int on_plane(double a, double b, double c, long i, int W, const double *disp, double thr) {
+    return fabs(a * (double)(i % W) + b * (double)(i / W) + c - disp[i]) <= thr;
+}
+
+/* plane_snap, stereonet.py:5498-5563 (SLIC with compactness 10 and 5
+ * iterations).  valid may be NULL.  Returns the number of snapped pixels. */
+EXPORT long or_plane_snap(const double *disp, const double *rgb, int W, int H,
+                          const uint8_t *valid, int step, double thr, double min_inl,
+                          int trials, uint32_t seed, double *out) {
+    const long N = (long)W * H;
+    int *lab = (int *)malloc(sizeof(int) * (N > 0 ? N : 1));
+    const int K = or_slic(rgb, W, H, step, 10.0, 5, lab);
+    /* groups in pixel order: counting sort by label */
+    long *start = (long *)calloc((size_t)K + 1, sizeof(long));
+    for (long i = 0; i < N; i++)
+        if (lab[i] >= 0) start[lab[i] + 1]++;
+    for (int g = 0; g < K; g++) start[g + 1] += start[g];
+    long *members = (long *)malloc(sizeof(long) * (N > 0 ? N : 1));
+    long *fill = (long *)malloc(sizeof(long) * ((size_t)K + 1));
+    memcpy(fill, start, sizeof(long) * ((size_t)K + 1));
+    for (long i = 0; i < N; i++)
+        if (lab[i] >= 0) members[fill[lab[i]]++] = i;
+    long *pts = (long *)malloc(sizeof(long) * (N > 0 ? N : 1));
+    memcpy(out, disp, sizeof(double) * N);
+    uint32_t s = seed;
+    long snapped = 0;
+    for (int g = 0; g < K; g++) {
+        const long *gm = members + start[g];
+        const long gn = start[g + 1] - start[g];
+        long n = 0;
+        for (long j = 0; j < gn; j++)
+            if (!valid || valid[gm[j]]) pts[n++] = gm[j];
+        if (n < 12) continue;
+        int have = 0;
+        long bestc = -1;
+        double best[3] = {0.0, 0.0, 0.0};
+        for (int t = 0; t < trials; t++) {
+            long tri[3];
+            for (int q = 0; q < 3; q++) {
+                s ^= s << 13;
+                s ^= s >> 17;
+                s ^= s << 5;
+                tri[q] = pts[s % (uint32_t)n];
+            }
+            double A[9], bv[3], pl[3];
+            for (int q = 0; q < 3; q++) {
+                A[3 * q] = (double)(tri[q] % W);
+                A[3 * q + 1] = (double)(tri[q] / W);
+                A[3 * q + 2] = 1.0;
+                bv[q] = disp[tri[q]];
+            }
+            if (!solve3(A, bv, pl)) continue;
+            long cnt = 0;
+            for (long j = 0; j < n; j++) cnt += on_plane(pl[0], pl[1], pl[2], pts[j], W, disp, thr);
+            if (cnt > bestc) {
+                bestc = cnt;
+                best[0] = pl[0];
+                best[1] = pl[1];
+                best[2] = pl[2];
+                have = 1;
+            }
+        }
+        if (!have || (double)bestc < min_inl * (double)n) continue;
+        long ni = 0;
+        for (long j = 0; j < n; j++)
+            if (on_plane(best[0], best[1], best[2], pts[j], W, disp, thr)) pts[ni++] = pts[j];
+        double fit[3];
+        lsq_plane(pts, ni, W, disp, fit);
+        for (long j = 0; j < gn; j++) {
+            const long i = gm[j];
+            const double v = fit[0] * (double)(i % W) + fit[1] * (double)(i / W) + fit[2];
+            if (fabs(v - disp[i]) <= thr || (valid && !valid[i])) {
+                out[i] = v > 0.0 ? v : 0.0;
+                snapped++;
+            }
+        }
+    }
+    free(lab);
+    free(start);
+    free(members);
+    free(fill);
+    free(pts);
+    return snapped;
+}

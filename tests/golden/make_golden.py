@@ -21,7 +21,10 @@ functions that no library routine, closed form or golden file pins:
 * the network left-right consistency: mirror_h, lr_consistency (clamped
   linear interpolation of the right map, the tolerance test) and
   fill_invalid run rules (:4974-5046), and predict_consistent's composition
-  of them with predict_auto / predict_at_scale on a small pair (:5049-5076).
+  of them with predict_auto / predict_at_scale on a small pair (:5049-5076),
+* SLIC labels, plane_snap (RANSAC with the reference's xorshift32 stream,
+  least-squares refit, snap rule) and refine_disparity with planes=True
+  (:5394-5563, :5865-5884).
 
 Inputs come from this package's own seeded generator.  Outputs are written as
 compressed .npz files (no pickles).  The network forward, layer gradients and
@@ -259,10 +262,55 @@ def consistency(SN):
     print("wrote consistency (valid fractions %.4f %.4f)" % (out["pcfrac_auto"], out["pcfrac_w64"]))
 
 
+def planes_rgb(gray, disp, dmax):
+    """An RGB image whose colours follow the disparity's planar regions
+    (three [0, 1] channels interleaved per pixel)."""
+    g = gray.astype(np.float64)
+    t = np.clip(disp / float(dmax), 0.0, 1.0)
+    rgb = np.stack([0.6 * g + 0.4 * t, 0.3 * g + 0.7 * (1.0 - t), 0.5 * g + 0.5 * t * t], axis=-1)
+    return np.round(rgb * 255.0) / 255.0
+
+
+def planes(SN):
+    """planes.npz: SLIC labels (step 12 and 8), plane_snap with and without
+    a validity mask, and refine_disparity(planes=True) on a 64x48 pair."""
+    out = {}
+    for name, (W, H, seed) in {"a": (61, 47, 41), "b": (64, 48, 42)}.items():
+        left, right, disp0, p = small_pair(seed, W, H, 3.0, 14.0)
+        rgb = planes_rgb(left, p["disp"], 14.0)
+        R = list(map(float, rgb.ravel()))
+        lab12, k12 = SN["slic"](R, W, H, 12)
+        lab8, k8 = SN["slic"](R, W, H, 8)
+        rng = np.random.default_rng(seed)
+        d = p["disp"].ravel() + rng.normal(0.0, 0.3, size=W * H)
+        d[rng.uniform(size=W * H) < 0.05] += 7.0         # outliers
+        valid = rng.uniform(size=W * H) > 0.15
+        D = list(map(float, d))
+        out.update({name + "_W": W, name + "_H": H, name + "_rgb": rgb.ravel(),
+                    name + "_lab12": np.array(lab12, dtype=np.int32), name + "_k12": k12,
+                    name + "_lab8": np.array(lab8, dtype=np.int32), name + "_k8": k8,
+                    name + "_disp": d, name + "_valid": valid,
+                    name + "_snap": np.array(SN["plane_snap"](D, R, W, H), dtype=np.float64),
+                    name + "_snapv": np.array(SN["plane_snap"](D, R, W, H, valid=list(map(bool, valid)),
+                                                            min_inl=0.4), dtype=np.float64)})
+        if name == "b":
+            Lg, Rg = list(map(float, left.ravel())), list(map(float, right.ravel()))
+            dd, st = SN["refine_disparity"](Lg, Rg, R, list(map(float, disp0.ravel())), W, H,
+                                            planes=True)
+            out.update(b_left=left.ravel(), b_right=right.ravel(), b_disp0=disp0.ravel(),
+                       b_refined=np.array(dd, dtype=np.float64),
+                       b_status=np.array(st, dtype=np.int32))
+    np.savez_compressed(os.path.join(HERE, "planes.npz"), **out)
+    print("wrote planes (%d and %d segments)" % (out["a_k12"], out["b_k12"]))
+
+
 if __name__ == "__main__":
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     if len(sys.argv) > 2 and sys.argv[2] == "consistency":
         consistency(load_reference(ref))
+    elif len(sys.argv) > 2 and sys.argv[2] == "planes":
+        planes(load_reference(ref))
     else:
         main(ref)
         consistency(load_reference(ref))
+        planes(load_reference(ref))

@@ -438,3 +438,36 @@ def test_predict_consistent_golden(name, ww):
                                            Wp, Hp, ww)
     assert frac == float(z["pcfrac_" + name])
     assert np.abs(d - z["pc_" + name]).max() <= 1e-12
+
+
+
+def test_planes_golden():
+    """SLIC labels (step 12 and 8), plane_snap with and without a validity
+    mask, and refine_disparity(planes=True) against the reference's own
+    outputs (tests/golden/planes.npz), bit for bit."""
+    z = np.load(os.path.join(GOLDEN, "planes.npz"))
+    for n in ("a", "b"):
+        W, H = int(z[n + "_W"]), int(z[n + "_H"])
+        for step in (12, 8):
+            lab, k = oracle.slic(z[n + "_rgb"], W, H, step)
+            assert k == int(z["%s_k%d" % (n, step)])
+            assert np.array_equal(lab, z["%s_lab%d" % (n, step)])
+        assert np.array_equal(oracle.plane_snap(z[n + "_disp"], z[n + "_rgb"], W, H), z[n + "_snap"])
+        assert np.array_equal(oracle.plane_snap(z[n + "_disp"], z[n + "_rgb"], W, H,
+                                                valid=z[n + "_valid"], min_inl=0.4), z[n + "_snapv"])
+    d, st = oracle.refine_disparity_planes(z["b_left"], z["b_right"], z["b_rgb"], z["b_disp0"], 64, 48)
+    assert np.array_equal(d, z["b_refined"]) and np.array_equal(st, z["b_status"])
+
+
+def test_plane_snap_closed_form():
+    """An exactly planar map with a uniform colour: every segment fits the
+    plane exactly, so plane_snap returns the plane's own values (floored at
+    0) everywhere SLIC labelled."""
+    W, H = 48, 36
+    yy, xx = np.mgrid[0:H, 0:W].astype(np.float64)
+    d = 0.25 * xx - 0.125 * yy + 3.0
+    rgb = np.full((H, W, 3), 0.5)
+    out = oracle.plane_snap(d.ravel(), rgb.ravel(), W, H)
+    lab, _ = oracle.slic(rgb.ravel(), W, H, 12)
+    m = lab >= 0
+    assert np.abs(out[m] - np.maximum(d.ravel()[m], 0.0)).max() < 1e-9
