@@ -87,7 +87,7 @@ def main():
                     help="apply these kernel variants instead of autotuning, e.g. "
                          "sgm_prefetch=8,sgm_htile=16,conv_rows=1,conv_tile_width=32")
     ap.add_argument("--workload", default="pipeline",
-                    choices=["pipeline", "band", "ragged64", "widened", "consistent"],
+                    choices=["pipeline", "band", "ragged64", "widened", "consistent", "planes"],
                     help="pipeline: 460x370 pairs (the headline); ragged64: 64 pairs of uneven "
                          "size in one captured graph")
     args = ap.parse_args()
@@ -101,6 +101,8 @@ def main():
         return widened_main(args)
     if args.workload == "consistent":
         return consistent_main(args)
+    if args.workload == "planes":
+        return planes_main(args)
     if args.workload == "band":
         return band_main(args)
     if args.tune_only and not args.tune_file:
@@ -631,6 +633,103 @@ def consistent_main(args):
         line["gpu_over_cpu"] = round((B / dt) / (n / tcpu), 1)
         line["check_vs_oracle"] = {"pairs": n, "max_valid_fraction_diff": round(dfrac, 5),
                                    "max_share_beyond_2e-3_px": round(worst, 5)}
+    print(json.dumps(line), flush=True)
+
+
+def planes_main(args):
+    """refine_disparity(planes=True) (stereonet.py:5865-5884) on B synthetic
+    460x370 pairs per step with an RGB image whose colours follow the
+    disparity's planar regions: lr_refine of both views, SLIC + RANSAC plane
+    snapping of the status-0 pixels, one 3x3 median.  Reports pairs/s, the
+    plane-snap stage time (HIP events), accuracy, and the float64 C oracle on
+    a few of the same pairs as the CPU baseline and the exactness check."""
+    import numpy as np
+    import torch
+
+    sv = importlib.import_module("from-scratch-stereo-vision_amd")
+    L = importlib.import_module("from-scratch-stereo-vision_amd._lib")
+    Rf = importlib.import_module("from-scratch-stereo-vision_amd.refine")
+    dev = L.device()
+    B = args.batch
+    pairs = [sv.synth.band_pair(s, W=W, H=H) for s in range(B)]
+
+    def rgb_of(p):
+        g = p["left"].astype(np.float64) / 255.0
+        t = np.clip(p["disp"] / 40.0, 0.0, 1.0)
+        rgb = np.stack([0.6 * g + 0.4 * t, 0.3 * g + 0.7 * (1.0 - t), 0.5 * g + 0.5 * t * t], -1)
+        return np.round(rgb * 255.0).astype(np.uint8)
+
+    rgbs = [rgb_of(p) for p in pairs]
+    L8 = torch.from_numpy(np.stack([p["left"] for p in pairs])).to(dev)
+    R8 = torch.from_numpy(np.stack([p["right"] for p in pairs])).to(dev)
+    C8 = torch.from_numpy(np.stack(rgbs)).to(dev)
+    D0 = torch.from_numpy(np.stack([np.asarray(p["disp0"], dtype=np.float64) for p in pairs])).to(dev)
+    gt = np.stack([p["disp"] for p in pairs]).reshape(B, -1)
+    gvalid = np.stack([p["valid"] for p in pairs]).reshape(B, -1)
+
+    def step():
+        return sv.refine_disparity(L8, R8, C8, D0, W, H, planes=True)
+
+    for _ in range(max(args.warmup, 1)):
+        out, st = step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out, st = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    # the plane-snap stage alone, on the lr_refine output of the last step
+    d, st1 = sv.lr_refine(L8, R8, D0, W, H, tol=2.0, R=10, P1=12.0, P2=150.0, rx=4, ry=3)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 10
+    e0.record()
+    for _ in range(reps):
+        Rf._snap_dev(d.reshape(-1).contiguous(), C8, L.SV_U8, B, W, H,
+                     st1.reshape(-1).contiguous(), 1, 12, 1.0, 0.4, 40, 12345)
+    e1.record()
+    torch.cuda.synchronize()
+    snap_ms = e0.elapsed_time(e1) / reps
+    o = out.reshape(B, -1).cpu().numpy()
+    plain, _ = sv.refine_disparity(L8, R8, None, D0, W, H, planes=False)
+    po = plain.reshape(B, -1).cpu().numpy()
+    err, perr = np.abs(o - gt), np.abs(po - gt)
+    log("refine_disparity(planes=True): %.1f pairs/s, plane_snap %.3f ms/step" % (B / dt, snap_ms))
+    line = {
+        "metric": "pairs/s @460x370: refine_disparity(planes=True)",
+        "value": round(B / dt, 2), "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int SGM / f64 planes",
+        "data": "synthetic",
+        "config": {"workload": "refine_disparity(planes=True): lr_refine R=10 census 9x7 + "
+                               "plane_snap(step 12, thr 1, min_inl 0.4) + median, 460x370, u8 RGB",
+                   "pairs_per_step": B, "execution": "eager, one stream"},
+        "plane_snap_ms_per_step": round(snap_ms, 3),
+        "accuracy": {"bad2_pct": round(float((err > 2.0)[gvalid].mean() * 100), 3),
+                     "epe_px": round(float(err[gvalid].mean()), 4),
+                     "planes_false_bad2_pct": round(float((perr > 2.0)[gvalid].mean() * 100), 3),
+                     "planes_false_epe_px": round(float(perr[gvalid].mean()), 4)},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        import oracle
+        n = min(max(args.cpu_pairs // 6, 1), B)
+        tc = time.perf_counter()
+        same = 0
+        for i in range(n):
+            p = pairs[i]
+            rd, rst = oracle.refine_disparity_planes(
+                sv.synth.gray(p["left"]), sv.synth.gray(p["right"]),
+                rgbs[i].astype(np.float64).ravel() * (1.0 / 255.0), p["disp0"], W, H)
+            same += int(np.array_equal(rd, o[i]) and
+                        np.array_equal(rst, st.reshape(B, -1)[i].cpu().numpy().astype(np.int32)))
+        tcpu = time.perf_counter() - tc
+        line["cpu_baseline"] = {"value": round(n / tcpu, 3), "unit": "pairs/s", "cores": 1,
+                                "kind": "port",
+                                "sample": "%d of the same pairs through the float64 C oracle "
+                                          "(refine_disparity planes=True), %.1f s" % (n, tcpu)}
+        line["gpu_over_cpu"] = round((B / dt) / (n / tcpu), 1)
+        line["identical_to_oracle"] = "%d/%d pairs" % (same, n)
     print(json.dumps(line), flush=True)
 
 

@@ -263,6 +263,7 @@ __global__ void __launch_bounds__(64) k_slic_update(const RT* __restrict__ rgb, 
                                                     int* __restrict__ err) {
     __shared__ StrayList sl;
     __shared__ int mi[64];
+    __shared__ double mc[3][64];  // the chunk's colours, staged by every lane
     const int ci = blockIdx.x, b = blockIdx.y, lane = threadIdx.x;
     const long long N = (long long)g.W * g.H;
     const double* c = cs_in + ((long long)b * g.K + ci) * 5;
@@ -275,46 +276,71 @@ __global__ void __launch_bounds__(64) k_slic_update(const RT* __restrict__ rgb, 
         if (lane == 0) atomicOr(err, 1);
         return;
     }
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0, a5 = 0.0;
+    // colour sums in member order (lane 0, over each chunk's members
+    // compacted by ballot); coordinate sums and the count are integers,
+    // exact in any order, so every lane adds its own
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    long long sx = 0, sy = 0, cnt = 0;
     int sp = 0;
-    auto add = [&](int i) {
+    auto add_stray = [&](int i) {
         a0 += rgb_at(rgbb, 3LL * i);
         a1 += rgb_at(rgbb, 3LL * i + 1);
         a2 += rgb_at(rgbb, 3LL * i + 2);
-        a3 += (double)(i % g.W);
-        a4 += (double)(i / g.W);
-        a5 += 1.0;
+        sx += i % g.W;
+        sy += i / g.W;
+        cnt += 1;
     };
+    const uint64_t below = (1ull << lane) - 1ull;
     for (int e0 = 0; e0 < n; e0 += 64) {
         const int e = e0 + lane;
-        int i = -1;
+        bool m = false;
+        int p = 0;
         if (e < n) {
             const int yy = y0 + e / ww, xx = x0 + e % ww;
-            const int p = yy * g.W + xx;
-            if (labb[p] == ci) i = p;
+            p = yy * g.W + xx;
+            m = labb[p] == ci;
         }
-        mi[lane] = i;
+        const uint64_t bal = __ballot(m);
+        if (m) {
+            const int pos = __popcll(bal & below);
+            mi[pos] = p;
+            mc[0][pos] = rgb_at(rgbb, 3LL * p);
+            mc[1][pos] = rgb_at(rgbb, 3LL * p + 1);
+            mc[2][pos] = rgb_at(rgbb, 3LL * p + 2);
+            sx += p % g.W;
+            sy += p / g.W;
+            cnt += 1;
+        }
         __builtin_amdgcn_wave_barrier();
         if (lane == 0) {
-            const int m = min(64, n - e0);
-            for (int k = 0; k < m; ++k) {
-                const int p = mi[k];
-                if (p < 0) continue;
-                while (sp < sl.n && sl.idx[sp] < p) add(sl.idx[sp++]);
-                add(p);
+            const int nmem = __popcll(bal);
+            for (int k = 0; k < nmem; ++k) {
+                if (sp < sl.n)
+                    while (sp < sl.n && sl.idx[sp] < mi[k]) add_stray(sl.idx[sp++]);
+                a0 += mc[0][k];
+                a1 += mc[1][k];
+                a2 += mc[2][k];
             }
         }
         __builtin_amdgcn_wave_barrier();
     }
+    if (lane == 0)
+        while (sp < sl.n) add_stray(sl.idx[sp++]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        sx += __shfl_xor(sx, o, 64);
+        sy += __shfl_xor(sy, o, 64);
+        cnt += __shfl_xor(cnt, o, 64);
+    }
     if (lane == 0) {
-        while (sp < sl.n) add(sl.idx[sp++]);
         double* o = cs_out + ((long long)b * g.K + ci) * 5;
-        if (a5 != 0.0) {
+        if (cnt != 0) {
+            const double a5 = (double)cnt;
             o[0] = a0 / a5 * 255.0;
             o[1] = a1 / a5 * 255.0;
             o[2] = a2 / a5 * 255.0;
-            o[3] = a3 / a5;
-            o[4] = a4 / a5;
+            o[3] = (double)sx / a5;
+            o[4] = (double)sy / a5;
         } else {
             for (int q = 0; q < 5; ++q) o[q] = c[q];
         }
@@ -322,11 +348,6 @@ __global__ void __launch_bounds__(64) k_slic_update(const RT* __restrict__ rgb, 
 }
 
 // ------------------------------------------------------------ plane_snap
-__device__ __forceinline__ bool on_plane(double a, double b, double c, int i, int W,
-                                         const double* __restrict__ d, double thr) {
-    return fabs(a * (double)(i % W) + b * (double)(i / W) + c - d[i]) <= thr;
-}
-
 __device__ __forceinline__ bool is_valid(const uint8_t* __restrict__ vb, int kind, int i) {
     if (!vb) return true;
     return kind == 0 ? vb[i] != 0 : vb[i] == 0;
@@ -416,8 +437,6 @@ __device__ bool solve3(const double A[9], const double bv[3], double out[3]) {
 // One wave per segment: member lists in LDS (window members merged with the
 // segment's strays, ascending), RANSAC (lane 0 draws and solves, the wave
 // counts), the inliers' least-squares plane (lane 0 sums in order), the snap.
-constexpr int PS_MAXW = 33 * 33;  // window of step <= 16
-constexpr int PS_MAXM = PS_MAXW + SCAP;
 
 __global__ void __launch_bounds__(64) k_ps_segment(const double* __restrict__ disp, SlicGeom g,
                                                    const double* __restrict__ cs,
@@ -429,9 +448,15 @@ __global__ void __launch_bounds__(64) k_ps_segment(const double* __restrict__ di
                                                    double min_inl, double* __restrict__ out,
                                                    int* __restrict__ err) {
     __shared__ StrayList sl;
-    __shared__ int mem[PS_MAXM];  // all members, ascending
-    __shared__ int pts[PS_MAXM];  // valid members, then the inliers
+    // dynamic LDS, sized for the step: all members (ascending), the valid
+    // members then the inliers, and their disparities
+    extern __shared__ __align__(16) unsigned char ps_dyn[];
+    const int maxm = (2 * g.step + 1) * (2 * g.step + 1) + SCAP;
+    double* pd = reinterpret_cast<double*>(ps_dyn);
+    int* mem = reinterpret_cast<int*>(pd + maxm);
+    int* pts = mem + maxm;
     __shared__ int mi[64];
+    __shared__ int mv[64];
     __shared__ int counts[2];
     __shared__ double plane[4];
     const int ci = blockIdx.x, b = blockIdx.y, lane = threadIdx.x;
@@ -447,34 +472,53 @@ __global__ void __launch_bounds__(64) k_ps_segment(const double* __restrict__ di
         if (lane == 0) atomicOr(err, 1);
         return;
     }
-    // ordered member lists
+    // ordered member lists: without strays, each chunk's members are
+    // appended by ballot compaction; with strays, lane 0 merges them in
     if (lane == 0) counts[0] = counts[1] = 0;
+    __builtin_amdgcn_wave_barrier();
+    const uint64_t below = (1ull << lane) - 1ull;
     int sp = 0;
     for (int e0 = 0; e0 < n; e0 += 64) {
         const int e = e0 + lane;
-        int i = -1;
+        bool m = false, v = false;
+        int p = 0;
         if (e < n) {
-            const int p = (y0 + e / ww) * g.W + x0 + e % ww;
-            if (labb[p] == ci) i = p;
+            p = (y0 + e / ww) * g.W + x0 + e % ww;
+            m = labb[p] == ci;
+            v = m && is_valid(vb, vkind, p);
         }
-        mi[lane] = i;
-        __builtin_amdgcn_wave_barrier();
-        if (lane == 0) {
-            const int m = min(64, n - e0);
-            int nm = counts[0], np = counts[1];
-            for (int k = 0; k < m; ++k) {
-                const int p = mi[k];
-                if (p < 0) continue;
-                while (sp < sl.n && sl.idx[sp] < p) {
-                    const int q = sl.idx[sp++];
-                    mem[nm++] = q;
-                    if (is_valid(vb, vkind, q)) pts[np++] = q;
-                }
-                mem[nm++] = p;
-                if (is_valid(vb, vkind, p)) pts[np++] = p;
+        const uint64_t bm = __ballot(m), bv = __ballot(v);
+        const int nm0 = counts[0], np0 = counts[1];
+        if (sl.n == 0) {
+            if (m) mem[nm0 + __popcll(bm & below)] = p;
+            if (v) pts[np0 + __popcll(bv & below)] = p;
+            __builtin_amdgcn_wave_barrier();
+            if (lane == 0) {
+                counts[0] = nm0 + __popcll(bm);
+                counts[1] = np0 + __popcll(bv);
             }
-            counts[0] = nm;
-            counts[1] = np;
+        } else {
+            if (m) {
+                mi[__popcll(bm & below)] = p;
+                mv[__popcll(bm & below)] = v ? 1 : 0;
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (lane == 0) {
+                int nm = nm0, np = np0;
+                const int c = __popcll(bm);
+                for (int k = 0; k < c; ++k) {
+                    const int q0 = mi[k];
+                    while (sp < sl.n && sl.idx[sp] < q0) {
+                        const int q = sl.idx[sp++];
+                        mem[nm++] = q;
+                        if (is_valid(vb, vkind, q)) pts[np++] = q;
+                    }
+                    mem[nm++] = q0;
+                    if (mv[k]) pts[np++] = q0;
+                }
+                counts[0] = nm;
+                counts[1] = np;
+            }
         }
         __builtin_amdgcn_wave_barrier();
     }
@@ -491,6 +535,13 @@ __global__ void __launch_bounds__(64) k_ps_segment(const double* __restrict__ di
     __builtin_amdgcn_wave_barrier();
     const int nm = counts[0], np = counts[1];
     if (np < 12) return;
+    for (int k = lane; k < np; k += 64) pd[k] = d[pts[k]];
+    __builtin_amdgcn_wave_barrier();
+    // the plane test of stereonet.py:5535/5547 on staged points
+    auto on_pt = [&](double a, double bb, double cc, int k) {
+        const int i = pts[k];
+        return fabs(a * (double)(i % g.W) + bb * (double)(i / g.W) + cc - pd[k]) <= thr;
+    };
     // RANSAC
     uint32_t s = state[(long long)b * g.K + ci];
     int bestc = -1;
@@ -505,7 +556,7 @@ __global__ void __launch_bounds__(64) k_ps_segment(const double* __restrict__ di
                 A[3 * q] = (double)(i % g.W);
                 A[3 * q + 1] = (double)(i / g.W);
                 A[3 * q + 2] = 1.0;
-                bv[q] = d[i];
+                bv[q] = pd[s % (uint32_t)np];
             }
             const bool ok = solve3(A, bv, pl);
             plane[0] = pl[0];
@@ -519,7 +570,7 @@ __global__ void __launch_bounds__(64) k_ps_segment(const double* __restrict__ di
         __builtin_amdgcn_wave_barrier();
         if (!ok) continue;
         int cnt = 0;
-        for (int k = lane; k < np; k += 64) cnt += on_plane(pa, pb, pc, pts[k], g.W, d, thr) ? 1 : 0;
+        for (int k = lane; k < np; k += 64) cnt += on_pt(pa, pb, pc, k) ? 1 : 0;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
         if (cnt > bestc) {
@@ -531,23 +582,50 @@ __global__ void __launch_bounds__(64) k_ps_segment(const double* __restrict__ di
         }
     }
     if (!have || (double)bestc < min_inl * (double)np) return;
-    // inliers, in order, over pts (lane 0)
+    // inliers, in order, compacted over pts by ballot
+    {
+        int base = 0;
+        for (int k0 = 0; k0 < np; k0 += 64) {
+            const int k = k0 + lane;
+            const bool in = k < np && on_pt(best[0], best[1], best[2], k);
+            const int q = in ? pts[k] : 0;
+            const double dq = in ? pd[k] : 0.0;
+            const uint64_t bi = __ballot(in);
+            __builtin_amdgcn_wave_barrier();  // every read of this chunk before the writes
+            if (in) {
+                pts[base + __popcll(bi & below)] = q;
+                pd[base + __popcll(bi & below)] = dq;
+            }
+            base += __popcll(bi);
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (lane == 0) counts[1] = base;
+    }
+    __builtin_amdgcn_wave_barrier();
+    const int ni = counts[1];
+    // _lsq_plane, stereonet.py:5466-5495: the coordinate sums are integers
+    // (exact in any order); the disparity and the centred sums go in order
+    long long isx = 0, isy = 0;
+    for (int k = lane; k < ni; k += 64) {
+        isx += pts[k] % g.W;
+        isy += pts[k] / g.W;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        isx += __shfl_xor(isx, o, 64);
+        isy += __shfl_xor(isy, o, 64);
+    }
     if (lane == 0) {
-        int ni = 0;
-        for (int k = 0; k < np; ++k)
-            if (on_plane(best[0], best[1], best[2], pts[k], g.W, d, thr)) pts[ni++] = pts[k];
-        // _lsq_plane, stereonet.py:5466-5495
         const double nn = (double)ni;
-        double sx = 0.0, sy = 0.0, sd = 0.0;
-        for (int k = 0; k < ni; ++k) sx += (double)(pts[k] % g.W);
-        for (int k = 0; k < ni; ++k) sy += (double)(pts[k] / g.W);
-        for (int k = 0; k < ni; ++k) sd += d[pts[k]];
+        const double sx = (double)isx, sy = (double)isy;
+        double sd = 0.0;
+        for (int k = 0; k < ni; ++k) sd += pd[k];
         const double mx = sx / nn, my = sy / nn, md = sd / nn;
         double sxx = 0.0, sxy = 0.0, syy = 0.0, sxd = 0.0, syd = 0.0;
         for (int k = 0; k < ni; ++k) {
             const double u = (double)(pts[k] % g.W) - mx;
             const double v = (double)(pts[k] / g.W) - my;
-            const double w = d[pts[k]] - md;
+            const double w = pd[k] - md;
             sxx += u * u;
             sxy += u * v;
             syy += v * v;
@@ -689,8 +767,9 @@ int plane_snap_t(const double* disp, const RT* rgb, const uint8_t* valid, int vk
     hipLaunchKernelGGL(k_ps_count, dim3(g.K, B), dim3(64), 0, st, g, last, w.lab, w.strays,
                        w.nstray, valid, vkind, w.npts);
     hipLaunchKernelGGL(k_ps_states, dim3(B), dim3(1024), 0, st, w.npts, g.K, seed, w.state);
-    hipLaunchKernelGGL(k_ps_segment, dim3(g.K, B), dim3(64), 0, st, disp, g, last, w.lab, w.strays,
-                       w.nstray, valid, vkind, w.state, thr, min_inl, out, w.err);
+    const size_t shm = (size_t)((2 * step + 1) * (2 * step + 1) + SCAP) * (8 + 4 + 4);
+    hipLaunchKernelGGL(k_ps_segment, dim3(g.K, B), dim3(64), shm, st, disp, g, last, w.lab,
+                       w.strays, w.nstray, valid, vkind, w.state, thr, min_inl, out, w.err);
     SV_CHECK_LAUNCH();
     return SV_OK;
 }
