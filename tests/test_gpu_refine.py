@@ -219,9 +219,14 @@ def test_errors(sv):
         sv.census(img, 20, 10, rx=5, ry=5)
     with pytest.raises(ValueError):
         sv.refine_fullres(img, img, torch.zeros(7, dtype=torch.float64, device="cuda"), 20, 10)
-    with pytest.raises(NotImplementedError):
-        sv.refine_disparity(img, img, [0.5] * 600, torch.zeros(200, dtype=torch.float64,
+    # planes=True with an RGB image of the wrong size
+    with pytest.raises(ValueError):
+        sv.refine_disparity(img, img, [0.5] * 599, torch.zeros(200, dtype=torch.float64,
                                                                    device="cuda"), 20, 10)
+    # plane_snap's SLIC step outside what the GPU kernels support
+    with pytest.raises(ValueError):
+        sv.plane_snap(torch.zeros(200, dtype=torch.float64, device="cuda"), [0.5] * 600, 20, 10,
+                      step=17)
 
 
 @pytest.mark.parametrize("W,H", [(700, 9), (3600, 3)])
