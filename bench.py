@@ -62,6 +62,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--cpu-pairs", type=int, default=24)
+    ap.add_argument("--threads", type=int, default=16,
+                    help="images workload: host decoding threads")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fullrange", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
@@ -87,7 +89,8 @@ def main():
                     help="apply these kernel variants instead of autotuning, e.g. "
                          "sgm_prefetch=8,sgm_htile=16,conv_rows=1,conv_tile_width=32")
     ap.add_argument("--workload", default="pipeline",
-                    choices=["pipeline", "band", "ragged64", "widened", "consistent", "planes"],
+                    choices=["pipeline", "band", "ragged64", "widened", "consistent", "planes",
+                             "images"],
                     help="pipeline: 460x370 pairs (the headline); ragged64: 64 pairs of uneven "
                          "size in one captured graph")
     args = ap.parse_args()
@@ -103,6 +106,8 @@ def main():
         return consistent_main(args)
     if args.workload == "planes":
         return planes_main(args)
+    if args.workload == "images":
+        return images_main(args)
     if args.workload == "band":
         return band_main(args)
     if args.tune_only and not args.tune_file:
@@ -729,6 +734,132 @@ def planes_main(args):
                                 "sample": "%d of the same pairs through the float64 C oracle "
                                           "(refine_disparity planes=True), %.1f s" % (n, tcpu)}
         line["gpu_over_cpu"] = round((B / dt) / (n / tcpu), 1)
+        line["identical_to_oracle"] = "%d/%d pairs" % (same, n)
+    print(json.dumps(line), flush=True)
+
+
+def _png_rgb(rgb):
+    """An 8-bit RGB PNG of rgb [H][W][3] uint8: rows filtered None, Sub, Up,
+    Average, Paeth in turn (every unfilter path), zlib level 6."""
+    import zlib
+    import numpy as np
+    Hh, Ww, _ = rgb.shape
+    a = rgb.reshape(Hh, Ww * 3).astype(np.int64)
+    out = bytearray()
+    for y in range(Hh):
+        f = y % 5
+        cur = a[y]
+        left = np.concatenate([np.zeros(3, np.int64), cur[:-3]])
+        up = a[y - 1] if y else np.zeros_like(cur)
+        ul = np.concatenate([np.zeros(3, np.int64), up[:-3]])
+        if f == 4:
+            p = left + up - ul
+            pa, pb, pc = np.abs(p - left), np.abs(p - up), np.abs(p - ul)
+            pred = np.where((pa <= pb) & (pa <= pc), left, np.where(pb <= pc, up, ul))
+        else:
+            pred = [np.zeros_like(cur), left, up, (left + up) >> 1][f]
+        out.append(f)
+        out += ((cur - pred) & 0xFF).astype(np.uint8).tobytes()
+
+    def chunk(t, b):
+        return len(b).to_bytes(4, "big") + t + b + (zlib.crc32(t + b) & 0xFFFFFFFF).to_bytes(4, "big")
+    ihdr = Ww.to_bytes(4, "big") + Hh.to_bytes(4, "big") + bytes([8, 2, 0, 0, 0])
+    return (b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", ihdr) + chunk(b"IDAT", zlib.compress(bytes(out), 6))
+            + chunk(b"IEND", b""))
+
+
+def images_main(args):
+    """Input decoding (read_image, stereonet.py:3499-3613): B synthetic
+    460x370 stereo pairs stored as 8-bit RGB PNG files (all five scanline
+    filters, dynamic-Huffman DEFLATE) decoded per step into float64 gray
+    and RGB planes on the GPU by read_images_gpu -- host inflate and
+    unfilter on --threads threads, one pinned upload, the expand kernel.
+    Reports pairs/s end to end from file bytes in host memory; the roofline
+    is the expand kernel's (HIP events); the CPU baseline is the oracle's
+    decoder (Python zlib + numpy) on one core, which also checks a sample
+    of the outputs for exact equality."""
+    import numpy as np
+    import torch
+
+    sv = importlib.import_module("from-scratch-stereo-vision_amd")
+    L = importlib.import_module("from-scratch-stereo-vision_amd._lib")
+    B = args.batch
+    pairs = [sv.synth.band_pair(s, W=W, H=H) for s in range(B)]
+
+    def rgb_of(img, disp):
+        g = img.astype(np.float64) / 255.0
+        t = np.clip(disp / 40.0, 0.0, 1.0)
+        rgb = np.stack([0.6 * g + 0.4 * t, 0.3 * g + 0.7 * (1.0 - t), 0.5 * g + 0.5 * t * t], -1)
+        return np.round(rgb * 255.0).astype(np.uint8)
+
+    lf = [_png_rgb(rgb_of(p["left"], p["disp"])) for p in pairs]
+    rf = [_png_rgb(rgb_of(p["right"], p["disp"])) for p in pairs]
+    log("images: %d pairs, %.1f KiB per PNG" % (B, sum(map(len, lf + rf)) / 2048.0 / B))
+
+    def step():
+        gl, cl, _, _ = sv.read_images_gpu(lf, threads=args.threads)
+        gr, cr, _, _ = sv.read_images_gpu(rf, threads=args.threads)
+        return gl, cl, gr, cr
+
+    for _ in range(max(args.warmup, 1)):
+        out = step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    # the expand kernel alone on resident samples, HIP events on its stream
+    npix = B * W * H
+    samp = torch.randint(0, 256, (3 * npix,), dtype=torch.int16, device=L.device())
+    gray = torch.empty(npix, dtype=torch.float64, device=L.device())
+    rgb = torch.empty(3 * npix, dtype=torch.float64, device=L.device())
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 20
+    e0.record(st)
+    for _ in range(reps):
+        L.check(L.lib().sv_image_expand(L.ptr(samp), npix, 0, 255, L.ptr(gray), L.ptr(rgb),
+                                        L.stream()))
+    e1.record(st)
+    torch.cuda.synchronize()
+    k_ms = e0.elapsed_time(e1) / reps
+    bytes_per_launch = 38.0 * npix  # 3 u16 samples in, 1 + 3 float64 out per pixel
+    achieved = bytes_per_launch / (k_ms * 1e-3) / 1e9
+    log("images: %.1f pairs/s, expand %.3f ms per %d px" % (B / dt, k_ms, npix))
+    line = {
+        "metric": "pairs/s @460x370: read_image (PNG RGB8) -> float64 gray + RGB on the GPU",
+        "value": round(B / dt, 2), "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u8 -> f64", "data": "synthetic",
+        "config": {"workload": "read_images_gpu: %d left + %d right 460x370 RGB8 PNG files "
+                               "(filters 0-4, zlib level 6) per step" % (B, B),
+                   "pairs_per_step": B, "host_threads": args.threads},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(achieved / 8000.0, 4), "traffic": None,
+                     "kernel": "k_image_expand", "kernel_ms": round(k_ms, 4),
+                     "algorithmic_bytes_per_launch": int(bytes_per_launch)},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        import oracle
+        n = min(max(args.cpu_pairs // 4, 1), B)
+        gl, cl, gr, cr = out
+        tc = time.perf_counter()
+        same = 0
+        for i in range(n):
+            g0, r0, _, _ = oracle.read_image_bytes(lf[i])
+            g1, r1, _, _ = oracle.read_image_bytes(rf[i])
+            same += int(np.array_equal(g0, gl[i].cpu().numpy().ravel())
+                        and np.array_equal(r0, cl[i].cpu().numpy().ravel())
+                        and np.array_equal(g1, gr[i].cpu().numpy().ravel())
+                        and np.array_equal(r1, cr[i].cpu().numpy().ravel()))
+        tcpu = time.perf_counter() - tc
+        line["cpu_baseline"] = {"value": round(n / tcpu, 3), "unit": "pairs/s", "cores": 1,
+                                "kind": "port",
+                                "sample": "%d of the same pairs through the oracle's decoder "
+                                          "(zlib + numpy unfilter), %.1f s incl. the comparison"
+                                          % (n, tcpu)}
         line["identical_to_oracle"] = "%d/%d pairs" % (same, n)
     print(json.dumps(line), flush=True)
 

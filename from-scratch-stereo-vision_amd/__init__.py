@@ -17,6 +17,7 @@ from .net import (StereoNet, Adam, LEVEL_WEIGHTS, CE_WEIGHT, train_grads, train_
 from .ragged import RaggedBatch, refine_ragged, net_band_ragged, CapturedPipeline
 from .consistency import (mirror_h, lr_consistency, fill_invalid, predict_consistent,
                           predict_consistent_batch)
+from .images import read_pnm, read_png, read_image, read_image_gpu, read_images_gpu
 from . import autotune, scenes, synth
 
 __all__ = ["census", "band_costs", "_aggregate_path", "refine_fullres", "_mirror1",
@@ -27,5 +28,6 @@ __all__ = ["census", "band_costs", "_aggregate_path", "refine_fullres", "_mirror
            "read_checkpoint_config", "load_weights", "load_model", "predict", "predict_batch",
            "predict_at_scale", "predict_auto", "predict_auto_batch", "widen", "RaggedBatch",
            "refine_ragged", "net_band_ragged", "CapturedPipeline", "mirror_h", "lr_consistency",
-           "fill_invalid", "predict_consistent", "predict_consistent_batch", "autotune", "scenes",
+           "fill_invalid", "predict_consistent", "predict_consistent_batch", "read_pnm", "read_png",
+           "read_image", "read_image_gpu", "read_images_gpu", "autotune", "scenes",
            "synth"]

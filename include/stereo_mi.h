@@ -317,6 +317,26 @@ int sv_axpby(const float* x, float* y, long long n, float a, float b, void* stre
 int sv_widen(const float* x, const int* dims, long long xstride, double* y, long long ystride,
              const float* mul, int N, int max_hw, void* stream);
 
+/* Image input, replacing read_pnm (stereonet.py:262-324), read_png with its
+ * zlib/DEFLATE inflater and scanline unfilter (:3204-3601) and read_image
+ * (:3604-3613).  The file bytes are decoded on the host (DEFLATE is a
+ * serial bit stream); the samples go to the GPU for the float64 planes.
+ *
+ * sv_image_size: width and height from the PNG IHDR or the Netpbm header
+ * (PNG chosen by its 8-byte signature, as read_image does).
+ * sv_image_decode: the whole file into samples[3*W*H] (host uint16, three
+ * channel values per pixel), the scale rule in *mode (0: value / maxval
+ * as value * (1/maxval); 1: palette entry / 255; 2: as 0, one-channel
+ * Netpbm whose gray is the sample itself) and *maxval.  Malformed files
+ * return SV_EINVAL with the reference's message where it has one.
+ * sv_image_expand: on the GPU, samples (device) -> gray[npix] and
+ * rgb[3*npix] float64 (either may be NULL), bit-identical to the lists
+ * read_pnm / read_png return. */
+int sv_image_size(const uint8_t* data, size_t n, int* W, int* H);
+int sv_image_decode(const uint8_t* data, size_t n, uint16_t* samples, int* mode, int* maxval);
+int sv_image_expand(const uint16_t* samples, long long npix, int mode, int maxval, double* gray,
+                    double* rgb, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

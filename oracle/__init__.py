@@ -502,3 +502,154 @@ def refine_disparity_planes(left, right, rgb, disp0, W, H, R=10, P1=12.0, P2=150
     d = plane_snap(d, rgb, W, H, valid=np.asarray(st) == 0, step=plane_step, thr=plane_thr,
                    min_inl=plane_min_inl)
     return median_filter(d, W, H, 1, 1), np.asarray(st)
+
+
+# ------------------------------------------------------------ image input
+# Pinned against the reference's own read_image outputs on 73 generated
+# PNG / Netpbm files (tests/golden/images.npz).  Inflate is Python's zlib
+# (an independent DEFLATE implementation); the rest is restated here.
+
+PNG_SIG = b"\x89PNG\r\n\x1a\n"
+
+
+def _paeth(a, b, c):
+    """stereonet.py:3436-3453."""
+    p = a + b - c
+    pa, pb, pc = abs(p - a), abs(p - b), abs(p - c)
+    if pa <= pb and pa <= pc:
+        return a
+    return b if pb <= pc else c
+
+
+def _unfilter(raw, H, bpp, stride):
+    """stereonet.py:3456-3496, one row at a time; Sub as a per-lane cumulative
+    sum mod 256, Up vectorised, Average and Paeth as loops."""
+    out = np.zeros((H, stride), dtype=np.int64)
+    raw = np.frombuffer(raw, dtype=np.uint8)
+    for y in range(H):
+        f = int(raw[y * (stride + 1)])
+        row = raw[y * (stride + 1) + 1:(y + 1) * (stride + 1)].astype(np.int64)
+        up = out[y - 1] if y > 0 else np.zeros(stride, dtype=np.int64)
+        cur = out[y]
+        if f == 0:
+            cur[:] = row
+        elif f == 1:
+            for lane in range(min(bpp, stride)):
+                cur[lane::bpp] = np.cumsum(row[lane::bpp]) & 0xFF
+        elif f == 2:
+            cur[:] = (row + up) & 0xFF
+        elif f in (3, 4):
+            r, u = row.tolist(), up.tolist()
+            o = [0] * stride
+            for i in range(stride):
+                a = o[i - bpp] if i >= bpp else 0
+                if f == 3:
+                    o[i] = (r[i] + ((a + u[i]) >> 1)) & 0xFF
+                else:
+                    c = u[i - bpp] if i >= bpp else 0
+                    o[i] = (r[i] + _paeth(a, u[i], c)) & 0xFF
+            cur[:] = o
+        else:
+            raise ValueError("unknown PNG filter type %d on row %d" % (f, y))
+    return out
+
+
+def _planes(r, g, b):
+    return (0.299 * r + 0.587 * g + 0.114 * b,
+            np.stack([r, g, b], axis=-1).reshape(-1))
+
+
+def read_png_bytes(data):
+    """read_png, stereonet.py:3499-3601: (gray, rgb, W, H) as float64 numpy."""
+    import zlib
+    if data[0:8] != PNG_SIG:
+        raise ValueError("not a PNG file")
+    pos, W, H, depth, ctype, interlace, pal, idat = 8, 0, 0, 0, 0, 0, None, b""
+    while pos < len(data):
+        n = int.from_bytes(data[pos:pos + 4], "big")
+        t, body = data[pos + 4:pos + 8], data[pos + 8:pos + 8 + n]
+        pos += 12 + n
+        if t == b"IHDR":
+            W, H = int.from_bytes(body[0:4], "big"), int.from_bytes(body[4:8], "big")
+            depth, ctype, interlace = body[8], body[9], body[12]
+        elif t == b"PLTE":
+            pal = np.frombuffer(body, dtype=np.uint8)
+        elif t == b"IDAT":
+            idat += body
+        elif t == b"IEND":
+            break
+    if interlace:
+        raise ValueError("Adam7 interlaced PNG is not supported")
+    if depth not in (1, 2, 4, 8, 16):
+        raise ValueError("unsupported PNG bit depth %d" % depth)
+    ch = {0: 1, 2: 3, 3: 1, 4: 2, 6: 4}.get(ctype)
+    if ch is None:
+        raise ValueError("unsupported PNG colour type %d" % ctype)
+    cmf, flg = idat[0], idat[1]                      # zlib_decompress :3421-3433
+    if (cmf & 0x0F) != 8 or ((cmf << 8) | flg) % 31 or flg & 0x20:
+        raise ValueError("bad zlib header")
+    raw = zlib.decompressobj(-15).decompress(idat[2:])
+    stride = (W * ch * depth + 7) // 8
+    img = _unfilter(raw, H, max(1, ch * depth // 8), stride)
+    if depth == 8:
+        s = img[:, :W * ch].reshape(H, W, ch)
+    elif depth == 16:
+        s = (img[:, 0:2 * W * ch:2] << 8 | img[:, 1:2 * W * ch:2]).reshape(H, W, ch)
+    else:
+        bit = np.arange(W) * depth
+        shift = 8 - depth - (bit & 7)
+        s = ((img[:, bit >> 3] >> shift) & ((1 << depth) - 1)).reshape(H, W, 1)
+    if ctype == 3:
+        if pal is None:
+            raise ValueError("indexed PNG with no PLTE chunk")
+        p = 3 * s[..., 0].reshape(-1)
+        r, g, b = pal[p] / 255.0, pal[p + 1] / 255.0, pal[p + 2] / 255.0
+    else:
+        inv = 1.0 / 65535.0 if depth == 16 else 1.0 / ((1 << depth) - 1)
+        v = s.reshape(-1, ch) * inv
+        r, g, b = (v[:, 0], v[:, 0], v[:, 0]) if ctype in (0, 4) else (v[:, 0], v[:, 1], v[:, 2])
+    gray, rgb = _planes(r, g, b)
+    return gray, rgb, W, H
+
+
+def read_pnm_bytes(data):
+    """read_pnm, stereonet.py:244-324 (tokens, comments, one whitespace byte
+    after maxval, 8- or 16-bit binary samples, ASCII samples)."""
+    magic = data[0:2]
+    if magic not in (b"P2", b"P3", b"P5", b"P6"):
+        raise ValueError("not a PGM/PPM file")
+    toks, i = [], 2
+    while len(toks) < 3 and i < len(data):
+        c = data[i]
+        if c == 35:
+            while i < len(data) and data[i] not in (10, 13):
+                i += 1
+        elif c in (32, 9, 10, 13):
+            i += 1
+        else:
+            j = i
+            while j < len(data) and data[j] not in (32, 9, 10, 13):
+                j += 1
+            toks.append(int(data[i:j]))
+            i = j
+    W, H, maxval = toks
+    ch = 3 if magic in (b"P3", b"P6") else 1
+    count = W * H * ch
+    if magic in (b"P2", b"P3"):
+        v = np.array([int(t) for t in data[i:].split()[:count]], dtype=np.int64)
+    elif maxval < 256:
+        v = np.frombuffer(data, dtype=np.uint8, count=count, offset=i + 1).astype(np.int64)
+    else:
+        v = np.frombuffer(data, dtype=">u2", count=count, offset=i + 1).astype(np.int64)
+    s = v * (1.0 / maxval)
+    if ch == 3:
+        s3 = s.reshape(-1, 3)
+        return _planes(s3[:, 0], s3[:, 1], s3[:, 2])[0], s, W, H
+    return s, np.repeat(s, 3), W, H
+
+
+def read_image_bytes(data):
+    """read_image, stereonet.py:3604-3613: PNG by its signature, else Netpbm."""
+    if data[0:8] == PNG_SIG:
+        return read_png_bytes(data)
+    return read_pnm_bytes(data)
