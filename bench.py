@@ -90,7 +90,7 @@ def main():
                          "sgm_prefetch=8,sgm_htile=16,conv_rows=1,conv_tile_width=32")
     ap.add_argument("--workload", default="pipeline",
                     choices=["pipeline", "band", "ragged64", "widened", "consistent", "planes",
-                             "images"],
+                             "images", "eval"],
                     help="pipeline: 460x370 pairs (the headline); ragged64: 64 pairs of uneven "
                          "size in one captured graph")
     args = ap.parse_args()
@@ -108,6 +108,8 @@ def main():
         return planes_main(args)
     if args.workload == "images":
         return images_main(args)
+    if args.workload == "eval":
+        return eval_main(args)
     if args.workload == "band":
         return band_main(args)
     if args.tune_only and not args.tune_file:
@@ -861,6 +863,122 @@ def images_main(args):
                                           "(zlib + numpy unfilter), %.1f s incl. the comparison"
                                           % (n, tcpu)}
         line["identical_to_oracle"] = "%d/%d pairs" % (same, n)
+    print(json.dumps(line), flush=True)
+
+
+def eval_main(args):
+    """Middlebury scoring (eval_real.py:78-127) of B synthetic 460x370 pairs
+    per step: both views' ground-truth PNGs (grey level = 3 x disparity)
+    decoded by read_images_gpu, ground truth and known masks, the
+    non-occluded mask, then EPE / bad1 / bad2 / bad4 over the non-occluded
+    and known masks and the constant baseline, for float32 predictions
+    already on the GPU.  Reports pairs/s (decode included) and the scoring
+    kernels alone; roofline on k_score (HIP events); CPU baseline the
+    oracle's scoring on a few of the same pairs."""
+    import numpy as np
+    import torch
+
+    sv = importlib.import_module("from-scratch-stereo-vision_amd")
+    L = importlib.import_module("from-scratch-stereo-vision_amd._lib")
+    ev = sv.evaluation
+    B = args.batch
+    pairs = [sv.synth.band_pair(s, W=W, H=H) for s in range(B)]
+
+    def gt_png(d, valid):
+        lv = np.where(valid, np.clip(np.round(d * 3.0), 0, 255), 0).astype(np.uint8)
+        return _png_rgb(np.repeat(lv[:, :, None], 3, axis=2))
+
+    fl, fr = [], []
+    for p in pairs:
+        d = np.asarray(p["disp"], np.float64).reshape(H, W)
+        fl.append(gt_png(d, np.asarray(p["valid"], bool).reshape(H, W)))
+        dr = np.asarray(sv.left_to_right(torch.from_numpy(d.ravel()).cuda(), W, H).cpu()).reshape(H, W)
+        fr.append(gt_png(dr, dr > 0))
+    rng = np.random.default_rng(1)
+    pred = torch.from_numpy(np.stack([np.asarray(p["disp"], np.float64).reshape(H, W)
+                                      + rng.normal(0, 1.5, (H, W)) for p in pairs])).float().cuda()
+
+    def scoring(gl, gr):
+        dl, kl = ev.gt_from_gray(gl, 3.0)
+        dr, kr = ev.gt_from_gray(gr, 3.0)
+        no = ev.nonocc_mask(dl, kl, dr, kr, W, H)
+        s_no, g, m = ev.score_stats(pred, dl, no, B=B)
+        s_all, _, _ = ev.score_stats(pred, dl, kl, B=B)
+        ad = ev.abs_dev_sums(g, m, s_no, B=B)
+        return dl, kl, no, s_no, s_all, ad
+
+    def step():
+        gl = sv.read_images_gpu(fl, threads=args.threads)[0]
+        gr = sv.read_images_gpu(fr, threads=args.threads)[0]
+        return scoring(gl, gr)
+
+    for _ in range(max(args.warmup, 1)):
+        out = step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    gl = sv.read_images_gpu(fl, threads=args.threads)[0]
+    gr = sv.read_images_gpu(fr, threads=args.threads)[0]
+    st = torch.cuda.current_stream()
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    reps = 20
+    e0.record(st)
+    for _ in range(reps):
+        scoring(gl, gr)
+    e1.record(st)
+    dl, kl, no = out[0], out[1], out[2]
+    for _ in range(reps):
+        ev.score_stats(pred, dl, no, B=B)
+    e2.record(st)
+    torch.cuda.synchronize()
+    sc_ms = e0.elapsed_time(e1) / reps
+    k_ms = e1.elapsed_time(e2) / reps  # k_score + its 1-block partial sum
+    npix = B * W * H
+    bytes_per_launch = 13.0 * npix  # f32 prediction, float64 ground truth, u8 mask
+    achieved = bytes_per_launch / (k_ms * 1e-3) / 1e9
+    log("eval: %.1f pairs/s (scoring alone %.3f ms per step)" % (B / dt, sc_ms))
+    line = {
+        "metric": "pairs/s @460x370: Middlebury scoring (load_gt + masks + score + constant)",
+        "value": round(B / dt, 2), "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "eval_real scoring: %d pairs of 460x370 ground-truth PNGs decoded, "
+                               "masks, EPE/bad1/2/4 over nonocc and all, constant" % B,
+                   "pairs_per_step": B, "host_threads": args.threads},
+        "scoring_ms_per_step": round(sc_ms, 4),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(achieved / 8000.0, 4), "traffic": None,
+                     "kernel": "k_score<float> (+ k_sum_parts)", "kernel_ms": round(k_ms, 4),
+                     "algorithmic_bytes_per_launch": int(bytes_per_launch)},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        import oracle
+        n = min(max(args.cpu_pairs // 4, 1), B)
+        s_no, s_all, ad = out[3].cpu().numpy(), out[4].cpu().numpy(), out[5].cpu().numpy()
+        tc = time.perf_counter()
+        same = 0
+        for i in range(n):
+            g0, _, _, _ = oracle.read_image_bytes(fl[i])
+            g1, _, _, _ = oracle.read_image_bytes(fr[i])
+            d0, k0 = oracle.gt_from_gray(g0, 3.0)
+            d1, k1 = oracle.gt_from_gray(g1, 3.0)
+            n0 = oracle.nonocc(d0, k0, d1, k1, W, H)
+            p = pred[i].double().cpu().numpy().ravel()
+            r_no, r_all, c = oracle.score(p, d0, n0), oracle.score(p, d0, k0), oracle.constant(d0, n0)
+            s, a = s_no[i], s_all[i]
+            same += int(s[0] == int(n0.sum()) and abs(s[1] / s[0] - r_no[0]) <= 1e-12 * r_no[0]
+                        and 100.0 * s[3] / s[0] == r_no[2] and 100.0 * a[3] / a[0] == r_all[2]
+                        and abs(ad[i] / s[0] - c) <= 1e-12 * c)
+        tcpu = time.perf_counter() - tc
+        line["cpu_baseline"] = {"value": round(n / tcpu, 3), "unit": "pairs/s", "cores": 1,
+                                "kind": "port",
+                                "sample": "%d of the same pairs through the oracle (PNG decode, "
+                                          "masks, sequential sums), %.1f s" % (n, tcpu)}
+        line["identical_to_oracle"] = "%d/%d pairs (counts exact, sums within 1e-12)" % (same, n)
     print(json.dumps(line), flush=True)
 
 

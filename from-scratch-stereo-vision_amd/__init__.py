@@ -17,8 +17,9 @@ from .net import (StereoNet, Adam, LEVEL_WEIGHTS, CE_WEIGHT, train_grads, train_
 from .ragged import RaggedBatch, refine_ragged, net_band_ragged, CapturedPipeline
 from .consistency import (mirror_h, lr_consistency, fill_invalid, predict_consistent,
                           predict_consistent_batch)
-from .images import read_pnm, read_png, read_image, read_image_gpu, read_images_gpu
-from . import autotune, scenes, synth
+from .images import (read_pnm, read_png, read_image, read_image_gpu, read_images_gpu, write_pgm,
+                     write_ppm)
+from . import autotune, scenes, synth, evaluation
 
 __all__ = ["census", "band_costs", "_aggregate_path", "refine_fullres", "_mirror1",
            "left_to_right", "lr_refine", "median_filter", "refine_disparity", "slic", "plane_snap",
@@ -29,5 +30,5 @@ __all__ = ["census", "band_costs", "_aggregate_path", "refine_fullres", "_mirror
            "predict_at_scale", "predict_auto", "predict_auto_batch", "widen", "RaggedBatch",
            "refine_ragged", "net_band_ragged", "CapturedPipeline", "mirror_h", "lr_consistency",
            "fill_invalid", "predict_consistent", "predict_consistent_batch", "read_pnm", "read_png",
-           "read_image", "read_image_gpu", "read_images_gpu", "autotune", "scenes",
-           "synth"]
+           "read_image", "read_image_gpu", "read_images_gpu", "write_pgm", "write_ppm",
+           "autotune", "scenes", "synth", "evaluation"]

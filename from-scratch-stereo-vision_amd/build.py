@@ -25,7 +25,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wno-unused-command-line-argument"]
 # per-file floating-point contraction: exact files never fuse multiply-adds
-EXACT = {"refine.hip", "common.hip", "planes.hip", "imageio.hip"}
+EXACT = {"refine.hip", "common.hip", "planes.hip", "imageio.hip", "evalmetrics.hip"}
 
 
 def sources():

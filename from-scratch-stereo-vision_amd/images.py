@@ -144,3 +144,33 @@ def read_image(path):
     if data[0:2] not in (b"P2", b"P3", b"P5", b"P6"):
         raise ValueError("not a PGM/PPM file: " + str(path))
     return _lists(path, data)
+
+
+def _quantise(v, lo, hi):
+    """write_pgm's mapping: int(255 (v - lo) / span + 0.5) clamped to 0..255."""
+    t = np.trunc(255.0 * (v - lo) / (hi - lo if hi - lo > 1.0e-12 else 1.0) + 0.5)
+    return np.clip(t, 0, 255).astype(np.uint8)
+
+
+def write_pgm(path, gray, width, height, lo=None, hi=None):
+    """write_pgm, stereonet.py:326-355: binary P5, scaled from [lo, hi]
+    (the values' own range when not given) to 0..255."""
+    v = gray.detach().double().cpu().numpy() if isinstance(gray, torch.Tensor) else \
+        np.asarray(gray, dtype=np.float64)
+    v = v.ravel()
+    lo = float(v.min()) if lo is None else lo
+    hi = float(v.max()) if hi is None else hi
+    with open(path, "wb") as f:
+        f.write(("P5\n%d %d\n255\n" % (width, height)).encode("ascii"))
+        f.write(_quantise(v, lo, hi).tobytes())
+
+
+def write_ppm(path, rgb, width, height):
+    """write_ppm, stereonet.py:358-372: binary P6 from interleaved [0, 1]
+    floats, int(255 v + 0.5) clamped."""
+    v = rgb.detach().double().cpu().numpy() if isinstance(rgb, torch.Tensor) else \
+        np.asarray(rgb, dtype=np.float64)
+    t = np.clip(np.trunc(255.0 * v.ravel() + 0.5), 0, 255).astype(np.uint8)
+    with open(path, "wb") as f:
+        f.write(("P6\n%d %d\n255\n" % (width, height)).encode("ascii"))
+        f.write(t.tobytes())

@@ -337,6 +337,30 @@ int sv_image_decode(const uint8_t* data, size_t n, uint16_t* samples, int* mode,
 int sv_image_expand(const uint16_t* samples, long long npix, int mode, int maxval, double* gray,
                     double* rgb, void* stream);
 
+/* Middlebury scoring, replacing eval_real.py's load_gt (:78-87), masks
+ * (:90-102), score (:105-121) and constant (:124-127); device pointers.
+ *
+ * sv_gt_from_gray: level = gray * 255; known[i] = level >= 0.5, disp[i] =
+ * level / scale where known, else 0.
+ * sv_gt_nonocc: for B maps of W x H, nonocc[i] = known left pixel whose
+ * xr = (int)(x - dl + 0.5) (toward zero) is inside the row, known on the
+ * right, with |dl - dr(xr)| <= 1.
+ * sv_score: per map, out[6b..6b+5] = (n, sum |pred - gt|, count > 1,
+ * count > 2, count > 4, sum gt) over mask != 0; pred f32 or f64 (SV_F32 /
+ * SV_F64).  sv_abs_dev: out[b] = sum |gt - m| over the mask with m =
+ * stats[6b+5] / stats[6b] (stats from sv_score).  Sums are float64 in a
+ * fixed order (deterministic; equal to the reference's left-to-right sums
+ * to rounding); both take a workspace of sv_score_workspace_bytes. */
+int sv_gt_from_gray(const double* gray, long long n, double scale, double* disp, uint8_t* known,
+                    void* stream);
+int sv_gt_nonocc(const double* dl, const uint8_t* kl, const double* dr, const uint8_t* kr, int B,
+                 int W, int H, uint8_t* nonocc, void* stream);
+int sv_score_workspace_bytes(int B, long long npix, size_t* bytes);
+int sv_score(const void* pred, int pred_dtype, const double* gt, const uint8_t* mask, int B,
+             long long npix, double* out, void* ws, size_t ws_bytes, void* stream);
+int sv_abs_dev(const double* gt, const uint8_t* mask, const double* stats, int B, long long npix,
+               double* out, void* ws, size_t ws_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -653,3 +653,53 @@ def read_image_bytes(data):
     if data[0:8] == PNG_SIG:
         return read_png_bytes(data)
     return read_pnm_bytes(data)
+
+
+# ------------------------------------------------------ Middlebury scoring
+# Pinned against the reference harness's own load_gt / masks / score /
+# constant outputs (tests/golden/eval.npz).
+
+def gt_from_gray(g, scale):
+    """load_gt, eval_real.py:78-87, from the decoded gray plane."""
+    lv = np.asarray(g, dtype=np.float64) * 255.0
+    known = lv >= 0.5
+    return np.where(known, lv / scale, 0.0), known
+
+
+def nonocc(dl, kl, dr, kr, W, H):
+    """masks, eval_real.py:90-102 (int() truncates toward zero)."""
+    dl, dr = dl.reshape(H, W), dr.reshape(H, W)
+    kl, kr = kl.reshape(H, W).astype(bool), kr.reshape(H, W).astype(bool)
+    t = np.arange(W)[None, :] - dl + 0.5
+    inside = kl & (t > -1.0) & (t < W)
+    xr = np.where(inside, np.trunc(np.where(inside, t, 0.0)), 0).astype(np.int64)
+    rows = np.arange(H)[:, None].repeat(W, 1)
+    ok = inside & kr[rows, xr] & (np.abs(dl - dr[rows, xr]) <= 1.0)
+    return ok.reshape(-1)
+
+
+def score(pred, gt, mask):
+    """score, eval_real.py:105-121, summed left to right like the reference."""
+    d = np.abs(np.asarray(pred, dtype=np.float64) - np.asarray(gt, dtype=np.float64))
+    d = d[np.asarray(mask, dtype=bool).ravel()]
+    if d.size == 0:
+        return None
+    e = 0.0
+    for v in d.tolist():
+        e += v
+    n = d.size
+    return (e / n, 100.0 * int((d > 1.0).sum()) / n, 100.0 * int((d > 2.0).sum()) / n,
+            100.0 * int((d > 4.0).sum()) / n)
+
+
+def constant(gt, mask):
+    """constant, eval_real.py:124-127, sequential sums."""
+    v = np.asarray(gt, dtype=np.float64)[np.asarray(mask, dtype=bool).ravel()].tolist()
+    s = 0.0
+    for x in v:
+        s += x
+    m = s / len(v)
+    a = 0.0
+    for x in v:
+        a += abs(x - m)
+    return a / len(v)
