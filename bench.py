@@ -90,7 +90,7 @@ def main():
                          "sgm_prefetch=8,sgm_htile=16,conv_rows=1,conv_tile_width=32")
     ap.add_argument("--workload", default="pipeline",
                     choices=["pipeline", "band", "ragged64", "widened", "consistent", "planes",
-                             "images", "eval"],
+                             "images", "eval", "mesh"],
                     help="pipeline: 460x370 pairs (the headline); ragged64: 64 pairs of uneven "
                          "size in one captured graph")
     args = ap.parse_args()
@@ -110,6 +110,8 @@ def main():
         return images_main(args)
     if args.workload == "eval":
         return eval_main(args)
+    if args.workload == "mesh":
+        return mesh_main(args)
     if args.workload == "band":
         return band_main(args)
     if args.tune_only and not args.tune_file:
@@ -979,6 +981,94 @@ def eval_main(args):
                                 "sample": "%d of the same pairs through the oracle (PNG decode, "
                                           "masks, sequential sums), %.1f s" % (n, tcpu)}
         line["identical_to_oracle"] = "%d/%d pairs (counts exact, sums within 1e-12)" % (same, n)
+    print(json.dumps(line), flush=True)
+
+
+def mesh_main(args):
+    """The viewer's grid mesh (grid_mesh, stereonet.py:5770-5862; step 2 with
+    the background fill) for B refined-quality 460x370 disparity maps per
+    step (synthetic truth + noise), each map through robust_min_disparity,
+    the main grid, background_region and the fill grid on the GPU; the
+    vertex and triangle counts are read back once per map.  Roofline on the
+    row min filter (k_minrow, radius W/16 = 28) timed alone with HIP events
+    over all B maps in one launch; CPU baseline the oracle's grid_mesh."""
+    import numpy as np
+    import torch
+
+    sv = importlib.import_module("from-scratch-stereo-vision_amd")
+    L = importlib.import_module("from-scratch-stereo-vision_amd._lib")
+    G = sv.geometry
+    B = args.batch
+    rng = np.random.default_rng(2)
+    maps = np.stack([np.asarray(sv.synth.pipeline_pair(s)["disp"], np.float64).reshape(H, W)
+                     + rng.normal(0, 0.1, (H, W)) for s in range(B)])
+    D = torch.from_numpy(maps).cuda()
+    cam = sv.Camera.synthetic(W, H)
+
+    def step():
+        return [G.grid_mesh_gpu(D[b], W, H, cam, step=2, fill=True) for b in range(B)]
+
+    for _ in range(max(args.warmup, 1)):
+        out = step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    flat = D.reshape(-1)
+    mo = torch.empty_like(flat)
+    mi = torch.empty(flat.numel(), dtype=torch.int32, device=flat.device)
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 20
+    r = int(W / 16)
+    e0.record(st)
+    for _ in range(reps):
+        L.check(L.lib().sv_min_filter_arg(L.ptr(flat), B, W, H, r, 0, L.ptr(mo), L.ptr(mi), None,
+                                          None, L.stream()))
+    e1.record(st)
+    torch.cuda.synchronize()
+    k_ms = e0.elapsed_time(e1) / reps
+    bytes_per_launch = 20.0 * B * W * H  # disparity in, minimum and source index out
+    achieved = bytes_per_launch / (k_ms * 1e-3) / 1e9
+    nv = sum(g["pos"].shape[0] for g in out) / B
+    nt = sum(g["main_tris"] + g["fill_tris"] for g in out) / B
+    log("mesh: %.1f maps/s, %.0f vertices and %.0f triangles per map; k_minrow %.3f ms"
+        % (B / dt, nv, nt, k_ms))
+    line = {
+        "metric": "pairs/s @460x370: grid_mesh (step 2, background fill) of the refined disparity",
+        "value": round(B / dt, 2), "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "grid_mesh(step=2, fill=True) of %d 460x370 disparity maps per "
+                               "step, one map per call (counts read back per map)" % B,
+                   "pairs_per_step": B, "vertices_per_map": round(nv), "triangles_per_map": round(nt)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(achieved / 8000.0, 4), "traffic": None,
+                     "kernel": "k_minrow (r = 28, %d maps per launch)" % B,
+                     "kernel_ms": round(k_ms, 4),
+                     "algorithmic_bytes_per_launch": int(bytes_per_launch)},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        import oracle
+        n = min(max(args.cpu_pairs // 8, 1), B)
+        camt = (cam.fx, cam.cx, cam.cy, cam.baseline, cam.doffs)
+        tc = time.perf_counter()
+        same = 0
+        for i in range(n):
+            ref = oracle.grid_mesh(maps[i].ravel(), W, H, camt, step=2, fill=True)
+            g = out[i]
+            same += int(all(np.array_equal(g[f].cpu().numpy(), ref[f])
+                            for f in ("pos", "uv", "disp", "fill", "idx"))
+                        and g["pivot"] == ref["pivot"])
+        tcpu = time.perf_counter() - tc
+        line["cpu_baseline"] = {"value": round(n / tcpu, 3), "unit": "pairs/s", "cores": 1,
+                                "kind": "port",
+                                "sample": "%d of the same maps through the oracle's grid_mesh "
+                                          "(numpy filters, Python vertex loops), %.1f s" % (n, tcpu)}
+        line["identical_to_oracle"] = "%d/%d maps" % (same, n)
     print(json.dumps(line), flush=True)
 
 

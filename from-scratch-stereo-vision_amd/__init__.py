@@ -19,7 +19,9 @@ from .consistency import (mirror_h, lr_consistency, fill_invalid, predict_consis
                           predict_consistent_batch)
 from .images import (read_pnm, read_png, read_image, read_image_gpu, read_images_gpu, write_pgm,
                      write_ppm)
-from . import autotune, scenes, synth, evaluation
+from .geometry import (Camera, read_calib, robust_min_disparity, disparity_to_points,
+                       min_filter_arg, smooth_background, background_region, grid_mesh)
+from . import autotune, scenes, synth, evaluation, geometry
 
 __all__ = ["census", "band_costs", "_aggregate_path", "refine_fullres", "_mirror1",
            "left_to_right", "lr_refine", "median_filter", "refine_disparity", "slic", "plane_snap",
@@ -31,4 +33,6 @@ __all__ = ["census", "band_costs", "_aggregate_path", "refine_fullres", "_mirror
            "refine_ragged", "net_band_ragged", "CapturedPipeline", "mirror_h", "lr_consistency",
            "fill_invalid", "predict_consistent", "predict_consistent_batch", "read_pnm", "read_png",
            "read_image", "read_image_gpu", "read_images_gpu", "write_pgm", "write_ppm",
-           "autotune", "scenes", "synth", "evaluation"]
+           "Camera", "read_calib", "robust_min_disparity", "disparity_to_points", "min_filter_arg",
+           "smooth_background", "background_region", "grid_mesh",
+           "autotune", "scenes", "synth", "evaluation", "geometry"]

@@ -25,7 +25,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wno-unused-command-line-argument"]
 # per-file floating-point contraction: exact files never fuse multiply-adds
-EXACT = {"refine.hip", "common.hip", "planes.hip", "imageio.hip", "evalmetrics.hip"}
+EXACT = {"refine.hip", "common.hip", "planes.hip", "imageio.hip", "evalmetrics.hip",
+         "geometry.hip"}
 
 
 def sources():

@@ -361,6 +361,53 @@ int sv_score(const void* pred, int pred_dtype, const double* gt, const uint8_t* 
 int sv_abs_dev(const double* gt, const uint8_t* mask, const double* stats, int B, long long npix,
                double* out, void* ws, size_t ws_bytes, void* stream);
 
+/* Disparity to 3D, replacing Camera (stereonet.py:1951-1992),
+ * robust_min_disparity (:1989-2003), disparity_to_points (:2006-2021),
+ * min_filter_arg (:5566-5605), smooth_background (:5608-5634),
+ * background_region (:5637-5671) and grid_mesh (:5770-5862).  Device
+ * pointers except cam: a host array {fx, cx, cy, baseline, doffs}.  float64
+ * throughout, bit-identical to the reference.
+ *
+ * sv_disparity_to_points: pts[3i..3i+2] = (X, Y, Z), valid[i] = 0 (pts
+ * NaN) where d < min_disp or Z is not in (0, max_depth].
+ * sv_kth_f64: *out (device) = the k-th smallest (0-based) of x[0],
+ * x[stride], ..., x[(n-1)*stride]; robust_min_disparity's median and
+ * grid_mesh's pivot.
+ * sv_min_filter_arg: B maps; minimum over the clipped window (radius r
+ * along rows, rv along columns), idx = flat in-image index of the first
+ * strict minimum (rows scanned first); rv > 0 needs tmp (B*W*H doubles)
+ * and tmp_idx (B*W*H ints).
+ * sv_smooth_background: box means of radius r over rows then columns,
+ * clamped to at most disp; tmp holds B*W*H doubles.
+ * sv_background_region: mask (uint8), m (smoothed background disparity),
+ * src (int32 source pixel) for B maps; tmp (doubles) and tmp_mask (bytes)
+ * of B*W*H each.
+ * sv_grid_mesh_build: one build() of grid_mesh (main: region = src = NULL;
+ * fill: the background mask and source pixels, val = m).  Vertices land
+ * after *vbase and triangles after *tbase (device ints, NULL = 0), in the
+ * reference's order; totals[0..1] (device) receive this build's vertex and
+ * triangle counts.  pos[3n], uv[2n], flag[n], dv[n] and idx[3t] must hold
+ * nx*ny vertices and 2*(nx-1)*(ny-1) triangles per build (nx, ny the grid
+ * sizes); ws is sv_grid_mesh_workspace_bytes. */
+int sv_disparity_to_points(const double* disp, int B, int W, int H, const double* cam,
+                           double min_disp, double max_depth, double* pts, uint8_t* valid,
+                           void* stream);
+int sv_kth_f64_workspace_bytes(size_t* bytes);
+int sv_kth_f64(const double* x, long long stride, long long n, long long k, double* out, void* ws,
+               size_t ws_bytes, void* stream);
+int sv_min_filter_arg(const double* v, int B, int W, int H, int r, int rv, double* out, int* idx,
+                      double* tmp, int* tmp_idx, void* stream);
+int sv_smooth_background(const double* m, const double* disp, int B, int W, int H, int r,
+                         double* tmp, double* out, void* stream);
+int sv_background_region(const double* disp, int B, int W, int H, int radius, double jump_ratio,
+                         double min_jump, uint8_t* mask, double* m, int* src, double* tmp,
+                         uint8_t* tmp_mask, void* stream);
+int sv_grid_mesh_workspace_bytes(int W, int H, int step, size_t* bytes);
+int sv_grid_mesh_build(const double* val, const uint8_t* region, const int* src, int W, int H,
+                       int step, const double* cam, double min_disp, double jump, double jump_rel,
+                       const int* vbase, const int* tbase, double* pos, double* uv, uint8_t* flag,
+                       double* dv, int* idx, int* totals, void* ws, size_t ws_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -703,3 +703,185 @@ def constant(gt, mask):
     for x in v:
         a += abs(x - m)
     return a / len(v)
+
+
+# -------------------------------------------------------- disparity to 3D
+# Pinned against the reference's own outputs (tests/golden/geometry.npz).
+
+def _depth(cam, d):
+    """Camera.depth_from_disparity, stereonet.py:1982-1986 (cam = fx, cx, cy,
+    baseline, doffs)."""
+    fx, cx, cy, bl, doffs = cam
+    den = np.asarray(d, dtype=np.float64) + doffs
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return np.where(den < 1.0e-6, 0.0, (fx * bl) / den)
+
+
+def disparity_to_points(disp, W, H, cam, min_disp=0.5, max_depth=1.0e6):
+    """stereonet.py:2006-2021: (points [n][3] with NaN rows, valid)."""
+    d = np.asarray(disp, dtype=np.float64).ravel()
+    fx, cx, cy = cam[0], cam[1], cam[2]
+    Z = _depth(cam, d)
+    ok = ~(d < min_disp) & ~((Z <= 0.0) | (Z > max_depth))
+    x = (np.arange(d.size) % W).astype(np.float64)
+    y = ((np.arange(d.size) // W) % H).astype(np.float64)
+    pts = np.stack([(x - cx) * Z / fx, (y - cy) * Z / fx, Z], -1)
+    pts[~ok] = np.nan
+    return pts, ok
+
+
+def robust_min_disparity(disp, floor=0.4, factor=0.38):
+    """stereonet.py:1989-2003."""
+    v = np.sort(np.asarray(disp, dtype=np.float64).ravel())
+    m = v[v.size // 2] * factor
+    return floor if floor > m else m
+
+
+def _win_first_min(v, r):
+    """Per row of v [rows][W]: first strict minimum over [x - r, x + r]
+    clipped, and its column."""
+    rows, W = v.shape
+    x = np.arange(W)
+    a = np.maximum(x - r, 0)
+    b = np.minimum(x + r, W - 1)
+    bv = v[:, a].copy()
+    bi = np.broadcast_to(a, v.shape).copy()
+    for k in range(1, 2 * r + 1):
+        xx = a + k
+        inb = xx <= b
+        cand = v[:, np.minimum(xx, W - 1)]
+        upd = inb[None, :] & (cand < bv)
+        bv = np.where(upd, cand, bv)
+        bi = np.where(upd, np.minimum(xx, W - 1)[None, :], bi)
+    return bv, bi
+
+
+def min_filter_arg(v, W, H, r, rv=None):
+    """min_filter_arg, stereonet.py:5566-5605 (rows, then columns)."""
+    rv = r if rv is None else rv
+    v2 = np.asarray(v, dtype=np.float64).reshape(H, W)
+    rowv, rc = _win_first_min(v2, r)
+    ri = np.arange(H)[:, None] * W + rc
+    if rv == 0:
+        return rowv.ravel(), ri.ravel()
+    cv, cr = _win_first_min(rowv.T, rv)  # [W][H]: rows of the transpose are columns
+    out = cv.T
+    src_rows = cr.T  # the row the minimum came from, per pixel
+    oi = ri[src_rows, np.arange(W)[None, :]]
+    return out.ravel(), oi.ravel()
+
+
+def _box_sum_means(v, r):
+    """Per row: sum over [x - r, x + r] clipped, left to right, / count."""
+    rows, W = v.shape
+    x = np.arange(W)
+    a = np.maximum(x - r, 0)
+    b = np.minimum(x + r, W - 1)
+    s = np.zeros_like(v)
+    for k in range(0, 2 * r + 1):
+        xx = a + k
+        inb = (xx <= b)[None, :]
+        s = np.where(inb, s + v[:, np.minimum(xx, W - 1)], s)
+    return s / (b - a + 1).astype(np.float64)[None, :]
+
+
+def smooth_background(m, disp, W, H, r):
+    """smooth_background, stereonet.py:5608-5634."""
+    t = _box_sum_means(np.asarray(m, dtype=np.float64).reshape(H, W), r)
+    v = _box_sum_means(t.T, r).T.ravel()
+    d = np.asarray(disp, dtype=np.float64).ravel()
+    return np.where(v < d, v, d)
+
+
+def background_region(disp, W, H, radius=None, jump_ratio=0.15, min_jump=2.0):
+    """background_region, stereonet.py:5637-5671: (mask, m, src)."""
+    if radius is None:
+        radius = int(W / 16) if W >= 64 else 4
+    d = np.asarray(disp, dtype=np.float64).ravel()
+    m, src = min_filter_arg(d, W, H, radius, 0)
+    m = smooth_background(m, d, W, H, radius // 2 if radius >= 2 else 1)
+    thr = np.where(jump_ratio * d < min_jump, min_jump, jump_ratio * d)
+    mask = (d - m > thr).reshape(H, W)
+    for _ in range(2):
+        g = mask.copy()
+        g[:, 1:] |= mask[:, :-1]
+        g[:, :-1] |= mask[:, 1:]
+        g[1:, :] |= mask[:-1, :]
+        g[:-1, :] |= mask[1:, :]
+        mask = g
+    return mask.ravel(), m, src
+
+
+def grid_mesh(disp, W, H, cam, step=2, fill=True, jump=1.5, jump_rel=0.06, min_disp=None):
+    """grid_mesh, stereonet.py:5770-5862, as plain loops; returns the dict
+    with numpy arrays (pos [n][3], uv [n][2], fill, disp, idx [t][3])."""
+    d = np.asarray(disp, dtype=np.float64).ravel()
+    fx, cx, cy = cam[0], cam[1], cam[2]
+    if min_disp is None:
+        min_disp = robust_min_disparity(d)
+    xs = list(range(0, W, step))
+    if xs[-1] != W - 1:
+        xs.append(W - 1)
+    ys = list(range(0, H, step))
+    if ys[-1] != H - 1:
+        ys.append(H - 1)
+    nx, ny = len(xs), len(ys)
+    pos, uv, flag, dv, idx = [], [], [], [], []
+
+    def build(val, is_fill, region, src):
+        vid = [-1] * (nx * ny)
+        for j, y in enumerate(ys):
+            for i, x in enumerate(xs):
+                p = y * W + x
+                if region is not None and not region[p]:
+                    continue
+                v = float(val[p])
+                if not v >= min_disp or (not is_fill and x - v < 0):
+                    continue
+                Z = float(_depth(cam, v))
+                if Z <= 0.0:
+                    continue
+                vid[j * nx + i] = len(dv)
+                pos.append(((x - cx) * Z / fx, (y - cy) * Z / fx, Z))
+                if is_fill:
+                    s = int(src[p])
+                    uv.append(((s % W + 0.5) / W, (s // W + 0.5) / H))
+                else:
+                    uv.append(((x + 0.5) / W, (y + 0.5) / H))
+                flag.append(1 if is_fill else 0)
+                dv.append(v)
+
+        def ok(a, b, c):
+            if min(a, b, c) < 0:
+                return False
+            da, db, dc = dv[a], dv[b], dv[c]
+            lo = da if da < (db if db < dc else dc) else (db if db < dc else dc)
+            hi = da if da > (db if db > dc else dc) else (db if db > dc else dc)
+            lim = jump if jump > jump_rel * hi else jump_rel * hi
+            return hi - lo <= lim
+
+        n = 0
+        for j in range(ny - 1):
+            for i in range(nx - 1):
+                a, b = vid[j * nx + i], vid[j * nx + i + 1]
+                c, e = vid[(j + 1) * nx + i], vid[(j + 1) * nx + i + 1]
+                if ok(a, b, c):
+                    idx.append((a, c, b))
+                    n += 1
+                if ok(b, c, e):
+                    idx.append((b, c, e))
+                    n += 1
+        return n
+
+    main = build(d, False, None, None)
+    nfill = 0
+    if fill:
+        mask, m, src = background_region(d, W, H)
+        nfill = build(m, True, mask, src)
+    zs = sorted(pos[k][2] for k in range(len(flag)) if not flag[k])
+    pivot = zs[len(zs) // 2] if zs else 1.0
+    return {"pos": np.array(pos, dtype=np.float64).reshape(-1, 3),
+            "uv": np.array(uv, dtype=np.float64).reshape(-1, 2),
+            "fill": np.array(flag, dtype=np.int64), "disp": np.array(dv, dtype=np.float64),
+            "idx": np.array(idx, dtype=np.int64).reshape(-1, 3), "main_tris": main,
+            "fill_tris": nfill, "pivot": pivot}
