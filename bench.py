@@ -987,9 +987,10 @@ def eval_main(args):
 def mesh_main(args):
     """The viewer's grid mesh (grid_mesh, stereonet.py:5770-5862; step 2 with
     the background fill) for B refined-quality 460x370 disparity maps per
-    step (synthetic truth + noise), each map through robust_min_disparity,
-    the main grid, background_region and the fill grid on the GPU; the
-    vertex and triangle counts are read back once per map.  Roofline on the
+    step (synthetic truth + noise): robust_min_disparity, the main grid,
+    background_region, the fill grid and the pivot for all B maps with one
+    launch per stage (grid_mesh_batch); the counts and pivots are read back
+    once per step.  Roofline on the
     row min filter (k_minrow, radius W/16 = 28) timed alone with HIP events
     over all B maps in one launch; CPU baseline the oracle's grid_mesh."""
     import numpy as np
@@ -1006,7 +1007,7 @@ def mesh_main(args):
     cam = sv.Camera.synthetic(W, H)
 
     def step():
-        return [G.grid_mesh_gpu(D[b], W, H, cam, step=2, fill=True) for b in range(B)]
+        return G.grid_mesh_batch(D, W, H, cam, step=2, fill=True)
 
     for _ in range(max(args.warmup, 1)):
         out = step()
@@ -1042,7 +1043,7 @@ def mesh_main(args):
         "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "grid_mesh(step=2, fill=True) of %d 460x370 disparity maps per "
-                               "step, one map per call (counts read back per map)" % B,
+                               "step in one batch (counts read back once)" % B,
                    "pairs_per_step": B, "vertices_per_map": round(nv), "triangles_per_map": round(nt)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
                      "frac": round(achieved / 8000.0, 4), "traffic": None,

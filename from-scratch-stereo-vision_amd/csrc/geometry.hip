@@ -62,7 +62,7 @@ constexpr int kSelPer = kSelBins / 256;  // bins per thread in the pick
 
 struct Sel64 {
     unsigned long long prefix;
-    long long k;
+    long long k, n;
     unsigned done;  // blocks finished in the current pass
 };
 
@@ -81,24 +81,37 @@ __device__ __forceinline__ void sel_digit(int pass, int& shift, int& width) {
     shift = pass < kSelPasses - 1 ? 64 - kSelBits * (pass + 1) : 0;
 }
 
-__global__ void k_sel64_init(unsigned* __restrict__ hist, Sel64* __restrict__ st, long long k) {
-    const int t = blockIdx.x * 256 + threadIdx.x;
-    if (t < kSelBins) hist[t] = 0;
+// map blockIdx.y: n values (ns[map * nstride] when ns is given, k = n / 2
+// then; an empty map yields empty_value)
+__global__ void k_sel64_init(unsigned* __restrict__ hist, Sel64* __restrict__ st, long long n,
+                             long long k, const int* __restrict__ ns, int nstride,
+                             double empty_value, double* __restrict__ out) {
+    const int b = blockIdx.y, t = blockIdx.x * 256 + threadIdx.x;
+    if (t < kSelBins) hist[(size_t)b * kSelBins + t] = 0;
     if (t == 0) {
-        st->prefix = 0;
-        st->k = k;
-        st->done = 0;
+        const long long nb = ns ? (long long)ns[(size_t)b * nstride] : n;
+        st[b].prefix = 0;
+        st[b].n = nb;
+        st[b].k = ns ? nb / 2 : k;
+        st[b].done = 0;
+        if (nb == 0) out[b] = empty_value;
     }
 }
 
-__global__ void __launch_bounds__(256) k_sel64_pass(const double* __restrict__ x, long long stride,
-                                                    long long n, int pass, Sel64* __restrict__ st,
-                                                    unsigned* __restrict__ hist,
+__global__ void __launch_bounds__(256) k_sel64_pass(const double* __restrict__ x, long long mstride,
+                                                    long long stride, int pass,
+                                                    Sel64* __restrict__ sts,
+                                                    unsigned* __restrict__ hists,
                                                     double* __restrict__ out) {
     __shared__ unsigned h[kSelBins];
     __shared__ long long s[256];
     __shared__ int last;
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, mb = blockIdx.y;
+    Sel64* st = sts + mb;
+    unsigned* hist = hists + (size_t)mb * kSelBins;
+    const long long n = st->n;
+    if (n == 0) return;
+    x += mb * mstride;
     int shift, width;
     sel_digit(pass, shift, width);
     const unsigned long long hi = pass ? ~0ULL << (shift + width) : 0ULL;
@@ -106,9 +119,31 @@ __global__ void __launch_bounds__(256) k_sel64_pass(const double* __restrict__ x
     const unsigned dmask = (1u << width) - 1u;
     for (int b = t; b < kSelBins; b += 256) h[b] = 0;
     __syncthreads();
-    for (long long i = blockIdx.x * 256LL + t; i < n; i += (long long)gridDim.x * 256) {
-        const unsigned long long key = key_of(x[i * stride]);
-        if ((key & hi) == want) atomicAdd(&h[(key >> shift) & dmask], 1u);
+    // wave-uniform loop; lanes sharing a bin add once (up to four bins per
+    // wave and step: the early digits put most keys in a handful of bins),
+    // the rest add one by one
+    const int lane = t & 63;
+    for (long long i0 = blockIdx.x * 256LL; i0 < n; i0 += (long long)gridDim.x * 256) {
+        const long long i = i0 + t;
+        unsigned bin = 0;
+        bool pend = false;
+        if (i < n) {
+            const unsigned long long key = key_of(x[i * stride]);
+            pend = (key & hi) == want;
+            bin = (unsigned)(key >> shift) & dmask;
+        }
+#pragma unroll
+        for (int it = 0; it < 4; ++it) {
+            const unsigned long long act = __ballot(pend);
+            if (!act) break;
+            const int leader = __ffsll((long long)act) - 1;
+            const unsigned lb = __shfl(bin, leader, 64);
+            const bool eq = pend && bin == lb;
+            const unsigned long long same = __ballot(eq);
+            if (lane == leader) atomicAdd(&h[lb], (unsigned)__popcll(same));
+            if (eq) pend = false;
+        }
+        if (pend) atomicAdd(&h[bin], 1u);
     }
     __syncthreads();
     for (int b = t; b < kSelBins; b += 256)
@@ -146,7 +181,7 @@ __global__ void __launch_bounds__(256) k_sel64_pass(const double* __restrict__ x
                     st->prefix | ((unsigned long long)(t * kSelPer + b) << shift);
                 st->prefix = pre;
                 st->k = k - run;
-                if (pass == kSelPasses - 1 && out) *out = value_of(pre);
+                if (pass == kSelPasses - 1) out[mb] = value_of(pre);
                 break;
             }
             run += c[b];
@@ -292,67 +327,84 @@ __global__ void k_bg_grow(const uint8_t* __restrict__ in, int B, int W, int H,
 
 // ------------------------------------------------------------ grid mesh
 struct Grid {
-    const double* val;      // disparity (main) or background disparity (fill)
+    const double* val;      // disparity (main) or background disparity (fill), map-major
     const uint8_t* region;  // fill: the background mask; main: null
     const int* src;         // fill: source pixel of each background value
+    const double* min_disp; // per map
     int W, H, step, nx, ny, nx0, ny0, is_fill;
-    double min_disp;
     Cam cam;
 };
 
-__device__ __forceinline__ bool grid_vertex(const Grid& g, int i, int j, double& d, double& Z,
-                                            int& x, int& y, int& p) {
+// where map b's vertices and triangles go: its own region of vcap vertices
+// and tcap triangles, after base[b * 4 + {0, 1}] (the main build's counts)
+// when building the fill
+struct Out {
+    double* pos;
+    double* uv;
+    uint8_t* flag;
+    double* dv;
+    int* idx;
+    long long vcap, tcap;
+};
+
+__device__ __forceinline__ bool grid_vertex(const Grid& g, int b, int i, int j, double& d,
+                                            double& Z, int& x, int& y, int& p) {
     x = i < g.nx0 ? i * g.step : g.W - 1;
     y = j < g.ny0 ? j * g.step : g.H - 1;
     p = y * g.W + x;
-    if (g.region && !g.region[p]) return false;
-    d = g.val[p];
-    if (!(d >= g.min_disp)) return false;
+    const long long q = (long long)b * g.W * g.H + p;
+    if (g.region && !g.region[q]) return false;
+    d = g.val[q];
+    if (!(d >= g.min_disp[b])) return false;
     if (!g.is_fill && (double)x - d < 0.0) return false;
     Z = depth_of(g.cam, d);
     return !(Z <= 0.0);
 }
 
-// grid_mesh's build(), :5806-5829: one wave per grid row.  Count pass
-// (WRITE = false) -> cnt[row]; write pass -> vid table and vertex arrays at
-// offs[row] (the scan of the counts, plus the vertices built before)
+// grid_mesh's build(), :5806-5829: one wave per (grid row, map).  Count pass
+// (WRITE = false) -> cnt[map][row]; write pass -> the map's vid table and
+// vertex arrays at offs[map][row] (the scan of the counts, plus the
+// vertices built before)
 template <bool WRITE>
-__global__ void __launch_bounds__(64) k_grid_vert(Grid g, int* __restrict__ cnt,
+__global__ void __launch_bounds__(64) k_grid_vert(Grid g, Out o, int* __restrict__ cnt,
                                                   const int* __restrict__ offs,
-                                                  int* __restrict__ vid, double* __restrict__ pos,
-                                                  double* __restrict__ uv, uint8_t* __restrict__ flag,
-                                                  double* __restrict__ dv) {
-    const int j = blockIdx.x, l = threadIdx.x;
-    int run = WRITE ? offs[j] : 0;
+                                                  int* __restrict__ vid) {
+    const int j = blockIdx.x, b = blockIdx.y, l = threadIdx.x;
+    const size_t row = (size_t)b * g.ny + j;
+    int run = WRITE ? offs[row] : 0;
+    int* vt = vid + (size_t)b * g.nx * g.ny;
+    const size_t vb = (size_t)b * o.vcap;
     for (int i0 = 0; i0 < g.nx; i0 += 64) {
         const int i = i0 + l;
         double d = 0.0, Z = 0.0;
         int x = 0, y = 0, p = 0;
-        const bool v = i < g.nx && grid_vertex(g, i, j, d, Z, x, y, p);
+        const bool v = i < g.nx && grid_vertex(g, b, i, j, d, Z, x, y, p);
         const unsigned long long bal = __ballot(v);
         if (WRITE && i < g.nx) {
             if (v) {
                 const int id = run + __popcll(bal & ((1ULL << l) - 1ULL));
-                vid[j * g.nx + i] = id;
-                pos[3 * (size_t)id] = ((double)x - g.cam.cx) * Z / g.cam.fx;
-                pos[3 * (size_t)id + 1] = ((double)y - g.cam.cy) * Z / g.cam.fx;
-                pos[3 * (size_t)id + 2] = Z;
+                vt[j * g.nx + i] = id;
+                const size_t a = vb + id;
+                o.pos[3 * a] = ((double)x - g.cam.cx) * Z / g.cam.fx;
+                o.pos[3 * a + 1] = ((double)y - g.cam.cy) * Z / g.cam.fx;
+                o.pos[3 * a + 2] = Z;
                 int ux = x, uy = y;
                 if (g.is_fill) {
-                    ux = g.src[p] % g.W;
-                    uy = g.src[p] / g.W;
+                    const int sp = g.src[(size_t)b * g.W * g.H + p];
+                    ux = sp % g.W;
+                    uy = sp / g.W;
                 }
-                uv[2 * (size_t)id] = ((double)ux + 0.5) / (double)g.W;
-                uv[2 * (size_t)id + 1] = ((double)uy + 0.5) / (double)g.H;
-                flag[id] = (uint8_t)g.is_fill;
-                dv[id] = d;
+                o.uv[2 * a] = ((double)ux + 0.5) / (double)g.W;
+                o.uv[2 * a + 1] = ((double)uy + 0.5) / (double)g.H;
+                o.flag[a] = (uint8_t)g.is_fill;
+                o.dv[a] = d;
             } else {
-                vid[j * g.nx + i] = -1;
+                vt[j * g.nx + i] = -1;
             }
         }
         run += __popcll(bal);
     }
-    if (!WRITE && l == 0) cnt[j] = run;
+    if (!WRITE && l == 0) cnt[row] = run;
 }
 
 __device__ __forceinline__ double fmax_(double a, double b) { return a > b ? a : b; }
@@ -367,24 +419,27 @@ __device__ __forceinline__ bool tri_ok(int a, int b, int c, const double* dv, do
     return hi - lo <= fmax_(jump, jump_rel * hi);
 }
 
-// the cells of grid row j (one wave): (a, c, b) then (b, c, e) when they pass
+// the cells of grid row j of map b (one wave): (a, c, b) then (b, c, e)
+// when they pass
 template <bool WRITE>
-__global__ void __launch_bounds__(64) k_grid_tri(const int* __restrict__ vid, int nx,
-                                                 const double* __restrict__ dv, double jump,
-                                                 double jump_rel, int* __restrict__ cnt,
-                                                 const int* __restrict__ offs,
-                                                 int* __restrict__ idx) {
-    const int j = blockIdx.x, l = threadIdx.x;
-    int run = WRITE ? offs[j] : 0;
+__global__ void __launch_bounds__(64) k_grid_tri(const int* __restrict__ vid, int nx, int ny,
+                                                 Out o, double jump, double jump_rel,
+                                                 int* __restrict__ cnt, const int* __restrict__ offs) {
+    const int j = blockIdx.x, mb = blockIdx.y, l = threadIdx.x;
+    const size_t row = (size_t)mb * (ny - 1) + j;
+    int run = WRITE ? offs[row] : 0;
+    const int* vt = vid + (size_t)mb * nx * ny;
+    const double* dv = o.dv + (size_t)mb * o.vcap;
+    int* idx = o.idx + 3 * (size_t)mb * o.tcap;
     const unsigned long long lt = (1ULL << l) - 1ULL;
     for (int i0 = 0; i0 < nx - 1; i0 += 64) {
         const int i = i0 + l;
         int a = -1, b = -1, c = -1, e = -1;
         if (i < nx - 1) {
-            a = vid[j * nx + i];
-            b = vid[j * nx + i + 1];
-            c = vid[(j + 1) * nx + i];
-            e = vid[(j + 1) * nx + i + 1];
+            a = vt[j * nx + i];
+            b = vt[j * nx + i + 1];
+            c = vt[(j + 1) * nx + i];
+            e = vt[(j + 1) * nx + i + 1];
         }
         const bool o1 = tri_ok(a, b, c, dv, jump, jump_rel);
         const bool o2 = tri_ok(b, c, e, dv, jump, jump_rel);
@@ -405,16 +460,18 @@ __global__ void __launch_bounds__(64) k_grid_tri(const int* __restrict__ vid, in
         }
         run += __popcll(b1) + __popcll(b2);
     }
-    if (!WRITE && l == 0) cnt[j] = run;
+    if (!WRITE && l == 0) cnt[row] = run;
 }
 
-// exclusive scan of n row counts (one block), offset by *base when given;
-// *total = the sum
-__global__ void __launch_bounds__(1024) k_scan_rows(const int* __restrict__ cnt, int n,
+// exclusive scan of map blockIdx.x's n row counts (one block per map),
+// offset by base[map * 4] when given; total[map * 4] = the sum
+__global__ void __launch_bounds__(1024) k_scan_rows(const int* __restrict__ cnts, int n,
                                                     const int* __restrict__ base,
-                                                    int* __restrict__ offs, int* __restrict__ total) {
+                                                    int* __restrict__ offss, int* __restrict__ total) {
     __shared__ int s[1024];
-    const int t = threadIdx.x, seg = (n + 1023) / 1024;
+    const int t = threadIdx.x, mb = blockIdx.x, seg = (n + 1023) / 1024;
+    const int* cnt = cnts + (size_t)mb * n;
+    int* offs = offss + (size_t)mb * n;
     const int lo = t * seg < n ? t * seg : n, hi = lo + seg < n ? lo + seg : n;
     int sum = 0;
     for (int i = lo; i < hi; ++i) sum += cnt[i];
@@ -426,12 +483,12 @@ __global__ void __launch_bounds__(1024) k_scan_rows(const int* __restrict__ cnt,
         s[t] += v;
         __syncthreads();
     }
-    int run = (t ? s[t - 1] : 0) + (base ? *base : 0);
+    int run = (t ? s[t - 1] : 0) + (base ? base[(size_t)mb * 4] : 0);
     for (int i = lo; i < hi; ++i) {
         offs[i] = run;
         run += cnt[i];
     }
-    if (t == 1023) *total = s[1023];
+    if (t == 1023) total[(size_t)mb * 4] = s[1023];
 }
 
 unsigned blocks(long long n) { return (unsigned)((n + 255) / 256); }
@@ -472,29 +529,42 @@ int sv_disparity_to_points(const double* disp, int B, int W, int H, const double
     return SV_OK;
 }
 
-int sv_kth_f64_workspace_bytes(size_t* bytes) {
-    SV_REQUIRE(bytes, "sv_kth_f64_workspace_bytes: bad arguments");
-    *bytes = kSelBins * sizeof(unsigned) + 256;
+int sv_kth_f64_workspace_bytes(int B, size_t* bytes) {
+    SV_REQUIRE(bytes && B > 0, "sv_kth_f64_workspace_bytes: bad arguments");
+    *bytes = (size_t)B * (kSelBins * sizeof(unsigned) + 64);
+    return SV_OK;
+}
+
+int sv_kth_f64_batch(const double* x, long long mstride, long long stride, int B, long long n,
+                     long long k, const int* ns, int nstride, double empty_value, double* out,
+                     void* ws, size_t ws_bytes, void* stream) {
+    SV_REQUIRE(x && out && ws && stride > 0 && B > 0 && B <= 65535 && n > 0,
+               "sv_kth_f64_batch: bad arguments");
+    SV_REQUIRE(ns || (k >= 0 && k < n), "sv_kth_f64_batch: k must be in [0, n)");
+    static_assert(sizeof(Sel64) <= 64, "selection state");
+    SV_REQUIRE(ws_bytes >= (size_t)B * (kSelBins * sizeof(unsigned) + 64),
+               "sv_kth_f64_batch: workspace too small");
+    unsigned* hist = (unsigned*)ws;
+    Sel64* st = (Sel64*)((char*)ws + (size_t)B * kSelBins * sizeof(unsigned));
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_sel64_init, dim3(kSelBins / 256, B), dim3(256), 0, s, hist, st, n, k, ns,
+                       nstride, empty_value, out);
+    SV_CHECK_LAUNCH();
+    // about 32 keys per thread: each block pays a fixed cost (histogram
+    // clear and flush, the completion counter) that dominates small blocks
+    const long long nb = (n + 8191) / 8192;
+    const unsigned g = (unsigned)(nb < 256 ? nb : 256);
+    for (int pass = 0; pass < kSelPasses; ++pass) {
+        hipLaunchKernelGGL(k_sel64_pass, dim3(g, B), dim3(256), 0, s, x, mstride, stride, pass, st,
+                           hist, out);
+        SV_CHECK_LAUNCH();
+    }
     return SV_OK;
 }
 
 int sv_kth_f64(const double* x, long long stride, long long n, long long k, double* out, void* ws,
                size_t ws_bytes, void* stream) {
-    SV_REQUIRE(x && out && ws && stride > 0 && n > 0 && k >= 0 && k < n,
-               "sv_kth_f64: bad arguments (k must be in [0, n))");
-    SV_REQUIRE(ws_bytes >= kSelBins * sizeof(unsigned) + 256, "sv_kth_f64: workspace too small");
-    unsigned* hist = (unsigned*)ws;
-    Sel64* st = (Sel64*)((char*)ws + kSelBins * sizeof(unsigned));
-    hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_sel64_init, dim3(kSelBins / 256), dim3(256), 0, s, hist, st, k);
-    SV_CHECK_LAUNCH();
-    const long long nb = (n + 1023) / 1024;  // about four keys per thread
-    const unsigned g = (unsigned)(nb < 512 ? nb : 512);
-    for (int pass = 0; pass < kSelPasses; ++pass) {
-        hipLaunchKernelGGL(k_sel64_pass, dim3(g), dim3(256), 0, s, x, stride, n, pass, st, hist, out);
-        SV_CHECK_LAUNCH();
-    }
-    return SV_OK;
+    return sv_kth_f64_batch(x, 0, stride, 1, n, k, nullptr, 0, 0.0, out, ws, ws_bytes, stream);
 }
 
 int sv_min_filter_arg(const double* v, int B, int W, int H, int r, int rv, double* out, int* idx,
@@ -559,60 +629,70 @@ int sv_background_region(const double* disp, int B, int W, int H, int radius, do
     return SV_OK;
 }
 
-int sv_grid_mesh_workspace_bytes(int W, int H, int step, size_t* bytes) {
-    SV_REQUIRE(bytes && W > 0 && H > 0 && step > 0, "sv_grid_mesh_workspace_bytes: bad arguments");
+int sv_grid_mesh_workspace_bytes(int B, int W, int H, int step, size_t* bytes) {
+    SV_REQUIRE(bytes && B > 0 && W > 0 && H > 0 && step > 0,
+               "sv_grid_mesh_workspace_bytes: bad arguments");
     const long long nx = (W + step - 1) / step + ((W - 1) % step ? 1 : 0);
     const long long ny = (H + step - 1) / step + ((H - 1) % step ? 1 : 0);
-    *bytes = (size_t)(nx * ny + 2 * ny + 8) * sizeof(int);
+    *bytes = (size_t)B * (nx * ny + 2 * ny + 8) * sizeof(int);
     return SV_OK;
 }
 
-// One build of grid_mesh (main: val = disparity, region/src null; fill:
-// val = background disparity, region = mask, src = source pixels).  Vertices
-// are appended after *vbase (device int, or null for 0), triangles after
-// *tbase; totals[0] / totals[1] receive this build's vertex and triangle
-// counts.  Output arrays must hold the worst case: nx*ny vertices and
-// 2*(nx-1)*(ny-1) triangles per build.
-int sv_grid_mesh_build(const double* val, const uint8_t* region, const int* src, int W, int H,
-                       int step, const double* cam, double min_disp, double jump, double jump_rel,
-                       const int* vbase, const int* tbase, double* pos, double* uv, uint8_t* flag,
-                       double* dv, int* idx, int* totals, void* ws, size_t ws_bytes, void* stream) {
-    SV_REQUIRE(val && cam && pos && uv && flag && dv && idx && totals && ws && W > 0 && H > 0 &&
-                   step > 0,
+// One build of grid_mesh for B maps (main: val = disparity, region/src
+// null; fill: val = background disparity, region = mask, src = source
+// pixels).  Map b's vertices go to [b * vcap, ...) of pos/uv/flag/dv and its
+// triangles to [b * tcap, ...) of idx, vertex ids counted within the map.
+// totals is [B][4] on the device: (main vertices, main triangles, fill
+// vertices, fill triangles); the main build writes the first two, the fill
+// build appends after them and writes the last two.
+int sv_grid_mesh_build(const double* val, const uint8_t* region, const int* src, int B, int W,
+                       int H, int step, const double* cam, const double* min_disp, double jump,
+                       double jump_rel, double* pos, double* uv, uint8_t* flag, double* dv, int* idx,
+                       long long vcap, long long tcap, int* totals, void* ws, size_t ws_bytes,
+                       void* stream) {
+    SV_REQUIRE(val && cam && min_disp && pos && uv && flag && dv && idx && totals && ws && B > 0 &&
+                   B <= 65535 && W > 0 && H > 0 && step > 0,
                "sv_grid_mesh_build: bad arguments");
     SV_REQUIRE((region == nullptr) == (src == nullptr), "sv_grid_mesh_build: region and src go together");
     const int nx0 = (W + step - 1) / step, ny0 = (H + step - 1) / step;
     const int nx = nx0 + ((W - 1) % step ? 1 : 0), ny = ny0 + ((H - 1) % step ? 1 : 0);
+    const bool fill = region != nullptr;
+    SV_REQUIRE(vcap >= (long long)nx * ny * (fill ? 2 : 1) &&
+                   tcap >= 2LL * (nx - 1) * (ny - 1) * (fill ? 2 : 1),
+               "sv_grid_mesh_build: output capacity below the worst case");
     size_t need = 0;
-    sv_grid_mesh_workspace_bytes(W, H, step, &need);
+    sv_grid_mesh_workspace_bytes(B, W, H, step, &need);
     SV_REQUIRE(ws_bytes >= need, "sv_grid_mesh_build: workspace too small");
     SV_REQUIRE(ny <= 1 << 20, "sv_grid_mesh_build: too many grid rows");
     int* vid = (int*)ws;
-    int* cnt = vid + (size_t)nx * ny;
-    int* offs = cnt + ny;
-    Grid g{val, region, src, W, H, step, nx, ny, nx0, ny0, region ? 1 : 0, min_disp,
+    int* cnt = vid + (size_t)B * nx * ny;
+    int* offs = cnt + (size_t)B * ny;
+    Grid g{val, region, src, min_disp, W, H, step, nx, ny, nx0, ny0, fill ? 1 : 0,
            Cam{cam[0], cam[1], cam[2], cam[3], cam[4]}};
+    Out o{pos, uv, flag, dv, idx, vcap, tcap};
+    const int* vbase = fill ? totals : nullptr;
+    const int* tbase = fill ? totals + 1 : nullptr;
+    int* vtot = totals + (fill ? 2 : 0);
+    int* ttot = totals + (fill ? 3 : 1);
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_grid_vert<false>, dim3(ny), dim3(64), 0, s, g, cnt, offs, vid, pos, uv, flag,
-                       dv);
+    hipLaunchKernelGGL(k_grid_vert<false>, dim3(ny, B), dim3(64), 0, s, g, o, cnt, offs, vid);
     SV_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_scan_rows, dim3(1), dim3(1024), 0, s, cnt, ny, vbase, offs, totals);
+    hipLaunchKernelGGL(k_scan_rows, dim3(B), dim3(1024), 0, s, cnt, ny, vbase, offs, vtot);
     SV_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_grid_vert<true>, dim3(ny), dim3(64), 0, s, g, cnt, offs, vid, pos, uv, flag,
-                       dv);
+    hipLaunchKernelGGL(k_grid_vert<true>, dim3(ny, B), dim3(64), 0, s, g, o, cnt, offs, vid);
     SV_CHECK_LAUNCH();
     if (ny < 2 || nx < 2) {  // no cells: zero triangles
-        hipLaunchKernelGGL(k_scan_rows, dim3(1), dim3(1024), 0, s, cnt, 0, tbase, offs, totals + 1);
+        hipLaunchKernelGGL(k_scan_rows, dim3(B), dim3(1024), 0, s, cnt, 0, tbase, offs, ttot);
         SV_CHECK_LAUNCH();
         return SV_OK;
     }
-    hipLaunchKernelGGL(k_grid_tri<false>, dim3(ny - 1), dim3(64), 0, s, vid, nx, dv, jump, jump_rel,
-                       cnt, offs, idx);
+    hipLaunchKernelGGL(k_grid_tri<false>, dim3(ny - 1, B), dim3(64), 0, s, vid, nx, ny, o, jump,
+                       jump_rel, cnt, offs);
     SV_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_scan_rows, dim3(1), dim3(1024), 0, s, cnt, ny - 1, tbase, offs, totals + 1);
+    hipLaunchKernelGGL(k_scan_rows, dim3(B), dim3(1024), 0, s, cnt, ny - 1, tbase, offs, ttot);
     SV_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_grid_tri<true>, dim3(ny - 1), dim3(64), 0, s, vid, nx, dv, jump, jump_rel,
-                       cnt, offs, idx);
+    hipLaunchKernelGGL(k_grid_tri<true>, dim3(ny - 1, B), dim3(64), 0, s, vid, nx, ny, o, jump,
+                       jump_rel, cnt, offs);
     SV_CHECK_LAUNCH();
     return SV_OK;
 }

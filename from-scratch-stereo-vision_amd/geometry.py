@@ -24,14 +24,15 @@ _P, _I, _D, _LL, _SZ = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_
     ctypes.c_size_t
 L.extra_signatures({
     "sv_disparity_to_points": (_I, [_P, _I, _I, _I, _P, _D, _D, _P, _P, _P]),
-    "sv_kth_f64_workspace_bytes": (_I, [ctypes.POINTER(_SZ)]),
+    "sv_kth_f64_workspace_bytes": (_I, [_I, ctypes.POINTER(_SZ)]),
     "sv_kth_f64": (_I, [_P, _LL, _LL, _LL, _P, _P, _SZ, _P]),
+    "sv_kth_f64_batch": (_I, [_P, _LL, _LL, _I, _LL, _LL, _P, _I, _D, _P, _P, _SZ, _P]),
     "sv_min_filter_arg": (_I, [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P]),
     "sv_smooth_background": (_I, [_P, _P, _I, _I, _I, _I, _P, _P, _P]),
     "sv_background_region": (_I, [_P, _I, _I, _I, _I, _D, _D, _P, _P, _P, _P, _P, _P]),
-    "sv_grid_mesh_workspace_bytes": (_I, [_I, _I, _I, ctypes.POINTER(_SZ)]),
-    "sv_grid_mesh_build": (_I, [_P, _P, _P, _I, _I, _I, _P, _D, _D, _D, _P, _P, _P, _P, _P, _P,
-                                _P, _P, _P, _SZ, _P]),
+    "sv_grid_mesh_workspace_bytes": (_I, [_I, _I, _I, _I, ctypes.POINTER(_SZ)]),
+    "sv_grid_mesh_build": (_I, [_P, _P, _P, _I, _I, _I, _I, _P, _P, _D, _D, _P, _P, _P, _P, _P,
+                                _LL, _LL, _P, _P, _SZ, _P]),
 })
 
 
@@ -134,7 +135,7 @@ def kth_smallest(x, k, stride=1, n=None):
     if not 0 <= k < n:
         raise ValueError("k = %d outside [0, %d)" % (k, n))
     nb = ctypes.c_size_t(0)
-    L.check(L.lib().sv_kth_f64_workspace_bytes(ctypes.byref(nb)))
+    L.check(L.lib().sv_kth_f64_workspace_bytes(1, ctypes.byref(nb)))
     ws = L.workspace.get(nb.value)
     out = torch.empty((), dtype=torch.float64, device=x.device)
     L.check(L.lib().sv_kth_f64(L.ptr(x), stride, n, k, L.ptr(out), L.ptr(ws), nb.value,
@@ -248,6 +249,71 @@ def _grid_dims(W, H, step):
     return nx, ny
 
 
+def _kth_batch(x, mstride, stride, B, n, k=0, ns=None, nstride=0, empty=0.0):
+    nb = ctypes.c_size_t(0)
+    L.check(L.lib().sv_kth_f64_workspace_bytes(B, ctypes.byref(nb)))
+    ws = L.workspace.get(nb.value)
+    out = torch.empty(B, dtype=torch.float64, device=x.device)
+    L.check(L.lib().sv_kth_f64_batch(L.ptr(x), mstride, stride, B, n, k, L.ptr(ns), nstride,
+                                     float(empty), L.ptr(out), L.ptr(ws), nb.value, L.stream()))
+    return out
+
+
+def grid_mesh_batch(disp, W, H, cam, step=2, fill=True, jump=1.5, jump_rel=0.06,
+                    min_disp=None):
+    """grid_mesh for B maps of one size ([B][H][W] or B*W*H values) with one
+    launch per stage for all maps.  Returns a list of B dicts as
+    grid_mesh_gpu's (tensor views into shared buffers).  The counts and
+    pivots are read back once for the whole batch."""
+    d = _dev(disp)
+    B = _count(d, W, H)
+    if step < 1:
+        raise ValueError("step must be >= 1")
+    dev = d.device
+    npix = W * H
+    if min_disp is None:  # robust_min_disparity per map: max(0.4, median * 0.38)
+        med = _kth_batch(d, npix, 1, B, npix, npix // 2)
+        m = med * 0.38
+        mind = torch.where(torch.full_like(m, 0.4) > m, torch.full_like(m, 0.4), m)
+    else:
+        mind = torch.full((B,), float(min_disp), dtype=torch.float64, device=dev)
+    nx, ny = _grid_dims(W, H, step)
+    builds = 2 if fill else 1
+    vcap, tcap = builds * nx * ny, max(builds * 2 * (nx - 1) * (ny - 1), 1)
+    pos = torch.empty((B, vcap, 3), dtype=torch.float64, device=dev)
+    uv = torch.empty((B, vcap, 2), dtype=torch.float64, device=dev)
+    flag = torch.empty((B, vcap), dtype=torch.uint8, device=dev)
+    dv = torch.empty((B, vcap), dtype=torch.float64, device=dev)
+    idx = torch.empty((B, tcap, 3), dtype=torch.int32, device=dev)
+    totals = torch.zeros((B, 4), dtype=torch.int32, device=dev)
+    nb = ctypes.c_size_t(0)
+    L.check(L.lib().sv_grid_mesh_workspace_bytes(B, W, H, step, ctypes.byref(nb)))
+    ws = L.workspace.get(nb.value)
+    camh = cam._host()
+    lib = L.lib()
+    outs = (L.ptr(pos), L.ptr(uv), L.ptr(flag), L.ptr(dv), L.ptr(idx), vcap, tcap, L.ptr(totals))
+    L.check(lib.sv_grid_mesh_build(L.ptr(d), None, None, B, W, H, step, camh, L.ptr(mind),
+                                   float(jump), float(jump_rel), *outs, L.ptr(ws), nb.value,
+                                   L.stream()))
+    if fill:
+        mask, m, src = background_region_gpu(d, W, H)
+        L.check(lib.sv_grid_mesh_build(L.ptr(m), L.ptr(mask), L.ptr(src), B, W, H, step, camh,
+                                       L.ptr(mind), float(jump), float(jump_rel), *outs, L.ptr(ws),
+                                       nb.value, L.stream()))
+    # pivot: the median depth of each map's main vertices (1.0 when none)
+    piv = _kth_batch(pos.reshape(-1)[2:], 3 * vcap, 3, B, vcap, ns=totals, nstride=4, empty=1.0)
+    tot = totals.cpu().tolist()
+    pivots = piv.cpu().tolist()
+    res = []
+    for b in range(B):
+        mv, mt, fv, ft = tot[b]
+        n_v, n_t = mv + fv, mt + ft
+        res.append({"pos": pos[b, :n_v], "uv": uv[b, :n_v], "fill": flag[b, :n_v],
+                    "disp": dv[b, :n_v], "idx": idx[b, :n_t], "main_tris": mt, "fill_tris": ft,
+                    "pivot": pivots[b], "fx": cam.fx, "cx": cam.cx, "cy": cam.cy, "W": W, "H": H})
+    return res
+
+
 def grid_mesh_gpu(disp, W, H, cam, step=2, fill=True, jump=1.5, jump_rel=0.06, min_disp=None):
     """grid_mesh with every array left on the GPU: a dict with pos [n][3],
     uv [n][2], fill [n] uint8, disp [n], idx [t][3] int32 tensors, the
@@ -256,43 +322,7 @@ def grid_mesh_gpu(disp, W, H, cam, step=2, fill=True, jump=1.5, jump_rel=0.06, m
     if d.numel() != W * H:
         raise ValueError("grid_mesh takes one W x H = %d x %d map, not %d values"
                          % (W, H, d.numel()))
-    if step < 1:
-        raise ValueError("step must be >= 1")
-    dev = d.device
-    if min_disp is None:
-        min_disp = robust_min_disparity(d)
-    nx, ny = _grid_dims(W, H, step)
-    builds = 2 if fill else 1
-    nv, nt = builds * nx * ny, builds * 2 * max(nx - 1, 0) * max(ny - 1, 0)
-    pos = torch.empty((max(nv, 1), 3), dtype=torch.float64, device=dev)
-    uv = torch.empty((max(nv, 1), 2), dtype=torch.float64, device=dev)
-    flag = torch.empty(max(nv, 1), dtype=torch.uint8, device=dev)
-    dv = torch.empty(max(nv, 1), dtype=torch.float64, device=dev)
-    idx = torch.empty((max(nt, 1), 3), dtype=torch.int32, device=dev)
-    totals = torch.zeros(4, dtype=torch.int32, device=dev)  # main v, main t, fill v, fill t
-    nb = ctypes.c_size_t(0)
-    L.check(L.lib().sv_grid_mesh_workspace_bytes(W, H, step, ctypes.byref(nb)))
-    ws = L.workspace.get(nb.value)
-    camh = cam._host()
-    lib = L.lib()
-    L.check(lib.sv_grid_mesh_build(L.ptr(d), None, None, W, H, step, camh, float(min_disp),
-                                   float(jump), float(jump_rel), None, None, L.ptr(pos), L.ptr(uv),
-                                   L.ptr(flag), L.ptr(dv), L.ptr(idx), L.ptr(totals), L.ptr(ws),
-                                   nb.value, L.stream()))
-    if fill:
-        mask, m, src = background_region_gpu(d, W, H)
-        t2 = totals[2:]
-        L.check(lib.sv_grid_mesh_build(L.ptr(m), L.ptr(mask), L.ptr(src), W, H, step, camh,
-                                       float(min_disp), float(jump), float(jump_rel),
-                                       L.ptr(totals), L.ptr(totals[1:]), L.ptr(pos), L.ptr(uv),
-                                       L.ptr(flag), L.ptr(dv), L.ptr(idx), L.ptr(t2), L.ptr(ws),
-                                       nb.value, L.stream()))
-    mv, mt, fv, ft = (int(v) for v in totals.cpu().tolist())
-    n_v, n_t = mv + fv, mt + ft
-    pivot = float(kth_smallest(pos[:mv].reshape(-1)[2:], mv // 2, stride=3, n=mv)) if mv else 1.0
-    return {"pos": pos[:n_v], "uv": uv[:n_v], "fill": flag[:n_v], "disp": dv[:n_v],
-            "idx": idx[:n_t], "main_tris": mt, "fill_tris": ft, "pivot": pivot,
-            "fx": cam.fx, "cx": cam.cx, "cy": cam.cy, "W": W, "H": H}
+    return grid_mesh_batch(d, W, H, cam, step, fill, jump, jump_rel, min_disp)[0]
 
 
 def grid_mesh(disp, W, H, cam, step=2, fill=True, jump=1.5, jump_rel=0.06, min_disp=None):

@@ -372,7 +372,10 @@ int sv_abs_dev(const double* gt, const uint8_t* mask, const double* stats, int B
  * NaN) where d < min_disp or Z is not in (0, max_depth].
  * sv_kth_f64: *out (device) = the k-th smallest (0-based) of x[0],
  * x[stride], ..., x[(n-1)*stride]; robust_min_disparity's median and
- * grid_mesh's pivot.
+ * grid_mesh's pivot.  sv_kth_f64_batch: the same for B sets at x + b *
+ * mstride, out[b]; with ns given, set b holds ns[b * nstride] values (at
+ * most n) and k is half of that (empty sets give empty_value).  Workspace:
+ * sv_kth_f64_workspace_bytes(B).
  * sv_min_filter_arg: B maps; minimum over the clipped window (radius r
  * along rows, rv along columns), idx = flat in-image index of the first
  * strict minimum (rows scanned first); rv > 0 needs tmp (B*W*H doubles)
@@ -382,19 +385,25 @@ int sv_abs_dev(const double* gt, const uint8_t* mask, const double* stats, int B
  * sv_background_region: mask (uint8), m (smoothed background disparity),
  * src (int32 source pixel) for B maps; tmp (doubles) and tmp_mask (bytes)
  * of B*W*H each.
- * sv_grid_mesh_build: one build() of grid_mesh (main: region = src = NULL;
- * fill: the background mask and source pixels, val = m).  Vertices land
- * after *vbase and triangles after *tbase (device ints, NULL = 0), in the
- * reference's order; totals[0..1] (device) receive this build's vertex and
- * triangle counts.  pos[3n], uv[2n], flag[n], dv[n] and idx[3t] must hold
- * nx*ny vertices and 2*(nx-1)*(ny-1) triangles per build (nx, ny the grid
- * sizes); ws is sv_grid_mesh_workspace_bytes. */
+ * sv_grid_mesh_build: one build() of grid_mesh for B maps (main: region =
+ * src = NULL; fill: the background mask and source pixels, val = m), with
+ * min_disp[b] per map (device).  Map b's vertices go to [b*vcap, ...) of
+ * pos (3 per vertex), uv (2), flag, dv and its triangles to [b*tcap, ...)
+ * of idx (3 per triangle), vertex ids counted within the map, in the
+ * reference's order.  totals[B][4] (device) = main vertices, main
+ * triangles, fill vertices, fill triangles: the main build writes the
+ * first two, the fill build appends after them and writes the last two.
+ * vcap >= nx*ny per build and tcap >= 2*(nx-1)*(ny-1) per build (nx, ny the
+ * grid sizes); ws is sv_grid_mesh_workspace_bytes(B, W, H, step). */
 int sv_disparity_to_points(const double* disp, int B, int W, int H, const double* cam,
                            double min_disp, double max_depth, double* pts, uint8_t* valid,
                            void* stream);
-int sv_kth_f64_workspace_bytes(size_t* bytes);
+int sv_kth_f64_workspace_bytes(int B, size_t* bytes);
 int sv_kth_f64(const double* x, long long stride, long long n, long long k, double* out, void* ws,
                size_t ws_bytes, void* stream);
+int sv_kth_f64_batch(const double* x, long long mstride, long long stride, int B, long long n,
+                     long long k, const int* ns, int nstride, double empty_value, double* out,
+                     void* ws, size_t ws_bytes, void* stream);
 int sv_min_filter_arg(const double* v, int B, int W, int H, int r, int rv, double* out, int* idx,
                       double* tmp, int* tmp_idx, void* stream);
 int sv_smooth_background(const double* m, const double* disp, int B, int W, int H, int r,
@@ -402,11 +411,12 @@ int sv_smooth_background(const double* m, const double* disp, int B, int W, int 
 int sv_background_region(const double* disp, int B, int W, int H, int radius, double jump_ratio,
                          double min_jump, uint8_t* mask, double* m, int* src, double* tmp,
                          uint8_t* tmp_mask, void* stream);
-int sv_grid_mesh_workspace_bytes(int W, int H, int step, size_t* bytes);
-int sv_grid_mesh_build(const double* val, const uint8_t* region, const int* src, int W, int H,
-                       int step, const double* cam, double min_disp, double jump, double jump_rel,
-                       const int* vbase, const int* tbase, double* pos, double* uv, uint8_t* flag,
-                       double* dv, int* idx, int* totals, void* ws, size_t ws_bytes, void* stream);
+int sv_grid_mesh_workspace_bytes(int B, int W, int H, int step, size_t* bytes);
+int sv_grid_mesh_build(const double* val, const uint8_t* region, const int* src, int B, int W,
+                       int H, int step, const double* cam, const double* min_disp, double jump,
+                       double jump_rel, double* pos, double* uv, uint8_t* flag, double* dv, int* idx,
+                       long long vcap, long long tcap, int* totals, void* ws, size_t ws_bytes,
+                       void* stream);
 
 #ifdef __cplusplus
 }

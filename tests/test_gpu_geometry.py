@@ -146,3 +146,29 @@ def test_row_filters_both_paths(sv, W, H):
     m0, _ = oracle.min_filter_arg(v, W, H, 9, 0)
     s = sv.smooth_background(torch.from_numpy(m0).cuda(), t, W, H, 4)
     assert np.array_equal(s.cpu().numpy(), oracle.smooth_background(m0, v, W, H, 4))
+
+
+def test_batch_equals_single(sv):
+    """grid_mesh_batch over five maps equals five grid_mesh_gpu calls: the
+    maps differ in their vertex and triangle counts, one has no usable
+    pixel (every count 0, pivot 1.0)."""
+    W, H = 83, 61
+    rng = np.random.default_rng(12)
+    maps = []
+    for b in range(5):
+        p = sv.synth.band_pair(20 + b, W=W, H=H)
+        maps.append(np.asarray(p["disp"], np.float64) + rng.normal(0, 0.1, (H, W)))
+    maps[3][:] = 0.0  # nothing above the floor
+    D = torch.from_numpy(np.stack(maps)).cuda()
+    cam = sv.Camera.synthetic(W, H)
+    res = sv.geometry.grid_mesh_batch(D, W, H, cam, step=2, fill=True)
+    for b in range(5):
+        one = sv.geometry.grid_mesh_gpu(D[b], W, H, cam, step=2, fill=True)
+        ref = oracle.grid_mesh(maps[b].ravel(), W, H, (cam.fx, cam.cx, cam.cy, cam.baseline,
+                                                       cam.doffs), step=2, fill=True)
+        for f in ("pos", "uv", "fill", "disp", "idx"):
+            assert torch.equal(res[b][f], one[f]), (b, f)
+            assert np.array_equal(res[b][f].cpu().numpy(), ref[f]), (b, f)
+        assert (res[b]["main_tris"], res[b]["fill_tris"], res[b]["pivot"]) == \
+            (ref["main_tris"], ref["fill_tris"], ref["pivot"])
+    assert res[3]["pos"].shape[0] == 0 and res[3]["pivot"] == 1.0
