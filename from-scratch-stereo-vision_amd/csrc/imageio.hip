@@ -16,6 +16,7 @@
 // canonical codes decoded one bit at a time through per-length first-code
 // counts (the count/symbol form of the RFC's code construction).
 #include <algorithm>
+#include <new>
 #include <cstdint>
 #include <cstring>
 #include <string>
@@ -141,7 +142,7 @@ const int kDistExtra[30] = {0, 0, 0, 0, 1, 1, 2, 2, 3, 3, 4, 4, 5, 5, 6,
                             6, 7, 7, 8, 8, 9, 9, 10, 10, 11, 11, 12, 12, 13, 13};
 const int kClenOrder[19] = {16, 17, 18, 0, 8, 7, 9, 6, 10, 5, 11, 4, 12, 3, 13, 2, 14, 1, 15};
 
-bool codes(Bits& b, const Code& lit, const Code& dist, std::vector<uint8_t>& out,
+bool codes(Bits& b, const Code& lit, const Code& dist, std::vector<uint8_t>& out, size_t limit,
            std::string& err) {
     for (;;) {
         const int sym = decode(b, lit);
@@ -150,6 +151,10 @@ bool codes(Bits& b, const Code& lit, const Code& dist, std::vector<uint8_t>& out
             return false;
         }
         if (sym < 256) {
+            if (out.size() >= limit) {
+                err = "DEFLATE output far beyond the image size";
+                return false;
+            }
             out.push_back((uint8_t)sym);
         } else if (sym == 256) {
             if (b.bad()) {
@@ -175,6 +180,10 @@ bool codes(Bits& b, const Code& lit, const Code& dist, std::vector<uint8_t>& out
                 return false;
             }
             const size_t o = out.size();
+            if (o + length > limit) {
+                err = "DEFLATE output far beyond the image size";
+                return false;
+            }
             if (distance > o) {
                 err = "back-reference before start of output";
                 return false;
@@ -187,7 +196,10 @@ bool codes(Bits& b, const Code& lit, const Code& dist, std::vector<uint8_t>& out
     }
 }
 
-bool inflate(const uint8_t* d, size_t n, std::vector<uint8_t>& out, std::string& err) {
+// limit: the most output accepted (a bound on memory for corrupt or hostile
+// streams; a valid image needs far less)
+bool inflate(const uint8_t* d, size_t n, std::vector<uint8_t>& out, size_t limit,
+             std::string& err) {
     Bits b{d, n};
     int last = 0;
     while (!last) {
@@ -214,6 +226,10 @@ bool inflate(const uint8_t* d, size_t n, std::vector<uint8_t>& out, std::string&
                 err = "truncated stored block";
                 return false;
             }
+            if (out.size() + len > limit) {
+                err = "DEFLATE output far beyond the image size";
+                return false;
+            }
             out.insert(out.end(), d + b.pos, d + b.pos + len);
             b.pos += len;
         } else if (type == 1) {
@@ -228,7 +244,7 @@ bool inflate(const uint8_t* d, size_t n, std::vector<uint8_t>& out, std::string&
                 }
             };
             static const Fixed fixed;  // built once (thread-safe static initialisation)
-            if (!codes(b, fixed.lit, fixed.dist, out, err)) return false;
+            if (!codes(b, fixed.lit, fixed.dist, out, limit, err)) return false;
         } else if (type == 2) {
             const int hlit = b.get(5) + 257, hdist = b.get(5) + 1, hclen = b.get(4) + 4;
             int cl[19] = {0};
@@ -274,7 +290,7 @@ bool inflate(const uint8_t* d, size_t n, std::vector<uint8_t>& out, std::string&
                 err = "over-subscribed Huffman code";
                 return false;
             }
-            if (!codes(b, lit, dist, out, err)) return false;
+            if (!codes(b, lit, dist, out, limit, err)) return false;
         } else {
             err = "reserved DEFLATE block type 3";
             return false;
@@ -284,6 +300,7 @@ bool inflate(const uint8_t* d, size_t n, std::vector<uint8_t>& out, std::string&
 }
 
 // ------------------------------------------------------------- PNG / PNM
+constexpr long long kMaxPixels = 1LL << 28;  // 268 M pixels: 1.6 GB of samples
 struct Decoded {
     int W = 0, H = 0;
     // 0: value * (1 / maxval), 1: value / 255 (palette), 2: as 0 with gray =
@@ -341,6 +358,10 @@ bool png(const uint8_t* d, size_t n, Decoded& o, std::string& err, bool header_o
         err = "PNG without a valid IHDR";
         return false;
     }
+    if ((long long)o.W * o.H > kMaxPixels) {
+        err = "image too large";
+        return false;
+    }
     if (interlace) {
         err = "Adam7 interlaced PNG is not supported; re-save without interlacing";
         return false;
@@ -362,8 +383,9 @@ bool png(const uint8_t* d, size_t n, Decoded& o, std::string& err, bool header_o
         return false;
     }
     std::vector<uint8_t> raw;
-    raw.reserve(((size_t)o.W * channels * depth + 7) / 8 * o.H + o.H);
-    if (!inflate(idat.data() + 2, idat.size() - 2, raw, err)) return false;
+    const size_t need = ((size_t)o.W * channels * depth + 7) / 8 * o.H + o.H;
+    raw.reserve(need);
+    if (!inflate(idat.data() + 2, idat.size() - 2, raw, 4 * need + (1u << 20), err)) return false;
     const size_t stride = ((size_t)o.W * channels * depth + 7) / 8;
     const int bpp = std::max(1, channels * depth / 8);
     if (raw.size() < (stride + 1) * (size_t)o.H) {
@@ -477,7 +499,7 @@ bool pnm(const uint8_t* d, size_t n, Decoded& o, std::string& err, bool header_o
         } else {
             long v = 0;
             while (i < n && !ws(d[i])) {
-                if (d[i] < '0' || d[i] > '9') {
+                if (d[i] < '0' || d[i] > '9' || v > (1L << 31)) {
                     err = "bad Netpbm header";
                     return false;
                 }
@@ -489,6 +511,10 @@ bool pnm(const uint8_t* d, size_t n, Decoded& o, std::string& err, bool header_o
     }
     if (nt < 3 || tok[0] <= 0 || tok[1] <= 0 || tok[2] <= 0 || tok[2] > 65535) {
         err = "bad Netpbm header";
+        return false;
+    }
+    if (tok[0] > kMaxPixels || tok[1] > kMaxPixels || tok[0] * tok[1] > kMaxPixels) {
+        err = "image too large";
         return false;
     }
     o.W = (int)tok[0];
@@ -611,11 +637,19 @@ int sv_image_size(const uint8_t* data, size_t n, int* W, int* H) {
     return SV_OK;
 }
 
-int sv_image_decode(const uint8_t* data, size_t n, uint16_t* samples, int* mode, int* maxval) {
+int sv_image_decode(const uint8_t* data, size_t n, uint16_t* samples, size_t capacity, int* mode,
+                    int* maxval) {
     SV_REQUIRE(data && samples && mode && maxval, "sv_image_decode: bad arguments");
     Decoded o;
     std::string err;
-    if (!decode_any(data, n, o, err, false)) return sv::set_error(SV_EINVAL, "%s", err.c_str());
+    try {
+        if (!decode_any(data, n, o, err, false)) return sv::set_error(SV_EINVAL, "%s", err.c_str());
+    } catch (const std::bad_alloc&) {
+        return sv::set_error(SV_EINVAL, "image too large for host memory");
+    }
+    if (o.s.size() > capacity)
+        return sv::set_error(SV_EINVAL, "sv_image_decode: %zu samples do not fit in %zu", o.s.size(),
+                             capacity);
     std::memcpy(samples, o.s.data(), o.s.size() * sizeof(uint16_t));
     *mode = o.mode;
     *maxval = o.maxval;

@@ -18,7 +18,7 @@ from . import _lib as L
 _I, _P, _SZ, _LL = ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_longlong
 L.extra_signatures({
     "sv_image_size": (_I, [_P, _SZ, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
-    "sv_image_decode": (_I, [_P, _SZ, _P, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "sv_image_decode": (_I, [_P, _SZ, _P, _SZ, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "sv_image_expand": (_I, [_P, _LL, _I, _I, _P, _P, _P]),
 })
 
@@ -38,7 +38,7 @@ def _decode_into(buf, s, path):
     scale rule.  ctypes drops the GIL for the call, so threads overlap."""
     mode, maxval = ctypes.c_int(0), ctypes.c_int(0)
     lib = L.lib()
-    if lib.sv_image_decode(buf.ctypes.data_as(_P), buf.size, s.ctypes.data_as(_P),
+    if lib.sv_image_decode(buf.ctypes.data_as(_P), buf.size, s.ctypes.data_as(_P), s.size,
                            ctypes.byref(mode), ctypes.byref(maxval)):
         raise ValueError("%s: %s" % (path, lib.sv_last_error().decode(errors="replace")))
     return mode.value, maxval.value

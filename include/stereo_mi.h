@@ -324,16 +324,18 @@ int sv_widen(const float* x, const int* dims, long long xstride, double* y, long
  *
  * sv_image_size: width and height from the PNG IHDR or the Netpbm header
  * (PNG chosen by its 8-byte signature, as read_image does).
- * sv_image_decode: the whole file into samples[3*W*H] (host uint16, three
- * channel values per pixel), the scale rule in *mode (0: value / maxval
+ * sv_image_decode: the whole file into samples (host uint16, capacity
+ * values; 3*W*H are written, three channel values per pixel), the scale rule in *mode (0: value / maxval
  * as value * (1/maxval); 1: palette entry / 255; 2: as 0, one-channel
  * Netpbm whose gray is the sample itself) and *maxval.  Malformed files
- * return SV_EINVAL with the reference's message where it has one.
+ * return SV_EINVAL with the reference's message where it has one; so do
+ * images of more than 2^28 pixels.
  * sv_image_expand: on the GPU, samples (device) -> gray[npix] and
  * rgb[3*npix] float64 (either may be NULL), bit-identical to the lists
  * read_pnm / read_png return. */
 int sv_image_size(const uint8_t* data, size_t n, int* W, int* H);
-int sv_image_decode(const uint8_t* data, size_t n, uint16_t* samples, int* mode, int* maxval);
+int sv_image_decode(const uint8_t* data, size_t n, uint16_t* samples, size_t capacity, int* mode,
+                    int* maxval);
 int sv_image_expand(const uint16_t* samples, long long npix, int mode, int maxval, double* gray,
                     double* rgb, void* stream);
 

@@ -129,6 +129,7 @@ def test_errors(tmp_path):
         "huffman": (_png(hdr(), b"\x78\x01" + b"\xff" * 8), "DEFLATE"),
         "short": (_png(hdr(H=3), ok_raw), "too short"),
         "pnm": (b"P5\n4 1\n255\n\x00", "too short"),
+        "bomb": (_png(hdr(), zlib.compress(bytes(8 << 20), 9)), "far beyond"),
     }
     for name, (blob, msg) in cases.items():
         p = tmp_path / name
@@ -145,3 +146,21 @@ def test_errors(tmp_path):
         im.read_pnm(str(p))
     with pytest.raises(ValueError, match="not a PGM/PPM file"):
         im.read_image(str(p))
+
+
+def test_size_limits_and_capacity(tmp_path):
+    """Headers that claim more than 2^28 pixels are rejected before any
+    allocation; a sample buffer smaller than the image is refused."""
+    im = _images()
+    huge = _png(struct.pack(">IIBBBBB", 1 << 20, 1 << 20, 8, 2, 0, 0, 0), zlib.compress(b"\x00"))
+    with pytest.raises(ValueError, match="too large"):
+        im._decode_bytes(huge, "huge")
+    with pytest.raises(ValueError, match="too large"):
+        im._decode_bytes(b"P5\n100000 100000\n255\n", "huge")
+    with pytest.raises(ValueError, match="bad Netpbm header"):
+        im._decode_bytes(b"P5\n99999999999999999999 2\n255\n", "overflow")
+    f = make_images.make_pnm(np.random.default_rng(0), 4, 3, "P5", 255)
+    buf = np.frombuffer(f, dtype=np.uint8)
+    small = np.empty(3 * 4 * 3 - 1, dtype=np.uint16)
+    with pytest.raises(ValueError, match="do not fit"):
+        im._decode_into(buf, small, "small")
