@@ -448,13 +448,14 @@ __global__ void __launch_bounds__(64) k_ps_segment(const double* __restrict__ di
                                                    double min_inl, double* __restrict__ out,
                                                    int* __restrict__ err) {
     __shared__ StrayList sl;
-    // dynamic LDS, sized for the step: all members (ascending), the valid
-    // members then the inliers, and their disparities
+    // dynamic LDS, sized for the step: the valid members (ascending) then
+    // the inliers, and their disparities.  The full member list is not kept:
+    // the final snap re-scans the window (its writes are per pixel, in any
+    // order), which leaves room for more segments per CU
     extern __shared__ __align__(16) unsigned char ps_dyn[];
     const int maxm = (2 * g.step + 1) * (2 * g.step + 1) + SCAP;
     double* pd = reinterpret_cast<double*>(ps_dyn);
-    int* mem = reinterpret_cast<int*>(pd + maxm);
-    int* pts = mem + maxm;
+    int* pts = reinterpret_cast<int*>(pd + maxm);
     __shared__ int mi[64];
     __shared__ int mv[64];
     __shared__ int counts[2];
@@ -490,7 +491,6 @@ __global__ void __launch_bounds__(64) k_ps_segment(const double* __restrict__ di
         const uint64_t bm = __ballot(m), bv = __ballot(v);
         const int nm0 = counts[0], np0 = counts[1];
         if (sl.n == 0) {
-            if (m) mem[nm0 + __popcll(bm & below)] = p;
             if (v) pts[np0 + __popcll(bv & below)] = p;
             __builtin_amdgcn_wave_barrier();
             if (lane == 0) {
@@ -510,10 +510,10 @@ __global__ void __launch_bounds__(64) k_ps_segment(const double* __restrict__ di
                     const int q0 = mi[k];
                     while (sp < sl.n && sl.idx[sp] < q0) {
                         const int q = sl.idx[sp++];
-                        mem[nm++] = q;
+                        ++nm;
                         if (is_valid(vb, vkind, q)) pts[np++] = q;
                     }
-                    mem[nm++] = q0;
+                    ++nm;
                     if (mv[k]) pts[np++] = q0;
                 }
                 counts[0] = nm;
@@ -526,14 +526,14 @@ __global__ void __launch_bounds__(64) k_ps_segment(const double* __restrict__ di
         int nm = counts[0], np = counts[1];
         while (sp < sl.n) {
             const int q = sl.idx[sp++];
-            mem[nm++] = q;
+            ++nm;
             if (is_valid(vb, vkind, q)) pts[np++] = q;
         }
         counts[0] = nm;
         counts[1] = np;
     }
     __builtin_amdgcn_wave_barrier();
-    const int nm = counts[0], np = counts[1];
+    const int np = counts[1];
     if (np < 12) return;
     for (int k = lane; k < np; k += 64) pd[k] = d[pts[k]];
     __builtin_amdgcn_wave_barrier();
@@ -542,45 +542,43 @@ __global__ void __launch_bounds__(64) k_ps_segment(const double* __restrict__ di
         const int i = pts[k];
         return fabs(a * (double)(i % g.W) + bb * (double)(i / g.W) + cc - pd[k]) <= thr;
     };
-    // RANSAC
-    uint32_t s = state[(long long)b * g.K + ci];
-    int bestc = -1;
-    bool have = false;
-    double best[3] = {0.0, 0.0, 0.0};
-    for (int t = 0; t < PS_TRIALS; ++t) {
-        if (lane == 0) {
-            double A[9], bv[3], pl[3];
-            for (int q = 0; q < 3; ++q) {
-                s = xs_step(s);
-                const int i = pts[s % (uint32_t)np];
-                A[3 * q] = (double)(i % g.W);
-                A[3 * q + 1] = (double)(i / g.W);
-                A[3 * q + 2] = 1.0;
-                bv[q] = pd[s % (uint32_t)np];
-            }
-            const bool ok = solve3(A, bv, pl);
-            plane[0] = pl[0];
-            plane[1] = pl[1];
-            plane[2] = pl[2];
-            plane[3] = ok ? 1.0 : 0.0;
+    // RANSAC, the trials side by side: lane t runs trial t, whose three
+    // draws are steps 3t+1..3t+3 of the segment's xorshift stream, and counts
+    // its inliers over all points.  The best trial is the highest count, the
+    // earliest among equals (a later trial replaces the best only when
+    // strictly better); trials whose 3x3 system is singular take no part
+    int cnt = -1;
+    double pl[3] = {0.0, 0.0, 0.0};
+    if (lane < PS_TRIALS) {
+        uint32_t s = state[(long long)b * g.K + ci];
+        for (int q = 0; q < 3 * lane; ++q) s = xs_step(s);
+        double A[9], bv[3];
+        for (int q = 0; q < 3; ++q) {
+            s = xs_step(s);
+            const int i = pts[s % (uint32_t)np];
+            A[3 * q] = (double)(i % g.W);
+            A[3 * q + 1] = (double)(i / g.W);
+            A[3 * q + 2] = 1.0;
+            bv[q] = pd[s % (uint32_t)np];
         }
-        __builtin_amdgcn_wave_barrier();
-        const double pa = plane[0], pb = plane[1], pc = plane[2];
-        const bool ok = plane[3] != 0.0;
-        __builtin_amdgcn_wave_barrier();
-        if (!ok) continue;
-        int cnt = 0;
-        for (int k = lane; k < np; k += 64) cnt += on_pt(pa, pb, pc, k) ? 1 : 0;
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
-        if (cnt > bestc) {
-            bestc = cnt;
-            best[0] = pa;
-            best[1] = pb;
-            best[2] = pc;
-            have = true;
+        if (solve3(A, bv, pl)) {
+            cnt = 0;
+            for (int k = 0; k < np; ++k) cnt += on_pt(pl[0], pl[1], pl[2], k) ? 1 : 0;
         }
     }
+    int bestc = cnt, bt = lane;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const int oc = __shfl_xor(bestc, o, 64), ot = __shfl_xor(bt, o, 64);
+        if (oc > bestc || (oc == bestc && ot < bt)) {
+            bestc = oc;
+            bt = ot;
+        }
+    }
+    const bool have = bestc >= 0;
+    double best[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) best[q] = __shfl(pl[q], bt, 64);
     if (!have || (double)bestc < min_inl * (double)np) return;
     // inliers, in order, compacted over pts by ballot
     {
@@ -648,11 +646,15 @@ __global__ void __launch_bounds__(64) k_ps_segment(const double* __restrict__ di
     __builtin_amdgcn_wave_barrier();
     const double fa = plane[0], fb = plane[1], fc = plane[2];
     double* ob = out + (long long)b * N;
-    for (int k = lane; k < nm; k += 64) {
-        const int i = mem[k];
+    auto snap = [&](int i) {
         const double v = fa * (double)(i % g.W) + fb * (double)(i / g.W) + fc;
         if (fabs(v - d[i]) <= thr || (vb && !is_valid(vb, vkind, i))) ob[i] = v > 0.0 ? v : 0.0;
+    };
+    for (int e = lane; e < n; e += 64) {  // the window's members
+        const int i = (y0 + e / ww) * g.W + x0 + e % ww;
+        if (labb[i] == ci) snap(i);
     }
+    for (int k = lane; k < sl.n; k += 64) snap(sl.idx[k]);  // and the strays
 }
 
 __global__ void k_copy_i32(const int* __restrict__ a, int* __restrict__ o) {
@@ -767,7 +769,7 @@ int plane_snap_t(const double* disp, const RT* rgb, const uint8_t* valid, int vk
     hipLaunchKernelGGL(k_ps_count, dim3(g.K, B), dim3(64), 0, st, g, last, w.lab, w.strays,
                        w.nstray, valid, vkind, w.npts);
     hipLaunchKernelGGL(k_ps_states, dim3(B), dim3(1024), 0, st, w.npts, g.K, seed, w.state);
-    const size_t shm = (size_t)((2 * step + 1) * (2 * step + 1) + SCAP) * (8 + 4 + 4);
+    const size_t shm = (size_t)((2 * step + 1) * (2 * step + 1) + SCAP) * (8 + 4);
     hipLaunchKernelGGL(k_ps_segment, dim3(g.K, B), dim3(64), shm, st, disp, g, last, w.lab,
                        w.strays, w.nstray, valid, vkind, w.state, thr, min_inl, out, w.err);
     SV_CHECK_LAUNCH();
