@@ -66,10 +66,6 @@ class Camera:
     def unproject(self, u, v, Z):
         return (u - self.cx) * Z / self.fx, (v - self.cy) * Z / self.fx, Z
 
-    def _dev(self, dev):
-        return torch.tensor([self.fx, self.cx, self.cy, self.baseline, self.doffs],
-                            dtype=torch.float64).to(dev)
-
     def _host(self):
         return (ctypes.c_double * 5)(self.fx, self.cx, self.cy, self.baseline, self.doffs)
 

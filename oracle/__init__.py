@@ -2,7 +2,12 @@
 
 A float64 C restatement of the reference (stereonet.py) lives in
 ``stereo_oracle.c``; this module builds it with gcc and exposes numpy
-wrappers.  Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+wrappers.  Image decoding (read_image: Python's zlib for inflate), the
+Middlebury scoring of eval_real.py and the disparity-to-3D functions
+(Camera arithmetic, background_region, grid_mesh) are restated in numpy and
+plain Python at the end of this module; each is pinned by golden vectors
+the reference itself wrote (tests/test_images.py, test_eval_harness.py,
+test_geometry.py).  Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
 leg may import this package, and only as the checker or the timed CPU
 baseline; the shipped library never calls it.
 
