@@ -314,11 +314,18 @@ uint32_t be32(const uint8_t* p) {
     return ((uint32_t)p[0] << 24) | ((uint32_t)p[1] << 16) | ((uint32_t)p[2] << 8) | p[3];
 }
 
-int paeth(int a, int b, int c) {
-    const int p = a + b - c;
-    const int pa = p > a ? p - a : a - p, pb = p > b ? p - b : b - p, pc = p > c ? p - c : c - p;
-    if (pa <= pb && pa <= pc) return a;
-    return pb <= pc ? b : c;
+// the Paeth predictor without branches (unpredictable on image data): with
+// p = a + b - c, |p - a| = |b - c|, |p - b| = |a - c|, |p - c| = |a + b - 2c|;
+// a wins ties with both, then b with c
+inline int paeth(int a, int b, int c) {
+    int pa = b - c, pb = a - c, pc = pa + pb;
+    pa = pa < 0 ? -pa : pa;
+    pb = pb < 0 ? -pb : pb;
+    pc = pc < 0 ? -pc : pc;
+    const int mbc = -(int)(pb <= pc);
+    const int r = c ^ ((b ^ c) & mbc);
+    const int ma = -(int)((pa <= pb) & (pa <= pc));
+    return r ^ ((a ^ r) & ma);
 }
 
 bool png(const uint8_t* d, size_t n, Decoded& o, std::string& err, bool header_only) {
@@ -416,11 +423,15 @@ bool png(const uint8_t* d, size_t n, Decoded& o, std::string& err, bool header_o
                 }
                 break;
             case 4:
-                for (size_t i = 0; i < stride; ++i) {
-                    const int a = i >= B ? cur[i - B] : 0, u = up ? up[i] : 0;
-                    const int c = (up && i >= B) ? up[i - B] : 0;
-                    cur[i] = (uint8_t)(row[i] + paeth(a, u, c));
+                if (!up) {  // first row: the predictor is the left byte
+                    for (size_t i = 0; i < stride; ++i)
+                        cur[i] = (uint8_t)(row[i] + paeth(i >= B ? cur[i - B] : 0, 0, 0));
+                    break;
                 }
+                for (size_t i = 0; i < B && i < stride; ++i)
+                    cur[i] = (uint8_t)(row[i] + paeth(0, up[i], 0));
+                for (size_t i = B; i < stride; ++i)
+                    cur[i] = (uint8_t)(row[i] + paeth(cur[i - B], up[i], up[i - B]));
                 break;
             default:
                 err = "unknown PNG filter type " + std::to_string(f) + " on row " + std::to_string(y);
