@@ -51,10 +51,10 @@ __global__ void k_points(const double* __restrict__ disp, long long total, int W
 
 // ------------------------------------------------- k-th smallest float64
 // Radix select over order-preserving 64-bit keys in six digit passes
-// (five of 11 bits, one of 9).  Each pass is one launch: blocks count the
+// (five of 11 bits, one of 9).  Each pass is two launches: blocks count the
 // keys that share the digits chosen so far in an LDS histogram and add it
-// to a global one; the last block to finish picks the digit holding the
-// k-th key, clears the global histogram and its counter for the next pass.
+// to the set's global one; then one block per set picks the digit holding
+// the k-th key and clears the histogram for the next pass.
 constexpr int kSelBits = 11;
 constexpr int kSelBins = 1 << kSelBits;
 constexpr int kSelPasses = 6;
@@ -63,7 +63,6 @@ constexpr int kSelPer = kSelBins / 256;  // bins per thread in the pick
 struct Sel64 {
     unsigned long long prefix;
     long long k, n;
-    unsigned done;  // blocks finished in the current pass
 };
 
 __device__ __forceinline__ unsigned long long key_of(double v) {
@@ -93,7 +92,6 @@ __global__ void k_sel64_init(unsigned* __restrict__ hist, Sel64* __restrict__ st
         st[b].prefix = 0;
         st[b].n = nb;
         st[b].k = ns ? nb / 2 : k;
-        st[b].done = 0;
         if (nb == 0) out[b] = empty_value;
     }
 }
@@ -104,8 +102,6 @@ __global__ void __launch_bounds__(256) k_sel64_pass(const double* __restrict__ x
                                                     unsigned* __restrict__ hists,
                                                     double* __restrict__ out) {
     __shared__ unsigned h[kSelBins];
-    __shared__ long long s[256];
-    __shared__ int last;
     const int t = threadIdx.x, mb = blockIdx.y;
     Sel64* st = sts + mb;
     unsigned* hist = hists + (size_t)mb * kSelBins;
@@ -148,18 +144,26 @@ __global__ void __launch_bounds__(256) k_sel64_pass(const double* __restrict__ x
     __syncthreads();
     for (int b = t; b < kSelBins; b += 256)
         if (h[b]) atomicAdd(&hist[b], h[b]);
-    __threadfence();
-    __syncthreads();
-    if (t == 0) last = atomicAdd(&st->done, 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    // the last block: thread t owns bins [kSelPer t, kSelPer (t + 1))
+}
+
+// one block per set: the digit holding the k-th key, then the histogram
+// cleared for the next pass (a kernel of its own: the boundary orders it
+// after every block's histogram, with no device-wide fence in the counting)
+__global__ void __launch_bounds__(256) k_sel64_pick(int pass, Sel64* __restrict__ sts,
+                                                    unsigned* __restrict__ hists,
+                                                    double* __restrict__ out) {
+    __shared__ long long s[256];
+    const int t = threadIdx.x, mb = blockIdx.x;
+    Sel64* st = sts + mb;
+    unsigned* hist = hists + (size_t)mb * kSelBins;
+    if (st->n == 0) return;
+    int shift, width;
+    sel_digit(pass, shift, width);
+    // thread t owns bins [kSelPer t, kSelPer (t + 1))
     unsigned c[kSelPer];
     long long sum = 0;
 #pragma unroll
-    for (int b = 0; b < kSelPer; ++b)
-        c[b] = __hip_atomic_load(&hist[t * kSelPer + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int b = 0; b < kSelPer; ++b) c[b] = hist[t * kSelPer + b];
 #pragma unroll
     for (int b = 0; b < kSelPer; ++b) sum += c[b];
     s[t] = sum;
@@ -187,10 +191,8 @@ __global__ void __launch_bounds__(256) k_sel64_pass(const double* __restrict__ x
             run += c[b];
         }
     }
-    __syncthreads();
 #pragma unroll
     for (int b = 0; b < kSelPer; ++b) hist[t * kSelPer + b] = 0;
-    if (t == 0) st->done = 0;
 }
 
 // ------------------------------------------------------ background layer
@@ -557,6 +559,8 @@ int sv_kth_f64_batch(const double* x, long long mstride, long long stride, int B
     for (int pass = 0; pass < kSelPasses; ++pass) {
         hipLaunchKernelGGL(k_sel64_pass, dim3(g, B), dim3(256), 0, s, x, mstride, stride, pass, st,
                            hist, out);
+        SV_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_sel64_pick, dim3(B), dim3(256), 0, s, pass, st, hist, out);
         SV_CHECK_LAUNCH();
     }
     return SV_OK;
