@@ -172,3 +172,20 @@ def test_batch_equals_single(sv):
         assert (res[b]["main_tris"], res[b]["fill_tris"], res[b]["pivot"]) == \
             (ref["main_tris"], ref["fill_tris"], ref["pivot"])
     assert res[3]["pos"].shape[0] == 0 and res[3]["pivot"] == 1.0
+
+
+def test_row_min_with_nans(sv):
+    """NaNs follow the scan's rules: a NaN where a window starts sticks, a
+    NaN later in the window is passed over; the oracle's scan agrees."""
+    W, H = 300, 6
+    rng = np.random.default_rng(31)
+    v = np.round(rng.normal(10, 3, (H, W)))
+    v[1, 0] = np.nan  # the scan's start: sticks for the windows that begin there
+    v[3, 150] = np.nan
+    v[5, [7, 8, 299]] = np.nan
+    t = torch.from_numpy(v.ravel()).cuda()
+    for r in (1, 5, 28, 400):
+        m, i = sv.min_filter_arg(t, W, H, r, 0)
+        m0, i0 = oracle.min_filter_arg(v.ravel(), W, H, r, 0)
+        assert np.array_equal(m.cpu().numpy(), m0, equal_nan=True)
+        assert np.array_equal(i.cpu().numpy(), i0)
