@@ -328,6 +328,25 @@ inline int paeth(int a, int b, int c) {
     return r ^ ((a ^ r) & ma);
 }
 
+// one Paeth-filtered row with bpp fixed: the left and upper-left bytes of
+// each channel stay in registers (no reload of the byte just written)
+template <int BPP>
+void paeth_row(const uint8_t* row, const uint8_t* up, uint8_t* cur, size_t stride) {
+    int a[BPP], c[BPP];
+#pragma unroll
+    for (int k = 0; k < BPP; ++k) a[k] = c[k] = 0;
+    for (size_t i = 0; i + BPP <= stride; i += BPP) {
+#pragma unroll
+        for (int k = 0; k < BPP; ++k) {
+            const int u = up[i + k];
+            const int v = (row[i + k] + paeth(a[k], u, c[k])) & 0xFF;
+            cur[i + k] = (uint8_t)v;
+            a[k] = v;
+            c[k] = u;
+        }
+    }
+}
+
 bool png(const uint8_t* d, size_t n, Decoded& o, std::string& err, bool header_only) {
     size_t pos = 8;
     int depth = 0, ctype = 0, interlace = 0;
@@ -426,6 +445,14 @@ bool png(const uint8_t* d, size_t n, Decoded& o, std::string& err, bool header_o
                 if (!up) {  // first row: the predictor is the left byte
                     for (size_t i = 0; i < stride; ++i)
                         cur[i] = (uint8_t)(row[i] + paeth(i >= B ? cur[i - B] : 0, 0, 0));
+                    break;
+                }
+                if (stride % B == 0 && (B == 3 || B == 4 || B == 6 || B == 8)) {
+                    // whole pixels per row (8- and 16-bit colour)
+                    if (B == 3) paeth_row<3>(row, up, cur, stride);
+                    else if (B == 4) paeth_row<4>(row, up, cur, stride);
+                    else if (B == 6) paeth_row<6>(row, up, cur, stride);
+                    else paeth_row<8>(row, up, cur, stride);
                     break;
                 }
                 for (size_t i = 0; i < B && i < stride; ++i)
