@@ -123,15 +123,27 @@ __global__ void __launch_bounds__(kThreads) k_score(const P* __restrict__ pred,
         gs += g;
     };
     if constexpr (V4) {
-        for (long long i = lo + 4 * threadIdx.x; i < hi; i += 4 * kThreads) {
+        // two groups of four pixels per step, all their loads issued first
+        for (long long i = lo + 4 * threadIdx.x; i < hi; i += 8 * kThreads) {
+            const long long i2 = i + 4 * kThreads;
+            const bool two = i2 < hi;
             const uint32_t mk = *reinterpret_cast<const uint32_t*>(mask + base + i);
-            if (!mk) continue;
-            double pv[4], g[4];
+            const uint32_t mk2 = two ? *reinterpret_cast<const uint32_t*>(mask + base + i2) : 0u;
+            double pv[4], g[4], pv2[4], g2[4];
             load4(pred + base + i, pv);
             load4(gt + base + i, g);
+            if (two) {
+                load4(pred + base + i2, pv2);
+                load4(gt + base + i2, g2);
+            }
 #pragma unroll
             for (int j = 0; j < 4; ++j)
                 if ((mk >> (8 * j)) & 0xFFu) add(pv[j], g[j]);
+            if (two) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if ((mk2 >> (8 * j)) & 0xFFu) add(pv2[j], g2[j]);
+            }
         }
     } else {
         for (long long i = lo + threadIdx.x; i < hi; i += kThreads)
