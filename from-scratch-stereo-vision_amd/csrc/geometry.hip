@@ -304,6 +304,35 @@ __global__ void k_smooth_col(const double* __restrict__ t, const double* __restr
     out[i] = v < disp[i] ? v : disp[i];
 }
 
+// the same with a tile of 64 columns x 32 rows (plus r rows above and below)
+// staged in LDS: each input is read from HBM once per tile instead of once
+// per output row that covers it
+constexpr int kColTile = 32, kColRMax = 32;
+
+__global__ void __launch_bounds__(256) k_smooth_col_lds(const double* __restrict__ t,
+                                                        const double* __restrict__ disp, int W,
+                                                        int H, int r, double* __restrict__ out) {
+    extern __shared__ double tile[];  // (kColTile + 2r) rows x 64 columns
+    const int cx = threadIdx.x & 63, ry = threadIdx.x >> 6;
+    const int x = blockIdx.x * 64 + cx, y0 = blockIdx.y * kColTile;
+    const long long img = (long long)blockIdx.z * W * H;
+    const int ya = y0 - r > 0 ? y0 - r : 0;
+    const int ylast = y0 + kColTile - 1 < H - 1 ? y0 + kColTile - 1 : H - 1;
+    const int yb = ylast + r < H - 1 ? ylast + r : H - 1;
+    for (int rr = ry; rr <= yb - ya; rr += 4)
+        tile[rr * 64 + cx] = x < W ? t[img + (long long)(ya + rr) * W + x] : 0.0;
+    __syncthreads();
+    if (x >= W) return;
+    for (int y = y0 + ry; y <= ylast; y += 4) {
+        const int a = y - r > 0 ? y - r : 0, b = y + r < H - 1 ? y + r : H - 1;
+        double s = 0.0;
+        for (int yy = a; yy <= b; ++yy) s += tile[(yy - ya) * 64 + cx];
+        const double v = s / (double)(b - a + 1);
+        const long long i = img + (long long)y * W + x;
+        out[i] = v < disp[i] ? v : disp[i];
+    }
+}
+
 // background_region, :5637-5671: the foreground side of a depth edge
 __global__ void k_bg_thresh(const double* __restrict__ disp, const double* __restrict__ m,
                             long long total, double jump_ratio, double min_jump,
@@ -506,6 +535,19 @@ void launch_minrow(hipStream_t s, const double* v, int B, int W, int H, int r, d
                            idx);
 }
 
+void launch_smooth_col(hipStream_t s, const double* t, const double* disp, int B, int W, int H,
+                       int r, double* out) {
+    if (r <= kColRMax && B <= 65535) {
+        const dim3 grid((unsigned)((W + 63) / 64), (unsigned)((H + kColTile - 1) / kColTile), B);
+        const size_t shm = (size_t)(kColTile + 2 * r) * 64 * sizeof(double);
+        hipLaunchKernelGGL(k_smooth_col_lds, grid, dim3(256), shm, s, t, disp, W, H, r, out);
+    } else {
+        const long long total = (long long)B * W * H;
+        hipLaunchKernelGGL(k_smooth_col, dim3(blocks(total)), dim3(256), 0, s, t, disp, B, W, H, r,
+                           out);
+    }
+}
+
 void launch_smooth_row(hipStream_t s, const double* m, int B, int W, int H, int r, double* t) {
     const long long rows = (long long)B * H;
     if (W <= kRowMax && rows <= 0x7FFFFFFFLL)
@@ -600,7 +642,7 @@ int sv_smooth_background(const double* m, const double* disp, int B, int W, int 
     const long long total = (long long)B * W * H;
     launch_smooth_row(s, m, B, W, H, r, tmp);
     SV_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_smooth_col, dim3(blocks(total)), dim3(256), 0, s, tmp, disp, B, W, H, r, out);
+    launch_smooth_col(s, tmp, disp, B, W, H, r, out);
     SV_CHECK_LAUNCH();
     return SV_OK;
 }
@@ -621,7 +663,7 @@ int sv_background_region(const double* disp, int B, int W, int H, int radius, do
     const int rs = radius >= 2 ? radius / 2 : 1;
     launch_smooth_row(s, m, B, W, H, rs, tmp);
     SV_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_smooth_col, dim3(blocks(total)), dim3(256), 0, s, tmp, disp, B, W, H, rs, m);
+    launch_smooth_col(s, tmp, disp, B, W, H, rs, m);
     SV_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_bg_thresh, dim3(blocks(total)), dim3(256), 0, s, disp, m, total,
                        jump_ratio, min_jump, mask);
