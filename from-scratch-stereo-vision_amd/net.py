@@ -653,6 +653,13 @@ class _Targets:
 
 
 def train_grads_batch(net, lefts, rights, disps, valids, dims, sync=True):
+    """train_grads over a batch (see _train_grads_batch); returns the
+    per-sample losses and the plan."""
+    loss, pl, _ = _train_grads_batch(net, lefts, rights, disps, valids, dims, sync)
+    return loss, pl
+
+
+def _train_grads_batch(net, lefts, rights, disps, valids, dims, sync=True):
     """train_grads (stereonet.py:1619-1653) over a batch: per-sample losses
     (three pseudo-Huber levels with LEVEL_WEIGHTS, cross-entropy with
     CE_WEIGHT), gradients averaged over the batch like train_parallel.py's
@@ -682,33 +689,113 @@ def train_grads_batch(net, lefts, rights, disps, valids, dims, sync=True):
           _p(loss), _p(part), _p(scale), nblk, L.stream())
     pl.has_extra = True
     pl.backward()
-    return (loss.cpu().numpy() if sync else loss), pl
+    return (loss.cpu().numpy() if sync else loss), pl, tg
+
+
+def _targets_out(pl, tg, kind):
+    """The targets dict build_targets returns (stereonet.py:1595-1612) for
+    sample 0 of a plan: per-level disparities 'd' and validity 'v' (quarter,
+    half, full), the guides and the (W, H) dims of each level."""
+    q = tg.qlv.hw[0]
+    hw = (q, pl.half.hw[0], pl.full.hw[0])
+    d, v = [], []
+    for lvl in range(3):
+        n = hw[lvl][0] * hw[lvl][1]
+        dd, vv = tg.d[lvl][:n], tg.v[lvl][:n]
+        if kind == "torch":
+            d.append(dd)
+            v.append(vv.bool())
+        else:
+            da = dd.detach().cpu().numpy().astype(np.float64)
+            va = vv.detach().cpu().numpy().astype(bool)
+            d.append(da if kind == "numpy" else da.tolist())
+            v.append(va if kind == "numpy" else va.tolist())
+    h2, w2 = pl.half.hw[0]
+    h, w = pl.full.hw[0]
+    gh = pl.rin[0][h2 * w2:2 * h2 * w2]
+    gf = pl.rin[1][h * w:2 * h * w]
+    return {"d": d, "v": v, "guide_half": _ret(gh, kind), "guide_full": _ret(gf, kind),
+            "dims": [(hw[0][1], hw[0][0]), (w2, h2), (w, h)]}
 
 
 def train_grads(net, sample):
     """train_grads (stereonet.py:1619-1653) on one sample dict
     {left, right, disp, valid, W, H}: returns (loss, d2, targets) with the
-    gradient left in net.grads."""
+    gradient left in net.grads.  targets is build_targets' dict (:1595-1612),
+    in the sample's kind (lists, numpy arrays or device tensors)."""
     W, H = sample["W"], sample["H"]
+    kind = _kind(sample["left"])
     l, r = _img(sample["left"]), _img(sample["right"])
     d = _img(sample["disp"], torch.float32)
-    v = _img(np.asarray(sample["valid"], dtype=np.uint8), torch.uint8)
-    losses, pl = train_grads_batch(net, [l], [r], [d], [v], [(H, W)])
-    return float(losses[0]), _ret(pl.out(2, 0), _kind(sample["left"])), None
+    v = _img(np.asarray(sample["valid"], dtype=np.uint8)
+             if not isinstance(sample["valid"], torch.Tensor) else sample["valid"], torch.uint8)
+    losses, pl, tg = _train_grads_batch(net, [l], [r], [d], [v], [(H, W)])
+    return float(losses[0]), _ret(pl.out(2, 0), kind), _targets_out(pl, tg, kind)
 
 
 class Adam:
-    """Adam (stereonet.py:1354-1387) over the network's flat FP32 parameter
-    vector: elementwise clip, moments, running-product bias correction."""
+    """Adam (stereonet.py:1354-1387): elementwise clip, moments,
+    running-product bias correction, on the device.
+
+    params is what the reference passes, net.all_params(): a list of
+    (param, grad) pairs, here float32 device views.  Blocks that lie back to
+    back in one storage (all_params' views of the net's flat vectors) are
+    merged, so the whole network is one launch.  A StereoNet may be passed
+    instead of its all_params()."""
 
     def __init__(self, params, lr=1.0e-3, b1=0.9, b2=0.999, eps=1.0e-8):
         if isinstance(params, StereoNet):
-            self.net = params
-        else:
-            raise TypeError("Adam here optimises a StereoNet's flat parameters: pass the net")
+            params = params.all_params()
+        blocks = list(params)
+        for i, blk in enumerate(blocks):
+            if not (isinstance(blk, (tuple, list)) and len(blk) == 2):
+                raise TypeError("Adam expects (param, grad) pairs as all_params() returns")
+            p, g = blk
+            if not (isinstance(p, torch.Tensor) and isinstance(g, torch.Tensor)):
+                raise TypeError("Adam block %d: param and grad must be device tensors "
+                                "(views from StereoNet.all_params())" % i)
+            if p.dtype != torch.float32 or g.dtype != torch.float32 or not p.is_cuda or not g.is_cuda:
+                raise TypeError("Adam block %d: float32 device tensors expected" % i)
+            if not (p.is_contiguous() and g.is_contiguous()) or p.numel() != g.numel():
+                raise ValueError("Adam block %d: contiguous param/grad views of equal length "
+                                 "expected" % i)
+        self.params = blocks
         self.lr, self.b1, self.b2, self.eps = lr, b1, b2, eps
-        self.m = torch.zeros_like(self.net.params)
-        self.v = torch.zeros_like(self.net.params)
+        # runs of consecutive blocks adjacent in memory: (param ptr owner,
+        # grad owner, element count); one launch each
+        runs = []
+        for p, g in blocks:
+            if p.numel() == 0:
+                continue
+            if runs:
+                rp, rg, n = runs[-1]
+                if (p.data_ptr() == rp.data_ptr() + 4 * n and g.data_ptr() == rg.data_ptr() + 4 * n
+                        and p.untyped_storage().data_ptr() == rp.untyped_storage().data_ptr()
+                        and g.untyped_storage().data_ptr() == rg.untyped_storage().data_ptr()):
+                    runs[-1] = (rp, rg, n + p.numel())
+                    continue
+            runs.append((p, g, p.numel()))
+        self._runs = []
+        self.m, self.v = [], []
+        for p, g, n in runs:
+            m = torch.zeros(n, dtype=torch.float32, device=p.device)
+            v = torch.zeros(n, dtype=torch.float32, device=p.device)
+            self._runs.append((p, g, n, m, v))
+        # per-block moment views, as the reference's lists of lists
+        k = 0
+        for p, g in blocks:
+            if p.numel() == 0:
+                self.m.append(p.new_zeros(0))
+                self.v.append(p.new_zeros(0))
+                continue
+            while True:
+                rp, rg, n, m, v = self._runs[k]
+                off = (p.data_ptr() - rp.data_ptr()) // 4
+                if 0 <= off < n and p.untyped_storage().data_ptr() == rp.untyped_storage().data_ptr():
+                    break
+                k += 1
+            self.m.append(m[off:off + p.numel()])
+            self.v.append(v[off:off + p.numel()])
         self.b1t = 1.0
         self.b2t = 1.0
 
@@ -717,9 +804,9 @@ class Adam:
         self.b2t *= self.b2
         c1 = 1.0 / (1.0 - self.b1t)
         c2 = 1.0 / (1.0 - self.b2t)
-        p = self.net.params
-        _call("sv_adam_step", _p(p), _p(self.net.grads), _p(self.m), _p(self.v), p.numel(),
-              self.lr, self.b1, self.b2, self.eps, c1, c2, clip, L.stream())
+        for p, g, n, m, v in self._runs:
+            _call("sv_adam_step", p.data_ptr(), g.data_ptr(), _p(m), _p(v), n,
+                  self.lr, self.b1, self.b2, self.eps, c1, c2, clip, L.stream())
 
 
 def train_step(net, opt, sample):

@@ -141,6 +141,56 @@ def test_adam_matches_oracle(sv, N):
     assert np.abs(_host_params(net) - ref.p).max() <= 1e-5
 
 
+def test_adam_all_params_form(sv, N):
+    """Adam(net.all_params(), lr) as train_parallel.py:121 and
+    finetune_real.py:181 call it: the same steps as Adam(net), also when the
+    blocks come in another order (one launch per block then)."""
+    nets = [N.StereoNet(None, feat_C=4, cost_C=3, D=4, refine_C=3, refine_dilations=(1, 2), seed=5)
+            for _ in range(3)]
+    opts = [N.Adam(nets[0], lr=1e-2), N.Adam(nets[1].all_params(), lr=1e-2),
+            N.Adam(list(reversed(nets[2].all_params())), lr=1e-2)]
+    assert len(opts[1]._runs) == 1 and len(opts[2]._runs) == len(nets[2].all_params())
+    assert [m.numel() for m in opts[1].m] == [p.numel() for p, _ in nets[1].all_params()]
+    rng = np.random.default_rng(1)
+    for _ in range(5):
+        g = rng.normal(size=nets[0].nparams) * 3.0
+        for net, opt in zip(nets, opts):
+            N.set_flat_grads(net, g)
+            opt.step()
+    assert torch.equal(nets[0].params, nets[1].params)
+    assert torch.equal(nets[0].params, nets[2].params)
+    with pytest.raises(TypeError):
+        N.Adam([([0.0], [0.0])])
+
+
+def test_train_grads_returns_targets(sv, N):
+    """train_grads returns build_targets' dict (stereonet.py:1595-1612), so
+    the reference's metrics(pred, tgt['d'][2], tgt['v'][2]) call works."""
+    net = N.StereoNet(None, feat_C=4, cost_C=3, D=4, refine_C=3, refine_dilations=(1, 2), seed=3)
+    p = sv.synth.make_pair(40, 32, 4, dmin=1.0, dmax=8.0, coarse_noise=0)
+    gl = sv.synth.gray(p["left"])
+    sample = {"left": gl.ravel().tolist(), "right": sv.synth.gray(p["right"]).ravel().tolist(),
+              "disp": p["disp"].ravel().tolist(), "valid": p["valid"].ravel().tolist(),
+              "W": 40, "H": 32}
+    _, pred, tgt = N.train_grads(net, sample)
+    assert isinstance(pred, list) and len(pred) == 40 * 32
+    assert tgt["dims"] == [(10, 8), (20, 16), (40, 32)]
+    d = p["disp"].astype(np.float32).astype(np.float64)
+    v = p["valid"].astype(bool)
+    for lvl in (2, 1, 0):
+        assert np.array_equal(np.asarray(tgt["v"][lvl]), v.ravel())
+        assert np.abs(np.asarray(tgt["d"][lvl]) - d.ravel()).max() <= 1e-5
+        h, w = d.shape
+        d = 0.5 * 0.25 * (d[0:h // 2 * 2:2, 0:w // 2 * 2:2] + d[0:h // 2 * 2:2, 1:w // 2 * 2:2]
+                          + d[1:h // 2 * 2:2, 0:w // 2 * 2:2] + d[1:h // 2 * 2:2, 1:w // 2 * 2:2])
+        v = (v[0:h // 2 * 2:2, 0:w // 2 * 2:2] & v[0:h // 2 * 2:2, 1:w // 2 * 2:2]
+             & v[1:h // 2 * 2:2, 0:w // 2 * 2:2] & v[1:h // 2 * 2:2, 1:w // 2 * 2:2])
+    g = gl.astype(np.float32).astype(np.float64)
+    assert np.abs(np.asarray(tgt["guide_full"]) - g.ravel()).max() <= 1e-6
+    gh = 0.25 * (g[0::2, 0::2] + g[0::2, 1::2] + g[1::2, 0::2] + g[1::2, 1::2])
+    assert np.abs(np.asarray(tgt["guide_half"]) - gh.ravel()).max() <= 1e-6
+
+
 def test_checkpoint_round_trip(sv, N, tmp_path):
     net = N.StereoNet(None, feat_C=6, cost_C=3, D=6, max_disp=24, refine_C=5, refine_dilations=(1, 2))
     path = str(tmp_path / "ck.txt")
@@ -185,7 +235,7 @@ def test_loss_falls_on_one_scene(sv, N):
     on one 32x32 scene."""
     net = N.StereoNet(None, feat_C=6, cost_C=3, D=6, max_disp=24, refine_C=4, refine_dilations=(1, 2),
                       seed=2024)
-    opt = N.Adam(net, lr=3e-3)
+    opt = N.Adam(net.all_params(), lr=3e-3)      # test_stereonet.py:450
     p = sv.synth.make_pair(32, 32, 9, dmin=1.0, dmax=8.0, coarse_noise=0)
     sample = {"left": sv.synth.gray(p["left"]).astype(np.float32),
               "right": sv.synth.gray(p["right"]).astype(np.float32),
