@@ -8,7 +8,7 @@ arithmetic runs in hand-written HIP kernels (libstereo_mi.so, built by
 build.py); torch supplies device memory and streams only.
 """
 
-from .refine import (census, band_costs, _aggregate_path, refine_fullres, _mirror1,
+from .refine import (census, band_costs, _aggregate_path, refine_fullres, full_range, _mirror1,
                      left_to_right, lr_refine, median_filter, refine_disparity, slic, plane_snap)
 from .net import (StereoNet, Adam, LEVEL_WEIGHTS, CE_WEIGHT, train_grads, train_grads_batch,
                   train_step, flat_params, set_flat_params, flat_grads, set_flat_grads,
@@ -23,7 +23,7 @@ from .geometry import (Camera, read_calib, robust_min_disparity, disparity_to_po
                        min_filter_arg, smooth_background, background_region, grid_mesh)
 from . import autotune, scenes, synth, evaluation, geometry
 
-__all__ = ["census", "band_costs", "_aggregate_path", "refine_fullres", "_mirror1",
+__all__ = ["census", "band_costs", "_aggregate_path", "refine_fullres", "full_range", "_mirror1",
            "left_to_right", "lr_refine", "median_filter", "refine_disparity", "slic", "plane_snap",
            "StereoNet", "Adam",
            "LEVEL_WEIGHTS", "CE_WEIGHT", "train_grads", "train_grads_batch", "train_step",

@@ -34,6 +34,10 @@ SIGNATURES = {
     "sv_refine_workspace_bytes": (_I, [_I, _I, _I, _I, _I, _D, _D, _D, _D, _I, _I, _I, _D, _D, _PSZ]),
     "sv_refine_fullres": (_I, [_P, _P, _I, _P, _I, _I, _I, _I, _D, _D, _D, _D, _I, _D, _I, _I,
                                _D, _D, _P, _P, _P, _SZ, _P]),
+    "sv_full_range_workspace_bytes": (_I, [_I, _I, _I, _I, _I, _D, _D, _I, _I, _I, _PSZ]),
+    "sv_full_range_fast": (_I, [_I, _I, _I, _I, _D, _D, _I, _I, _I, ctypes.POINTER(_I)]),
+    "sv_full_range": (_I, [_I, _P, _P, _I, _I, _I, _I, _I, _P, _D, _D, _I, _I, _I, _D, _P, _P, _P,
+                           _P, _SZ, _P]),
     "sv_left_to_right": (_I, [_P, _I, _I, _I, _P, _P]),
     "sv_lr_refine": (_I, [_P, _P, _I, _P, _I, _I, _I, _D, _I, _D, _D, _D, _D, _I, _D, _I, _I,
                           _D, _D, _P, _P, _P, _SZ, _P]),

@@ -1177,8 +1177,69 @@ __device__ __forceinline__ void packed_step2(u16x2 (&L)[PackedBand2<K>::NPH], ui
     bprev = bc;
 }
 
-// k_sgm_p with two lanes per scanline (32 lines per wave).
-template <int K, int PF>
+// packed_step2 for a constant band centre: full-range matching (every
+// pixel's band is disparities 0..K-1 around the centre R) has re-centring
+// shift 0 at every step, so the neighbour minima M stay in the lane's
+// registers and no LDS slot is used.  A launch may serve a band shorter than
+// K: its entries from the real length up to K hold cost 255 in the cost
+// array (soft pads).  On the u8 path every real path cost is at most
+// invalid + P2 <= 255 and a soft pad's is at least its cost, 255, so soft
+// pads never lower a real entry's minimum, its M or the line minimum, and
+// the real entries equal those of a launch of the exact length.
+template <int K>
+__device__ __forceinline__ void packed_step2c(u16x2 (&L)[PackedBand2<K>::NPH],
+                                              u16x2 (&M)[PackedBand2<K>::NPH], uint32_t& mprev,
+                                              const uint32_t (&cw)[PackedBand2<K>::NW],
+                                              bool border, int P1, int P2, int h,
+                                              uint32_t (&out)[PackedBand2<K>::NW]) {
+    using PB = PackedBand2<K>;
+    constexpr int NPH = PB::NPH, NW = PB::NW;
+    const u16x2 p1v = {(unsigned short)P1, (unsigned short)P1};
+    u16x2 c[NPH];
+#pragma unroll
+    for (int j = 0; j < NPH; ++j)
+        c[j] = as_u16x2(__builtin_amdgcn_perm(0u, cw[j >> 1], (j & 1) ? 0x0c030c02u : 0x0c010c00u));
+    if (border) {
+#pragma unroll
+        for (int j = 0; j < NPH; ++j) L[j] = c[j];
+    } else {
+        const uint32_t cap = mprev + (uint32_t)P2;
+        const u16x2 capv = {(unsigned short)cap, (unsigned short)cap};
+        const u16x2 mpv = {(unsigned short)mprev, (unsigned short)mprev};
+#pragma unroll
+        for (int j = 0; j < NPH; ++j) L[j] = c[j] + __builtin_elementwise_min(M[j], capv) - mpv;
+    }
+#pragma unroll
+    for (int j = 0; j < NPH; ++j)
+        if (PB::pad1(j)) L[j] = as_u16x2(as_u32(L[j]) | (h ? PB::pad1(j) : 0u));
+    u16x2 mm = L[0];
+#pragma unroll
+    for (int j = 1; j < NPH; ++j) mm = __builtin_elementwise_min(mm, L[j]);
+    uint32_t m = min((uint32_t)mm.x, (uint32_t)mm.y);
+    m = min(m, swap_pair(m));
+    const uint32_t recv = swap_pair(as_u32(h ? L[0] : L[NPH - 1]));
+#pragma unroll
+    for (int j = 0; j < NPH; ++j) {
+        const uint32_t lo = j > 0 ? as_u32(L[j - 1]) : (h ? recv : 0xFFFFFFFFu);
+        const uint32_t hi = j + 1 < NPH ? as_u32(L[j + 1]) : (h ? 0xFFFFFFFFu : recv);
+        const u16x2 left = as_u16x2(__builtin_amdgcn_alignbit(as_u32(L[j]), lo, 16));
+        const u16x2 right = as_u16x2(__builtin_amdgcn_alignbit(hi, as_u32(L[j]), 16));
+        const u16x2 nb = __builtin_elementwise_add_sat(__builtin_elementwise_min(left, right), p1v);
+        M[j] = __builtin_elementwise_min(L[j], nb);
+    }
+#pragma unroll
+    for (int q = 0; q < NW; ++q) {
+        uint32_t lo = as_u32(L[2 * q]), hi = as_u32(L[2 * q + 1]);
+        if (PB::pad1(2 * q)) lo &= h ? ~PB::pad1(2 * q) : 0xFFFFFFFFu;
+        if (PB::pad1(2 * q + 1)) hi &= h ? ~PB::pad1(2 * q + 1) : 0xFFFFFFFFu;
+        out[q] = __builtin_amdgcn_perm(hi, lo, 0x06040200u);
+    }
+    mprev = m;
+}
+
+// k_sgm_p with two lanes per scanline (32 lines per wave).  CB:
+// constant band centre (packed_step2c; the band centres are not read).
+template <int K, int PF, bool CB = false>
 __global__ void __launch_bounds__(64) k_sgm_p2(const uint8_t* __restrict__ C,
                                                const int* __restrict__ base,
                                                uint8_t* __restrict__ Lout, Geom g, int Kp,
@@ -1201,7 +1262,8 @@ __global__ void __launch_bounds__(64) k_sgm_p2(const uint8_t* __restrict__ C,
     if (lp.line >= nLines) return;  // both lanes of a pair leave together
     u32_alias* sl32 = reinterpret_cast<u32_alias*>(lds + ln * SD);
     u16_alias* s16 = reinterpret_cast<u16_alias*>(sl32);
-    for (int i = h; i < SD; i += 2) sl32[i] = 0xFFFFFFFFu;
+    if constexpr (!CB)
+        for (int i = h; i < SD; i += 2) sl32[i] = 0xFFFFFFFFu;
     const long long plane = g.slot;
     const uint8_t* Cb = C + (long long)blockIdx.z * plane * Kp + 4 * NW * h;
     const int* Bb = base + (long long)blockIdx.z * plane;
@@ -1214,9 +1276,10 @@ __global__ void __launch_bounds__(64) k_sgm_p2(const uint8_t* __restrict__ C,
         const long long pq = lp.at(u < len ? u : 0, bd);
 #pragma unroll
         for (int q = 0; q < NW; ++q) ring[u][q] = *reinterpret_cast<const uint32_t*>(Cb + pq * Kp + 4 * q);
-        rb[u] = Bb[pq];
+        rb[u] = CB ? 0 : Bb[pq];
     }
     u16x2 L[NPH];
+    u16x2 M[CB ? NPH : 1];
     uint32_t mprev = 0;
     int bprev = 0;
     auto step = [&](int i, const uint32_t (&cr)[NW], int b) {
@@ -1226,7 +1289,10 @@ __global__ void __launch_bounds__(64) k_sgm_p2(const uint8_t* __restrict__ C,
 #pragma unroll
         for (int q = 0; q < NW; ++q) cw[q] = cr[q];
         uint32_t o[NW];
-        packed_step2<K>(L, mprev, bprev, sl32, s16, cw, b, border, P1, P2, h, o);
+        if constexpr (CB)
+            packed_step2c<K>(L, M, mprev, cw, border, P1, P2, h, o);
+        else
+            packed_step2<K>(L, mprev, bprev, sl32, s16, cw, b, border, P1, P2, h, o);
 #pragma unroll
         for (int q = 0; q < NW; ++q) *reinterpret_cast<uint32_t*>(Lb + pix * Kp + 4 * q) = o[q];
     };
@@ -1235,7 +1301,7 @@ __global__ void __launch_bounds__(64) k_sgm_p2(const uint8_t* __restrict__ C,
         const long long pq = lp.at(inext, bd);
 #pragma unroll
         for (int q = 0; q < NW; ++q) ring[u][q] = *reinterpret_cast<const uint32_t*>(Cb + pq * Kp + 4 * q);
-        rb[u] = Bb[pq];
+        rb[u] = CB ? 0 : Bb[pq];
     };
     int i0 = 0;
     for (; i0 + PF <= len; i0 += PF) {
@@ -1252,7 +1318,7 @@ __global__ void __launch_bounds__(64) k_sgm_p2(const uint8_t* __restrict__ C,
 
 // k_sgm_ph with two lanes per row: 32 rows per wave, the row tiles staged
 // through LDS as in k_sgm_ph.
-template <int K, int TC>
+template <int K, int TC, bool CB = false>
 __global__ void __launch_bounds__(64) k_sgm_ph2(const uint8_t* __restrict__ C,
                                                 const int* __restrict__ base,
                                                 uint8_t* __restrict__ Lout, Geom g, int Kp,
@@ -1268,15 +1334,16 @@ __global__ void __launch_bounds__(64) k_sgm_ph2(const uint8_t* __restrict__ C,
     static_assert(RS % 2 == 0, "8-byte aligned rows");
     static_assert(4 * NW * 2 <= 8 * NB, "a lane pair's words fit the pixel's chunks");
     __shared__ __align__(16) uint32_t tile[ROWS * RS];
-    __shared__ int btile[ROWS * BS];
-    __shared__ uint32_t lds[ROWS * SD];
+    __shared__ int btile[CB ? 1 : ROWS * BS];
+    __shared__ uint32_t lds[CB ? 1 : ROWS * SD];
     const int dx = c_dx[dir0 + (int)blockIdx.y];
     const int lane = threadIdx.x, h = lane & 1, ln = lane >> 1;
     const int y0 = blockIdx.x * ROWS;
     const int rows = min(ROWS, H - y0);
-    u32_alias* sl32 = reinterpret_cast<u32_alias*>(lds + ln * SD);
+    u32_alias* sl32 = reinterpret_cast<u32_alias*>(lds + (CB ? 0 : ln * SD));
     u16_alias* s16 = reinterpret_cast<u16_alias*>(sl32);
-    for (int i = h; i < SD; i += 2) sl32[i] = 0xFFFFFFFFu;
+    if constexpr (!CB)
+        for (int i = h; i < SD; i += 2) sl32[i] = 0xFFFFFFFFu;
     const long long plane = g.slot;
     const uint8_t* Cb = C + (long long)blockIdx.z * plane * Kp + (long long)y0 * W * Kp;
     const int* Bb = base + (long long)blockIdx.z * plane + (long long)y0 * W;
@@ -1284,12 +1351,13 @@ __global__ void __launch_bounds__(64) k_sgm_ph2(const uint8_t* __restrict__ C,
                   (long long)y0 * W * Kp;
     const long long rowBytes = (long long)W * Kp;
     u16x2 L[NPH];
+    u16x2 M[CB ? NPH : 1];
     uint32_t mprev = 0;
     int bprev = 0;
     // this lane's words of a pixel: u32 offset 2*NB*col + NW*h in its row
     const uint32_t* mine = tile + ln * RS + NW * h;
     uint32_t* mine_w = tile + ln * RS + NW * h;
-    const int* bmine = btile + ln * BS;
+    const int* bmine = btile + (CB ? 0 : ln * BS);
     constexpr int CU = T * NB;          // 8-byte cost units of a full row run
     constexpr int TOT = ROWS * CU;      // units of a full tile
     constexpr int TB = ROWS * T;        // band centres of a full tile
@@ -1317,7 +1385,7 @@ __global__ void __launch_bounds__(64) k_sgm_ph2(const uint8_t* __restrict__ C,
             for (int u = 0; u < 8; ++u) *reinterpret_cast<uint2*>(tile + off[u]) = v[u];
         }
 #pragma unroll
-        for (int it = 0; it < TB; it += 64 * 4) {
+        for (int it = 0; it < (CB ? 0 : TB); it += 64 * 4) {
             int v[4], off[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
@@ -1337,7 +1405,11 @@ __global__ void __launch_bounds__(64) k_sgm_ph2(const uint8_t* __restrict__ C,
 #pragma unroll
             for (int q = 0; q < NW; ++q) cw[q] = mine[col * 2 * NB + q];
             uint32_t o[NW];
-            packed_step2<K>(L, mprev, bprev, sl32, s16, cw, bmine[col], c0 + j == 0, P1, P2, h, o);
+            if constexpr (CB)
+                packed_step2c<K>(L, M, mprev, cw, c0 + j == 0, P1, P2, h, o);
+            else
+                packed_step2<K>(L, mprev, bprev, sl32, s16, cw, bmine[col], c0 + j == 0, P1, P2,
+                                h, o);
 #pragma unroll
             for (int q = 0; q < NW; ++q) mine_w[col * 2 * NB + q] = o[q];
         }
@@ -1546,6 +1618,164 @@ __global__ void __launch_bounds__(64) k_wta_k(const LT* __restrict__ Lv, long lo
     }
     out[pix] = o;
     if (conf) conf[pix] = cf;
+}
+
+
+// ------------------------------------------- constant band centre (full range)
+// band_costs with the centre R at every pixel (full-range matching,
+// eval_real.py:143-146): candidate k is disparity k.  A wave makes the costs
+// of 64 pixels of one row: the partner row's census words its candidates
+// reach (64 + Kp - 1, clamped into the row) are staged in the wave's LDS
+// window, each lane takes its Kp words from there, entries k in [K, Kp) are
+// set to 255 (the soft pads of packed_step2c) and the row run is written
+// back as contiguous 8-byte units through the wave's LDS stage.
+template <bool RIGHT>
+__global__ void __launch_bounds__(256) k_costs_cb(const uint64_t* __restrict__ cA,
+                                                  const uint64_t* __restrict__ cB, Geom g, int K,
+                                                  const int* __restrict__ rtab, int Kp,
+                                                  int invalid, uint8_t* __restrict__ C) {
+    extern __shared__ __align__(16) uint32_t cb_lds[];
+    const int lane = threadIdx.x;
+    const int nwin = 64 + Kp - 1;
+    const int wave_u32 = 64 * (Kp / 4) + 2 * ((nwin + 1) & ~1);
+    uint32_t* cstage = cb_lds + threadIdx.y * wave_u32;
+    uint64_t* win = reinterpret_cast<uint64_t*>(cstage + 64 * (Kp / 4));
+    const int x0 = blockIdx.x * 64;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int W = g.w(blockIdx.z);
+    if (x0 >= W || y >= g.h(blockIdx.z)) return;  // whole wave outside this image
+    if (rtab) K = 2 * rtab[blockIdx.z] + 1;        // this pair's own band length
+    const long long row0 = (long long)blockIdx.z * g.slot + (long long)y * W;
+    const uint64_t* rowB = cB + row0;
+    // window element i is partner column w0 + i
+    const int w0 = RIGHT ? x0 : x0 - (Kp - 1);
+    for (int i = lane; i < nwin; i += 64) win[i] = rowB[min(max(w0 + i, 0), W - 1)];
+    __builtin_amdgcn_wave_barrier();
+    const int x = x0 + lane;
+    if (x < W) {
+        const uint64_t a = cA[row0 + x];
+        for (int k0 = 0; k0 < Kp; k0 += 4) {
+            uint32_t packed = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int k = k0 + j;
+                const int xr = RIGHT ? x + k : x - k;
+                uint32_t v = 255u;
+                if (k < K) {
+                    const uint64_t w = win[RIGHT ? lane + k : lane + Kp - 1 - k];
+                    v = (xr >= 0 && xr < W) ? (uint32_t)__popcll(a ^ w) : (uint32_t)invalid;
+                }
+                packed |= v << (8 * j);
+            }
+            cstage[lane * (Kp / 4) + k0 / 4] = packed;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    const int units = (min(64, W - x0) * Kp) / 8;
+    const uint2* src = reinterpret_cast<const uint2*>(cstage);
+    uint2* dst = reinterpret_cast<uint2*>(C + (row0 + x0) * Kp);
+    for (int u = lane; u < units; u += 64) dst[u] = src[u];
+}
+
+// refine_fullres epilogue (stereonet.py:5271-5302) for constant-centre
+// bands of K <= KB entries: the NDIR u8 path costs of a pixel (Kp bytes
+// each) are summed as packed u16 pairs (at most 8 * 255), then first
+// minimum over the K real entries, band-edge passthrough (the centre R, the
+// value of disp0 everywhere), second best outside bk +- 1 and parabola as in
+// k_wta_k.  The output is base - R + bk + off with base = R, i.e. bk + off.
+template <int KB, int NDIR>
+__global__ void __launch_bounds__(64) k_wta_cb(const uint8_t* __restrict__ Lv, long long dirStride,
+                                               Geom g, int K, const int* __restrict__ rtab,
+                                               int Kp, int R, double* __restrict__ out,
+                                               double* __restrict__ conf) {
+    constexpr int NU = PackedBand<KB>::NB;   // 8-byte units per pixel and path
+    constexpr int NPR = 4 * NU;              // u16 pairs
+    const int bv = blockIdx.y;
+    const int p = blockIdx.x * 64 + threadIdx.x;
+    if (p >= g.w(bv) * g.h(bv)) return;
+    if (rtab) {  // this pair's own centre and band length
+        R = rtab[bv % g.B];
+        K = 2 * R + 1;
+    }
+    const long long pix = (long long)bv * g.slot + p;
+    u16x2 S[NPR];
+#pragma unroll
+    for (int j = 0; j < NPR; ++j) S[j] = (u16x2){0, 0};
+#pragma unroll
+    for (int t0 = 0; t0 < NDIR; t0 += 4) {
+        uint2 raw[4][NU];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const uint2* src = reinterpret_cast<const uint2*>(Lv + (long long)(t0 + t) * dirStride + pix * Kp);
+#pragma unroll
+            for (int q = 0; q < NU; ++q) raw[t][q] = src[q];
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int q = 0; q < NU; ++q) {
+                const uint32_t w2[2] = {raw[t][q].x, raw[t][q].y};
+#pragma unroll
+                for (int hh = 0; hh < 2; ++hh) {
+                    S[4 * q + 2 * hh] += as_u16x2(__builtin_amdgcn_perm(0u, w2[hh], 0x0c010c00u));
+                    S[4 * q + 2 * hh + 1] += as_u16x2(__builtin_amdgcn_perm(0u, w2[hh], 0x0c030c02u));
+                }
+            }
+    }
+    // entries at or past K out of every minimum
+#pragma unroll
+    for (int j = 0; j < NPR; ++j) {
+        if (2 * j >= K) S[j].x = 0xFFFF;
+        if (2 * j + 1 >= K) S[j].y = 0xFFFF;
+    }
+    u16x2 mm = S[0];
+#pragma unroll
+    for (int j = 1; j < NPR; ++j) mm = __builtin_elementwise_min(mm, S[j]);
+    const int bvv = min((int)mm.x, (int)mm.y);
+    int bk = 2 * NPR;
+#pragma unroll
+    for (int j = NPR - 1; j >= 0; --j) {
+        if ((int)S[j].y == bvv) bk = 2 * j + 1;
+        if ((int)S[j].x == bvv) bk = 2 * j;
+    }
+    double o, cf;
+    if (bk == 0 || bk == K - 1) {
+        o = (double)R;
+        cf = 0.0;
+    } else {
+        int sv = 0x7FFFFFFF, a = 0, bb = 0;
+        bool found = false;
+#pragma unroll
+        for (int j = 0; j < NPR; ++j) {
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const int k = 2 * j + e;
+                const int v = e ? (int)S[j].y : (int)S[j].x;
+                if (k < K && (k < bk - 1 || k > bk + 1)) {
+                    found = true;
+                    sv = min(sv, v);
+                }
+                if (k == bk - 1) a = v;
+                if (k == bk + 1) bb = v;
+            }
+        }
+        const double bvd = (double)bvv;
+        cf = found ? ((double)sv - bvd) / (bvd + 1.0e-6) : 1.0;
+        const double ad = (double)a, bd = (double)bb;
+        const double den = ad - 2.0 * bvd + bd;
+        double off = 0.0;
+        if (den > 1.0e-9) off = 0.5 * (ad - bd) / den;
+        o = (double)bk + off;
+    }
+    out[pix] = o;
+    if (conf) conf[pix] = cf;
+}
+
+// a[i] = v, or the pair's own centre rtab[i / slot] when rtab is given
+__global__ void k_fill_f64(double* __restrict__ a, long long n, double v,
+                           const int* __restrict__ rtab, long long slot) {
+    const long long i = blockIdx.x * 256LL + threadIdx.x;
+    if (i < n) a[i] = rtab ? (double)rtab[i / slot] : v;
 }
 
 // ------------------------------------------------------------ LR check
@@ -2376,6 +2606,154 @@ int refine_views(const Plan& pl, const RefineBufs& bf, const void* left, const v
                       outs, conf, st);
 }
 
+
+// ------------------------------------------- constant band centre (full range)
+// Band lengths with constant-centre kernels: a band of K entries runs on the
+// smallest bucket KB >= K (soft pads in between, see packed_step2c).
+constexpr int kCbBuckets[] = {45, 65, 85, 125};
+
+int cb_bucket(int K) {
+    for (int kb : kCbBuckets)
+        if (K <= kb) return kb;
+    return 0;
+}
+
+int cb_kp(int kb) {
+    switch (kb) {
+        case 45: return 8 * PackedBand<45>::NB;
+        case 65: return 8 * PackedBand<65>::NB;
+        case 85: return 8 * PackedBand<85>::NB;
+        case 125: return 8 * PackedBand<125>::NB;
+    }
+    return 0;
+}
+
+struct CbBufs {
+    uint64_t* cen;  // [2][B][HW]
+    uint8_t* C;     // [V][B][HW][Kp]
+    uint8_t* L;     // [ndir][V][B][HW][Kp]
+    double* d0;     // [B][HW]   the centre R (lr only: fill value and check input)
+    double* d0r;    // [B][HW]   left_to_right(d0) (lr only)
+    double* disp;   // [V][B][HW] (lr only)
+    size_t bytes;
+};
+
+CbBufs carve_cb(void* ws, int Kp, int ndir, int V, int B, long long slot, bool lr) {
+    sv::Carver cv(ws);
+    const size_t hw = (size_t)slot;
+    CbBufs r;
+    r.cen = cv.take<uint64_t>(2 * (size_t)B * hw);
+    r.C = cv.take<uint8_t>((size_t)V * B * hw * Kp);
+    r.L = cv.take<uint8_t>((size_t)ndir * V * B * hw * Kp);
+    r.d0 = lr ? cv.take<double>((size_t)B * hw) : nullptr;
+    r.d0r = lr ? cv.take<double>((size_t)B * hw) : nullptr;
+    r.disp = lr ? cv.take<double>((size_t)V * B * hw) : nullptr;
+    r.bytes = cv.used;
+    return r;
+}
+
+int launch_costs_cb(bool right, const uint64_t* cA, const uint64_t* cB, int B, const Geom& g,
+                    int K, const int* rtab, int Kp, int invalid, uint8_t* C, hipStream_t st) {
+    const int rb = 4;
+    const int nwin = 64 + Kp - 1;
+    const size_t wave_u32 = 64 * (size_t)(Kp / 4) + 2 * (size_t)((nwin + 1) & ~1);
+    dim3 grid((g.W + 63) / 64, (g.H + rb - 1) / rb, B);
+    const size_t shm = rb * wave_u32 * 4;
+    if (right)
+        hipLaunchKernelGGL((k_costs_cb<true>), grid, dim3(64, rb), shm, st, cA, cB, g, K, rtab, Kp,
+                           invalid, C);
+    else
+        hipLaunchKernelGGL((k_costs_cb<false>), grid, dim3(64, rb), shm, st, cA, cB, g, K, rtab, Kp,
+                           invalid, C);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+template <int KB>
+int sgm_cb_t(const uint8_t* C, uint8_t* L, int BV, const Geom& g, int Kp, int P1, int P2,
+             int ndir, hipStream_t st) {
+    const long long dirStride = (long long)BV * g.slot * Kp;
+    dim3 hgrid((g.H + 31) / 32, 2, BV);
+    hipLaunchKernelGGL((k_sgm_ph2<KB, 8, true>), hgrid, dim3(64), 0, st, C, nullptr, L, g, Kp, P1,
+                       P2, 0, dirStride);
+    dim3 vgrid((g.W + 31) / 32, ndir - 2, BV);
+    hipLaunchKernelGGL((k_sgm_p2<KB, 4, true>), vgrid, dim3(64), 0, st, C, nullptr,
+                       L + 2 * dirStride, g, Kp, P1, P2, 2, dirStride);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+template <int KB>
+int wta_cb_t(const uint8_t* L, int BV, int ndir, const Geom& g, int K, const int* rtab, int Kp,
+             int R, double* out, double* conf, hipStream_t st) {
+    const long long dirStride = (long long)BV * g.slot * Kp;
+    dim3 grid((unsigned)((g.slot + 63) / 64), BV);
+    if (ndir == 8)
+        hipLaunchKernelGGL((k_wta_cb<KB, 8>), grid, dim3(64), 0, st, L, dirStride, g, K, rtab, Kp, R,
+                           out, conf);
+    else
+        hipLaunchKernelGGL((k_wta_cb<KB, 4>), grid, dim3(64), 0, st, L, dirStride, g, K, rtab, Kp, R,
+                           out, conf);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+// V views of B pairs with the constant centre R (or pair n's own centre
+// rtab[n], device int32, all at most R): census, costs, the packed
+// constant-centre SGM and its epilogue.  View 1 matches right against left.
+int refine_views_cb(const CbBufs& bf, const void* left, const void* right, int dtype, int V,
+                    const Geom& g, int R, const int* rtab, int kb, int Kp, int ndir, int P1, int P2,
+                    int rx, int ry, double* outs, double* conf, hipStream_t st) {
+    const int B = g.B;
+    const int K = 2 * R + 1;
+    const int invalid = (2 * rx + 1) * (2 * ry + 1) - 1 + 4;
+    const size_t hw = (size_t)g.slot;
+    uint64_t* cL = bf.cen;
+    uint64_t* cR = bf.cen + (size_t)B * hw;
+    int rc;
+    {
+        StageTimer t(ST_CENSUS, st);
+        if ((rc = census_any(left, dtype, B, g, rx, ry, cL, st))) return rc;
+        if ((rc = census_any(right, dtype, B, g, rx, ry, cR, st))) return rc;
+    }
+    {
+        StageTimer t(ST_COSTS, st);
+        for (int v = 0; v < V; ++v) {
+            uint8_t* C = bf.C + (size_t)v * B * hw * Kp;
+            rc = v == 0 ? launch_costs_cb(false, cL, cR, B, g, K, rtab, Kp, invalid, C, st)
+                        : launch_costs_cb(true, cR, cL, B, g, K, rtab, Kp, invalid, C, st);
+            if (rc) return rc;
+        }
+    }
+    {
+        StageTimer t(ST_SGM, st);
+        switch (kb) {
+            case 45: rc = sgm_cb_t<45>(bf.C, bf.L, V * B, g, Kp, P1, P2, ndir, st); break;
+            case 65: rc = sgm_cb_t<65>(bf.C, bf.L, V * B, g, Kp, P1, P2, ndir, st); break;
+            case 85: rc = sgm_cb_t<85>(bf.C, bf.L, V * B, g, Kp, P1, P2, ndir, st); break;
+            case 125: rc = sgm_cb_t<125>(bf.C, bf.L, V * B, g, Kp, P1, P2, ndir, st); break;
+            default: return sv::set_error(SV_EUNSUPPORTED, "no constant-centre bucket %d", kb);
+        }
+        if (rc) return rc;
+    }
+    StageTimer t(ST_WTA, st);
+    switch (kb) {
+        case 45: return wta_cb_t<45>(bf.L, V * B, ndir, g, K, rtab, Kp, R, outs, conf, st);
+        case 65: return wta_cb_t<65>(bf.L, V * B, ndir, g, K, rtab, Kp, R, outs, conf, st);
+        case 85: return wta_cb_t<85>(bf.L, V * B, ndir, g, K, rtab, Kp, R, outs, conf, st);
+        case 125: return wta_cb_t<125>(bf.L, V * B, ndir, g, K, rtab, Kp, R, outs, conf, st);
+    }
+    return sv::set_error(SV_EUNSUPPORTED, "no constant-centre bucket %d", kb);
+}
+
+// Whether the constant-centre kernels serve this full-range call: integer u8
+// path (census costs, integral P1/P2 with invalid + P2 <= 255) and a bucket
+// for K.  The right view's centre is left_to_right of the constant map,
+// again the constant R when W >= R (some pixel lands inside the view).
+bool cb_applies(const Plan& pl, int W, int R, bool lr) {
+    return pl.integer && pl.lt == 1 && cb_bucket(pl.K) != 0 && (!lr || W >= R);
+}
+
 }  // namespace
 
 // =============================================================== C ABI
@@ -2793,6 +3171,98 @@ int sv_refine_disparity(const void* left, const void* right, int dtype, const do
     SV_REQUIRE(B > 0 && W > 0 && H > 0, "sv_refine_disparity: bad sizes");
     return lr_core(left, right, dtype, disp0, uniform_geom(B, W, H), tol, R, P1, P2, 0.0, 20.0, 8,
                    0.0, rx, ry, 0.0, 4.0, out, status, ws, ws_bytes, true, (hipStream_t)stream);
+}
+
+// Full-range SGM: refine_fullres (mode 0) or lr_refine (mode 1) with disp0 =
+// the constant R at every pixel, K = 2R + 1 -- the control of eval_real.py:
+// 143-146.  Results equal sv_refine_fullres / sv_lr_refine given that
+// constant map (ad_weight = prior_w = 0, uniq = 0); conf (mode 0, may be
+// null) as refine_fullres' return_conf.  rtab (device int32 [B], may be
+// null): pair n's own centre, each in [0, R], all pairs in one launch (the
+// constant-centre kernels only); R then sizes the band.
+int sv_full_range_workspace_bytes(int mode, int B, int W, int H, int R, double P1, double P2,
+                                  int paths, int rx, int ry, size_t* bytes) {
+    SV_REQUIRE(bytes && B > 0 && (mode == 0 || mode == 1), "sv_full_range_workspace_bytes: bad arguments");
+    Plan pl{};
+    int rc = make_plan(W, H, R, P1, P2, 0.0, 0.0, 20.0, 4.0, paths, rx, ry, &pl);
+    if (rc) return rc;
+    const bool lr = mode == 1;
+    const int V = lr ? 2 : 1;
+    if (cb_applies(pl, W, R, lr)) {
+        *bytes = carve_cb(nullptr, cb_kp(cb_bucket(pl.K)), pl.ndir, V, B, (long long)W * H, lr).bytes;
+        return SV_OK;
+    }
+    pl.Kp = pl.G * pl.KC;
+    // the general path plus the constant centre map
+    *bytes = carve(nullptr, pl, V, B, (long long)W * H, lr, false).bytes +
+             sv::align_up((size_t)B * W * H * sizeof(double));
+    return SV_OK;
+}
+
+// Whether sv_full_range runs the constant-centre kernels for these
+// arguments (1) or the general ones (0); per-pair centres need 1.
+int sv_full_range_fast(int mode, int W, int H, int R, double P1, double P2, int paths, int rx,
+                       int ry, int* fast) {
+    SV_REQUIRE(fast && (mode == 0 || mode == 1), "sv_full_range_fast: bad arguments");
+    Plan pl{};
+    int rc = make_plan(W, H, R, P1, P2, 0.0, 0.0, 20.0, 4.0, paths, rx, ry, &pl);
+    if (rc) return rc;
+    *fast = cb_applies(pl, W, R, mode == 1) ? 1 : 0;
+    return SV_OK;
+}
+
+int sv_full_range(int mode, const void* left, const void* right, int dtype, int B, int W, int H,
+                  int R, const int* rtab, double P1, double P2, int paths, int rx, int ry,
+                  double tol, double* out, double* conf, uint8_t* status, void* ws,
+                  size_t ws_bytes, void* stream) {
+    SV_REQUIRE(left && right && out && B > 0 && W > 0 && H > 0 && (mode == 0 || mode == 1),
+               "sv_full_range: bad arguments");
+    SV_REQUIRE(mode == 0 || status, "sv_full_range: lr mode needs a status buffer");
+    Plan pl{};
+    int rc = make_plan(W, H, R, P1, P2, 0.0, 0.0, 20.0, 4.0, paths, rx, ry, &pl);
+    if (rc) return rc;
+    const bool lr = mode == 1;
+    const int V = lr ? 2 : 1;
+    const Geom g = uniform_geom(B, W, H);
+    hipStream_t st = (hipStream_t)stream;
+    const long long n = (long long)B * W * H;
+    const dim3 fgrid((unsigned)((n + 255) / 256));
+    if (!cb_applies(pl, W, R, lr)) {
+        if (rtab)
+            return sv::set_error(SV_EUNSUPPORTED, "per-pair centres need the constant-centre "
+                                 "kernels (integer u8 path, 2R + 1 <= 125, W >= R in lr mode)");
+        pl.Kp = pl.G * pl.KC;
+        RefineBufs bf = carve(ws, pl, V, B, (long long)W * H, lr, false);
+        const size_t need = bf.bytes + sv::align_up((size_t)n * sizeof(double));
+        if ((rc = check_ws(need, ws, ws_bytes))) return rc;
+        double* d0 = reinterpret_cast<double*>((char*)ws + bf.bytes);
+        hipLaunchKernelGGL(k_fill_f64, fgrid, dim3(256), 0, st, d0, n, (double)R, nullptr, g.slot);
+        SV_CHECK_LAUNCH();
+        if (!lr)
+            return refine_views(pl, bf, left, right, dtype, d0, nullptr, 1, g, R, P1, P2, 0.0, 20.0,
+                                0.0, rx, ry, 0.0, 4.0, out, conf, st);
+        return lr_core(left, right, dtype, d0, g, tol, R, P1, P2, 0.0, 20.0, paths, 0.0, rx, ry, 0.0,
+                       4.0, out, status, bf.cen, bf.bytes, false, st);
+    }
+    const int kb = cb_bucket(pl.K), Kp = cb_kp(kb);
+    CbBufs bf = carve_cb(ws, Kp, pl.ndir, V, B, (long long)W * H, lr);
+    if ((rc = check_ws(bf.bytes, ws, ws_bytes))) return rc;
+    if (!lr)
+        return refine_views_cb(bf, left, right, dtype, 1, g, R, rtab, kb, Kp, pl.ndir, (int)P1,
+                               (int)P2, rx, ry, out, conf, st);
+    {
+        StageTimer t(ST_LR, st);
+        hipLaunchKernelGGL(k_fill_f64, fgrid, dim3(256), 0, st, bf.d0, n, (double)R, rtab, g.slot);
+        launch_l2r(bf.d0, g, bf.d0r, st);
+        SV_CHECK_LAUNCH();
+    }
+    rc = refine_views_cb(bf, left, right, dtype, 2, g, R, rtab, kb, Kp, pl.ndir, (int)P1, (int)P2,
+                         rx, ry, bf.disp, nullptr, st);
+    if (rc) return rc;
+    StageTimer t(ST_LR, st);
+    launch_lr_check(bf.disp, bf.disp + (size_t)n, bf.d0, g, tol, out, status, st);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
 }
 
 // Ragged batches: pair n is dims[n] = (h, w) (device int32 pairs, every h <=

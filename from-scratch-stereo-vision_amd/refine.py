@@ -188,6 +188,94 @@ def refine_fullres(left, right, disp0, W, H, R=3, P1=1.0, P2=12.0,
     return _ret(out, kind, shape)
 
 
+def full_range(left, right, W, H, R, P1=12.0, P2=150.0, rx=4, ry=3, paths=8, lr=False,
+               tol=1.0, return_conf=False):
+    """Full-range SGM, the control of eval_real.py:143-146: equals
+    refine_fullres(left, right, [float(R)] * (W*H), W, H, R=R, P1=P1, P2=P2,
+    rx=rx, ry=ry, paths=paths) (and with return_conf its conf), or with
+    lr=True lr_refine(..., tol=tol) on that constant centre, returning
+    (disparity, status).  R may be one int or one per pair of a batch
+    (B, H, W): the pairs then run in one launch when the constant-centre
+    kernels apply, else in one call per distinct R.  The centre map is never
+    built on the host."""
+    kind = _kind(left)
+    l, dt = _image(left)
+    r, dt2 = _image(right)
+    if dt2 != dt:
+        r = r.to(l.dtype)
+    B = _batch(l, W, H)
+    if r.numel() != l.numel():
+        raise ValueError("left and right must have the same size")
+    shape = _shape_of(left)
+    per_pair = not isinstance(R, (int, np.integer))
+    Rs = [int(v) for v in R] if per_pair else [int(R)] * B
+    if len(Rs) != B or min(Rs) < 0:
+        raise ValueError("R must be a non-negative int or one per pair (%d pairs)" % B)
+    Rmax = max(Rs)
+    mode = 1 if lr else 0
+    out = torch.empty(B * W * H, dtype=torch.float64, device=l.device)
+    conf = torch.empty_like(out) if (return_conf and not lr) else None
+    status = torch.empty(B * W * H, dtype=torch.uint8, device=l.device) if lr else None
+    fast = ctypes.c_int(0)
+    L.check(L.lib().sv_full_range_fast(mode, W, H, Rmax, float(P1), float(P2), int(paths), rx, ry,
+                                       ctypes.byref(fast)))
+    uniform = len(set(Rs)) == 1
+    if uniform or fast.value:
+        rtab = None if uniform else _rtab(Rs, l.device)
+        groups = [(list(range(B)), Rmax, rtab)]
+    else:
+        groups = [([i for i in range(B) if Rs[i] == v], v, None) for v in sorted(set(Rs))]
+    hw = W * H
+    l2, r2 = l.reshape(B, hw), r.reshape(B, hw)
+    o2 = out.reshape(B, hw)
+    for idx, Rg, rtab in groups:
+        whole = len(idx) == B
+        sl, sr = (l2, r2) if whole else (l2[idx].contiguous(), r2[idx].contiguous())
+        go = o2 if whole else torch.empty(len(idx), hw, dtype=torch.float64, device=l.device)
+        gc = None if conf is None else (conf.reshape(B, hw) if whole else torch.empty_like(go))
+        gs = None if status is None else (status.reshape(B, hw) if whole else
+                                          torch.empty(len(idx), hw, dtype=torch.uint8,
+                                                      device=l.device))
+        n = ctypes.c_size_t(0)
+        L.check(L.lib().sv_full_range_workspace_bytes(mode, len(idx), W, H, Rg, float(P1),
+                                                      float(P2), int(paths), rx, ry,
+                                                      ctypes.byref(n)))
+        ws = L.workspace.get(n.value)
+        L.check(L.lib().sv_full_range(mode, L.ptr(sl), L.ptr(sr), dt, len(idx), W, H, Rg,
+                                      L.ptr(rtab), float(P1), float(P2), int(paths), rx, ry,
+                                      float(tol), L.ptr(go), L.ptr(gc), L.ptr(gs), L.ptr(ws),
+                                      n.value, L.stream()))
+        if not whole:
+            ix = torch.as_tensor(idx, device=l.device)
+            o2[ix] = go
+            if gc is not None:
+                conf.reshape(B, hw)[ix] = gc
+            if gs is not None:
+                status.reshape(B, hw)[ix] = gs
+    if lr:
+        st = status if kind == "torch" else status.to(torch.int32)
+        return _ret(out, kind, shape), _ret(st, kind, shape)
+    if return_conf:
+        return _ret(out, kind, shape), _ret(conf, kind, shape)
+    return _ret(out, kind, shape)
+
+
+_RTABS = {}
+
+
+def _rtab(Rs, dev):
+    """Device int32 table of per-pair centres, cached (a host-to-device copy
+    synchronises)."""
+    key = (tuple(Rs), str(dev))
+    t = _RTABS.get(key)
+    if t is None:
+        if len(_RTABS) > 256:
+            _RTABS.clear()
+        t = torch.tensor(Rs, dtype=torch.int32, device=dev)
+        _RTABS[key] = t
+    return t
+
+
 def _mirror1(img, W, H):
     """_mirror1, stereonet.py:5308-5314 (horizontal flip)."""
     kind, shape = _kind(img), _shape_of(img)

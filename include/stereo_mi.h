@@ -86,6 +86,26 @@ int sv_refine_fullres(const void* left, const void* right, int dtype, const doub
                       double prior_t, double* out, double* conf, void* ws, size_t ws_bytes,
                       void* stream);
 
+/* Full-range SGM, the control of eval_real.py:143-146:
+ * refine_fullres(left, right, [float(R)] * (W*H), W, H, R=R, P1, P2, rx, ry)
+ * (mode 0; conf as return_conf=True when not NULL) or lr_refine with that
+ * constant centre (mode 1; status [B][H][W] u8).  Results equal
+ * sv_refine_fullres / sv_lr_refine given the constant map.  The integer u8
+ * path with 2R + 1 <= 125 (and W >= R in mode 1) runs constant-centre
+ * kernels, which never read a centre map (sv_full_range_fast says which);
+ * anything else the general kernels.  rtab (device int32 [B], may be NULL)
+ * gives pair n its own centre in [0, R] within one launch (constant-centre
+ * kernels only; R sizes the band).  No counterpart function in the
+ * reference: it replaces the harness's refine_fullres call per pair. */
+int sv_full_range_workspace_bytes(int mode, int B, int W, int H, int R, double P1, double P2,
+                                  int paths, int rx, int ry, size_t* bytes);
+int sv_full_range_fast(int mode, int W, int H, int R, double P1, double P2, int paths, int rx,
+                       int ry, int* fast);
+int sv_full_range(int mode, const void* left, const void* right, int dtype, int B, int W, int H,
+                  int R, const int* rtab, double P1, double P2, int paths, int rx, int ry,
+                  double tol, double* out, double* conf, uint8_t* status, void* ws,
+                  size_t ws_bytes, void* stream);
+
 /* left_to_right(dl, W, H), stereonet.py:5317-5340. */
 int sv_left_to_right(const double* dl, int B, int W, int H, double* dr, void* stream);
 
