@@ -33,7 +33,8 @@ KNOBS = {"sgm_prefetch": (0, (2, 4, 8)),
          "sgm_overlap": (5, (2, 1)),
          "conv_engine": (6, (1, 2, 3, 4)),
          "conv_mgroups": (7, (1, 2, 4)),
-         "sgm_split": (8, (1, 2))}
+         "sgm_split": (8, (1, 2)),
+         "wta_reads": (9, (1, 2))}
 
 
 def set_knob(name, value):

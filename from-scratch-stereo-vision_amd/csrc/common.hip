@@ -46,7 +46,7 @@ int sv_version(void) { return 1; }
 int sv_tune_set(int knob, int value) {
     static const int allowed[SV_TUNE_COUNT][4] = {
         {2, 4, 8, -1}, {8, 16, -1, -1}, {1, 4, -1, -1}, {16, 32, 64, -1}, {1, 2, -1, -1},
-        {1, 2, -1, -1}, {1, 2, 3, 4}, {1, 2, 4, -1}, {1, 2, -1, -1}};
+        {1, 2, -1, -1}, {1, 2, 3, 4}, {1, 2, 4, -1}, {1, 2, -1, -1}, {1, 2, -1, -1}};
     if (knob < 0 || knob >= SV_TUNE_COUNT)
         return sv::set_error(SV_EINVAL, "tuning knob %d does not exist", knob);
     bool ok = value == 0;

@@ -1677,70 +1677,104 @@ __global__ void __launch_bounds__(256) k_costs_cb(const uint64_t* __restrict__ c
     for (int u = lane; u < units; u += 64) dst[u] = src[u];
 }
 
-// refine_fullres epilogue (stereonet.py:5271-5302) for constant-centre
-// bands of K <= KB entries: the NDIR u8 path costs of a pixel (Kp bytes
-// each) are summed as packed u16 pairs (at most 8 * 255), then first
-// minimum over the K real entries, band-edge passthrough (the centre R, the
-// value of disp0 everywhere), second best outside bk +- 1 and parabola as in
-// k_wta_k.  The output is base - R + bk + off with base = R, i.e. bk + off.
-template <int KB, int NDIR>
-__global__ void __launch_bounds__(64) k_wta_cb(const uint8_t* __restrict__ Lv, long long dirStride,
-                                               Geom g, int K, const int* __restrict__ rtab,
-                                               int Kp, int R, double* __restrict__ out,
-                                               double* __restrict__ conf) {
-    constexpr int NU = PackedBand<KB>::NB;   // 8-byte units per pixel and path
-    constexpr int NPR = 4 * NU;              // u16 pairs
+// The refine_fullres epilogue (stereonet.py:5271-5302) on u8 path costs of
+// KP bytes per pixel with coalesced reads.  A wave takes 64 consecutive
+// pixels of one image; for each path their 64 * KP bytes are one contiguous
+// run, which lane l reads as the 4-byte words l, l + 64, ...  Each lane sums
+// its words over the paths as packed u16 (at most 8 * 255); the sums go
+// through LDS so that lane l then holds pixel l's KP sums, and the
+// selection follows k_wta_k: first minimum over the K real entries,
+// band-edge passthrough (disp0; the centre R itself for constant centres),
+// second best outside bk +- 1, confidence, uniqueness, parabola, output
+// base - R + bk + off (CB: base = R, so bk + off).  Integer sums do not
+// depend on the order of the paths, so the mirrored view needs no
+// reordering.
+template <int KP, int NDIR, bool CB>
+__global__ void __launch_bounds__(64) k_wta_lds(const uint8_t* __restrict__ Lv, long long dirStride,
+                                                const int* __restrict__ base,
+                                                const double* __restrict__ d0a,
+                                                const double* __restrict__ d0b, Geom g, int K,
+                                                const int* __restrict__ rtab, int R, double uniq,
+                                                double* __restrict__ out,
+                                                double* __restrict__ conf) {
+    static_assert(KP % 4 == 0, "whole words per pixel");
+    constexpr int NWD = KP / 4;          // words per pixel and path
+    constexpr int NPR = KP / 2;          // u16 pairs per pixel
+    constexpr int SP = NPR | 1;          // LDS row stride in pairs (odd: no bank conflicts)
+    __shared__ uint32_t sums[64 * SP];
     const int bv = blockIdx.y;
-    const int p = blockIdx.x * 64 + threadIdx.x;
-    if (p >= g.w(bv) * g.h(bv)) return;
-    if (rtab) {  // this pair's own centre and band length
+    const int HW = g.w(bv) * g.h(bv);
+    const int p0 = blockIdx.x * 64;
+    if (p0 >= HW) return;
+    if (CB && rtab) {  // this pair's own centre and band length
         R = rtab[bv % g.B];
         K = 2 * R + 1;
     }
-    const long long pix = (long long)bv * g.slot + p;
-    u16x2 S[NPR];
+    const int np = min(64, HW - p0);
+    const int nwords = np * NWD;
+    const int lane = threadIdx.x;
+    const long long pix0 = (long long)bv * g.slot + p0;
+    u16x2 S[2 * NWD];
 #pragma unroll
-    for (int j = 0; j < NPR; ++j) S[j] = (u16x2){0, 0};
+    for (int j = 0; j < 2 * NWD; ++j) S[j] = (u16x2){0, 0};
 #pragma unroll
     for (int t0 = 0; t0 < NDIR; t0 += 4) {
-        uint2 raw[4][NU];
+        uint32_t raw[4][NWD];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            const uint2* src = reinterpret_cast<const uint2*>(Lv + (long long)(t0 + t) * dirStride + pix * Kp);
+            const uint32_t* src = reinterpret_cast<const uint32_t*>(Lv + (long long)(t0 + t) * dirStride + pix0 * KP);
 #pragma unroll
-            for (int q = 0; q < NU; ++q) raw[t][q] = src[q];
+            for (int i = 0; i < NWD; ++i) {
+                const int w = lane + 64 * i;
+                raw[t][i] = w < nwords ? src[w] : 0u;
+            }
         }
 #pragma unroll
         for (int t = 0; t < 4; ++t)
 #pragma unroll
-            for (int q = 0; q < NU; ++q) {
-                const uint32_t w2[2] = {raw[t][q].x, raw[t][q].y};
-#pragma unroll
-                for (int hh = 0; hh < 2; ++hh) {
-                    S[4 * q + 2 * hh] += as_u16x2(__builtin_amdgcn_perm(0u, w2[hh], 0x0c010c00u));
-                    S[4 * q + 2 * hh + 1] += as_u16x2(__builtin_amdgcn_perm(0u, w2[hh], 0x0c030c02u));
-                }
+            for (int i = 0; i < NWD; ++i) {
+                S[2 * i] += as_u16x2(__builtin_amdgcn_perm(0u, raw[t][i], 0x0c010c00u));
+                S[2 * i + 1] += as_u16x2(__builtin_amdgcn_perm(0u, raw[t][i], 0x0c030c02u));
             }
     }
-    // entries at or past K out of every minimum
+#pragma unroll
+    for (int i = 0; i < NWD; ++i) {
+        const int w = lane + 64 * i;
+        if (w < nwords) {
+            const int pw = w / NWD;
+            const int j = 2 * (w - pw * NWD);
+            sums[pw * SP + j] = as_u32(S[2 * i]);
+            sums[pw * SP + j + 1] = as_u32(S[2 * i + 1]);
+        }
+    }
+    __syncthreads();
+    if (lane >= np) return;
+    u16x2 T[NPR];
+#pragma unroll
+    for (int j = 0; j < NPR; ++j) T[j] = as_u16x2(sums[lane * SP + j]);
 #pragma unroll
     for (int j = 0; j < NPR; ++j) {
-        if (2 * j >= K) S[j].x = 0xFFFF;
-        if (2 * j + 1 >= K) S[j].y = 0xFFFF;
+        if (2 * j >= K) T[j].x = 0xFFFF;
+        if (2 * j + 1 >= K) T[j].y = 0xFFFF;
     }
-    u16x2 mm = S[0];
+    u16x2 mm = T[0];
 #pragma unroll
-    for (int j = 1; j < NPR; ++j) mm = __builtin_elementwise_min(mm, S[j]);
+    for (int j = 1; j < NPR; ++j) mm = __builtin_elementwise_min(mm, T[j]);
     const int bvv = min((int)mm.x, (int)mm.y);
-    int bk = 2 * NPR;
+    int bk = KP;
 #pragma unroll
     for (int j = NPR - 1; j >= 0; --j) {
-        if ((int)S[j].y == bvv) bk = 2 * j + 1;
-        if ((int)S[j].x == bvv) bk = 2 * j;
+        if ((int)T[j].y == bvv) bk = 2 * j + 1;
+        if ((int)T[j].x == bvv) bk = 2 * j;
     }
+    const long long pix = pix0 + lane;
     double o, cf;
+    const int v = bv >= g.B ? 1 : 0;
+    auto disp0 = [&]() {
+        return CB ? (double)R : (v ? d0b : d0a)[(long long)(bv - v * g.B) * g.slot + p0 + lane];
+    };
     if (bk == 0 || bk == K - 1) {
-        o = (double)R;
+        o = disp0();
         cf = 0.0;
     } else {
         int sv = 0x7FFFFFFF, a = 0, bb = 0;
@@ -1750,22 +1784,26 @@ __global__ void __launch_bounds__(64) k_wta_cb(const uint8_t* __restrict__ Lv, l
 #pragma unroll
             for (int e = 0; e < 2; ++e) {
                 const int k = 2 * j + e;
-                const int v = e ? (int)S[j].y : (int)S[j].x;
+                const int val = e ? (int)T[j].y : (int)T[j].x;
                 if (k < K && (k < bk - 1 || k > bk + 1)) {
                     found = true;
-                    sv = min(sv, v);
+                    sv = min(sv, val);
                 }
-                if (k == bk - 1) a = v;
-                if (k == bk + 1) bb = v;
+                if (k == bk - 1) a = val;
+                if (k == bk + 1) bb = val;
             }
         }
         const double bvd = (double)bvv;
         cf = found ? ((double)sv - bvd) / (bvd + 1.0e-6) : 1.0;
-        const double ad = (double)a, bd = (double)bb;
-        const double den = ad - 2.0 * bvd + bd;
-        double off = 0.0;
-        if (den > 1.0e-9) off = 0.5 * (ad - bd) / den;
-        o = (double)bk + off;
+        if (cf < uniq) {
+            o = disp0();
+        } else {
+            const double ad = (double)a, bd = (double)bb;
+            const double den = ad - 2.0 * bvd + bd;
+            double off = 0.0;
+            if (den > 1.0e-9) off = 0.5 * (ad - bd) / den;
+            o = (double)((CB ? R : base[pix]) - R + bk) + off;
+        }
     }
     out[pix] = o;
     if (conf) conf[pix] = cf;
@@ -2502,6 +2540,19 @@ int wta_k_t(const Plan& pl, const void* L, int BV, int mirrorFrom, const int* ba
 int launch_wta(const Plan& pl, const void* L, int BV, int mirrorFrom, const int* base,
                const double* d0a, const double* d0b, const Geom& g, int R, double uniq,
                double* out, double* conf, hipStream_t st) {
+    if (has_k_specialisation(pl) && pl.K == 21 && pl.Kp == 24 && pl.lt == 1 &&
+        sv::tune(SV_TUNE_WTA_READS) == 2) {
+        const long long dirStride = (long long)BV * g.slot * pl.Kp;
+        dim3 grid((unsigned)((g.slot + 63) / 64), BV);
+        if (pl.ndir == 8)
+            hipLaunchKernelGGL((k_wta_lds<24, 8, false>), grid, dim3(64), 0, st, (const uint8_t*)L,
+                               dirStride, base, d0a, d0b, g, pl.K, nullptr, R, uniq, out, conf);
+        else
+            hipLaunchKernelGGL((k_wta_lds<24, 4, false>), grid, dim3(64), 0, st, (const uint8_t*)L,
+                               dirStride, base, d0a, d0b, g, pl.K, nullptr, R, uniq, out, conf);
+        SV_CHECK_LAUNCH();
+        return SV_OK;
+    }
     if (has_k_specialisation(pl)) {
         if (pl.K == 21)
             return pl.lt == 1 ? wta_k_t<21, uint8_t>(pl, L, BV, mirrorFrom, base, d0a, d0b, g, R, uniq, out, conf, st)
@@ -2686,14 +2737,16 @@ int sgm_cb_t(const uint8_t* C, uint8_t* L, int BV, const Geom& g, int Kp, int P1
 template <int KB>
 int wta_cb_t(const uint8_t* L, int BV, int ndir, const Geom& g, int K, const int* rtab, int Kp,
              int R, double* out, double* conf, hipStream_t st) {
+    constexpr int KP = 8 * PackedBand<KB>::NB;
+    if (Kp != KP) return sv::set_error(SV_EINVAL, "constant-centre bucket %d needs %d-byte rows", KB, KP);
     const long long dirStride = (long long)BV * g.slot * Kp;
     dim3 grid((unsigned)((g.slot + 63) / 64), BV);
     if (ndir == 8)
-        hipLaunchKernelGGL((k_wta_cb<KB, 8>), grid, dim3(64), 0, st, L, dirStride, g, K, rtab, Kp, R,
-                           out, conf);
+        hipLaunchKernelGGL((k_wta_lds<KP, 8, true>), grid, dim3(64), 0, st, L, dirStride, nullptr,
+                           nullptr, nullptr, g, K, rtab, R, 0.0, out, conf);
     else
-        hipLaunchKernelGGL((k_wta_cb<KB, 4>), grid, dim3(64), 0, st, L, dirStride, g, K, rtab, Kp, R,
-                           out, conf);
+        hipLaunchKernelGGL((k_wta_lds<KP, 4, true>), grid, dim3(64), 0, st, L, dirStride, nullptr,
+                           nullptr, nullptr, g, K, rtab, R, 0.0, out, conf);
     SV_CHECK_LAUNCH();
     return SV_OK;
 }

@@ -202,7 +202,9 @@ int sv_refine_ragged(int mode, const void* left, const void* right, int dtype, c
  * cores with one depth per tile, 4 vector ALUs staging one input channel at
  * a time); 7: 64-pixel groups per wave of the
  * matrix-core convolution (1, 2, 4); 8: lanes per scanline of the packed
- * SGM kernels (1, 2).  Value 0 restores the built-in
+ * SGM kernels (1, 2); 9: path-cost reads of the band epilogue (K = 21, u8
+ * path costs): one lane per pixel (1) or coalesced 64-pixel runs summed per
+ * lane and handed out through LDS (2).  Value 0 restores the built-in
  * heuristic (LDS weights; horizontal paths after the others; matrix cores
  * for 4 to 16 output channels, 3D layers sliding over depth chunks; the
  * per-channel vector kernel when the input channels do not fit one staging
