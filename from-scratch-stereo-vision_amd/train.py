@@ -296,7 +296,7 @@ def main(argv=None):
     a = ap.parse_args(argv)
     log = lambda s: print(s, file=sys.stderr, flush=True)
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-    init = a.init or os.path.join(root, "tests", "golden", "stereonet_weights_v5.txt")
+    init = a.init or os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "stereonet_weights_v5.txt")
     W, H = a.width, a.height
     t = time.time()
     # the worker pool forks before this process touches the GPU

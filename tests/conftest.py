@@ -9,6 +9,8 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 GOLDEN = os.path.join(ROOT, "tests", "golden")
+# the reference's v5 checkpoint (stereonet_weights.txt), shipped with the package
+V5 = os.path.join(ROOT, "from-scratch-stereo-vision_amd", "weights", "stereonet_weights_v5.txt")
 
 
 def pytest_configure(config):

@@ -55,7 +55,7 @@ def main(ref_root):
         out[key + "_raw"] = raw
     np.savez_compressed(os.path.join(HERE, "demo_page.npz"), **out)
     shutil.copyfile(os.path.join(ref_root, "stereonet_weights.txt"),
-                    os.path.join(HERE, "stereonet_weights_v5.txt"))
+                    os.path.join(HERE, "..", "..", "from-scratch-stereo-vision_amd", "weights", "stereonet_weights_v5.txt"))
     print("wrote demo_page.npz and stereonet_weights_v5.txt")
 
 

@@ -249,7 +249,7 @@ def consistency(SN):
         v = SN["lr_consistency"](Ld, Rd, W, H, tol)
         out["valid_" + name] = np.array(v, dtype=bool)
         out["fill_" + name] = np.array(SN["fill_invalid"](Ld, v, W, H), dtype=np.float64)
-    net = SN["load_model"](os.path.join(HERE, "stereonet_weights_v5.txt"))
+    net = SN["load_model"](os.path.join(HERE, "..", "..", "from-scratch-stereo-vision_amd", "weights", "stereonet_weights_v5.txt"))
     Wp, Hp = 96, 64
     left, right, _, _ = small_pair(31, Wp, Hp, 3.0, 14.0)
     Lp, Rp = list(map(float, left.ravel())), list(map(float, right.ravel()))

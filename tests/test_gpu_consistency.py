@@ -11,7 +11,7 @@ import pytest
 import torch
 
 import oracle
-from conftest import GOLDEN, package
+from conftest import GOLDEN, package, V5
 
 pytestmark = pytest.mark.gpu
 
@@ -82,11 +82,11 @@ def test_mirror_uint8_batch(sv):
 
 
 def _v5(sv):
-    return sv.load_model(os.path.join(GOLDEN, "stereonet_weights_v5.txt"))
+    return sv.load_model(V5)
 
 
 def _oracle_net():
-    cfg, blocks = oracle.load_weights_txt(os.path.join(GOLDEN, "stereonet_weights_v5.txt"))
+    cfg, blocks = oracle.load_weights_txt(V5)
     return oracle.Arch(cfg["feat_C"], cfg["cost_C"], cfg["D"], cfg["refine_C"],
                        cfg["refine_dil"], cfg["target_peak"]), np.concatenate(blocks)
 

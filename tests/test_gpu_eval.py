@@ -11,7 +11,7 @@ import pytest
 import torch
 
 import oracle
-from conftest import GOLDEN, package
+from conftest import GOLDEN, package, V5
 
 import sys  # noqa: E402
 sys.path.insert(0, GOLDEN)
@@ -128,7 +128,7 @@ def _scene(sv, tmp_path, W=96, H=64):
                                        ("sgm", False)])
 def test_evaluate(sv, tmp_path, mode, cons):
     ev = sv.evaluation
-    net = sv.load_model(os.path.join(GOLDEN, "stereonet_weights_v5.txt"))
+    net = sv.load_model(V5)
     sc = _scene(sv, tmp_path)
     r = ev.evaluate(net, sc, consistency=cons, mode=mode, dump=str(tmp_path / "dump"),
                     save_dir=str(tmp_path / "out"))

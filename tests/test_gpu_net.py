@@ -17,7 +17,7 @@ import pytest
 import torch
 
 import oracle
-from conftest import GOLDEN, package
+from conftest import GOLDEN, package, V5
 
 pytestmark = pytest.mark.gpu
 
@@ -36,7 +36,7 @@ def N(sv):
 
 
 def _v5(N):
-    return N.load_model(os.path.join(GOLDEN, "stereonet_weights_v5.txt"))
+    return N.load_model(V5)
 
 
 def _arch(net, tp=34.0):
@@ -201,7 +201,7 @@ def test_checkpoint_round_trip(sv, N, tmp_path):
     assert torch.equal(net.params, net2.params)
     # the v5 file reads into exactly its float32 values
     v5 = _v5(N)
-    _, blocks = oracle.load_weights_txt(os.path.join(GOLDEN, "stereonet_weights_v5.txt"))
+    _, blocks = oracle.load_weights_txt(V5)
     assert np.array_equal(v5.params.cpu().numpy(), np.concatenate(blocks).astype(np.float32))
 
 

@@ -20,7 +20,7 @@ import torch
 import torch.nn.functional as F
 
 import oracle
-from conftest import GOLDEN
+from conftest import GOLDEN, V5
 
 REFINE_CASES = ["refine_band_r10", "refine_defaults", "refine_fullrange",
                 "refine_float", "refine_paths4", "refine_jumps",
@@ -374,7 +374,7 @@ def test_predict_auto_matches_demo_page(key):
     on real Middlebury pairs as round(d * 64); the oracle must reproduce every
     quantised value."""
     dp = np.load(os.path.join(GOLDEN, "demo_page.npz"))
-    cfg, blocks = oracle.load_weights_txt(os.path.join(GOLDEN, "stereonet_weights_v5.txt"))
+    cfg, blocks = oracle.load_weights_txt(V5)
     P = np.concatenate(blocks)
     assert np.array_equal(P, dp["ckpt_flat"])
     arch = oracle.Arch(cfg["feat_C"], cfg["cost_C"], cfg["D"], cfg["refine_C"],
@@ -430,7 +430,7 @@ def test_predict_consistent_golden(name, ww):
     output: same valid fraction, disparities within 1e-12 px (the oracle's
     libm exp differs from the reference's hand-made one by ~1e-16)."""
     z = _consistency_golden()
-    cfg, blocks = oracle.load_weights_txt(os.path.join(GOLDEN, "stereonet_weights_v5.txt"))
+    cfg, blocks = oracle.load_weights_txt(V5)
     arch = oracle.Arch(cfg["feat_C"], cfg["cost_C"], cfg["D"], cfg["refine_C"],
                        cfg["refine_dil"], cfg["target_peak"])
     Wp, Hp = int(z["Wp"]), int(z["Hp"])
