@@ -48,6 +48,8 @@ SIGNATURES = {
     "sv_profile_read": (_I, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I)]),
     "sv_refine_disparity": (_I, [_P, _P, _I, _P, _I, _I, _I, _I, _D, _D, _I, _I, _D, _P, _P, _P,
                                  _SZ, _P]),
+    "sv_refine_disparity_noalloc": (_I, [_P, _P, _I, _P, _I, _I, _I, _I, _D, _D, _I, _I, _D, _P,
+                                         _P, _P]),
 }
 
 _lib = None

@@ -2128,6 +2128,9 @@ struct ProfPair {
 enum { ST_CENSUS, ST_COSTS, ST_SGM, ST_WTA, ST_LR, ST_MEDIAN, ST_COUNT };
 static const char* kStageNames[ST_COUNT] = {"census", "costs", "sgm", "wta", "lr", "median"};
 static bool g_prof_on = false;
+// set for the duration of an allocation-free call on this thread: no stage
+// events are created and the SGM paths never fork onto the side stream
+static thread_local bool t_noalloc = false;
 static ProfPair* g_prof = nullptr;
 static int g_prof_n = 0, g_prof_cap = 0;
 
@@ -2136,7 +2139,7 @@ struct StageTimer {
     int idx = -1;
     hipStream_t st;
     StageTimer(int stage, hipStream_t s) : st(s) {
-        if (!g_prof_on) return;
+        if (!g_prof_on || t_noalloc) return;
         if (g_prof_n == g_prof_cap) {
             const int cap = g_prof_cap ? 2 * g_prof_cap : 256;
             ProfPair* np = (ProfPair*)realloc(g_prof, sizeof(ProfPair) * cap);
@@ -2408,7 +2411,7 @@ int sgm_k_t(const Plan& pl, const void* C, const int* base, void* L, int BV, con
         // except while the caller's stream is being captured into a graph
         // (the one side stream per device would then join every capturing
         // stream that reaches here; such captures run both kinds in order)
-        bool fork = hEnd > dir0 && vEnd > vBeg && sv::tune(SV_TUNE_SGM_OVERLAP) == 2;
+        bool fork = hEnd > dir0 && vEnd > vBeg && sv::tune(SV_TUNE_SGM_OVERLAP) == 2 && !t_noalloc;
         if (fork) {
             hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
             if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
@@ -2806,6 +2809,39 @@ int refine_views_cb(const CbBufs& bf, const void* left, const void* right, int d
 bool cb_applies(const Plan& pl, int W, int R, bool lr) {
     return pl.integer && pl.lt == 1 && cb_bucket(pl.K) != 0 && (!lr || W >= R);
 }
+
+// --------------------------------------------------- allocation-free calls
+// The workspace of sv_refine_disparity_noalloc: a file-scope device array
+// (each GPU the library runs on holds its own copy until the process ends).
+// Host code never uses its name as an address: the device address comes
+// from hipGetSymbolAddress, once per device, with that device current.
+constexpr size_t kScratchBytes = (size_t)96 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[64];
+
+int scratch_ptr(void** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= 64)
+        return sv::set_error(SV_EHIP, "scratch: no current device");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return sv::set_error(SV_EHIP, "scratch: hipGetSymbolAddress: %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return SV_OK;
+}
+
+struct NoAllocScope {
+    NoAllocScope() { t_noalloc = true; }
+    ~NoAllocScope() { t_noalloc = false; }
+};
 
 }  // namespace
 
@@ -3316,6 +3352,37 @@ int sv_full_range(int mode, const void* left, const void* right, int dtype, int 
     launch_lr_check(bf.disp, bf.disp + (size_t)n, bf.d0, g, tol, out, status, st);
     SV_CHECK_LAUNCH();
     return SV_OK;
+}
+
+// refine_disparity(planes=False) as sv_refine_disparity, with the workspace
+// in the library's static device scratch (kScratchBytes per GPU): no
+// allocation of any kind (no stage events, no side stream).  Sizes whose
+// workspace exceeds the scratch are rejected before anything is queued.
+// Every call on one GPU shares that GPU's scratch: calls that overlap on one
+// GPU (other streams, other threads) race.  No kernel reads scratch memory it
+// has not written in the same call, so nothing is cleared between calls.
+int sv_refine_disparity_noalloc(const void* left, const void* right, int dtype,
+                                const double* disp0, int B, int W, int H, int R, double P1,
+                                double P2, int rx, int ry, double tol, double* out,
+                                uint8_t* status, void* stream) {
+    SV_REQUIRE(left && right && disp0 && out && status && B > 0 && W > 0 && H > 0,
+               "sv_refine_disparity_noalloc: bad arguments");
+    Plan pl{};
+    int rc = make_plan(W, H, R, P1, P2, 0.0, 0.0, 20.0, 4.0, 8, rx, ry, &pl);
+    if (rc) return rc;
+    pl.Kp = pl.G * pl.KC;
+    const size_t need = carve(nullptr, pl, 2, B, (long long)W * H, true, true).bytes;
+    if (need > kScratchBytes)
+        return sv::set_error(SV_EWORKSPACE,
+                             "sv_refine_disparity_noalloc: %zu bytes of workspace needed, the "
+                             "scratch holds %zu (B*W*H too large for R=%d)",
+                             need, kScratchBytes, R);
+    void* ws = nullptr;
+    if ((rc = scratch_ptr(&ws))) return rc;
+    NoAllocScope scope;
+    return lr_core(left, right, dtype, disp0, uniform_geom(B, W, H), tol, R, P1, P2, 0.0, 20.0, 8,
+                   0.0, rx, ry, 0.0, 4.0, out, status, ws, kScratchBytes, true,
+                   (hipStream_t)stream);
 }
 
 // Ragged batches: pair n is dims[n] = (h, w) (device int32 pairs, every h <=

@@ -174,6 +174,21 @@ int sv_refine_disparity(const void* left, const void* right, int dtype, const do
                         double tol, double* out, uint8_t* status, void* ws, size_t ws_bytes,
                         void* stream);
 
+/* No counterpart in the reference: refine_disparity(planes=False) exactly
+ * as sv_refine_disparity, for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * workspace is the library's static device scratch, 96 MiB per GPU, held
+ * until the process ends; no allocation of any kind is made (no stage
+ * events, no side stream).  Limit: the workspace of B*W*H pixels at band
+ * half-width R must fit the scratch (about 488 bytes per pixel at R = 10:
+ * one 460x370 pair, or seven 192x144 pairs); larger calls return
+ * SV_EWORKSPACE before anything is queued.  All calls on one GPU share its
+ * scratch: calls that overlap on one GPU race. */
+int sv_refine_disparity_noalloc(const void* left, const void* right, int dtype,
+                                const double* disp0, int B, int W, int H, int R, double P1,
+                                double P2, int rx, int ry, double tol, double* out,
+                                uint8_t* status, void* stream);
+
 /* The three refinement drivers over a ragged batch (pairs of different
  * sizes in one launch per stage): refine_fullres (mode 0, paths=8,
  * uniq=0), lr_refine (mode 1) and refine_disparity planes=False (mode 2),
