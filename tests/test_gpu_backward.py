@@ -1,0 +1,291 @@
+"""Each backward kernel of the network on its own, against float64
+references, one parameter block (or input gradient) at a time.
+
+References: torch float64 autograd for the convolutions (an independent
+implementation of stereonet.py:528-568 and :643-692 -- the same sums); the
+float64 C oracle for the cost volume, soft-argmin, bilinear upsampling and
+the two losses (pinned by central differences at the reference's own
+tolerances, test_stereonet.py:99-215, in tests/test_oracle.py).
+
+Bound: per block, max |GPU - reference| <= 5e-6 * max(1, max |reference|).
+The reference checks its float64 backward against central differences at
+rel 1e-6 (1e-5 for the cross-entropy); these kernels compute in FP32, whose
+rounding over the few thousand terms of a weight-gradient sum measured at
+most 4e-7 of the block's scale here, while a wrong tap, sign, stride or
+channel gives errors of the gradient's own size.  Every case runs on a ragged batch of two
+samples of different sizes.
+"""
+
+import ctypes
+import importlib
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from conftest import package
+
+pytestmark = pytest.mark.gpu
+
+TOL = 5e-6
+
+
+@pytest.fixture(scope="module")
+def sv():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return package()
+
+
+@pytest.fixture(scope="module")
+def N(sv):
+    return importlib.import_module("from-scratch-stereo-vision_amd.net")
+
+
+def _close(got, want, what):
+    got = np.asarray(got, dtype=np.float64).ravel()
+    want = np.asarray(want, dtype=np.float64).ravel()
+    assert got.shape == want.shape, what
+    err = float(np.abs(got - want).max()) if got.size else 0.0
+    scale = max(1.0, float(np.abs(want).max()) if want.size else 0.0)
+    print("%s: max |err| %.2e (scale %.2f)" % (what, err, scale))
+    assert err <= TOL * scale, "%s: %.3e > %.3e" % (what, err, TOL * scale)
+
+
+def _pack(arrs, stride):
+    """Samples of different sizes into one [N][stride] float32 device buffer."""
+    buf = torch.zeros(len(arrs), stride, dtype=torch.float32)
+    for n, a in enumerate(arrs):
+        a = torch.as_tensor(np.asarray(a, dtype=np.float32).ravel())
+        buf[n, :a.numel()] = a
+    return buf.cuda()
+
+
+SIZES = [(11, 14), (7, 9)]
+
+
+# IC, OC, KD (1: 2D, 3: 3D), stride, dilation, depth
+_CONV = [(2, 3, 1, 2, 1, 1), (2, 2, 1, 1, 2, 1), (2, 2, 1, 1, 1, 1), (1, 12, 1, 2, 1, 1),
+         (12, 12, 1, 2, 1, 1), (8, 8, 1, 1, 4, 1), (8, 1, 1, 1, 1, 1), (2, 2, 3, 1, 1, 4),
+         (6, 6, 3, 1, 1, 5), (6, 1, 3, 1, 1, 5), (48, 24, 3, 1, 1, 3)]
+
+
+@pytest.mark.parametrize("case", _CONV, ids=lambda c: "ic%d_oc%d_kd%d_s%d_d%d_D%d" % c)
+def test_conv_backward_blocks(sv, N, case):
+    """sv_conv_bwd_data (input gradient) and sv_conv_bwd_weight (weight and
+    bias gradients, accumulated) against autograd of F.conv2d / F.conv3d."""
+    import torch.nn.functional as F
+    IC, OC, KD, s, dil, D = case
+    print("conv bwd case", case, flush=True)
+    g = np.random.default_rng(sum(case) * 7 + KD)
+    p = dil if KD == 1 else 1
+    outs = [((h + 2 * p - dil * 2 - 1) // s + 1, (w + 2 * p - dil * 2 - 1) // s + 1) for h, w in SIZES]
+    lvi, lvo = N._Level(SIZES, N.L.device()), N._Level(outs, N.L.device())
+    xs = [g.normal(size=(IC, D, h, w)) for h, w in SIZES]
+    gys = [g.normal(size=(OC, D, ho, wo)) for ho, wo in outs]
+    w = g.normal(size=(OC, IC, KD, 3, 3)) * 0.3 if KD == 3 else g.normal(size=(OC, IC, 3, 3)) * 0.3
+    xstride, gstride = IC * D * lvi.slot, OC * D * lvo.slot
+    xb, gyb = _pack(xs, xstride), _pack(gys, gstride)
+    wd = torch.as_tensor(w.astype(np.float32).ravel()).cuda()
+    gx = torch.full((2, xstride), 9.0, dtype=torch.float32, device="cuda")
+    N._call("sv_conv_bwd_data", N._p(gyb), N._p(lvo.table), gstride, N._p(wd), IC, OC, D, KD, s,
+            p, dil, N._p(gx), N._p(lvi.table), xstride, 2, lvi.maxh, lvi.maxw, 0, N.L.stream())
+    nb = ctypes.c_size_t(0)
+    nchunk = 3
+    N._call("sv_conv_bwd_weight_workspace_bytes", 2, IC, OC, KD, nchunk, ctypes.byref(nb))
+    ws = torch.empty(max(1, nb.value), dtype=torch.uint8, device="cuda")
+    gw = torch.full((w.size,), 0.25, dtype=torch.float32, device="cuda")   # accumulates
+    gb = torch.full((OC,), -0.5, dtype=torch.float32, device="cuda")
+    N._call("sv_conv_bwd_weight", N._p(xb), None, N._p(lvi.table), xstride, IC, D, KD, N._p(gyb),
+            N._p(lvo.table), gstride, OC, s, p, dil, 2, nchunk, N._p(gw), N._p(gb), N._p(ws),
+            nb.value, N.L.stream())
+    torch.cuda.synchronize()
+    wt = torch.tensor(w.astype(np.float32).astype(np.float64), requires_grad=True)
+    bt = torch.zeros(OC, dtype=torch.float64, requires_grad=True)
+    gw_ref = np.zeros(w.size)
+    gb_ref = np.zeros(OC)
+    for n, ((h, wdt), x, gy) in enumerate(zip(SIZES, xs, gys)):
+        xt = torch.tensor(x.astype(np.float32).astype(np.float64), requires_grad=True)
+        gyt = torch.tensor(gy.astype(np.float32).astype(np.float64))
+        if KD == 3:
+            y = F.conv3d(xt[None], wt, bt, padding=1)[0]
+        else:
+            y = F.conv2d(xt[:, 0][None], wt, bt, stride=s, padding=p, dilation=dil)[0][:, None]
+        (y * gyt).sum().backward()
+        got = gx[n, :IC * D * h * wdt].double().cpu().numpy()
+        _close(got, xt.grad.numpy(), "gx sample %d" % n)
+        gw_ref += wt.grad.numpy().ravel()
+        gb_ref += bt.grad.numpy()
+        wt.grad = None
+        bt.grad = None
+    _close(gw.double().cpu().numpy() - 0.25, gw_ref, "gw")
+    _close(gb.double().cpu().numpy() + 0.5, gb_ref, "gb")
+
+
+def test_costvol_weight_grad_fused(sv, N):
+    """The first 3D layer reads the cost volume on the fly (xr = right
+    features): its weight gradient equals autograd through
+    |fl(x) - fr(x - d)| (2.0 where x < d)."""
+    import torch.nn.functional as F
+    C, OC, D = 4, 3, 5
+    g = np.random.default_rng(17)
+    lv = N._Level(SIZES, N.L.device())
+    fls = [g.normal(size=(C, h, w)) for h, w in SIZES]
+    frs = [g.normal(size=(C, h, w)) for h, w in SIZES]
+    gys = [g.normal(size=(OC, D, h, w)) for h, w in SIZES]
+    w = g.normal(size=(OC, C, 3, 3, 3)) * 0.3
+    fst = C * lv.slot
+    flb, frb, gyb = _pack(fls, fst), _pack(frs, fst), _pack(gys, OC * D * lv.slot)
+    nb = ctypes.c_size_t(0)
+    N._call("sv_conv_bwd_weight_workspace_bytes", 2, C, OC, 3, 2, ctypes.byref(nb))
+    ws = torch.empty(max(1, nb.value), dtype=torch.uint8, device="cuda")
+    gw = torch.zeros(w.size, dtype=torch.float32, device="cuda")
+    gb = torch.zeros(OC, dtype=torch.float32, device="cuda")
+    N._call("sv_conv_bwd_weight", N._p(flb), N._p(frb), N._p(lv.table), fst, C, D, 3, N._p(gyb),
+            N._p(lv.table), OC * D * lv.slot, OC, 1, 1, 1, 2, 2, N._p(gw), N._p(gb), N._p(ws),
+            nb.value, N.L.stream())
+    torch.cuda.synchronize()
+    wt = torch.tensor(w.astype(np.float32).astype(np.float64), requires_grad=True)
+    for (h, wd), fl, fr, gy in zip(SIZES, fls, frs, gys):
+        fl = torch.tensor(fl.astype(np.float32).astype(np.float64))
+        fr = torch.tensor(fr.astype(np.float32).astype(np.float64))
+        cv = torch.full((C, D, h, wd), 2.0, dtype=torch.float64)
+        for d in range(D):
+            cv[:, d, :, d:] = (fl[:, :, d:] - fr[:, :, :wd - d]).abs()
+        y = F.conv3d(cv[None], wt, padding=1)[0]
+        (y * torch.tensor(gy.astype(np.float32).astype(np.float64))).sum().backward()
+    _close(gw.double().cpu().numpy(), wt.grad.numpy(), "gw (fused cost volume)")
+
+
+def test_costvol_backward(sv, N):
+    """sv_costvol_bwd against the oracle's CostVolume.backward (:801-825),
+    with some features equal across views (the sign is +1 at a zero
+    difference) and x < d cells (no gradient)."""
+    C, D = 3, 4
+    g = np.random.default_rng(5)
+    lv = N._Level(SIZES, N.L.device())
+    fls = [g.normal(size=(C, h, w)) for h, w in SIZES]
+    frs = [g.normal(size=(C, h, w)) for h, w in SIZES]
+    frs[0][:, :, 1:] = fls[0][:, :, :-1]     # fl(x) == fr(x - 1): zero differences at d = 1
+    gvs = [g.normal(size=(C, D, h, w)) for h, w in SIZES]
+    fst = C * lv.slot
+    flb, frb, gvb = _pack(fls, fst), _pack(frs, fst), _pack(gvs, C * D * lv.slot)
+    gfl = torch.full((2, fst), 9.0, dtype=torch.float32, device="cuda")
+    gfr = torch.full((2, fst), 9.0, dtype=torch.float32, device="cuda")
+    N._call("sv_costvol_bwd", N._p(flb), N._p(frb), N._p(lv.table), fst, C, D, N._p(gvb),
+            C * D * lv.slot, N._p(gfl), N._p(gfr), 2, lv.maxh, lv.maxw, N.L.stream())
+    torch.cuda.synchronize()
+    for n, ((h, w), fl, fr, gv) in enumerate(zip(SIZES, fls, frs, gvs)):
+        f32 = lambda a: a.astype(np.float32).astype(np.float64)
+        rl, rr = oracle.costvol_bwd(f32(fl), f32(fr), C, D, h, w, f32(gv))
+        _close(gfl[n, :C * h * w].double().cpu().numpy(), rl, "gfl sample %d" % n)
+        _close(gfr[n, :C * h * w].double().cpu().numpy(), rr, "gfr sample %d" % n)
+
+
+def test_softargmin_backward(sv, N):
+    """sv_softargmin_fwd/bwd against the oracle's SoftArgmin (:861-901), with
+    and without the extra direct cost gradient of train_grads' CE term."""
+    D = 6
+    g = np.random.default_rng(8)
+    lv = N._Level(SIZES, N.L.device())
+    costs = [g.normal(size=(D, h, w)) * 2.0 for h, w in SIZES]
+    gds = [g.normal(size=(h, w)) for h, w in SIZES]
+    gex = [g.normal(size=(D, h, w)) * 0.1 for h, w in SIZES]
+    cst, dst = D * lv.slot, lv.slot
+    cb, gdb, geb = _pack(costs, cst), _pack(gds, dst), _pack(gex, cst)
+    disp = torch.zeros(2, dst, dtype=torch.float32, device="cuda")
+    probs = torch.zeros(2, cst, dtype=torch.float32, device="cuda")
+    N._call("sv_softargmin_fwd", N._p(cb), N._p(lv.table), cst, D, 0.0, N._p(disp), dst,
+            N._p(probs), 2, lv.slot, N.L.stream())
+    for extra in (None, geb):
+        gc = torch.full((2, cst), 9.0, dtype=torch.float32, device="cuda")
+        N._call("sv_softargmin_bwd", N._p(probs), N._p(disp), N._p(gdb), N._p(extra),
+                N._p(lv.table), cst, D, dst, N._p(gc), 2, lv.slot, N.L.stream())
+        torch.cuda.synchronize()
+        for n, ((h, w), c, gd, ge) in enumerate(zip(SIZES, costs, gds, gex)):
+            f32 = lambda a: a.astype(np.float32).astype(np.float64)
+            rd, rp = oracle.softargmin_fwd(f32(c), D, h, w)
+            _close(disp[n, :h * w].double().cpu().numpy(), rd, "disp sample %d" % n)
+            rg = oracle.softargmin_bwd(rp, rd, D, h, w, f32(gd))
+            if extra is not None:
+                rg = rg + f32(ge).ravel()
+            _close(gc[n, :D * h * w].double().cpu().numpy(), rg,
+                   "gcost sample %d%s" % (n, "" if extra is None else " + extra"))
+
+
+@pytest.mark.parametrize("mul", [1.0, 2.0])
+def test_bilinear_backward(sv, N, mul):
+    """sv_bilinear_bwd against the oracle's BilinearUpsample.backward
+    (:945-973): upsampling x2 (the network's) and an uneven resize, times the
+    disparity multiplier."""
+    g = np.random.default_rng(int(mul * 10))
+    ins = [(5, 7), (4, 4)]
+    outs = [(10, 14), (9, 7)]
+    lvi, lvo = N._Level(ins, N.L.device()), N._Level(outs, N.L.device())
+    gys = [g.normal(size=(ho, wo)) for ho, wo in outs]
+    gyb = _pack(gys, lvo.slot)
+    gx = torch.full((2, lvi.slot), 9.0, dtype=torch.float32, device="cuda")
+    N._call("sv_bilinear_bwd", N._p(gyb), N._p(lvo.table), lvo.slot, 0, 1, N._p(gx),
+            N._p(lvi.table), lvi.slot, mul, 2, lvi.maxh, lvi.maxw, N.L.stream())
+    torch.cuda.synchronize()
+    for n, ((h, w), (ho, wo), gy) in enumerate(zip(ins, outs, gys)):
+        ref = oracle.bilinear_bwd(gy.astype(np.float32).astype(np.float64), 1, h, w, ho, wo) * mul
+        _close(gx[n, :h * w].double().cpu().numpy(), ref, "gx sample %d" % n)
+
+
+def test_losses_and_gradients(sv, N):
+    """sv_pseudo_huber and sv_disparity_ce: per-sample loss (float64
+    reduction) and gradient against the oracle's pseudo_huber (:1326-1351)
+    and disparity_ce (:1274-1323), scaled by weight * gscale / count; a
+    sample with no valid pixel gives loss 0 and gradient 0."""
+    D = 6
+    g = np.random.default_rng(12)
+    sizes = SIZES + [(3, 5)]
+    lv = N._Level(sizes, N.L.device())
+    preds = [g.uniform(0, 5, size=(h, w)) for h, w in sizes]
+    tgts = [g.uniform(0.5, 4.5, size=(h, w)) for h, w in sizes]
+    vals = [g.random(size=(h, w)) > 0.3 for h, w in sizes]
+    vals[2][:] = False
+    probs = []
+    for h, w in sizes:
+        e = np.exp(g.normal(size=(D, h, w)))
+        probs.append(e / e.sum(axis=0))
+    pb, tb = _pack(preds, lv.slot), _pack(tgts, lv.slot)
+    vb = torch.zeros(3, lv.slot, dtype=torch.uint8)
+    for n, v in enumerate(vals):
+        vb[n, :v.size] = torch.as_tensor(v.ravel().astype(np.uint8))
+    vb = vb.cuda()
+    prb = _pack(probs, D * lv.slot)
+    nblk, wgt, gsc = 4, 0.7, 0.25
+    for name in ("ph", "ce"):
+        loss = torch.zeros(3, dtype=torch.float64, device="cuda")
+        part = torch.empty(2 * 3 * nblk, dtype=torch.float64, device="cuda")
+        scale = torch.empty(3, dtype=torch.float32, device="cuda")
+        if name == "ph":
+            grad = torch.full((3, lv.slot), 9.0, dtype=torch.float32, device="cuda")
+            N._call("sv_pseudo_huber", N._p(pb), N._p(tb), N._p(vb), N._p(lv.table), lv.slot, 3,
+                    1.0, wgt, gsc, N._p(grad), N._p(loss), N._p(part), N._p(scale), nblk,
+                    N.L.stream())
+        else:
+            grad = torch.full((3, D * lv.slot), 9.0, dtype=torch.float32, device="cuda")
+            N._call("sv_disparity_ce", N._p(prb), N._p(tb), N._p(vb), N._p(lv.table),
+                    D * lv.slot, lv.slot, D, 0.8, 3, wgt, gsc, N._p(grad), N._p(loss), N._p(part),
+                    N._p(scale), nblk, N.L.stream())
+        torch.cuda.synchronize()
+        for n, (h, w) in enumerate(sizes):
+            f32 = lambda a: a.astype(np.float32).astype(np.float64)
+            if name == "ph":
+                rl, rg, cnt = oracle.pseudo_huber(f32(preds[n]), f32(tgts[n]), vals[n])
+                k = 1
+            else:
+                rl, rg, cnt = oracle.disparity_ce(f32(probs[n]), f32(tgts[n]), vals[n], D, h, w)
+                k = D
+            want_loss = wgt * rl if cnt else 0.0
+            got_loss = float(loss[n])
+            print("%s sample %d: loss %.9f vs %.9f" % (name, n, got_loss, want_loss))
+            assert abs(got_loss - want_loss) <= 1e-6 * max(1.0, abs(want_loss))
+            # the oracle's gradient carries 1/count; the GPU's weight*gscale/count:
+            # compare at unit scale (times count / (weight*gscale))
+            _close(grad[n, :k * h * w].double().cpu().numpy() * (cnt or 1) / (gsc * wgt),
+                   np.asarray(rg) * (cnt or 1), "%s grad sample %d" % (name, n))
