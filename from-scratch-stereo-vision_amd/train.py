@@ -91,9 +91,10 @@ def make_dataset(seeds, W=260, H=192, peak=(22.0, 40.0), kind="mixed", workers=N
 
 def pseudo_labels(left, right, W, H, max_disp, tol=1.0, chunk=64):
     """Full-range SGM labels on both views (lr_refine with the centre R and
-    K = 2R + 1 of eval_real.py:143-146, P1=12, P2=150, census 9x7): returns
-    (disp float32 [N][H][W], valid uint8) on the GPU, valid where the
-    left-right check passes."""
+    K = 2R + 1 of eval_real.py:143-146, P1=12, P2=150, census 9x7, through
+    the constant-centre kernels of refine.full_range): returns (disp float32
+    [N][H][W], valid uint8) on the GPU, valid where the left-right check
+    passes."""
     dev = L.device()
     R = full_range_R(max_disp)
     n = left.shape[0]
@@ -103,8 +104,7 @@ def pseudo_labels(left, right, W, H, max_disp, tol=1.0, chunk=64):
         b = min(n, a + chunk)
         l = torch.as_tensor(left[a:b]).to(dev)
         r = torch.as_tensor(right[a:b]).to(dev)
-        c = torch.full((b - a, H, W), float(R), dtype=torch.float64, device=dev)
-        d, st = RF.lr_refine(l, r, c, W, H, tol=tol, R=R, P1=P1, P2=P2, rx=RX, ry=RY)
+        d, st = RF.full_range(l, r, W, H, R, P1=P1, P2=P2, rx=RX, ry=RY, lr=True, tol=tol)
         disp[a:b] = d.reshape(b - a, H, W).float()
         valid[a:b] = (st.reshape(b - a, H, W) == 0).to(torch.uint8)
     return disp, valid
@@ -162,20 +162,14 @@ def _bad2(pred, gt, valid):
 
 
 def full_range(left, right, W, H, max_disps):
-    """Full-range SGM control per pair (eval_real.py:143-146), pairs grouped
-    by their R.  Returns float64 [N][H][W] on the GPU."""
-    dev = left.device
+    """Full-range SGM control per pair (eval_real.py:143-146: each pair's R
+    from its maximum disparity), all pairs in one launch per stage
+    (refine.full_range with a per-pair centre table).  Returns float64
+    [N][H][W] on the GPU."""
     n = left.shape[0]
-    out = torch.empty(n, H, W, dtype=torch.float64, device=dev)
-    groups = {}
-    for i, m in enumerate(max_disps):
-        groups.setdefault(full_range_R(m), []).append(i)
-    for R, idx in groups.items():
-        it = torch.as_tensor(idx, device=dev)
-        c = torch.full((len(idx), H, W), float(R), dtype=torch.float64, device=dev)
-        o = RF.refine_fullres(left[it], right[it], c, W, H, R=R, P1=P1, P2=P2, rx=RX, ry=RY)
-        out[it] = o.reshape(len(idx), H, W)
-    return out
+    Rs = [full_range_R(m) for m in max_disps]
+    out = RF.full_range(left, right, W, H, Rs, P1=P1, P2=P2, rx=RX, ry=RY)
+    return out.reshape(n, H, W)
 
 
 class FallbackResult:
