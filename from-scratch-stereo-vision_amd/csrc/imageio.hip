@@ -355,13 +355,13 @@ bool png(const uint8_t* d, size_t n, Decoded& o, std::string& err, bool header_o
     std::vector<uint8_t> idat;
     bool ihdr = false;
     while (pos + 8 <= n) {
-        const uint32_t len = be32(d + pos);
+        // a chunk running past the end of the file keeps the bytes that are
+        // present (read_png slices the body and carries on, stereonet.py:
+        // 3521-3524); missing image data fails later as too short
+        const uint32_t len = (uint32_t)std::min<size_t>(be32(d + pos), n - pos - 8);
+        const size_t step = 12 + (size_t)be32(d + pos);
         const uint8_t* type = d + pos + 4;
         const uint8_t* body = d + pos + 8;
-        if (pos + 12 + (size_t)len > n) {
-            err = "truncated PNG chunk";
-            return false;
-        }
         if (!std::memcmp(type, "IHDR", 4) && len >= 13) {
             o.W = (int)be32(body);
             o.H = (int)be32(body + 4);
@@ -378,7 +378,7 @@ bool png(const uint8_t* d, size_t n, Decoded& o, std::string& err, bool header_o
         } else if (!std::memcmp(type, "IEND", 4)) {
             break;
         }
-        pos += 12 + (size_t)len;
+        pos += step;
     }
     if (!ihdr || o.W <= 0 || o.H <= 0) {
         err = "PNG without a valid IHDR";

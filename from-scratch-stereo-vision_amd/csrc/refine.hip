@@ -2203,8 +2203,9 @@ int make_plan(int W, int H, int R, double P1, double P2, double ad_weight, doubl
     pl->K = 2 * R + 1;
     pl->ndir = paths == 8 ? 8 : 4;
     pl->invalid = (double)pl->nbits + ad_weight * ad_trunc + prior_w * prior_t + 4.0;
+    // integer path: every path cost L <= invalid + P2 must fit u16 storage
     const bool integral = P1 == std::floor(P1) && P2 == std::floor(P2) && P1 >= 0 && P2 >= 0 &&
-                          P1 <= 65535 && P2 <= 65535;
+                          P1 <= 65535 && pl->invalid + P2 <= 65535.0;
     pl->integer = ad_weight == 0.0 && prior_w == 0.0 && integral;
     if (pl->integer) {
         const double lmax = pl->invalid + P2;  // L <= C + P2

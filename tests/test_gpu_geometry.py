@@ -189,3 +189,17 @@ def test_row_min_with_nans(sv):
         m0, i0 = oracle.min_filter_arg(v.ravel(), W, H, r, 0)
         assert np.array_equal(m.cpu().numpy(), m0, equal_nan=True)
         assert np.array_equal(i.cpu().numpy(), i0)
+
+
+@pytest.mark.parametrize("r", [31, 32, 33, 40])
+@pytest.mark.parametrize("W,H", [(300, 70), (97, 45)])
+def test_smooth_background_both_column_paths(sv, r, W, H):
+    """Column smoothing runs from an LDS tile for r <= 32 and from global
+    memory beyond (background_region's radius reaches 33 at W >= 1056):
+    both paths equal the oracle, on heights that are not multiples of 32."""
+    rng = np.random.default_rng(r * 1000 + W)
+    v = np.round(rng.normal(10, 3, W * H))
+    t = torch.from_numpy(v).cuda()
+    m0, _ = oracle.min_filter_arg(v, W, H, 9, 0)
+    s = sv.smooth_background(torch.from_numpy(m0).cuda(), t, W, H, r)
+    assert np.array_equal(s.cpu().numpy(), oracle.smooth_background(m0, v, W, H, r))

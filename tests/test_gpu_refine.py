@@ -195,6 +195,20 @@ def test_u16_path_large_p2(sv):
     assert np.array_equal(out.cpu().numpy().ravel(), oracle.refine_fullres(gl, gr, p["disp0"], W, H, **kw))
 
 
+@pytest.mark.parametrize("P2", [65469.0, 65470.0, 70000.0])
+def test_u16_path_limit(sv, P2):
+    """Path costs reach invalid + P2 (invalid = 66 for the 9x7 census): up to
+    65535 they stay on the u16 integer path, beyond it the float64 path
+    takes over; both exact against the oracle."""
+    p = sv.synth.band_pair(6, W=48, H=24)
+    W, H = 48, 24
+    gl, gr = sv.synth.gray(p["left"]), sv.synth.gray(p["right"])
+    kw = dict(R=5, P1=7.0, P2=P2, rx=4, ry=3)
+    out = sv.refine_fullres(_cu(p["left"], torch.uint8), _cu(p["right"], torch.uint8),
+                            _cu(p["disp0"]), W, H, **kw)
+    assert np.array_equal(out.cpu().numpy().ravel(), oracle.refine_fullres(gl, gr, p["disp0"], W, H, **kw))
+
+
 @pytest.mark.parametrize("W,H", [(460, 370), (203, 77), (64, 32), (65, 33), (7, 5), (9, 1),
                                  (1, 6), (4, 40), (130, 2)])
 def test_census_9x7_uint8_matches_oracle(sv, W, H):

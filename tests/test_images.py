@@ -104,6 +104,30 @@ def test_host_decoder_large(ct, depth, comp):
     assert np.array_equal(g, g0) and np.array_equal(r, r0)
 
 
+def test_truncated_trailing_chunks_decode_like_the_reference():
+    """read_png walks chunks by slicing (stereonet.py:3519-3537): a chunk that
+    runs past the end of the file keeps the bytes that are present.  A file
+    whose IEND lacks its CRC, or whose last ancillary chunk is cut short,
+    decodes to the golden output of the complete file; an IDAT cut short
+    fails only as missing image data."""
+    im = _images()
+    z = _golden()
+    for n in ("png_c2_d8_dynamic", "png_c3_d4_fixed", "png_c0_d16_stored"):
+        f = z["file_" + n].tobytes()
+        assert f[-12:-8] == b"\x00\x00\x00\x00" and f[-8:-4] == b"IEND"
+        cut = f[:-4]                                      # IEND without its CRC
+        extra = f[:-12] + make_images._chunk(b"tEXt", b"comment" * 9)[:30]
+        for blob in (cut, extra):
+            s, mode, maxval, W, H = im._decode_bytes(blob, n)
+            g, r = _implied(s, mode, maxval)
+            assert [W, H] == list(z["wh_" + n]), n
+            assert np.array_equal(g, z["gray_" + n]) and np.array_equal(r, z["rgb_" + n]), n
+    f = z["file_png_c2_d8_stored"].tobytes()
+    i = f.index(b"IDAT")
+    with pytest.raises(ValueError, match="too short|truncated|DEFLATE|zlib"):
+        im._decode_bytes(f[:i + 4 + 40], "cut idat")
+
+
 def _png(ihdr, idat, plte=None):
     c = make_images._chunk
     b = b"\x89PNG\r\n\x1a\n" + c(b"IHDR", ihdr)
