@@ -628,9 +628,93 @@ __global__ void k_softargmin_fwd(const float* __restrict__ cost, const int* __re
     for (int d = 0; d < D; ++d) {
         const float pv = ps[(long long)d * plane + i] * inv;
         ps[(long long)d * plane + i] = pv;
-        acc += (float)d * pv;
+        acc = __builtin_fmaf((float)d, pv, acc);
     }
     disp[(long long)n * dstride + i] = acc;
+}
+
+// The last cost-filter layer (Conv3d IC -> 1, 3x3x3, no activation,
+// stereonet.py:1079-1094) fused with SoftArgmin.forward and the invalid
+// mask (:861-889, :827-840).  A block makes a 16 x 16 pixel tile of one
+// sample, a thread one pixel with all D output depths in registers, so the
+// filtered costs never leave the kernel.  The input tile of one channel
+// (every depth, one-pixel halo, zero outside) is staged in LDS at a time.
+// The sums follow the convolution engines' order -- input channel, depth
+// tap, row, column, one fmaf per term (zero-padding terms add nothing and
+// are skipped), bias last -- and the soft-argmin is k_softargmin_fwd's, so
+// disparities and probabilities equal the two unfused launches bit for bit.
+template <int D>
+__global__ void __launch_bounds__(256) k_conv3_softargmin(
+    const float* __restrict__ x, const int* __restrict__ dims, long long xstride, int IC,
+    const float* __restrict__ w, const float* __restrict__ b, float penalty,
+    float* __restrict__ disp, long long dstride, float* __restrict__ probs, long long pstride) {
+    constexpr int T = 16, TH = T + 2, PL = TH * TH;
+    __shared__ float tile[D * PL];
+    const int n = blockIdx.z;
+    const int h = dh(dims, n), wd = dw(dims, n);
+    const int tx0 = blockIdx.x * T, ty0 = blockIdx.y * T;
+    if (tx0 >= wd || ty0 >= h) return;  // the whole block lies outside this sample
+    const int tx = threadIdx.x & (T - 1), ty = threadIdx.x / T;
+    const long long plane = (long long)h * wd;
+    const float* xs = x + (long long)n * xstride;
+    float acc[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) acc[d] = 0.f;
+    for (int c = 0; c < IC; ++c) {
+        for (int e = threadIdx.x; e < D * PL; e += 256) {
+            const int dd = e / PL;
+            const int r = e - dd * PL;
+            const int yy = r / TH, xx = r - yy * TH;
+            const int iy = ty0 - 1 + yy, ix = tx0 - 1 + xx;
+            tile[e] = (iy >= 0 && iy < h && ix >= 0 && ix < wd)
+                          ? xs[((long long)c * D + dd) * plane + (long long)iy * wd + ix] : 0.f;
+        }
+        __syncthreads();
+        const float* wc = w + c * 27;
+        const float* tp = tile + ty * TH + tx;
+#pragma unroll
+        for (int od = 0; od < D; ++od)
+#pragma unroll
+            for (int kd = 0; kd < 3; ++kd) {
+                const int dd = od + kd - 1;
+                if (dd < 0 || dd >= D) continue;
+#pragma unroll
+                for (int ky = 0; ky < 3; ++ky)
+#pragma unroll
+                    for (int kx = 0; kx < 3; ++kx)
+                        acc[od] = __builtin_fmaf(tp[dd * PL + ky * TH + kx], wc[(kd * 3 + ky) * 3 + kx],
+                                                 acc[od]);
+            }
+        __syncthreads();
+    }
+    const int ox = tx0 + tx, oy = ty0 + ty;
+    if (ox >= wd || oy >= h) return;
+    const float bias = b[0];
+    float cst[D];
+    float best = -INFINITY;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        cst[d] = (acc[d] + bias) + ((d >= 1 && ox < d) ? penalty : 0.f);
+        best = fmaxf(best, -cst[d]);
+    }
+    float e[D];
+    float total = 0.f;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        e[d] = __expf(-cst[d] - best);
+        total += e[d];
+    }
+    const float inv = 1.f / total;
+    const long long i = (long long)oy * wd + ox;
+    float* ps = probs + (long long)n * pstride + i;
+    float dsum = 0.f;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const float pv = e[d] * inv;
+        ps[(long long)d * plane] = pv;
+        dsum = __builtin_fmaf((float)d, pv, dsum);
+    }
+    disp[(long long)n * dstride + i] = dsum;
 }
 
 // SoftArgmin.backward (:891-901) plus an extra cost gradient (the
@@ -678,6 +762,25 @@ __device__ __forceinline__ void tap_sc(int o, int in, double sc, int& i0, int& i
     wt = (float)(f - a);
 }
 
+// One bilinear sample of plane xs (row length wi) from its taps: the
+// four-term sum of BilinearUpsample.forward (:913-960).  Every kernel that
+// forms a bilinear value calls this, so fused and unfused launches compute
+// it identically.  uint8 planes read as value / 255.
+template <typename T>
+__device__ __forceinline__ float bl_sample(const T* __restrict__ xs, int wi, int y0, int y1,
+                                           float wy, int x0, int x1, float wx) {
+    auto rd = [&](int yy, int xx) -> float {
+        const T v = xs[(long long)yy * wi + xx];
+        if constexpr (sizeof(T) == 1)
+            return (float)v * (1.0f / 255.0f);
+        else
+            return (float)v;
+    };
+    const float w00 = (1.f - wy) * (1.f - wx), w01 = (1.f - wy) * wx;
+    const float w10 = wy * (1.f - wx), w11 = wy * wx;
+    return rd(y0, x0) * w00 + rd(y0, x1) * w01 + rd(y1, x0) * w10 + rd(y1, x1) * w11;
+}
+
 // BilinearUpsample.forward (:913-960) / resize_gray (:3843-3845): y =
 // mul * bilinear(x) into output channels [och, och + C).  T is the input
 // element type (uint8 images are read as value / 255).  A thread makes
@@ -703,13 +806,6 @@ __global__ void k_bilinear_fwd(const T* __restrict__ x, const int* __restrict__ 
     tap_sc(oy, hi, (double)hi / (double)ho, y0, y1, wy);
     const double scx = (double)wi / (double)wo;
     const T* xs = x + (long long)n * xstride + (long long)c * hi * wi;
-    auto rd = [&](int yy, int xx) -> float {
-        const T v = xs[(long long)yy * wi + xx];
-        if constexpr (sizeof(T) == 1)
-            return (float)v * (1.0f / 255.0f);
-        else
-            return (float)v;
-    };
     float* yrow = y + (long long)n * ystride + (long long)(och + c) * ho * wo + (long long)oy * wo;
 #pragma unroll
     for (int k = 0; k < BL_X; ++k) {
@@ -718,10 +814,7 @@ __global__ void k_bilinear_fwd(const T* __restrict__ x, const int* __restrict__ 
         int x0, x1;
         float wx;
         tap_sc(ox, wi, scx, x0, x1, wx);
-        const float w00 = (1.f - wy) * (1.f - wx), w01 = (1.f - wy) * wx;
-        const float w10 = wy * (1.f - wx), w11 = wy * wx;
-        const float v = rd(y0, x0) * w00 + rd(y0, x1) * w01 + rd(y1, x0) * w10 + rd(y1, x1) * w11;
-        yrow[ox] = v * mul;
+        yrow[ox] = bl_sample(xs, wi, y0, y1, wy, x0, x1, wx) * mul;
     }
 }
 
@@ -807,6 +900,124 @@ __global__ void k_copy_chan(const T* __restrict__ x, const int* __restrict__ dim
     float v = (float)x[(long long)n * xstride + i];
     if constexpr (sizeof(T) == 1) v *= (1.0f / 255.0f);
     y[(long long)n * ystride + (long long)och * h * w + i] = v;
+}
+
+// The network's input for one pass at the working size (the resize of
+// predict_at_scale, :3950-3953, and the guides of predict, :5891-5897): both
+// views resized (resize_gray, :3843-3845) into img ([2N][fslot], left views
+// first), the left view again as channel 1 of the full-resolution
+// refinement input rin_f ([N][2][fslot]) and its 2x2 average (avgpool2,
+// :979-995) as channel 1 of the half-resolution one rin_h ([N][2][hslot]).
+// A thread makes one working-size pixel; those at even (x, y) inside the
+// half-size map also form the three neighbouring samples for the average.
+// Every value equals sv_bilinear_fwd / sv_copy_channel / sv_avgpool2 of the
+// same data (bl_sample; the sums in avgpool2's order).
+template <typename T>
+__global__ void k_load_pair(const T* __restrict__ l, const T* __restrict__ r,
+                            const int* __restrict__ sdims, long long sstride,
+                            float* __restrict__ img, long long fslot, const int* __restrict__ fdims,
+                            float* __restrict__ rin_h, long long hslot,
+                            const int* __restrict__ hdims, float* __restrict__ rin_f, int N) {
+    const int n = blockIdx.z;
+    const int oy = blockIdx.y * 4 + threadIdx.y;
+    const int ox = blockIdx.x * 64 + threadIdx.x;
+    const int hi = dh(sdims, n), wi = dw(sdims, n);
+    const int ho = dh(fdims, n), wo = dw(fdims, n);
+    if (ox >= wo || oy >= ho || hi <= 0 || wi <= 0) return;  // past the map, or a pair skipped
+    const double scy = (double)hi / (double)ho, scx = (double)wi / (double)wo;
+    const T* ls = l + (long long)n * sstride;
+    const T* rs = r + (long long)n * sstride;
+    int y0, y1, x0, x1;
+    float wy, wx;
+    tap_sc(oy, hi, scy, y0, y1, wy);
+    tap_sc(ox, wi, scx, x0, x1, wx);
+    const float vl = bl_sample(ls, wi, y0, y1, wy, x0, x1, wx);
+    const float vr = bl_sample(rs, wi, y0, y1, wy, x0, x1, wx);
+    const long long i = (long long)oy * wo + ox;
+    img[(long long)n * fslot + i] = vl;
+    img[(long long)(N + n) * fslot + i] = vr;
+    rin_f[(long long)n * 2 * fslot + (long long)ho * wo + i] = vl;
+    const int hh = dh(hdims, n), wh = dw(hdims, n);
+    if ((oy & 1) || (ox & 1) || (oy >> 1) >= hh || (ox >> 1) >= wh) return;
+    int ya, yb, xa, xb;
+    float wya, wxa;
+    tap_sc(oy + 1, hi, scy, ya, yb, wya);
+    tap_sc(ox + 1, wi, scx, xa, xb, wxa);
+    const float v01 = bl_sample(ls, wi, y0, y1, wy, xa, xb, wxa);
+    const float v10 = bl_sample(ls, wi, ya, yb, wya, x0, x1, wx);
+    const float v11 = bl_sample(ls, wi, ya, yb, wya, xa, xb, wxa);
+    const float sum = vl + v01 + v10 + v11;
+    rin_h[(long long)n * 2 * hslot + (long long)hh * wh + (long long)(oy >> 1) * wh + (ox >> 1)] =
+        0.25f * sum;
+}
+
+// Entry of one refinement level (StereoNet.forward :1220-1232): the
+// coarser disparity upsampled x2 with its values doubled
+// (BilinearUpsample, :913-960) becomes channel 0 of the level's input rin
+// ([N][2][slot]; kept there for the residual and the backward pass), and the
+// first refinement convolution (2 -> OC, 3x3, dilation dil, LeakyReLU,
+// :1125-1129) runs over [upsampled, guide], the guide being rin's channel 1.
+// A block makes a 16 x 16 output tile: both channels with a dil-pixel halo
+// (zero outside) go to LDS, the upsampled one formed there with
+// bl_sample.  Sums in the convolution engines' order (input channel, row,
+// column; one fmaf per term; bias last), so every value equals
+// sv_bilinear_fwd + sv_conv_fwd bit for bit.
+template <int OC>
+__global__ void __launch_bounds__(256) k_refine_in(
+    const float* __restrict__ src, const int* __restrict__ sdims, long long sslot,
+    float* __restrict__ rin, const int* __restrict__ dims, long long slot,
+    const float* __restrict__ w, const float* __restrict__ b, int dil, float slope,
+    float* __restrict__ y, long long ystride) {
+    constexpr int T = 16, MAXH = T + 8;
+    __shared__ float tile[2][MAXH * MAXH];
+    const int n = blockIdx.z;
+    const int h = dh(dims, n), wd = dw(dims, n);
+    const int tx0 = blockIdx.x * T, ty0 = blockIdx.y * T;
+    if (tx0 >= wd || ty0 >= h) return;  // the whole block lies outside this sample
+    const int hi = dh(sdims, n), wi = dw(sdims, n);
+    const int TH = T + 2 * dil;
+    const double scy = (double)hi / (double)h, scx = (double)wi / (double)wd;
+    const float* ss = src + (long long)n * sslot;
+    float* rn = rin + (long long)n * 2 * slot;
+    const long long plane = (long long)h * wd;
+    for (int e = threadIdx.x; e < TH * TH; e += 256) {
+        const int yy = e / TH, xx = e - yy * TH;
+        const int iy = ty0 - dil + yy, ix = tx0 - dil + xx;
+        float up = 0.f, gd = 0.f;
+        if (iy >= 0 && iy < h && ix >= 0 && ix < wd) {
+            int y0, y1, x0, x1;
+            float wy, wx;
+            tap_sc(iy, hi, scy, y0, y1, wy);
+            tap_sc(ix, wi, scx, x0, x1, wx);
+            up = bl_sample(ss, wi, y0, y1, wy, x0, x1, wx) * 2.0f;
+            gd = rn[plane + (long long)iy * wd + ix];
+            if (yy >= dil && yy < dil + T && xx >= dil && xx < dil + T)
+                rn[(long long)iy * wd + ix] = up;
+        }
+        tile[0][e] = up;
+        tile[1][e] = gd;
+    }
+    __syncthreads();
+    const int tx = threadIdx.x & (T - 1), ty = threadIdx.x / T;
+    const int ox = tx0 + tx, oy = ty0 + ty;
+    if (ox >= wd || oy >= h) return;
+    float acc[OC];
+#pragma unroll
+    for (int o = 0; o < OC; ++o) acc[o] = 0.f;
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int ky = 0; ky < 3; ++ky)
+#pragma unroll
+            for (int kx = 0; kx < 3; ++kx) {
+                const float v = tile[c][(ty + ky * dil) * TH + tx + kx * dil];
+#pragma unroll
+                for (int o = 0; o < OC; ++o)
+                    acc[o] = __builtin_fmaf(v, w[(o * 2 + c) * 9 + ky * 3 + kx], acc[o]);
+            }
+    float* ys = y + (long long)n * ystride + (long long)oy * wd + ox;
+#pragma unroll
+    for (int o = 0; o < OC; ++o) ys[(long long)o * plane] = conv_finish(acc[o], b[o], 1, slope, nullptr, 0);
 }
 
 // ------------------------------------------------------------------ losses
@@ -1463,6 +1674,74 @@ int sv_softargmin_fwd(const float* cost, const int* dims, long long cstride, int
     SV_REQUIRE(cost && dims && disp && probs && N > 0, "sv_softargmin_fwd: bad arguments");
     hipLaunchKernelGGL(k_softargmin_fwd, dim3((max_hw + 255) / 256, N), dim3(256), 0,
                        (hipStream_t)stream, cost, dims, cstride, D, penalty, disp, dstride, probs);
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+int sv_conv3_softargmin(const float* x, const int* dims, long long xstride, int IC, int D,
+                        const float* w, const float* b, float penalty, float* disp,
+                        long long dstride, float* probs, long long pstride, int N, int max_h,
+                        int max_w, void* stream) {
+    SV_REQUIRE(x && dims && w && b && disp && probs && N > 0 && IC > 0 && max_h > 0 && max_w > 0,
+               "sv_conv3_softargmin: bad arguments");
+    dim3 grid((max_w + 15) / 16, (max_h + 15) / 16, N);
+    switch (D) {
+        case 12:
+            hipLaunchKernelGGL(k_conv3_softargmin<12>, grid, dim3(256), 0, (hipStream_t)stream, x,
+                               dims, xstride, IC, w, b, penalty, disp, dstride, probs, pstride);
+            break;
+        case 8:
+            hipLaunchKernelGGL(k_conv3_softargmin<8>, grid, dim3(256), 0, (hipStream_t)stream, x,
+                               dims, xstride, IC, w, b, penalty, disp, dstride, probs, pstride);
+            break;
+        default:
+            return sv::set_error(SV_EUNSUPPORTED, "fused soft-argmin for %d disparities", D);
+    }
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+int sv_load_pair(const void* l, const void* r, int dtype, const int* sdims, long long sstride,
+                 float* img, long long fslot, const int* fdims, float* rin_h, long long hslot,
+                 const int* hdims, float* rin_f, int N, int max_h, int max_w, void* stream) {
+    SV_REQUIRE(l && r && sdims && img && fdims && rin_h && hdims && rin_f && N > 0 && max_h > 0 &&
+                   max_w > 0, "sv_load_pair: bad arguments");
+    dim3 grid((max_w + 63) / 64, (max_h + 3) / 4, N), blk(64, 4);
+    hipStream_t st = (hipStream_t)stream;
+    if (dtype == SV_U8)
+        hipLaunchKernelGGL(k_load_pair<uint8_t>, grid, blk, 0, st, (const uint8_t*)l,
+                           (const uint8_t*)r, sdims, sstride, img, fslot, fdims, rin_h, hslot,
+                           hdims, rin_f, N);
+    else if (dtype == SV_F32)
+        hipLaunchKernelGGL(k_load_pair<float>, grid, blk, 0, st, (const float*)l, (const float*)r,
+                           sdims, sstride, img, fslot, fdims, rin_h, hslot, hdims, rin_f, N);
+    else
+        return sv::set_error(SV_EINVAL, "sv_load_pair: images must be uint8 or float32");
+    SV_CHECK_LAUNCH();
+    return SV_OK;
+}
+
+int sv_refine_in(const float* src, const int* sdims, long long sslot, float* rin, const int* dims,
+                 long long slot, const float* w, const float* b, int OC, int dil, float slope,
+                 float* y, long long ystride, int N, int max_h, int max_w, void* stream) {
+    SV_REQUIRE(src && sdims && rin && dims && w && b && y && N > 0 && max_h > 0 && max_w > 0,
+               "sv_refine_in: bad arguments");
+    if (dil < 1 || dil > 4)
+        return sv::set_error(SV_EUNSUPPORTED, "sv_refine_in: dilation %d (1..4 supported)", dil);
+    dim3 grid((max_w + 15) / 16, (max_h + 15) / 16, N);
+    hipStream_t st = (hipStream_t)stream;
+    switch (OC) {
+        case 8:
+            hipLaunchKernelGGL(k_refine_in<8>, grid, dim3(256), 0, st, src, sdims, sslot, rin, dims,
+                               slot, w, b, dil, slope, y, ystride);
+            break;
+        case 32:
+            hipLaunchKernelGGL(k_refine_in<32>, grid, dim3(256), 0, st, src, sdims, sslot, rin,
+                               dims, slot, w, b, dil, slope, y, ystride);
+            break;
+        default:
+            return sv::set_error(SV_EUNSUPPORTED, "sv_refine_in: %d output channels", OC);
+    }
     SV_CHECK_LAUNCH();
     return SV_OK;
 }

@@ -37,6 +37,10 @@ L.extra_signatures({
     "sv_costvol_bwd": (_I, [_P, _P, _P, _LL, _I, _I, _P, _LL, _P, _P, _I, _I, _I, _P]),
     "sv_softargmin_fwd": (_I, [_P, _P, _LL, _I, _F, _P, _LL, _P, _I, _I, _P]),
     "sv_softargmin_bwd": (_I, [_P, _P, _P, _P, _P, _LL, _I, _LL, _P, _I, _I, _P]),
+    "sv_conv3_softargmin": (_I, [_P, _P, _LL, _I, _I, _P, _P, _F, _P, _LL, _P, _LL, _I, _I, _I,
+                                 _P]),
+    "sv_load_pair": (_I, [_P, _P, _I, _P, _LL, _P, _LL, _P, _P, _LL, _P, _P, _I, _I, _I, _P]),
+    "sv_refine_in": (_I, [_P, _P, _LL, _P, _P, _LL, _P, _P, _I, _I, _F, _P, _LL, _I, _I, _I, _P]),
     "sv_bilinear_fwd": (_I, [_P, _I, _P, _LL, _I, _P, _P, _LL, _I, _F, _P, _I, _I, _I, _P]),
     "sv_bilinear_bwd": (_I, [_P, _P, _LL, _I, _I, _P, _P, _LL, _F, _I, _I, _I, _P]),
     "sv_avgpool2": (_I, [_P, _I, _P, _LL, _P, _P, _LL, _I, _F, _I, _I, _I, _P]),
@@ -157,6 +161,10 @@ class StereoNet:
         # ((knob, value), ...); and a recorder of conv launches it uses
         self._conv_tune = {}
         self._conv_hook = None
+        # fused launches (identical results): the pass input with its
+        # guides, the last cost-filter layer with the soft-argmin, each
+        # refinement level's upsampling with its first convolution
+        self.fused = True
 
     # -- precision -------------------------------------------------------
     def use_bf16(self, on=True):
@@ -402,6 +410,17 @@ class _Plan:
         N = self.N
         if src is None:
             src = _Level([(H, W)] * N, self.dev)
+        if getattr(self.net, "fused", False):
+            l = left.reshape(N, -1)
+            r = right.reshape(N, -1)
+            if l.dtype not in (torch.uint8, torch.float32):
+                l, r = l.float(), r.float()
+            l, r = l.contiguous(), r.contiguous()
+            _call("sv_load_pair", _p(l), _p(r), L.SV_U8 if l.dtype == torch.uint8 else L.SV_F32,
+                  _p(src.table), src.slot, _p(self.img), self.full.slot, _p(self.full.table),
+                  _p(self.rin[0]), self.half.slot, _p(self.half.table), _p(self.rin[1]), N,
+                  self.full.maxh, self.full.maxw, L.stream())
+            return
         for side, imgs in ((0, left), (1, right)):
             s = imgs.reshape(N, -1)
             if s.dtype not in (torch.uint8, torch.float32):
@@ -483,19 +502,40 @@ class _Plan:
         c1, c2, c3 = net.filt
         self._conv(c1, fl, self.f4, self.ca[0], self.f4, N, 1, D=D, xr=fr)
         self._conv(c2, self.ca[0], self.f4, self.ca[1], self.f4, N, 1, D=D)
-        self._conv(c3, self.ca[1], self.f4, self.ca[2], self.f4, N, 0, D=D)
-        # soft-argmin with the invalid-disparity penalty (:1216-1218)
-        _call("sv_softargmin_fwd", _p(self.ca[2]), _p(self.f4.table), D * self.f4.slot, D,
-              INVALID_PENALTY, _p(self.d0), self.f4.slot, _p(self.probs), N, self.f4.slot,
-              L.stream())
+        if getattr(net, "fused", False) and c3.oc == 1 and D in (8, 12):
+            # last layer + soft-argmin in one launch (the filtered costs stay
+            # in registers)
+            _call("sv_conv3_softargmin", _p(self.ca[1]), _p(self.f4.table), c3.ic * D * self.f4.slot,
+                  c3.ic, D, _p(net.params[c3.w_off:]), _p(net.params[c3.b_off:]), INVALID_PENALTY,
+                  _p(self.d0), self.f4.slot, _p(self.probs), D * self.f4.slot, N, self.f4.maxh,
+                  self.f4.maxw, L.stream())
+        else:
+            self._conv(c3, self.ca[1], self.f4, self.ca[2], self.f4, N, 0, D=D)
+            # soft-argmin with the invalid-disparity penalty (:1216-1218)
+            _call("sv_softargmin_fwd", _p(self.ca[2]), _p(self.f4.table), D * self.f4.slot, D,
+                  INVALID_PENALTY, _p(self.d0), self.f4.slot, _p(self.probs), N, self.f4.slot,
+                  L.stream())
         # two refinement levels (:1220-1230)
         src, slv = self.d0, self.f4
         for r, (lv, dout) in enumerate(((self.half, self.d1), (self.full, self.d2))):
-            _call("sv_bilinear_fwd", _p(src), L.SV_F32, _p(slv.table), slv.slot, 1, _p(self.rin[r]),
-                  _p(lv.table), 2 * lv.slot, 0, 2.0, None, N, lv.maxh, lv.maxw, L.stream())
             convs = net.ref[r]
-            x, ic = self.rin[r], 2
+            c1 = convs[0]
+            first = 0
+            if getattr(net, "fused", False) and c1.ic == 2 and c1.oc in (8, 32) and 1 <= c1.dil <= 4:
+                # upsampling + first convolution in one launch
+                _call("sv_refine_in", _p(src), _p(slv.table), slv.slot, _p(self.rin[r]),
+                      _p(lv.table), lv.slot, _p(net.params[c1.w_off:]), _p(net.params[c1.b_off:]),
+                      c1.oc, c1.dil, SLOPE, _p(self.ra[r][0]), c1.oc * lv.slot, N, lv.maxh,
+                      lv.maxw, L.stream())
+                first = 1
+            else:
+                _call("sv_bilinear_fwd", _p(src), L.SV_F32, _p(slv.table), slv.slot, 1,
+                      _p(self.rin[r]), _p(lv.table), 2 * lv.slot, 0, 2.0, None, N, lv.maxh,
+                      lv.maxw, L.stream())
+            x = self.rin[r] if first == 0 else self.ra[r][0]
             for i, c in enumerate(convs[:-1]):
+                if i < first:
+                    continue
                 self._conv(c, x, lv, self.ra[r][i], lv, N, 1)
                 x = self.ra[r][i]
             self._conv(convs[-1], x, lv, dout, lv, N, 2, resid=self.rin[r], rstride=2 * lv.slot)

@@ -284,6 +284,34 @@ int sv_costvol_bwd(const float* fl, const float* fr, const int* dims, long long 
 int sv_softargmin_fwd(const float* cost, const int* dims, long long cstride, int D,
                       float penalty, float* disp, long long dstride, float* probs, int N,
                       int max_hw, void* stream);
+/* The last cost-filter layer (Conv3d IC -> 1, :1079-1094, no activation)
+ * fused with sv_softargmin_fwd: disp and probs as that call gives them after
+ * sv_conv_fwd of the layer, bit for bit, without the filtered cost volume
+ * (D = 8 or 12; SV_EUNSUPPORTED otherwise).  No counterpart function in the
+ * reference: two of its layers in one launch. */
+int sv_conv3_softargmin(const float* x, const int* dims, long long xstride, int IC, int D,
+                        const float* w, const float* b, float penalty, float* disp,
+                        long long dstride, float* probs, long long pstride, int N, int max_h,
+                        int max_w, void* stream);
+/* The network input of one pass at the working size (predict_at_scale's
+ * resize, :3950-3953, and predict's guides, :5891-5897) in one launch:
+ * both views resized (resize_gray, :3843-3845) into img [2N][fslot] (left
+ * first), the left again as channel 1 of rin_f [N][2][fslot] and its 2x2
+ * average (avgpool2, :979-995) as channel 1 of rin_h [N][2][hslot]; equal
+ * to sv_bilinear_fwd + sv_copy_channel + sv_avgpool2 bit for bit.  No
+ * counterpart function in the reference: three of its steps in one launch. */
+int sv_load_pair(const void* l, const void* r, int dtype, const int* sdims, long long sstride,
+                 float* img, long long fslot, const int* fdims, float* rin_h, long long hslot,
+                 const int* hdims, float* rin_f, int N, int max_h, int max_w, void* stream);
+/* Entry of a refinement level (StereoNet.forward :1220-1232): the coarser
+ * disparity upsampled x2 and doubled into channel 0 of rin [N][2][slot],
+ * then the first refinement convolution (2 -> OC, 3x3, dilation dil,
+ * LeakyReLU slope) over rin into y; equal to sv_bilinear_fwd + sv_conv_fwd
+ * bit for bit (OC 8 or 32, dilation 1..4).  No counterpart function in the
+ * reference: two of its steps in one launch. */
+int sv_refine_in(const float* src, const int* sdims, long long sslot, float* rin, const int* dims,
+                 long long slot, const float* w, const float* b, int OC, int dil, float slope,
+                 float* y, long long ystride, int N, int max_h, int max_w, void* stream);
 int sv_softargmin_bwd(const float* probs, const float* disp, const float* gdisp,
                       const float* gextra, const int* dims, long long cstride, int D,
                       long long dstride, float* gcost, int N, int max_hw, void* stream);

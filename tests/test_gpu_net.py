@@ -10,6 +10,7 @@ Tolerances (FP32 arithmetic against float64):
   quantisation.
 """
 
+import importlib
 import os
 
 import numpy as np
@@ -419,3 +420,37 @@ def test_conv_variants_bit_identical_and_match_float64(sv, N, case):
                     assert torch.equal(a, c), "variant %s differs" % (variant,)
     finally:
         A.reset()
+
+
+@pytest.mark.parametrize("wide", [False, True])
+def test_fused_launches_bit_identical(sv, N, wide):
+    """The fused launches (pass input + guides; last cost-filter layer +
+    soft-argmin; refinement upsampling + first convolution) give the
+    unfused launches' values bit for bit: predict_auto on a ragged batch
+    (every pass, the probe included) and a single-size forward, v5 and the
+    widened network."""
+    R = importlib.import_module("from-scratch-stereo-vision_amd.ragged")
+    net = _v5(N)
+    if wide:
+        net = N.widen(net, 48, 24, 32)
+    sizes = [(460, 370), (300, 201), (64, 51), (173, 129)]
+    ps = [sv.synth.make_pair(w, h, 60 + i, dmin=4.0, dmax=w / 6.0, n_boxes=1) for i, (w, h) in enumerate(sizes)]
+    b = R.RaggedBatch(sizes)
+    l = b.pack([p["left"] for p in ps])
+    r = b.pack([p["right"] for p in ps])
+    outs = {}
+    for fused in (False, True):
+        net.fused = fused
+        res = N.predict_auto_batch(net, l, r, None, None, batch=b)
+        pl = net.forward_batch([torch.as_tensor(ps[1]["left"]).cuda().float().reshape(-1) / 255.0],
+                               [torch.as_tensor(ps[1]["right"]).cuda().float().reshape(-1) / 255.0],
+                               [(201, 300)])
+        outs[fused] = ([t.clone() for t in b.unpack(res.full)], res.trace,
+                       pl.d0.clone(), pl.d1.clone(), pl.d2.clone(), pl.probs.clone(), pl.rin[0].clone(),
+                       pl.rin[1].clone(), pl.ra[1][0].clone())
+    net.fused = True
+    a, c = outs[False], outs[True]
+    assert a[1] == c[1]
+    assert all(torch.equal(u, v) for u, v in zip(a[0], c[0]))
+    for u, v in zip(a[2:], c[2:]):
+        assert torch.equal(u, v)
