@@ -635,66 +635,101 @@ __global__ void k_softargmin_fwd(const float* __restrict__ cost, const int* __re
 
 // The last cost-filter layer (Conv3d IC -> 1, 3x3x3, no activation,
 // stereonet.py:1079-1094) fused with SoftArgmin.forward and the invalid
-// mask (:861-889, :827-840).  A block makes a 16 x 16 pixel tile of one
-// sample, a thread one pixel with all D output depths in registers, so the
-// filtered costs never leave the kernel.  The input tile of one channel
-// (every depth, one-pixel halo, zero outside) is staged in LDS at a time.
-// The sums follow the convolution engines' order -- input channel, depth
-// tap, row, column, one fmaf per term (zero-padding terms add nothing and
-// are skipped), bias last -- and the soft-argmin is k_softargmin_fwd's, so
-// disparities and probabilities equal the two unfused launches bit for bit.
+// mask (:861-889, :827-840).  A block makes an 8 x 8 pixel tile of one
+// sample: wave g computes output depths [g*DG, (g+1)*DG) of its lane's
+// pixel, the four waves' costs meet in LDS, and wave 0 runs the soft-argmin
+// of each pixel, so the filtered costs never reach memory.  The input tile
+// of one channel (every depth, one-pixel halo, zero outside) is staged in
+// LDS at a time; each staged value is read once and fed to the (up to
+// three) outputs it touches.  The sums follow the convolution engines'
+// order -- input channel, depth tap, row, column, one fmaf per term
+// (zero-padding terms add nothing and are skipped), bias last -- and the
+// soft-argmin is k_softargmin_fwd's, so disparities and probabilities equal
+// the two unfused launches bit for bit.
 template <int D>
 __global__ void __launch_bounds__(256) k_conv3_softargmin(
     const float* __restrict__ x, const int* __restrict__ dims, long long xstride, int IC,
     const float* __restrict__ w, const float* __restrict__ b, float penalty,
     float* __restrict__ disp, long long dstride, float* __restrict__ probs, long long pstride) {
-    constexpr int T = 16, TH = T + 2, PL = TH * TH;
+    constexpr int T = 8, TH = T + 2, PL = TH * TH;
+    constexpr int DG = (D + 3) / 4;  // output depths per wave
     __shared__ float tile[D * PL];
+    __shared__ float cost[64 * (D + 1)];
     const int n = blockIdx.z;
     const int h = dh(dims, n), wd = dw(dims, n);
     const int tx0 = blockIdx.x * T, ty0 = blockIdx.y * T;
     if (tx0 >= wd || ty0 >= h) return;  // the whole block lies outside this sample
-    const int tx = threadIdx.x & (T - 1), ty = threadIdx.x / T;
+    const int lane = threadIdx.x & 63;
+    const int g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int tx = lane & (T - 1), ty = lane / T;
     const long long plane = (long long)h * wd;
     const float* xs = x + (long long)n * xstride;
-    float acc[D];
+    const int d0 = g * DG;
+    float acc[DG];
 #pragma unroll
-    for (int d = 0; d < D; ++d) acc[d] = 0.f;
+    for (int j = 0; j < DG; ++j) acc[j] = 0.f;
+    // this thread's staging positions (fixed over the channels) and the next
+    // channel's values, loaded while the current one is being used
+    constexpr int NE = (D * PL + 255) / 256;
+    long long off[NE];
+    bool in[NE];
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+        const int e = threadIdx.x + 256 * k;
+        const int dd = e / PL;
+        const int r = e - dd * PL;
+        const int yy = r / TH, xx = r - yy * TH;
+        const int iy = ty0 - 1 + yy, ix = tx0 - 1 + xx;
+        in[k] = e < D * PL && iy >= 0 && iy < h && ix >= 0 && ix < wd;
+        off[k] = (long long)dd * plane + (long long)iy * wd + ix;
+    }
+    float nxt[NE];
+#pragma unroll
+    for (int k = 0; k < NE; ++k) nxt[k] = in[k] ? xs[off[k]] : 0.f;
     for (int c = 0; c < IC; ++c) {
-        for (int e = threadIdx.x; e < D * PL; e += 256) {
-            const int dd = e / PL;
-            const int r = e - dd * PL;
-            const int yy = r / TH, xx = r - yy * TH;
-            const int iy = ty0 - 1 + yy, ix = tx0 - 1 + xx;
-            tile[e] = (iy >= 0 && iy < h && ix >= 0 && ix < wd)
-                          ? xs[((long long)c * D + dd) * plane + (long long)iy * wd + ix] : 0.f;
-        }
+#pragma unroll
+        for (int k = 0; k < NE; ++k)
+            if (threadIdx.x + 256 * k < D * PL) tile[threadIdx.x + 256 * k] = nxt[k];
         __syncthreads();
+        if (c + 1 < IC) {
+            const float* xn = xs + (long long)(c + 1) * D * plane;
+#pragma unroll
+            for (int k = 0; k < NE; ++k) nxt[k] = in[k] ? xn[off[k]] : 0.f;
+        }
         const float* wc = w + c * 27;
         const float* tp = tile + ty * TH + tx;
+        // input depths d0-1 .. d0+DG feed outputs d0 .. d0+DG-1 (od = dd - kd + 1)
 #pragma unroll
-        for (int od = 0; od < D; ++od)
+        for (int q = -1; q <= DG; ++q) {
+            const int dd = d0 + q;
+            if (dd < 0 || dd >= D) continue;
 #pragma unroll
-            for (int kd = 0; kd < 3; ++kd) {
-                const int dd = od + kd - 1;
-                if (dd < 0 || dd >= D) continue;
+            for (int ky = 0; ky < 3; ++ky)
 #pragma unroll
-                for (int ky = 0; ky < 3; ++ky)
+                for (int kx = 0; kx < 3; ++kx) {
+                    const float v = tp[dd * PL + ky * TH + kx];
 #pragma unroll
-                    for (int kx = 0; kx < 3; ++kx)
-                        acc[od] = __builtin_fmaf(tp[dd * PL + ky * TH + kx], wc[(kd * 3 + ky) * 3 + kx],
-                                                 acc[od]);
-            }
+                    for (int kd = 2; kd >= 0; --kd) {
+                        const int j = q - kd + 1;  // local output index
+                        if (j < 0 || j >= DG || d0 + j >= D) continue;
+                        acc[j] = __builtin_fmaf(v, wc[(kd * 3 + ky) * 3 + kx], acc[j]);
+                    }
+                }
+        }
         __syncthreads();
     }
-    const int ox = tx0 + tx, oy = ty0 + ty;
-    if (ox >= wd || oy >= h) return;
     const float bias = b[0];
+#pragma unroll
+    for (int j = 0; j < DG; ++j)
+        if (d0 + j < D) cost[lane * (D + 1) + d0 + j] = acc[j] + bias;
+    __syncthreads();
+    const int ox = tx0 + tx, oy = ty0 + ty;
+    if (g != 0 || ox >= wd || oy >= h) return;
     float cst[D];
     float best = -INFINITY;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        cst[d] = (acc[d] + bias) + ((d >= 1 && ox < d) ? penalty : 0.f);
+        cst[d] = cost[lane * (D + 1) + d] + ((d >= 1 && ox < d) ? penalty : 0.f);
         best = fmaxf(best, -cst[d]);
     }
     float e[D];
@@ -1684,7 +1719,7 @@ int sv_conv3_softargmin(const float* x, const int* dims, long long xstride, int 
                         int max_w, void* stream) {
     SV_REQUIRE(x && dims && w && b && disp && probs && N > 0 && IC > 0 && max_h > 0 && max_w > 0,
                "sv_conv3_softargmin: bad arguments");
-    dim3 grid((max_w + 15) / 16, (max_h + 15) / 16, N);
+    dim3 grid((max_w + 7) / 8, (max_h + 7) / 8, N);
     switch (D) {
         case 12:
             hipLaunchKernelGGL(k_conv3_softargmin<12>, grid, dim3(256), 0, (hipStream_t)stream, x,
