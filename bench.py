@@ -118,7 +118,7 @@ def main():
                          "sgm_prefetch=8,sgm_htile=16,conv_rows=1,conv_tile_width=32")
     ap.add_argument("--workload", default="pipeline",
                     choices=["pipeline", "band", "ragged64", "widened", "consistent", "planes",
-                             "images", "eval", "mesh"],
+                             "images", "eval", "mesh", "train"],
                     help="pipeline: 460x370 pairs (the headline); ragged64: 64 pairs of uneven "
                          "size in one captured graph")
     args = ap.parse_args()
@@ -144,6 +144,8 @@ def main():
         return mesh_main(args)
     if args.workload == "band":
         return band_main(args)
+    if args.workload == "train":
+        return train_main(args)
     if args.tune_only and not args.tune_file:
         print("--tune-only needs --tune-file", file=sys.stderr)
         sys.exit(2)
@@ -634,6 +636,68 @@ def cpu_baseline_one_core(args, sv, pairs):
     tcpu = time.perf_counter() - tc
     return {"value": round(n / tcpu, 4), "unit": "pairs/s", "cores": 1, "kind": "port",
             "sample": "%d of the same pairs, one process, %.1f s" % (n, tcpu)}
+
+
+def net_macs(net, H, W):
+    """Multiply-adds of one StereoNet forward at H x W (the layer formulas of
+    stereonet.py:459-1268: every conv output voxel costs ic * kd * 9)."""
+    def co(n, s, d):
+        return (n + 2 * d - (2 * d + 1)) // s + 1
+    macs = 0
+    h, w = H, W
+    for c in net.feat:  # both views
+        h, w = co(h, c.s, c.dil), co(w, c.s, c.dil)
+        macs += 2 * h * w * c.oc * c.ic * 9
+    for c in net.filt:
+        macs += h * w * net.D * c.oc * c.ic * 27
+    for r, (hh, ww) in enumerate(((H // 2, W // 2), (H, W))):
+        for c in net.ref[r]:
+            macs += hh * ww * c.oc * c.ic * 9
+    return macs
+
+
+def train_main(args):
+    """Configuration 5's training step: StereoNet v5 at 260x192, batch 16,
+    train_grads on the batch (forward, losses, hand-written backward) + one
+    Adam step (train.train_net's step, jitter on), labels from full-range
+    LR SGM on the same synthetic pairs.  value = training pairs/s;
+    FLOP/step = 3 x the forward's multiply-adds x 2 (forward, data and
+    weight gradients)."""
+    import numpy as np
+    import torch
+    sv = importlib.import_module("from-scratch-stereo-vision_amd")
+    L = importlib.import_module("from-scratch-stereo-vision_amd._lib")
+    N = importlib.import_module("from-scratch-stereo-vision_amd.net")
+    T = importlib.import_module("from-scratch-stereo-vision_amd.train")
+    dev = L.device()
+    Wt, Ht, Bt = 260, 192, 16
+    log("generating 64 planar training pairs %dx%d" % (Wt, Ht))
+    d = T.make_dataset([10001 + 2 * i for i in range(64)], Wt, Ht, kind="planar", workers=1)
+    disp, valid = T.pseudo_labels(d["left"], d["right"], Wt, Ht, max_disp=42.0)
+    left = torch.as_tensor(d["left"]).to(dev)
+    right = torch.as_tensor(d["right"]).to(dev)
+    net = N.load_model(V5_WEIGHTS)
+    warm = max(args.warmup, 1)
+    T.train_net(net, left, right, disp, valid, steps=warm, batch=Bt, lr=5e-4, log_every=10 ** 9)
+    torch.cuda.synchronize()
+    steps = max(args.steps, 10) * 5
+    t0 = time.perf_counter()
+    hist = T.train_net(net, left, right, disp, valid, steps=steps, batch=Bt, lr=5e-4, seed=1,
+                       log_every=steps)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    flops = 3 * 2 * net_macs(net, Ht, Wt) * Bt
+    rec = {"metric": "training pairs/s, StereoNet v5 260x192, batch 16 (train_grads + Adam)",
+           "value": round(Bt / dt, 2), "unit": "pairs/s", "n_gpus": 1, "steps": steps,
+           "warmup": warm, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "train_net step: 16 planar synthetic pairs 260x192 (peak "
+                                  "disparity 22-40), full-range LR SGM pseudo-labels, brightness "
+                                  "jitter, lr 5e-4", "pairs_per_step": Bt},
+           "flop_per_step": flops, "tflops": round(flops / dt / 1e12, 2),
+           "fp32_peak_frac": round(flops / dt / 1e12 / 157.3, 4),
+           "loss": hist[-1][1]}
+    print(json.dumps(rec), flush=True)
 
 
 def band_main(args):

@@ -500,11 +500,18 @@ __global__ void __launch_bounds__(WG_COLS) k_conv_bwd_weight(
         }
         if (col < ncol) {
             const int lim = (int)min((long long)POS_TILE, npos - base);
+            // (depth, row, column) of the tile's first position, then stepped
+            int od = (int)(base / oplane);
+            const int rem0 = (int)(base - (long long)od * oplane);
+            int oy = rem0 / wo, ox = rem0 - oy * wo;
             for (int q = 0; q < lim; ++q) {
-                const long long pos = base + q;
-                const int od = (int)(pos / oplane);
-                const int rem = (int)(pos - (long long)od * oplane);
-                const int oy = rem / wo, ox = rem - (rem / wo) * wo;
+                if (q > 0 && ++ox == wo) {
+                    ox = 0;
+                    if (++oy == ho) {
+                        oy = 0;
+                        ++od;
+                    }
+                }
                 const int dd = od + kd - pd;
                 const int iy = oy * s - p + ky * dil;
                 const int ix = ox * s - p + kx * dil;
@@ -538,19 +545,45 @@ __global__ void __launch_bounds__(WG_COLS) k_conv_bwd_weight(
 }
 
 // gw[o][col] += sum over (n, chunk) of partials, in a fixed order; the
-// bias column goes to gb.
-__global__ void k_reduce_partials(const float* __restrict__ partials, int nparts, int OC,
-                                  int ncol, float* __restrict__ gw, float* __restrict__ gb) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
+// bias column goes to gb.  A block takes 64 consecutive entries (lane l,
+// entry 64*blockIdx.x + l); wave w adds parts w, w + RP_WAVES, ... in
+// float64 (each part's 64 entries are one coalesced read, several parts in
+// flight), then wave 0 adds the waves' sums in wave order.
+constexpr int RP_WAVES = 16;
+
+__global__ void __launch_bounds__(64 * RP_WAVES) k_reduce_partials(
+    const float* __restrict__ partials, int nparts, int OC, int ncol, float* __restrict__ gw,
+    float* __restrict__ gb) {
+    __shared__ double ws[RP_WAVES][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int ncolp = ncol + 1;
-    if (i >= OC * ncolp) return;
+    const long long E = (long long)OC * ncolp;
+    const int i = blockIdx.x * 64 + lane;
     double s = 0.0;
-    for (int q = 0; q < nparts; ++q) s += (double)partials[(long long)q * OC * ncolp + i];
+    if (i < E) {
+        const float* p = partials + i;
+        int q = wave;
+        for (; q + 3 * RP_WAVES < nparts; q += 4 * RP_WAVES) {
+            const float a = p[(long long)q * E], b = p[(long long)(q + RP_WAVES) * E];
+            const float c = p[(long long)(q + 2 * RP_WAVES) * E], d = p[(long long)(q + 3 * RP_WAVES) * E];
+            s += (double)a;
+            s += (double)b;
+            s += (double)c;
+            s += (double)d;
+        }
+        for (; q < nparts; q += RP_WAVES) s += (double)p[(long long)q * E];
+    }
+    ws[wave][lane] = s;
+    __syncthreads();
+    if (wave != 0 || i >= E) return;
+    double t = ws[0][lane];
+#pragma unroll
+    for (int w = 1; w < RP_WAVES; ++w) t += ws[w][lane];
     const int o = i / ncolp, col = i - o * ncolp;
     if (col < ncol)
-        gw[(long long)o * ncol + col] += (float)s;
+        gw[(long long)o * ncol + col] += (float)t;
     else
-        gb[o] += (float)s;
+        gb[o] += (float)t;
 }
 
 // ------------------------------------------------------------ elementwise
@@ -1676,8 +1709,8 @@ int sv_conv_bwd_weight(const float* x, const float* xr, const int* idims, long l
 #undef LAUNCH_BW
     SV_CHECK_LAUNCH();
     const int tot = OC * (ncol + 1);
-    hipLaunchKernelGGL(k_reduce_partials, dim3((tot + 255) / 256), dim3(256), 0, st, partials,
-                       N * nchunk, OC, ncol, gw, gb);
+    hipLaunchKernelGGL(k_reduce_partials, dim3((tot + 63) / 64), dim3(64 * RP_WAVES), 0, st,
+                       partials, N * nchunk, OC, ncol, gw, gb);
     SV_CHECK_LAUNCH();
     return SV_OK;
 }
