@@ -445,6 +445,22 @@ __global__ void __launch_bounds__(256) k_conv_bwd_data(
     }
 }
 
+// Weights of the transposed convolution: the input gradient of a stride-1
+// convolution with same padding is the forward convolution of the output
+// gradient with every kernel flipped in all its taps and input and output
+// channels exchanged, wt[ic][oc][t] = w[oc][ic][TAPS - 1 - t] (first ICn input
+// channels), with a zero bias zb.
+__global__ void k_flip_weights(const float* __restrict__ w, int OC, int IC, int TAPS, int ICn,
+                               float* __restrict__ wt, float* __restrict__ zb) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < ICn) zb[i] = 0.f;
+    if (i >= ICn * OC * TAPS) return;
+    const int ic = i / (OC * TAPS);
+    const int r = i - ic * OC * TAPS;
+    const int o = r / TAPS, t = r - o * TAPS;
+    wt[i] = w[((long long)o * IC + ic) * TAPS + (TAPS - 1 - t)];
+}
+
 // --------------------------------------------------------- conv bwd weight
 // Weight and bias gradient as a split reduction.  Block (c, kt, n) owns the
 // weight columns kt*128 + t (one (ic, tap) pair per thread) for all output
@@ -1655,10 +1671,23 @@ int sv_conv_fwd(const float* x, const float* xr, const int* idims, long long xst
 
 int sv_conv_bwd_data(const float* g, const int* odims, long long gstride, const float* w, int IC,
                      int OC, int D, int KD, int s, int p, int dil, float* gx, const int* idims,
-                     long long gxstride, int N, int max_hi, int max_wi, int ic_count,
+                     long long gxstride, int N, int max_hi, int max_wi, int ic_count, float* wt_ws,
                      void* stream) {
     SV_REQUIRE(g && odims && w && gx && idims && N > 0, "sv_conv_bwd_data: bad arguments");
     const int ICn = ic_count > 0 ? ic_count : IC;
+    if (wt_ws && s == 1 && p == dil) {
+        // stride 1, same padding: the transposed convolution on the forward
+        // engines (matrix cores for 4..16 output channels)
+        const int TAPS = KD * 9;
+        float* wt = wt_ws;
+        float* zb = wt_ws + (size_t)ICn * OC * TAPS;
+        const int n = ICn * OC * TAPS;
+        hipLaunchKernelGGL(k_flip_weights, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                           w, OC, IC, TAPS, ICn, wt, zb);
+        SV_CHECK_LAUNCH();
+        return sv_conv_fwd(g, nullptr, odims, gstride, OC, D, KD, wt, zb, ICn, 1, p, dil, gx, idims,
+                           gxstride, N, max_hi, max_wi, 0, 0.f, nullptr, 0, stream);
+    }
     const int tY = tiles4(max_hi);
     dim3 grid((max_wi + 63) / 64, tY * D, N), blk(64, 4);
     hipStream_t st = (hipStream_t)stream;

@@ -29,7 +29,7 @@ L.extra_signatures({
     "sv_conv_fwd": (_I, [_P, _P, _P, _LL, _I, _I, _I, _P, _P, _I, _I, _I, _I, _P, _P, _LL, _I, _I,
                          _I, _I, _F, _P, _LL, _P]),
     "sv_conv_bwd_data": (_I, [_P, _P, _LL, _P, _I, _I, _I, _I, _I, _I, _I, _P, _P, _LL, _I, _I, _I,
-                              _I, _P]),
+                              _I, _P, _P]),
     "sv_conv_bwd_weight_workspace_bytes": (_I, [_I, _I, _I, _I, _I, ctypes.POINTER(_SZ)]),
     "sv_conv_bwd_weight": (_I, [_P, _P, _P, _LL, _I, _I, _I, _P, _P, _LL, _I, _I, _I, _I, _I, _I,
                                 _P, _P, _P, _SZ, _P]),
@@ -368,6 +368,7 @@ class _Plan:
         self.d1 = z(N * self.half.slot)
         self.d2 = z(N * self.full.slot)
         self._g = None
+        self._wt = None
         self.has_extra = False
 
     # slot strides
@@ -590,9 +591,17 @@ class _Plan:
               _p(net.grads[c.w_off:]), _p(net.grads[c.b_off:]), _p(ws), nb.value, L.stream())
 
     def _dgrad(self, c, gy, ylv, gx, xlv, N, D=1, ic_count=0):
+        # stride-1 layers run as transposed convolutions on the forward
+        # engines (flipped weights in a small per-plan buffer)
+        ws = None
+        if c.s == 1:
+            need = (ic_count or c.ic) * c.oc * c.kd * 9 + (ic_count or c.ic)
+            if self._wt is None or self._wt.numel() < need:
+                self._wt = torch.empty(max(need, 4096), dtype=torch.float32, device=self.dev)
+            ws = self._wt
         _call("sv_conv_bwd_data", _p(gy), _p(ylv.table), c.oc * D * ylv.slot, _p(self.net.params[c.w_off:]),
               c.ic, c.oc, D, c.kd, c.s, c.p, c.dil, _p(gx), _p(xlv.table),
-              (ic_count or c.ic) * D * xlv.slot, N, xlv.maxh, xlv.maxw, ic_count, L.stream())
+              (ic_count or c.ic) * D * xlv.slot, N, xlv.maxh, xlv.maxw, ic_count, _p(ws), L.stream())
 
     def _act(self, g, y, n, slope):
         _call("sv_act_bwd", _p(g), _p(y), n, slope, L.stream())

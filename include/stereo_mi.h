@@ -255,10 +255,14 @@ int sv_conv_fwd(const float* x, const float* xr, const int* idims, long long xst
                 int mode, float slope, const float* resid, long long rstride, void* stream);
 
 /* Conv2d/Conv3d.backward input gradient (:528-568, :643-692); ic_count > 0
- * computes only the first ic_count input channels. */
+ * computes only the first ic_count input channels.  wt_ws (may be NULL):
+ * device workspace of ICn*OC*KD*9 + ICn floats (ICn = ic_count or IC); given
+ * it, a stride-1 layer runs as the forward convolution of g with the
+ * flipped, channel-exchanged kernel on the forward engines; otherwise (or
+ * for stride 2) a gather kernel on the vector ALUs. */
 int sv_conv_bwd_data(const float* g, const int* odims, long long gstride, const float* w, int IC,
                      int OC, int D, int KD, int s, int p, int dil, float* gx, const int* idims,
-                     long long gxstride, int N, int max_hi, int max_wi, int ic_count,
+                     long long gxstride, int N, int max_hi, int max_wi, int ic_count, float* wt_ws,
                      void* stream);
 
 /* Conv2d/Conv3d.backward weight and bias gradients (accumulated into gw,

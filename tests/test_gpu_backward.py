@@ -90,7 +90,13 @@ def test_conv_backward_blocks(sv, N, case):
     wd = torch.as_tensor(w.astype(np.float32).ravel()).cuda()
     gx = torch.full((2, xstride), 9.0, dtype=torch.float32, device="cuda")
     N._call("sv_conv_bwd_data", N._p(gyb), N._p(lvo.table), gstride, N._p(wd), IC, OC, D, KD, s,
-            p, dil, N._p(gx), N._p(lvi.table), xstride, 2, lvi.maxh, lvi.maxw, 0, N.L.stream())
+            p, dil, N._p(gx), N._p(lvi.table), xstride, 2, lvi.maxh, lvi.maxw, 0, None, N.L.stream())
+    # the transposed-convolution form (stride 1) on the forward engines
+    wt = torch.empty(IC * OC * KD * 9 + IC, dtype=torch.float32, device="cuda")
+    gx2 = torch.full((2, xstride), 9.0, dtype=torch.float32, device="cuda")
+    N._call("sv_conv_bwd_data", N._p(gyb), N._p(lvo.table), gstride, N._p(wd), IC, OC, D, KD, s,
+            p, dil, N._p(gx2), N._p(lvi.table), xstride, 2, lvi.maxh, lvi.maxw, 0, N._p(wt),
+            N.L.stream())
     nb = ctypes.c_size_t(0)
     nchunk = 3
     N._call("sv_conv_bwd_weight_workspace_bytes", 2, IC, OC, KD, nchunk, ctypes.byref(nb))
@@ -115,6 +121,8 @@ def test_conv_backward_blocks(sv, N, case):
         (y * gyt).sum().backward()
         got = gx[n, :IC * D * h * wdt].double().cpu().numpy()
         _close(got, xt.grad.numpy(), "gx sample %d" % n)
+        got2 = gx2[n, :IC * D * h * wdt].double().cpu().numpy()
+        _close(got2, xt.grad.numpy(), "gx (transposed form) sample %d" % n)
         gw_ref += wt.grad.numpy().ravel()
         gb_ref += bt.grad.numpy()
         wt.grad = None
