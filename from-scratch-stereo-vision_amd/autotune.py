@@ -31,7 +31,7 @@ KNOBS = {"sgm_prefetch": (0, (2, 4, 8)),
          "conv_tile_width": (3, (64, 32, 16)),
          "conv_weights": (4, (1, 2)),
          "sgm_overlap": (5, (2, 1)),
-         "conv_engine": (6, (1, 2, 3, 4)),
+         "conv_engine": (6, (1, 2, 3, 4, 5)),
          "conv_mgroups": (7, (1, 2, 4)),
          "sgm_split": (8, (1, 2)),
          "wta_reads": (9, (1, 2))}
@@ -138,7 +138,8 @@ def _conv_variants():
     under each rows x tile width x weight source, its per-channel form under
     each rows x tile width, the matrix-core kernel
     (3D layers sliding over depth, or one depth per tile) under each pixel
-    groups per wave x tile width."""
+    groups per wave x tile width, and the 4-channel-block matrix-core
+    kernel (2D layers) under 1 or 2 pixel groups x tile width."""
     K = {k: v[0] for k, v in KNOBS.items()}
     out = []
     for r in KNOBS["conv_rows"][1]:
@@ -153,6 +154,9 @@ def _conv_variants():
         for g in KNOBS["conv_mgroups"][1]:
             for t in KNOBS["conv_tile_width"][1]:
                 out.append(((K["conv_engine"], e), (K["conv_mgroups"], g), (K["conv_tile_width"], t)))
+    for g in (1, 2):
+        for t in KNOBS["conv_tile_width"][1]:
+            out.append(((K["conv_engine"], 5), (K["conv_mgroups"], g), (K["conv_tile_width"], t)))
     return out
 
 

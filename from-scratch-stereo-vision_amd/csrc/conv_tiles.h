@@ -20,6 +20,7 @@
 #pragma once
 
 #include <algorithm>
+#include <type_traits>
 
 #include "common.h"
 
@@ -64,6 +65,13 @@ inline unsigned conv_grid(long long tiles_bound, long long shm) {
     if (G >= 8) G -= G % 8;
     return (unsigned)std::max<long long>(G, 1);
 }
+
+// 2D matrix-core convolution over 4-channel blocks (conv_c4.hip); returns
+// SV_EUNSUPPORTED when the shape does not fit it
+int conv_fwd_c4(const float* x, const int* idims, long long xstride, int IC, const float* w,
+                const float* b, int OC, int s, int p, int dil, float* y, const int* odims,
+                long long ystride, int N, int max_ho, int max_wo, int mode, float slope,
+                const float* resid, long long rstride, hipStream_t st);
 
 // matrix-core variant of the tiled forward convolution (mfma_conv.hip);
 // returns SV_EUNSUPPORTED when the shape does not fit it
@@ -220,6 +228,22 @@ __device__ __forceinline__ void conv_stage_inputs(float* tile, const float* __re
         }
     }
 }
+
+// Compile-time loop over taps TP .. TAPS-1: f(std::integral_constant<int, tp>)
+// (MFMA operand-select immediates need the tap as a constant).
+template <int TP, int TAPS>
+struct TapSteps {
+    template <class F>
+    __device__ __forceinline__ static void run(F& f) {
+        f(std::integral_constant<int, TP>{});
+        TapSteps<TP + 1, TAPS>::run(f);
+    }
+};
+template <int TAPS>
+struct TapSteps<TAPS, TAPS> {
+    template <class F>
+    __device__ __forceinline__ static void run(F&) {}
+};
 
 // conv epilogue: bias, then LeakyReLU (mode 1) or max(resid + v, 0) (mode 2)
 __device__ __forceinline__ float conv_finish(float acc, float bias, int mode, float slope,

@@ -149,20 +149,6 @@ __global__ void __launch_bounds__(256) k_conv_mfma(
 // A block (4 waves) makes tiles of TW columns x 4*P*(64/TW) rows, P pixel
 // groups per wave, over the launch's tile list (conv_tiles.h); OCG groups of
 // 4 output channels (OC <= 4*OCG).
-template <int TP, int TAPS>
-struct TapSteps {
-    template <class F>
-    __device__ __forceinline__ static void run(F& f) {
-        f(std::integral_constant<int, TP>{});
-        TapSteps<TP + 1, TAPS>::run(f);
-    }
-};
-template <int TAPS>
-struct TapSteps<TAPS, TAPS> {
-    template <class F>
-    __device__ __forceinline__ static void run(F&) {}
-};
-
 template <int KD, int SRC, int OCG, int TW, int P>
 __global__ void __launch_bounds__(256) k_conv_m4(
     const float* __restrict__ x, const float* __restrict__ xr, const int* __restrict__ idims,

@@ -1530,6 +1530,12 @@ int sv_conv_fwd(const float* x, const float* xr, const int* idims, long long xst
     // agree bit for bit.
     {
         const int eng = sv::tune(SV_TUNE_CONV_ENGINE);
+        if (eng == 5 && KD == 1 && D == 1 && !src) {
+            const int rc = sv::conv_fwd_c4(x, idims, xstride, IC, w, b, OC, s, p, dil, y, odims,
+                                           ystride, N, max_ho, max_wo, mode, slope, resid, rstride,
+                                           st);
+            if (rc != SV_EUNSUPPORTED) return rc;
+        }
         if (eng >= 2 || (eng == 0 && OC >= 4 && OC <= 16)) {
             const int rc = sv::conv_fwd_m4(x, xr, idims, xstride, IC, D, KD, w, b, OC, s, p, dil, y,
                                            odims, ystride, N, max_ho, max_wo, mode, slope, resid,

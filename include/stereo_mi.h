@@ -215,7 +215,9 @@ int sv_refine_ragged(int mode, const void* left, const void* right, int dtype, c
  * stream is being captured into a graph: then after the others); 6: convolution engine (1
  * vector ALUs, 2 matrix cores with 3D layers sliding over depth, 3 matrix
  * cores with one depth per tile, 4 vector ALUs staging one input channel at
- * a time); 7: 64-pixel groups per wave of the
+ * a time, 5 matrix cores reading 4-channel blocks per LDS read with the next
+ * tile prefetched, 2D layers of 2 to 12 output channels; other shapes fall
+ * back to 2/3); 7: 64-pixel groups per wave of the
  * matrix-core convolution (1, 2, 4); 8: lanes per scanline of the packed
  * SGM kernels (1, 2); 9: path-cost reads of the band epilogue (K = 21, u8
  * path costs): one lane per pixel (1) or coalesced 64-pixel runs summed per
