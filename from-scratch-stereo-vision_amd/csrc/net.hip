@@ -605,6 +605,147 @@ __global__ void __launch_bounds__(64 * RP_WAVES) k_reduce_partials(
 // ------------------------------------------------------------ elementwise
 // LeakyReLU.backward (:713-720) in place: the post-activation value has the
 // sign of the pre-activation for any slope >= 0.
+typedef float v4f32w __attribute__((ext_vector_type(4)));
+
+// Weight and bias gradient on the matrix cores (v_mfma_f32_4x4x1_16b_f32),
+// the same split reduction as k_conv_bwd_weight (partials per (sample,
+// chunk), added by k_reduce_partials).  Block (c, n): tiles c, c + nchunk,
+// ... of sample n's output (16 x 16 positions at one depth), each staged
+// with its input halo tile in LDS (conv_tiles.h staging; the cost volume
+// |fl - fr| formed there when SRC == 1).  Lanes are weight
+// columns k = (ic, kd, ky, kx); per 16 positions of a tile row, one VGPR holds
+// the output gradient of 4 output channels at all 16 positions (lane 4j + i:
+// g[oc0 + i][pos j]) and the MFMA's broadcast index picks position j:
+// acc[i] += g[oc0 + i][pos j] * x(column, pos j).  Work units (column group
+// of 64, row phase) are dealt to the 4 waves; a unit's rows are
+// r = phase, phase + PQ, ...; the phases of a column group are added in order
+// at the end, so the result does not depend on timing.
+constexpr int WG_T = 16;  // tile: 16 x 16 output positions
+constexpr int WG_UMAX = 2;  // work units per wave
+
+template <int KD, int SRC, int OCG>
+__global__ void __launch_bounds__(256) k_wgrad_m4(
+    const float* __restrict__ x, const float* __restrict__ xr, const int* __restrict__ idims,
+    long long xstride, int IC, int D, const float* __restrict__ g, const int* __restrict__ odims,
+    long long gstride, int OC, int s, int p, int dil, int nchunk, float* __restrict__ partials,
+    sv::ConvTiles ct) {
+    extern __shared__ __align__(16) float smem_f[];
+    const int IWt = ct.IWt;
+    const int plane = ct.IHt * IWt;
+    const int tsz = KD * plane;
+    float* tile = smem_f;                        // [IC][KD][IHt][IWt]
+    float* gt = smem_f + IC * tsz;               // [4*OCG][16][16]
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int n = blockIdx.z, c = blockIdx.x;
+    const int hin = dh(idims, n), win = dw(idims, n);
+    const int ho = dh(odims, n), wo = dw(odims, n);
+    const int ncol = IC * KD * 9;
+    const int NCG = (ncol + 63) / 64;
+    const int PQ = NCG >= 4 ? 1 : 4 / NCG;
+    const int nunits = NCG * PQ;
+    const int txn = (wo + WG_T - 1) / WG_T, tyn = (ho + WG_T - 1) / WG_T;
+    const int ntiles = (ho > 0 && wo > 0) ? txn * tyn * D : 0;
+    const long long oplane = (long long)ho * wo;
+    const float* xs = x + (long long)n * xstride;
+    const float* xrs = SRC == 1 ? xr + (long long)n * xstride : nullptr;
+    const float* gs = g + (long long)n * gstride;
+    // per unit: this lane's column and its offset in the staged tile
+    int coff[WG_UMAX], rph[WG_UMAX];
+    bool cok[WG_UMAX];
+#pragma unroll
+    for (int k = 0; k < WG_UMAX; ++k) {
+        const int u = wave + 4 * k;
+        const int cg = u % NCG;
+        rph[k] = u / NCG;
+        const int col = cg * 64 + lane;
+        cok[k] = u < nunits && col < ncol;
+        const int cc = cok[k] ? col : 0;
+        const int ic = cc / (KD * 9);
+        const int r = cc - ic * KD * 9;
+        const int kd = r / 9, ky = (r - kd * 9) / 3, kx = r - kd * 9 - ky * 3;
+        coff[k] = ic * tsz + kd * plane + ky * dil * IWt + kx * dil;
+    }
+    v4f32w acc[WG_UMAX][OCG];
+#pragma unroll
+    for (int k = 0; k < WG_UMAX; ++k)
+#pragma unroll
+        for (int q = 0; q < OCG; ++q) acc[k][q] = (v4f32w){0.f, 0.f, 0.f, 0.f};
+    float bsum = 0.f;  // bias: thread tid < 16*OC sums row tid%16 of channel tid/16
+    for (int t = c; t < ntiles; t += nchunk) {
+        const int od = t % D;
+        const int tt = t / D;
+        const int ty0 = (tt / txn) * WG_T, tx0 = (tt % txn) * WG_T;
+        __syncthreads();
+        conv_stage_inputs<KD, SRC>(tile, xs, xrs, 0, IC, D, od, ty0 * s - p, tx0 * s - p, hin, win,
+                                   ct, tid);
+        for (int e = tid; e < 4 * OCG * 256; e += 256) {
+            const int oc = e >> 8, r = (e >> 4) & 15, j = e & 15;
+            const int oy = ty0 + r, ox = tx0 + j;
+            gt[e] = (oc < OC && oy < ho && ox < wo)
+                        ? gs[(long long)oc * D * oplane + (long long)od * oplane + (long long)oy * wo + ox]
+                        : 0.f;
+        }
+        __syncthreads();
+        if (tid < 16 * OC) {
+            const float* gr = gt + (tid >> 4) * 256 + (tid & 15) * 16;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) bsum += gr[j];
+        }
+#pragma unroll
+        for (int k = 0; k < WG_UMAX; ++k) {
+            if (wave + 4 * k >= nunits) break;
+            for (int r = rph[k]; r < WG_T; r += PQ) {
+                float av[OCG];
+#pragma unroll
+                for (int q = 0; q < OCG; ++q) av[q] = gt[(4 * q + (lane & 3)) * 256 + r * 16 + (lane >> 2)];
+                const float* xb = tile + coff[k] + r * s * IWt;
+                float bv[16];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) bv[j] = xb[j * s];
+                auto step = [&](auto jc) {
+                    constexpr int j = decltype(jc)::value;
+#pragma unroll
+                    for (int q = 0; q < OCG; ++q)
+                        acc[k][q] = __builtin_amdgcn_mfma_f32_4x4x1f32(av[q], bv[j], acc[k][q], 4, j, 0);
+                };
+                TapSteps<0, 16>::run(step);
+            }
+        }
+    }
+    // combine: the row phases of each column group in order, then write
+    __syncthreads();
+    float* red = smem_f;  // [unit][4*OCG][64]
+#pragma unroll
+    for (int k = 0; k < WG_UMAX; ++k) {
+        const int u = wave + 4 * k;
+        if (u >= nunits) break;
+#pragma unroll
+        for (int q = 0; q < OCG; ++q)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) red[(u * 4 * OCG + 4 * q + i) * 64 + lane] = acc[k][q][i];
+    }
+    float* bred = smem_f + nunits * 4 * OCG * 64;  // [OC][16]
+    if (tid < 16 * OC) bred[tid] = bsum;
+    __syncthreads();
+    const int ncolp = ncol + 1;
+    float* out = partials + ((long long)n * nchunk + c) * (long long)OC * ncolp;
+    for (int e = tid; e < NCG * 64 * OC; e += 256) {
+        const int oc = e / (NCG * 64);
+        const int col = e - oc * (NCG * 64);
+        if (col >= ncol) continue;
+        const int cg = col >> 6, l = col & 63;
+        float v = 0.f;
+        for (int ph = 0; ph < PQ; ++ph) v += red[((ph * NCG + cg) * 4 * OCG + oc) * 64 + l];
+        out[(long long)oc * ncolp + col] = v;
+    }
+    if (tid < OC) {
+        float v = 0.f;
+        for (int r = 0; r < 16; ++r) v += bred[tid * 16 + r];
+        out[(long long)tid * ncolp + ncol] = v;
+    }
+}
+
 __global__ void k_act_bwd(float* __restrict__ g, const float* __restrict__ y, long long n,
                           float slope) {
     const long long i = blockIdx.x * 256LL + threadIdx.x;
@@ -1727,8 +1868,43 @@ int sv_conv_bwd_weight(const float* x, const float* xr, const int* idims, long l
     sv_conv_bwd_weight_workspace_bytes(N, IC, OC, KD, nchunk, &need);
     if (ws_bytes < need) return sv::set_error(SV_EWORKSPACE, "conv weight gradient needs %zu bytes", need);
     const int ncol = IC * KD * 9;
-    dim3 grid(nchunk, (ncol + WG_COLS - 1) / WG_COLS, N);
     hipStream_t st = (hipStream_t)stream;
+    // matrix-core kernel when every input channel's halo tile, the gradient
+    // tile and the combine buffer fit 64 KB and the columns fit its units
+    {
+        sv::ConvTiles ct;
+        const long long shm = sv::conv_tiles(KD, IC, WG_T, WG_T, s, dil, N, 0, 64 * 1024, ct);
+        const int OCG = (OC + 3) / 4;
+        const int NCG = (ncol + 63) / 64, PQ = NCG >= 4 ? 1 : 4 / NCG;
+        const long long tb = 4LL * IC * KD * ct.IHt * ct.IWt + 4LL * 4 * OCG * WG_T * WG_T;
+        const long long rb = 4LL * (NCG * PQ * 4 * OCG * 64 + OC * 16);
+        const long long shn = std::max(tb, rb);
+        if ((KD == 3 || !xr) && (KD == 1 || KD == 3) && shm > 0 && ct.ICG >= IC && OC <= 12 &&
+            NCG * PQ <= 4 * WG_UMAX && shn <= 64 * 1024) {
+            dim3 gn(nchunk, 1, N);
+#define LAUNCH_WG(KD_, SRC_, OCG_)                                                               \
+    hipLaunchKernelGGL((k_wgrad_m4<KD_, SRC_, OCG_>), gn, dim3(256), (size_t)shn, st, x, xr, idims, \
+                       xstride, IC, D, g, odims, gstride, OC, s, p, dil, nchunk, partials, ct)
+#define LAUNCH_WG_OC(KD_, SRC_)                                                                  \
+    do {                                                                                         \
+        if (OCG == 1) LAUNCH_WG(KD_, SRC_, 1);                                                   \
+        else if (OCG == 2) LAUNCH_WG(KD_, SRC_, 2);                                              \
+        else LAUNCH_WG(KD_, SRC_, 3);                                                            \
+    } while (0)
+            if (KD == 1) LAUNCH_WG_OC(1, 0);
+            else if (xr) LAUNCH_WG_OC(3, 1);
+            else LAUNCH_WG_OC(3, 0);
+#undef LAUNCH_WG_OC
+#undef LAUNCH_WG
+            SV_CHECK_LAUNCH();
+            const int tot = OC * (ncol + 1);
+            hipLaunchKernelGGL(k_reduce_partials, dim3((tot + 63) / 64), dim3(64 * RP_WAVES), 0, st,
+                               partials, N * nchunk, OC, ncol, gw, gb);
+            SV_CHECK_LAUNCH();
+            return SV_OK;
+        }
+    }
+    dim3 grid(nchunk, (ncol + WG_COLS - 1) / WG_COLS, N);
 #define LAUNCH_BW(SRC_, OCM_)                                                                  \
     hipLaunchKernelGGL((k_conv_bwd_weight<SRC_, OCM_>), grid, dim3(WG_COLS), 0, st, x, xr, idims, \
                        xstride, IC, D, KD, g, odims, gstride, OC, s, p, dil, nchunk, partials)

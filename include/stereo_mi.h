@@ -269,7 +269,10 @@ int sv_conv_bwd_data(const float* g, const int* odims, long long gstride, const 
 
 /* Conv2d/Conv3d.backward weight and bias gradients (accumulated into gw,
  * gb, like the reference's +=); partials is a workspace of the size the
- * query returns. */
+ * query returns.  Each (sample, position chunk) sums its share into a
+ * partial, on the matrix cores when the layer's input tile fits (up to 12
+ * output and 512 weight columns), else on the vector ALUs; the partials are
+ * then added in a fixed order, so the result is the same on every run. */
 int sv_conv_bwd_weight_workspace_bytes(int N, int IC, int OC, int KD, int nchunk, size_t* bytes);
 int sv_conv_bwd_weight(const float* x, const float* xr, const int* idims, long long xstride,
                        int IC, int D, int KD, const float* g, const int* odims, long long gstride,
