@@ -80,6 +80,13 @@ int conv_fwd_m4(const float* x, const float* xr, const int* idims, long long xst
                 float* y, const int* odims, long long ystride, int N, int max_ho, int max_wo,
                 int mode, float slope, const float* resid, long long rstride, hipStream_t st);
 
+// matrix-core 3D convolution over a column of output depths per tile
+// (conv3d_col.hip); returns SV_EUNSUPPORTED when the shape does not fit it
+int conv_fwd_col(const float* x, const float* xr, const int* idims, long long xstride, int IC,
+                 int D, int KD, const float* w, const float* b, int OC, int s, int p, int dil,
+                 float* y, const int* odims, long long ystride, int N, int max_ho, int max_wo,
+                 int mode, float slope, const float* resid, long long rstride, hipStream_t st);
+
 }  // namespace sv
 
 namespace {

@@ -1664,17 +1664,23 @@ int sv_conv_fwd(const float* x, const float* xr, const int* idims, long long xst
     hipStream_t st = (hipStream_t)stream;
     const int src = xr ? 1 : 0;
     SV_REQUIRE(xstride < (1LL << 31), "sv_conv_fwd: a sample's input must hold fewer than 2^31 values");
-    // engine: the matrix-core kernel when chosen (tuning knob) or, by
-    // default, for 4 to 16 output channels; the vector kernel otherwise or
-    // when the shape does not fit the matrix-core kernel.  Both sum every
-    // output in the same order with single-rounding multiply-adds, so they
-    // agree bit for bit.
+    // engine: the matrix-core kernels when chosen (tuning knob) or, by
+    // default, for 4 to 16 output channels (3D layers: one column of output
+    // depths per tile); the vector kernel otherwise or when the shape does
+    // not fit a matrix-core kernel.  All sum every output in the same order
+    // with single-rounding multiply-adds, so they agree bit for bit.
     {
         const int eng = sv::tune(SV_TUNE_CONV_ENGINE);
         if (eng == 5 && KD == 1 && D == 1 && !src) {
             const int rc = sv::conv_fwd_c4(x, idims, xstride, IC, w, b, OC, s, p, dil, y, odims,
                                            ystride, N, max_ho, max_wo, mode, slope, resid, rstride,
                                            st);
+            if (rc != SV_EUNSUPPORTED) return rc;
+        }
+        if (KD == 3 && (eng == 6 || (eng == 0 && OC >= 4 && OC <= 16))) {
+            const int rc = sv::conv_fwd_col(x, xr, idims, xstride, IC, D, KD, w, b, OC, s, p, dil, y,
+                                            odims, ystride, N, max_ho, max_wo, mode, slope, resid,
+                                            rstride, st);
             if (rc != SV_EUNSUPPORTED) return rc;
         }
         if (eng >= 2 || (eng == 0 && OC >= 4 && OC <= 16)) {

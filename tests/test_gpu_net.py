@@ -331,7 +331,10 @@ _CONV_CASES = [
     (8, 8, 1, 0, 1, 4, 1, 1),
     (8, 1, 1, 0, 1, 1, 1, 2),
     (12, 6, 3, 1, 1, 1, 5, 1),
+    (12, 6, 3, 1, 1, 1, 12, 1),
     (6, 6, 3, 0, 1, 1, 5, 1),
+    (6, 6, 3, 0, 1, 1, 12, 1),
+    (6, 6, 3, 0, 1, 1, 7, 0),
     (6, 1, 3, 0, 1, 1, 5, 0),
     (5, 16, 1, 0, 1, 2, 1, 1),
     (3, 13, 3, 0, 1, 1, 4, 0),
@@ -342,11 +345,12 @@ _CONV_CASES = [
 ]
 
 
-@pytest.mark.parametrize("case", _CONV_CASES, ids=lambda c: "ic%d_oc%d_kd%d_src%d_s%d_d%d" % c[:6])
+@pytest.mark.parametrize("case", _CONV_CASES, ids=lambda c: "ic%d_oc%d_kd%d_src%d_s%d_d%d_D%d" % c[:7])
 def test_conv_variants_bit_identical_and_match_float64(sv, N, case):
     """sv_conv_fwd on a ragged batch of 3 samples under every kernel variant
     the autotuner may pick (vector ALUs: rows per thread x tile width x
-    weight source; matrix cores: pixel groups per wave x tile width): every
+    weight source; matrix cores: pixel groups per wave x tile width; 3D
+    layers also the depth-column kernel per tile width x depth chunk): every
     variant's output is identical, and it equals the float64 convolution
     (torch conv2d/conv3d; cost volume |fl(x) - fr(x-d)|, 2.0 where x < d)
     to 2e-4 on unit-scale data."""
@@ -399,7 +403,7 @@ def test_conv_variants_bit_identical_and_match_float64(sv, N, case):
     y = torch.zeros(Nn, ystride, device=dev)
     first = None
     try:
-        for variant in A._conv_variants():
+        for variant in A._conv_variants(KD):
             A.reset()
             for knob, v in variant:
                 L.check(L.lib().sv_tune_set(knob, v))
