@@ -216,16 +216,18 @@ int sv_refine_ragged(int mode, const void* left, const void* right, int dtype, c
  * vector ALUs, 2 matrix cores with 3D layers sliding over depth, 3 matrix
  * cores with one depth per tile, 4 vector ALUs staging one input channel at
  * a time, 5 matrix cores reading 4-channel blocks per LDS read with the next
- * tile prefetched, 2D layers of 2 to 12 output channels; other shapes fall
- * back to 2/3); 7: 64-pixel groups per wave of the
+ * tile prefetched, 2D layers of 2 to 12 output channels; 6 matrix cores
+ * over a column of output depths per tile, 3D layers of up to 16 output
+ * channels; other shapes fall back to 2/3); 7: 64-pixel groups per wave of the
  * matrix-core convolution (1, 2, 4); 8: lanes per scanline of the packed
  * SGM kernels (1, 2); 9: path-cost reads of the band epilogue (K = 21, u8
  * path costs): one lane per pixel (1) or coalesced 64-pixel runs summed per
- * lane and handed out through LDS (2).  Value 0 restores the built-in
+ * lane and handed out through LDS (2); 10: output depths per tile of the
+ * depth-column 3D kernel (4, 6, 12).  Value 0 restores the built-in
  * heuristic (LDS weights; horizontal paths after the others; matrix cores
- * for 4 to 16 output channels, 3D layers sliding over depth chunks; the
- * per-channel vector kernel when the input channels do not fit one staging
- * pass).
+ * for 4 to 16 output channels, 3D layers over depth columns of 6, or 4 when
+ * 6 leaves CUs idle; the per-channel vector kernel when the input channels
+ * do not fit one staging pass).
  * Process-wide; set between launches, not while kernels are being queued
  * from other threads. */
 int sv_tune_set(int knob, int value);
