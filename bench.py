@@ -104,7 +104,7 @@ def main():
                     help="time the step as individual launches only (default: also capture it "
                          "in one HIP graph, check the replay equals the eager output, and report "
                          "the replay rate)")
-    ap.add_argument("--streams", type=int, default=2,
+    ap.add_argument("--streams", type=int, default=1,
                     help="graph replay: cut the batch into this many equal parts, each on its "
                          "own stream with its own network buffers, captured together so that one "
                          "part's kernels fill the other's gaps (1: the eager step as it is)")

@@ -180,9 +180,14 @@ def tune_layers(net, run, reps=3, log=None):
 
     calls = {}
 
-    def hook(key, fn, args, out):
+    def hook(key, fn, args, out, tables):
+        # the dims tables as they are at this launch (a later launch of the
+        # same plan may refill them on the device)
         if fn == "sv_conv_fwd" and key not in calls:
-            calls[key] = (args, out)
+            ti, to = tables[0].clone(), tables[1].clone()
+            args = list(args)
+            args[2], args[14] = N._p(ti), N._p(to)
+            calls[key] = (tuple(args), out, (ti, to))
 
     net._conv_hook = hook
     try:
@@ -192,7 +197,7 @@ def tune_layers(net, run, reps=3, log=None):
         net._conv_hook = None
     chosen = {}
     try:
-        for key, (args, out) in calls.items():
+        for key, (args, out, _) in calls.items():
             best, ref = None, None
             for variant in _conv_variants(args[6]):
                 reset()

@@ -370,6 +370,10 @@ class _Plan:
         self._g = None
         self._wt = None
         self.has_extra = False
+        # launch-shape tag of the per-layer kernel variants: the passes of
+        # predict_auto run one plan at different per-pair sizes (filled on
+        # the device), so each pass keys its own choices
+        self.tag = 0
 
     # slot strides
     def out(self, level, n):
@@ -467,9 +471,9 @@ class _Plan:
         args = (_p(x), _p(xr), _p(xlv.table), xst, c.ic, D, c.kd, _p(P[c.w_off:]), _p(P[c.b_off:]),
                 c.oc, c.s, c.p, c.dil, _p(y), _p(ylv.table), c.oc * D * ylv.slot, N, ylv.maxh,
                 ylv.maxw, mode, SLOPE, _p(resid), rstride, L.stream())
-        key = (c.w_off, D, N, ylv.maxh, ylv.maxw)
+        key = (c.w_off, D, N, ylv.maxh, ylv.maxw, self.tag)
         if self.net._conv_hook is not None:
-            self.net._conv_hook(key, fn, args, y[:c.oc * D * ylv.slot * N])
+            self.net._conv_hook(key, fn, args, y[:c.oc * D * ylv.slot * N], (xlv.table, ylv.table))
         variant = self.net._conv_tune.get(key)
         if variant is None:
             _call(fn, *args)
@@ -1118,6 +1122,7 @@ def predict_auto_batch(net, left, right, W, H, target_peak=None, probe_width=64,
               passes, _p(stt.works), _p(stt.active), _p(trace), _p(stt.tables), _p(stt.inv),
               L.stream())
         plw.load_resized(left, right, W, H, src)
+        plw.tag = it + 1
         plw.forward()
         net._cache = plw
         # back to full resolution, values scaled by 1/scale (:3957-3959);
