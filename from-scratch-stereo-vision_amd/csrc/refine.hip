@@ -803,6 +803,9 @@ struct PackedBand {
     static constexpr int OFF = K + 3;            // u16 index of M[0]; M[-1] at OFF-1
     static constexpr int SLOT16 = 3 * K + 6;
     static constexpr int SD = ((SLOT16 + 1) / 2) | 1;  // slot dwords per lane, odd
+    // the dword reads of the re-centred minima (u16 index OFF + shift,
+    // |shift| <= K + 2) start at or after dword 0 and end inside the slot
+    static_assert(OFF - (K + 2) >= 0 && (OFF + K + 2) / 2 + NP < SD, "minima reads in the slot");
 };
 
 // One step of the packed recurrence for one scanline: costs cw (u8, 2*NB
@@ -829,12 +832,17 @@ __device__ __forceinline__ void packed_step(u16x2 (&L)[PackedBand<K>::NP], uint3
         const uint32_t cap = mprev + (uint32_t)P2;
         const u16x2 capv = {(unsigned short)cap, (unsigned short)cap};
         const u16x2 mpv = {(unsigned short)mprev, (unsigned short)mprev};
-        const u16_alias* g = s16 + OFF + sh;
+        // the 2*NP neighbour minima from u16 index q0 on: NP + 1 aligned
+        // dwords, each pair funnel-shifted by 16 bits when q0 is odd
+        const int q0 = OFF + sh;
+        const u32_alias* gw = sl32 + (q0 >> 1);
+        const uint32_t fs = (uint32_t)(q0 & 1) << 4;
+        uint32_t wv[NP + 1];
+#pragma unroll
+        for (int j = 0; j <= NP; ++j) wv[j] = gw[j];
 #pragma unroll
         for (int j = 0; j < NP; ++j) {
-            u16x2 m2;
-            m2.x = g[2 * j];
-            m2.y = g[2 * j + 1];
+            const u16x2 m2 = as_u16x2(__builtin_amdgcn_alignbit(wv[j + 1], wv[j], fs));
             const u16x2 best = __builtin_elementwise_min(m2, capv);
             L[j] = c[j] + best - mpv;
         }
@@ -1103,6 +1111,8 @@ struct PackedBand2 {
         return (2 * NPH + 2 * j >= K ? 0x0000FFFFu : 0u) | (2 * NPH + 2 * j + 1 >= K ? 0xFFFF0000u : 0u);
     }
     static_assert(2 * NPH <= K, "lane 0 holds no pads");
+    static_assert(OFF - (K + 2) >= 0 && (OFF + K + 2 + 2 * NPH) / 2 + NPH < SD,
+                  "minima reads in the slot");
 };
 
 __device__ __forceinline__ uint32_t swap_pair(uint32_t v) {
@@ -1134,12 +1144,18 @@ __device__ __forceinline__ void packed_step2(u16x2 (&L)[PackedBand2<K>::NPH], ui
         const uint32_t cap = mprev + (uint32_t)P2;
         const u16x2 capv = {(unsigned short)cap, (unsigned short)cap};
         const u16x2 mpv = {(unsigned short)mprev, (unsigned short)mprev};
-        const u16_alias* gm = s16 + OFF + sh + 2 * NPH * h;
+        // the 2*NPH neighbour minima from u16 index q0 on: NPH + 1 aligned
+        // dwords, each pair funnel-shifted by 16 bits when q0 is odd
+        // (q0 >= OFF - (K + 2) > 0, and the last dword stays inside the slot)
+        const int q0 = OFF + sh + 2 * NPH * h;
+        const u32_alias* gw = sl32 + (q0 >> 1);
+        const uint32_t fs = (uint32_t)(q0 & 1) << 4;
+        uint32_t wv[NPH + 1];
+#pragma unroll
+        for (int j = 0; j <= NPH; ++j) wv[j] = gw[j];
 #pragma unroll
         for (int j = 0; j < NPH; ++j) {
-            u16x2 m2;
-            m2.x = gm[2 * j];
-            m2.y = gm[2 * j + 1];
+            const u16x2 m2 = as_u16x2(__builtin_amdgcn_alignbit(wv[j + 1], wv[j], fs));
             const u16x2 best = __builtin_elementwise_min(m2, capv);
             L[j] = c[j] + best - mpv;
         }
