@@ -31,7 +31,7 @@ KNOBS = {"sgm_prefetch": (0, (2, 4, 8)),
          "conv_tile_width": (3, (64, 32, 16)),
          "conv_weights": (4, (1, 2)),
          "sgm_overlap": (5, (2, 1)),
-         "conv_engine": (6, (1, 2, 3, 4, 5, 6)),
+         "conv_engine": (6, (1, 2, 3, 4, 5, 6, 7)),
          "conv_mgroups": (7, (1, 2, 4)),
          "sgm_split": (8, (1, 2)),
          "wta_reads": (9, (1, 2)),
@@ -134,15 +134,16 @@ def tune(run, key=None, knobs=None, reps=3, cache_path=None, log=None):
 
 
 
-def _conv_variants(kd=1):
+def _conv_variants(kd=1, oc=16):
     """Every convolution variant as ((knob, value), ...): the vector kernel
     under each rows x tile width x weight source, its per-channel form under
     each rows x tile width, the matrix-core kernel
     (3D layers sliding over depth, or one depth per tile) under each pixel
     groups per wave x tile width, the 4-channel-block matrix-core
-    kernel (2D layers) under 1 or 2 pixel groups x tile width, and for 3D
+    kernel (2D layers) under 1 or 2 pixel groups x tile width, for 3D
     layers (kd = 3) the depth-column matrix-core kernel under each tile
-    width x depth chunk."""
+    width x depth chunk, and for layers of at most 4 output channels the
+    direct kernel."""
     K = {k: v[0] for k, v in KNOBS.items()}
     out = []
     for r in KNOBS["conv_rows"][1]:
@@ -164,6 +165,8 @@ def _conv_variants(kd=1):
         for t in KNOBS["conv_tile_width"][1]:
             for dc in KNOBS["conv_dchunk"][1]:
                 out.append(((K["conv_engine"], 6), (K["conv_tile_width"], t), (K["conv_dchunk"], dc)))
+    if oc <= 4:
+        out.append(((K["conv_engine"], 7),))
     return out
 
 
@@ -199,7 +202,7 @@ def tune_layers(net, run, reps=3, log=None):
     try:
         for key, (args, out, _) in calls.items():
             best, ref = None, None
-            for variant in _conv_variants(args[6]):
+            for variant in _conv_variants(args[6], args[9]):
                 reset()
                 for knob, v in variant:
                     L.check(L.lib().sv_tune_set(knob, v))

@@ -27,7 +27,7 @@
 #define SV_TUNE_CONV_TW 3       // output columns per tile of the tiled convolution: 16, 32, 64
 #define SV_TUNE_CONV_WSRC 4     // weights of the tiled convolution: 1 LDS, 2 scalar cache
 #define SV_TUNE_SGM_OVERLAP 5   // horizontal SGM paths: 1 after the others, 2 beside them on a second stream
-#define SV_TUNE_CONV_ENGINE 6   // tiled convolution: 1 vector ALUs, 2 matrix cores (FP32 4x4x1 MFMA; 3D layers slide over depth), 3 matrix cores, one depth per tile, 4 vector ALUs, one channel per staging pass, 5 matrix cores over 4-channel blocks with the next tile prefetched (2D), 6 matrix cores over a column of output depths per tile (3D)
+#define SV_TUNE_CONV_ENGINE 6   // tiled convolution: 1 vector ALUs, 2 matrix cores (FP32 4x4x1 MFMA; 3D layers slide over depth), 3 matrix cores, one depth per tile, 4 vector ALUs, one channel per staging pass, 5 matrix cores over 4-channel blocks with the next tile prefetched (2D), 6 matrix cores over a column of output depths per tile (3D), 7 direct, one output position per thread, no LDS (1 to 4 output channels)
 #define SV_TUNE_CONV_MGROUPS 7  // matrix-core convolution: 64-pixel groups per wave, 1, 2, 4
 #define SV_TUNE_SGM_SPLIT 8     // packed SGM kernels: 1 one lane per scanline, 2 two lanes per scanline
 #define SV_TUNE_WTA_READS 9     // band epilogue (K = 21, u8): 1 one lane per pixel, 2 coalesced runs through LDS

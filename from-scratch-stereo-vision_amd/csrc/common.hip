@@ -44,15 +44,15 @@ int sv_version(void) { return 1; }
 
 // Allowed values per knob (0 always allowed: back to the heuristic).
 int sv_tune_set(int knob, int value) {
-    static const int allowed[SV_TUNE_COUNT][6] = {
-        {2, 4, 8, -1, -1, -1}, {8, 16, -1, -1, -1, -1}, {1, 4, -1, -1, -1, -1},
-        {16, 32, 64, -1, -1, -1}, {1, 2, -1, -1, -1, -1}, {1, 2, -1, -1, -1, -1},
-        {1, 2, 3, 4, 5, 6}, {1, 2, 4, -1, -1, -1}, {1, 2, -1, -1, -1, -1},
-        {1, 2, -1, -1, -1, -1}, {4, 6, 12, -1, -1, -1}};
+    static const int allowed[SV_TUNE_COUNT][7] = {
+        {2, 4, 8, -1, -1, -1, -1}, {8, 16, -1, -1, -1, -1, -1}, {1, 4, -1, -1, -1, -1, -1},
+        {16, 32, 64, -1, -1, -1, -1}, {1, 2, -1, -1, -1, -1, -1}, {1, 2, -1, -1, -1, -1, -1},
+        {1, 2, 3, 4, 5, 6, 7}, {1, 2, 4, -1, -1, -1, -1}, {1, 2, -1, -1, -1, -1, -1},
+        {1, 2, -1, -1, -1, -1, -1}, {4, 6, 12, -1, -1, -1, -1}};
     if (knob < 0 || knob >= SV_TUNE_COUNT)
         return sv::set_error(SV_EINVAL, "tuning knob %d does not exist", knob);
     bool ok = value == 0;
-    for (int i = 0; i < 6; ++i) ok = ok || allowed[knob][i] == value;
+    for (int i = 0; i < 7; ++i) ok = ok || allowed[knob][i] == value;
     if (!ok) return sv::set_error(SV_EINVAL, "value %d is not allowed for tuning knob %d", value, knob);
     sv::g_tune[knob].store(value);
     return SV_OK;

@@ -403,7 +403,7 @@ def test_conv_variants_bit_identical_and_match_float64(sv, N, case):
     y = torch.zeros(Nn, ystride, device=dev)
     first = None
     try:
-        for variant in A._conv_variants(KD):
+        for variant in A._conv_variants(KD, OC):
             A.reset()
             for knob, v in variant:
                 L.check(L.lib().sv_tune_set(knob, v))
