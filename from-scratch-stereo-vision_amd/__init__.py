@@ -21,7 +21,12 @@ from .images import (read_pnm, read_png, read_image, read_image_gpu, read_images
                      write_ppm)
 from .geometry import (Camera, read_calib, robust_min_disparity, disparity_to_points,
                        min_filter_arg, smooth_background, background_region, grid_mesh)
-from . import autotune, scenes, synth, evaluation, geometry
+from .mesh import (delaunay, cull_triangles, cull_slanted, select_points, write_obj,
+                   background_layer, merge_layer)
+from .splat import (GaussianCloud, gaussians_from_mesh, SplatRenderer, CloudAdam,
+                    refine_gaussians, look_at, cloud_centroid, orbit_views, reconstruct,
+                    ALPHA_MAX, IDENTITY_R, IDENTITY_T)
+from . import autotune, scenes, synth, evaluation, geometry, mesh, splat
 
 __all__ = ["census", "band_costs", "_aggregate_path", "refine_fullres", "full_range", "_mirror1",
            "left_to_right", "lr_refine", "median_filter", "refine_disparity", "slic", "plane_snap",
@@ -35,4 +40,8 @@ __all__ = ["census", "band_costs", "_aggregate_path", "refine_fullres", "full_ra
            "read_image", "read_image_gpu", "read_images_gpu", "write_pgm", "write_ppm",
            "Camera", "read_calib", "robust_min_disparity", "disparity_to_points", "min_filter_arg",
            "smooth_background", "background_region", "grid_mesh",
-           "autotune", "scenes", "synth", "evaluation", "geometry"]
+           "delaunay", "cull_triangles", "cull_slanted", "select_points", "write_obj",
+           "background_layer", "merge_layer", "GaussianCloud", "gaussians_from_mesh",
+           "SplatRenderer", "CloudAdam", "refine_gaussians", "look_at", "cloud_centroid",
+           "orbit_views", "reconstruct", "ALPHA_MAX", "IDENTITY_R", "IDENTITY_T",
+           "autotune", "scenes", "synth", "evaluation", "geometry", "mesh", "splat"]

@@ -26,7 +26,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
           "-Wno-unused-command-line-argument"]
 # per-file floating-point contraction: exact files never fuse multiply-adds
 EXACT = {"refine.hip", "common.hip", "planes.hip", "imageio.hip", "evalmetrics.hip",
-         "geometry.hip"}
+         "geometry.hip", "mesh.hip", "splat.hip"}
 
 
 def sources():

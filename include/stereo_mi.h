@@ -498,6 +498,67 @@ int sv_grid_mesh_build(const double* val, const uint8_t* region, const int* src,
                        long long vcap, long long tcap, int* totals, void* ws, size_t ws_bytes,
                        void* stream);
 
+/* Mesh and Gaussian cloud of the downstream 3D output (csrc/mesh.hip host
+ * code, csrc/splat.hip kernels); float64 with the reference's own fsqrt /
+ * fexp / flog, results equal to the reference's.
+ *
+ * sv_delaunay: delaunay, stereonet.py:2052-2141 (Bowyer-Watson); xy = n
+ *   (x, y) points (host); up to cap index triples to tris, *count = their
+ *   number (an error beyond cap).  Host code, no GPU call.
+ * sv_cull_triangles: cull_triangles, :2144-2199; vp = the triangles'
+ *   vertices' 3D points [nv][3] with valid flags vv (host), kept triples to
+ *   out in input order.  Host code.
+ * sv_cull_slanted: cull_slanted, :2324-2372 (same arrays).  Host code.
+ * sv_select_points: select_points, :2375-2482 on a host disparity map
+ *   (valid may be null); (x, y) pairs to xy.  Host code. */
+int sv_delaunay(const double* xy, int n, int* tris, int cap, int* count);
+int sv_cull_triangles(const double* vp, const uint8_t* vv, int nv, const int* tris, int nt,
+                      double max_z_ratio, double max_edge_factor, int* out, int* count);
+int sv_cull_slanted(const double* vp, const uint8_t* vv, int nv, const int* tris, int nt,
+                    double max_slant_cos, int* out, int* count);
+int sv_select_points(const double* disp, int W, int H, const uint8_t* valid, int target,
+                     double edge_fraction, int* xy, int cap, int* count);
+
+/* sv_gaussians_from_mesh: gaussians_from_mesh, :2546-2659; device arrays:
+ *   vertices xy [nv][2], triangles [nt][3], per-pixel points [W*H][3] with
+ *   valid flags, colour [W*H][3]; the kept triangles' Gaussians go to cloud
+ *   [14][cap] (mx my mz sxx sxy sxz syy syz szz cr cg cb raw_op raw_sc) in
+ *   triangle order, their number to *count (device int); raw_op and raw_sc
+ *   are the initial raw opacity and log scale.
+ * sv_splat_workspace_bytes / sv_splat_prepare: SplatRenderer._prepare,
+ *   :2717-2803: projection with the world-to-camera R (row-major 3x3) and t
+ *   (host), camera fx, cx, cy, blur and cutoff; the kept items ranked by
+ *   depth (ties in Gaussian order) into the workspace; info (device, 2
+ *   long longs, may be null) = kept items, their total footprint area.
+ * sv_splat_render: SplatRenderer.render, :2807-2852, after prepare: colour C
+ *   [H*W][3] and final transmittance T [H*W] (device).
+ * sv_splat_backward: SplatRenderer.backward, :2854-2962: adds the gradients
+ *   of sum(gC * C) to grads [5][cap] (colour r, g, b, raw opacity, raw
+ *   scale); contrib (5 * total doubles) and present (total bytes) are
+ *   scratch for the footprint area total of the prepared workspace.
+ * sv_photometric_mse: refine_gaussians' loss, :3017-3027: gC = 2 (C - target)
+ *   / n, mean squared error to *loss (device), part = 256 doubles of scratch.
+ * sv_cloud_adam: CloudAdam.step, :2982-3007, m and v [5][cap]; c1, c2 the
+ *   bias corrections 1 / (1 - b^t) of the caller's running products. */
+int sv_gaussians_from_mesh(const int* xy, int nv, const int* tris, int nt, const double* pts,
+                           const uint8_t* pvalid, const double* rgb, int W, int H,
+                           double thickness, double raw_op, double raw_sc, double* cloud, int cap,
+                           int* count, void* stream);
+int sv_splat_workspace_bytes(int n, size_t* bytes);
+int sv_splat_prepare(const double* cloud, int n, int cap, const double* R, const double* t,
+                     double fx, double cx, double cy, int W, int H, double blur, double cutoff,
+                     void* ws, size_t ws_bytes, long long* info, void* stream);
+int sv_splat_render(const void* ws, int n, int W, int H, double br, double bg, double bb,
+                    double* C, double* T, void* stream);
+int sv_splat_backward(const void* ws, int n, int W, int H, const double* gC, const double* T,
+                      double* contrib, uint8_t* present, long long total, double* grads, int cap,
+                      void* stream);
+int sv_photometric_mse(const double* C, const double* target, long long n, double* gC,
+                       double* loss, double* part, void* stream);
+int sv_cloud_adam(double* cloud, const double* grads, double* m, double* v, int n, int cap,
+                  double c1, double c2, double lr_colour, double lr_op, double lr_scale,
+                  void* stream);
+
 #ifdef __cplusplus
 }
 #endif
