@@ -15,9 +15,12 @@
 //   them in order exactly as the reference's inner loop does;
 // - backward: the same walk back to front per pixel; each pixel writes its
 //   contribution to every Gaussian it touches into that Gaussian's slot of
-//   a per-footprint buffer, and one thread per Gaussian then sums its
-//   footprint in the reference's row-major pixel order, so the gradients
-//   agree bit for bit (no atomics, no reassociation).
+//   a per-footprint buffer (zero where the pixel adds nothing), and one
+//   thread per Gaussian then sums its footprint in the reference's
+//   row-major pixel order, so the gradients agree bit for bit (no atomics,
+//   no reassociation; an added +0.0 changes no sum, since a sum that starts
+//   at +0.0 is never -0.0).
+#include <algorithm>
 #include <cstdint>
 
 #include "common.h"
@@ -246,33 +249,26 @@ __global__ void k_splat_project(const double* __restrict__ cloud, int n, int cap
 }
 
 // rank of each kept item: kept items in front (smaller depth), or equally
-// deep and earlier; the items land in depth order, ties in Gaussian order
+// deep and earlier; the items land in depth order, ties in Gaussian order.
+// One wavefront per item, its lanes counting over strided candidates.
 __global__ void __launch_bounds__(256) k_splat_rank(const double* __restrict__ raw,
                                                     const int* __restrict__ rawi,
                                                     const uint8_t* __restrict__ keep, int n,
                                                     double* __restrict__ items,
                                                     int* __restrict__ itemi) {
-    __shared__ double zs[256];
-    __shared__ uint8_t ks[256];
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    const bool mine = i < n && keep[i];
-    const double zi = mine ? raw[(long long)IZ * n + i] : 0.0;
-    int rank = 0;
-    for (int j0 = 0; j0 < n; j0 += 256) {
-        const int j = j0 + threadIdx.x;
-        zs[threadIdx.x] = j < n ? raw[(long long)IZ * n + j] : 0.0;
-        ks[threadIdx.x] = j < n ? keep[j] : 0;
-        __syncthreads();
-        const int m = min(256, n - j0);
-        for (int q = 0; q < m; ++q) {
-            const int jj = j0 + q;
-            if (ks[q] && (zs[q] < zi || (zs[q] == zi && jj < i))) ++rank;
-        }
-        __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n || !keep[i]) return;
+    const double zi = raw[(long long)IZ * n + i];
+    int cnt = 0;
+    for (int j = lane; j < n; j += 64) {
+        const double zj = raw[(long long)IZ * n + j];
+        cnt += (keep[j] && (zj < zi || (zj == zi && j < i))) ? 1 : 0;
     }
-    if (!mine) return;
-    for (int q = 0; q < NID; ++q) items[(long long)q * n + rank] = raw[(long long)q * n + i];
-    for (int q = 0; q < NJI; ++q) itemi[(long long)q * n + rank] = rawi[(long long)q * n + i];
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+    if (lane < NID) items[(long long)lane * n + cnt] = raw[(long long)lane * n + i];
+    else if (lane - NID < NJI)
+        itemi[(long long)(lane - NID) * n + cnt] = rawi[(long long)(lane - NID) * n + i];
 }
 
 // footprint offsets: area_off[r] = sum of the box areas of items 0 .. r-1
@@ -400,12 +396,13 @@ __global__ void __launch_bounds__(256) k_splat_render(const double* __restrict__
 }
 
 // backward sweep: per pixel, back to front; each touched item's five
-// contributions go to its footprint slot (present[] marks them)
+// contributions go to its footprint slot (the opacity and scale ones 0.0
+// where alpha was clipped, as the reference adds nothing there)
 __global__ void __launch_bounds__(256) k_splat_backward(
     const double* __restrict__ items, const int* __restrict__ itemi, int n,
     const int* __restrict__ nkept, int W, int H, const double* __restrict__ gC,
     const double* __restrict__ Tfinal, const long long* __restrict__ area_off,
-    double* __restrict__ contrib, uint8_t* __restrict__ present) {
+    double* __restrict__ contrib) {
     __shared__ int lst[256];
     __shared__ int wc[4];
     const int m = *nkept;
@@ -467,7 +464,6 @@ __global__ void __launch_bounds__(256) k_splat_backward(
             contrib[2 * total + slot] = gb * w;
             contrib[3 * total + slot] = dop;
             contrib[4 * total + slot] = dsc;
-            present[slot] = clipped ? 2 : 1;
             S0 += w * cr;
             S1 += w * cg;
             S2 += w * cb;
@@ -475,38 +471,121 @@ __global__ void __launch_bounds__(256) k_splat_backward(
     }
 }
 
-// one thread per item: its footprint's contributions summed in row-major
-// pixel order (opacity and scale only where alpha was not clipped), added
-// to the Gaussian's gradients
-__global__ void k_splat_reduce(const int* __restrict__ itemi, int n, const int* __restrict__ nkept,
-                               const long long* __restrict__ area_off,
-                               const double* __restrict__ contrib,
-                               const uint8_t* __restrict__ present, double* __restrict__ grads,
-                               int cap) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+// one block of NW wavefronts per item.  Its lanes load CH = NW * 64 * EPL
+// consecutive footprint entries of each of the five contributions (the next
+// chunk in flight while this one is summed), keep the entries that hold a
+// nonzero value, packed in row-major pixel order into LDS, and lanes 0-4 of
+// the first wavefront each sum one contribution over them, then add it to
+// the Gaussian's gradient.  Leaving out an entry whose five values are all
+// zero, and padding with +0.0, changes no sum that starts at +0.0, so the
+// result equals the sequential sum over the whole footprint; most of a box
+// lies outside the splat's support, and the sums are chains of dependent
+// adds, so the chain is shortened to the entries that count.
+template <int NW, int EPL>
+__global__ void __launch_bounds__(NW * 64) k_splat_reduce(const int* __restrict__ itemi, int n,
+                                                          const int* __restrict__ nkept,
+                                                          const long long* __restrict__ area_off,
+                                                          const double* __restrict__ contrib,
+                                                          double* __restrict__ grads, int cap) {
+    constexpr int NT = NW * 64, CH = NT * EPL, NG = NW * EPL;
+    constexpr int ROW = CH / 2 + 1;  // double2 per contribution row (the +1 spreads the banks)
+    __shared__ double2 buf[5 * ROW];
+    __shared__ int gcnt[NG];
+    const int r = blockIdx.x;
     const int m = *nkept;
     if (r >= m) return;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const long long total = area_off[m];
     const long long b = area_off[r], e = area_off[r + 1];
-    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    for (long long s = b; s < e; ++s) {
-        const uint8_t pr = present[s];
-        if (!pr) continue;
-        acc[0] += contrib[s];
-        acc[1] += contrib[total + s];
-        acc[2] += contrib[2 * total + s];
-        if (pr == 1) {
-            acc[3] += contrib[3 * total + s];
-            acc[4] += contrib[4 * total + s];
+    double nx[EPL][5];
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) {
+        const long long sk = b + k * NT + t;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) nx[k][q] = sk < e ? contrib[q * total + sk] : 0.0;
+    }
+    double* bufd = reinterpret_cast<double*>(buf);
+    double acc = 0.0;
+    for (long long s0 = b; s0 < e; s0 += CH) {
+        // entry k * NT + t of the chunk is group k * NW + w, lane `lane`
+        unsigned long long mk[EPL];
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) {
+            bool nz = false;
+#pragma unroll
+            for (int q = 0; q < 5; ++q) nz |= nx[k][q] != 0.0;
+            mk[k] = __ballot(nz);
+            if (lane == 0) gcnt[k * NW + w] = __popcll(mk[k]);
+        }
+        __syncthreads();
+        int cnt = 0, base[EPL];
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) base[k] = 0;
+        for (int g = 0; g < NG; ++g) {
+            const int c = gcnt[g];
+#pragma unroll
+            for (int k = 0; k < EPL; ++k) base[k] += g < k * NW + w ? c : 0;
+            cnt += c;
+        }
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) {
+            const int below = __builtin_amdgcn_mbcnt_hi(
+                (unsigned)(mk[k] >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mk[k], 0u));
+            if ((mk[k] >> lane) & 1ull) {
+                const int pos = base[k] + below;
+#pragma unroll
+                for (int q = 0; q < 5; ++q) bufd[q * 2 * ROW + pos] = nx[k][q];
+            }
+        }
+        const int nb = (cnt + 31) >> 5;  // 32-entry batches, the tail padded with +0.0
+        if (t < nb * 32 - cnt) {
+#pragma unroll
+            for (int q = 0; q < 5; ++q) bufd[q * 2 * ROW + cnt + t] = 0.0;
+        }
+        __syncthreads();
+        if (s0 + CH < e) {
+#pragma unroll
+            for (int k = 0; k < EPL; ++k) {
+                const long long sk = s0 + CH + k * NT + t;
+#pragma unroll
+                for (int q = 0; q < 5; ++q) nx[k][q] = sk < e ? contrib[q * total + sk] : 0.0;
+            }
+        }
+        if (t < 5 && nb > 0) {
+            const double2* v = buf + t * ROW;
+            double2 wa[16], wb[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) wa[j] = v[j];
+            int k = 0;
+            for (;;) {
+                if (k + 1 < nb) {
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) wb[j] = v[(k + 1) * 16 + j];
+                }
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    acc += wa[j].x;
+                    acc += wa[j].y;
+                }
+                if (++k >= nb) break;
+                if (k + 1 < nb) {
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) wa[j] = v[(k + 1) * 16 + j];
+                }
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    acc += wb[j].x;
+                    acc += wb[j].y;
+                }
+                if (++k >= nb) break;
+            }
         }
     }
-    const int gi = itemi[(long long)JG * n + r];
-    for (int q = 0; q < 5; ++q) grads[(long long)q * cap + gi] += acc[q];
+    if (t < 5) grads[(long long)t * cap + itemi[(long long)JG * n + r]] += acc;
 }
 
-__global__ void k_fill_u8(uint8_t* __restrict__ a, long long n) {
-    const long long i = blockIdx.x * 256LL + threadIdx.x;
-    if (i < n) a[i] = 0;
+__global__ void k_zero_f64(double* __restrict__ a, long long n) {
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += 256LL * gridDim.x) a[i] = 0.0;
 }
 
 // refine_gaussians' photometric loss: per-block partial sums of e^2 and
@@ -531,12 +610,16 @@ __global__ void __launch_bounds__(256) k_mse(const double* __restrict__ C,
     if (threadIdx.x == 0) part[blockIdx.x] = s[0];
 }
 
-__global__ void k_mse_final(const double* __restrict__ part, int nb, double inv,
-                            double* __restrict__ loss) {
-    if (threadIdx.x != 0) return;
-    double t = 0.0;
-    for (int i = 0; i < nb; ++i) t += part[i];
-    *loss = t * inv;
+__global__ void __launch_bounds__(256) k_mse_final(const double* __restrict__ part, int nb,
+                                                   double inv, double* __restrict__ loss) {
+    __shared__ double s[256];
+    s[threadIdx.x] = threadIdx.x < nb ? part[threadIdx.x] : 0.0;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *loss = s[0] * inv;
 }
 
 // CloudAdam.step: groups cr, cg, cb, raw_op, raw_sc with their learning
@@ -653,7 +736,7 @@ int sv_splat_prepare(const double* cloud, int n, int cap, const double* R, const
     if (n > 0) {
         hipLaunchKernelGGL(k_splat_project, dim3((n + 255) / 256), dim3(256), 0, st, cloud, n, cap,
                            pj, s.raw, s.rawi, s.keep);
-        hipLaunchKernelGGL(k_splat_rank, dim3((n + 255) / 256), dim3(256), 0, st, s.raw, s.rawi,
+        hipLaunchKernelGGL(k_splat_rank, dim3((n + 3) / 4), dim3(256), 0, st, s.raw, s.rawi,
                            s.keep, n, s.items, s.itemi);
     }
     hipLaunchKernelGGL(k_count_kept, dim3(1), dim3(256), 0, st, s.keep, n, s.nkept);
@@ -678,25 +761,24 @@ int sv_splat_render(const void* ws, int n, int W, int H, double br, double bg, d
 
 // SplatRenderer.backward: adds the gradients of sum(gC * C) to grads
 // [5][cap] (colour r, g, b, raw opacity, raw scale).  contrib holds
-// 5 * total doubles and present total bytes, total = the footprint area of
-// the prepared workspace (sv_splat_prepare's area_total).
+// 5 * total doubles, total = the footprint area of the prepared workspace
+// (sv_splat_prepare's info[1]).
 int sv_splat_backward(const void* ws, int n, int W, int H, const double* gC, const double* T,
-                      double* contrib, uint8_t* present, long long total, double* grads, int cap,
-                      void* stream) {
+                      double* contrib, long long total, double* grads, int cap, void* stream) {
     SV_REQUIRE(ws && gC && T && grads && W > 0 && H > 0 && n >= 0 && cap >= n && total >= 0 &&
-                   (total == 0 || (contrib && present)),
+                   (total == 0 || contrib),
                "sv_splat_backward: bad arguments");
     hipStream_t st = (hipStream_t)stream;
     SplatWs s = carve_splat(const_cast<void*>(ws), n);
     if (total > 0)
-        hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, present,
-                           total);
+        hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)std::min<long long>((5 * total + 255) / 256, 4096)),
+                           dim3(256), 0, st, contrib, 5 * total);
     dim3 grid((W + kTile - 1) / kTile, (H + kTile - 1) / kTile);
     hipLaunchKernelGGL(k_splat_backward, grid, dim3(256), 0, st, s.items, s.itemi, n, s.nkept, W, H,
-                       gC, T, s.area_off, contrib, present);
+                       gC, T, s.area_off, contrib);
     if (n > 0)
-        hipLaunchKernelGGL(k_splat_reduce, dim3((n + 255) / 256), dim3(256), 0, st, s.itemi, n,
-                           s.nkept, s.area_off, contrib, present, grads, cap);
+        hipLaunchKernelGGL((k_splat_reduce<8, 4>), dim3(n), dim3(512), 0, st, s.itemi, n,
+                           s.nkept, s.area_off, contrib, grads, cap);
     SV_CHECK_LAUNCH();
     return SV_OK;
 }
@@ -710,7 +792,7 @@ int sv_photometric_mse(const double* C, const double* target, long long n, doubl
     const int nb = 256;
     const double inv = 1.0 / (double)n;
     hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, st, C, target, n, inv, gC, part);
-    hipLaunchKernelGGL(k_mse_final, dim3(1), dim3(64), 0, st, part, nb, inv, loss);
+    hipLaunchKernelGGL(k_mse_final, dim3(1), dim3(256), 0, st, part, nb, inv, loss);
     SV_CHECK_LAUNCH();
     return SV_OK;
 }

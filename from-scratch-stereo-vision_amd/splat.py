@@ -35,7 +35,7 @@ L.extra_signatures({
     "sv_splat_workspace_bytes": (_I, [_I, ctypes.POINTER(_SZ)]),
     "sv_splat_prepare": (_I, [_P, _I, _I, _P, _P, _D, _D, _D, _I, _I, _D, _D, _P, _SZ, _P, _P]),
     "sv_splat_render": (_I, [_P, _I, _I, _I, _D, _D, _D, _P, _P, _P]),
-    "sv_splat_backward": (_I, [_P, _I, _I, _I, _P, _P, _P, _P, _LL, _P, _I, _P]),
+    "sv_splat_backward": (_I, [_P, _I, _I, _I, _P, _P, _P, _LL, _P, _I, _P]),
     "sv_photometric_mse": (_I, [_P, _P, _LL, _P, _P, _P, _P]),
     "sv_cloud_adam": (_I, [_P, _P, _P, _P, _I, _I, _D, _D, _D, _D, _D, _P]),
 })
@@ -178,10 +178,9 @@ class SplatRenderer:
         g = G._dev(gC)
         total = int(info[1].item())  # footprint area of the kept items
         contrib = torch.empty(max(5 * total, 1), dtype=torch.float64, device=ws.device)
-        present = torch.empty(max(total, 1), dtype=torch.uint8, device=ws.device)
         L.check(L.lib().sv_splat_backward(L.ptr(ws), n, self.W, self.H, L.ptr(g), L.ptr(T),
-                                          L.ptr(contrib), L.ptr(present), total,
-                                          L.ptr(gc.grads), gc.cap, L.stream()))
+                                          L.ptr(contrib), total, L.ptr(gc.grads), gc.cap,
+                                          L.stream()))
 
 
 class CloudAdam:

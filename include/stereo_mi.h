@@ -534,8 +534,8 @@ int sv_select_points(const double* disp, int W, int H, const uint8_t* valid, int
  *   [H*W][3] and final transmittance T [H*W] (device).
  * sv_splat_backward: SplatRenderer.backward, :2854-2962: adds the gradients
  *   of sum(gC * C) to grads [5][cap] (colour r, g, b, raw opacity, raw
- *   scale); contrib (5 * total doubles) and present (total bytes) are
- *   scratch for the footprint area total of the prepared workspace.
+ *   scale); contrib (5 * total doubles) is scratch for the footprint area
+ *   total of the prepared workspace.
  * sv_photometric_mse: refine_gaussians' loss, :3017-3027: gC = 2 (C - target)
  *   / n, mean squared error to *loss (device), part = 256 doubles of scratch.
  * sv_cloud_adam: CloudAdam.step, :2982-3007, m and v [5][cap]; c1, c2 the
@@ -551,8 +551,7 @@ int sv_splat_prepare(const double* cloud, int n, int cap, const double* R, const
 int sv_splat_render(const void* ws, int n, int W, int H, double br, double bg, double bb,
                     double* C, double* T, void* stream);
 int sv_splat_backward(const void* ws, int n, int W, int H, const double* gC, const double* T,
-                      double* contrib, uint8_t* present, long long total, double* grads, int cap,
-                      void* stream);
+                      double* contrib, long long total, double* grads, int cap, void* stream);
 int sv_photometric_mse(const double* C, const double* target, long long n, double* gC,
                        double* loss, double* part, void* stream);
 int sv_cloud_adam(double* cloud, const double* grads, double* m, double* v, int n, int cap,

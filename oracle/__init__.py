@@ -890,3 +890,149 @@ def grid_mesh(disp, W, H, cam, step=2, fill=True, jump=1.5, jump_rel=0.06, min_d
             "fill": np.array(flag, dtype=np.int64), "disp": np.array(dv, dtype=np.float64),
             "idx": np.array(idx, dtype=np.int64).reshape(-1, 3), "main_tris": main,
             "fill_tris": nfill, "pivot": pivot}
+
+
+# ------------------------------------------------------ Gaussian splatting
+# SplatRenderer.render (stereonet.py:2706-2852) restated in numpy: the
+# reference's scalar projection per Gaussian (its own fsqrt / fexp, stable
+# sort by depth), then each Gaussian composited over its footprint box with
+# elementwise array operations in the reference's order (exp by the same
+# 15-term series; the 2^k scaling is exact here since k >= -7).  Pinned by
+# the reference's own renders (tests/golden/splat.npz, tests/test_mesh.py).
+
+_LN2 = 0.6931471805599453
+
+
+def _rsqrt_ref(x):
+    if x <= 0.0:
+        return 0.0
+    y = x
+    if y > 4.0:
+        s = 1.0
+        while y > 4.0:
+            y *= 0.25
+            s *= 2.0
+        y = s
+    elif y < 0.25:
+        s = 1.0
+        while y < 0.25:
+            y *= 4.0
+            s *= 0.5
+        y = s
+    else:
+        y = 1.0
+    for _ in range(24):
+        y = 0.5 * (y + x / y)
+    return y
+
+
+def _exp_series(x):
+    """The reference's fexp on an array of values in [-700, 700]."""
+    x = np.asarray(x, dtype=np.float64)
+    k = np.trunc(x / _LN2 + np.where(x >= 0.0, 0.5, -0.5)).astype(np.int64)
+    r = x - k * _LN2
+    term = np.ones_like(x)
+    total = np.ones_like(x)
+    for n in range(1, 16):
+        term = term * r / n
+        total = total + term
+    return np.ldexp(total, k)
+
+
+def _rexp(x):
+    if x > 700.0:
+        return 1.0e308
+    if x < -700.0:
+        return 0.0
+    k = int(x / _LN2 + (0.5 if x >= 0.0 else -0.5))
+    r = x - k * _LN2
+    term = 1.0
+    total = 1.0
+    for n in range(1, 16):
+        term = term * r / n
+        total += term
+    if k >= 0:
+        for _ in range(k):
+            total *= 2.0
+    else:
+        for _ in range(-k):
+            total *= 0.5
+    return total
+
+
+def splat_render(cloud, cam, W, H, R, t, blur=0.30, cutoff=3.0, background=(0.0, 0.0, 0.0)):
+    """cloud [14][n] (mx my mz sxx sxy sxz syy syz szz cr cg cb raw_op
+    raw_sc), cam = (fx, cx, cy); returns C [H*W*3]."""
+    cl = np.asarray(cloud, dtype=np.float64)
+    fx, cx, cy = cam[0], cam[1], cam[2]
+    items = []
+    for i in range(cl.shape[1]):
+        X, Y, Z = cl[0, i], cl[1, i], cl[2, i]
+        xc = R[0] * X + R[1] * Y + R[2] * Z + t[0]
+        yc = R[3] * X + R[4] * Y + R[5] * Z + t[1]
+        zc = R[6] * X + R[7] * Y + R[8] * Z + t[2]
+        if zc <= 1.0e-4:
+            continue
+        u = fx * xc / zc + cx
+        v = fx * yc / zc + cy
+        f_z = fx / zc
+        f_z2 = -fx / (zc * zc)
+        jx = [f_z, 0.0, f_z2 * xc]
+        jy = [0.0, f_z, f_z2 * yc]
+        m0 = [jx[0] * R[0] + jx[1] * R[3] + jx[2] * R[6], jx[0] * R[1] + jx[1] * R[4] + jx[2] * R[7],
+              jx[0] * R[2] + jx[1] * R[5] + jx[2] * R[8]]
+        m1 = [jy[0] * R[0] + jy[1] * R[3] + jy[2] * R[6], jy[0] * R[1] + jy[1] * R[4] + jy[2] * R[7],
+              jy[0] * R[2] + jy[1] * R[5] + jy[2] * R[8]]
+        sxx, sxy, sxz, syy, syz, szz = (float(cl[k, i]) for k in (3, 4, 5, 6, 7, 8))
+
+        def sm(m):
+            return [sxx * m[0] + sxy * m[1] + sxz * m[2], sxy * m[0] + syy * m[1] + syz * m[2],
+                    sxz * m[0] + syz * m[1] + szz * m[2]]
+        sm0, sm1 = sm(m0), sm(m1)
+        a00 = m0[0] * sm0[0] + m0[1] * sm0[1] + m0[2] * sm0[2]
+        a01 = m0[0] * sm1[0] + m0[1] * sm1[1] + m0[2] * sm1[2]
+        a11 = m1[0] * sm1[0] + m1[1] * sm1[1] + m1[2] * sm1[2]
+        s = _rexp(float(cl[13, i]))
+        c00 = s * a00 + blur
+        c01 = s * a01
+        c11 = s * a11 + blur
+        det = c00 * c11 - c01 * c01
+        if det < 1.0e-12:
+            continue
+        inv = 1.0 / det
+        tr = 0.5 * (c00 + c11)
+        disc = _rsqrt_ref(tr * tr - det if tr * tr - det > 0.0 else 0.0)
+        lmax = tr + disc
+        rad = cutoff * _rsqrt_ref(lmax if lmax > 1.0e-12 else 1.0e-12)
+        if rad < 0.5:
+            rad = 0.5
+        if rad > 0.35 * (W + H):
+            continue
+        x0, x1 = int(u - rad), int(u + rad) + 1
+        y0, y1 = int(v - rad), int(v + rad) + 1
+        if x1 < 0 or y1 < 0 or x0 >= W or y0 >= H:
+            continue
+        x0, y0, x1, y1 = max(x0, 0), max(y0, 0), min(x1, W), min(y1, H)
+        rop = float(cl[12, i])
+        op = 1.0 / (1.0 + _rexp(-rop)) if rop >= 0.0 else _rexp(rop) / (1.0 + _rexp(rop))
+        items.append((zc, i, u, v, c11 * inv, -c01 * inv, c00 * inv, x0, x1, y0, y1, op))
+    items.sort(key=lambda e: e[0])
+    T = np.ones((H, W))
+    C = np.zeros((H, W, 3))
+    for (zc, i, u, v, i00, i01, i11, x0, x1, y0, y1, op) in items:
+        Tp = T[y0:y1, x0:x1]
+        dy = (np.arange(y0, y1, dtype=np.float64) - v)[:, None]
+        dx = (np.arange(x0, x1, dtype=np.float64) - u)[None, :]
+        q = i00 * dx * dx + 2.0 * i01 * dx * dy + i11 * dy * dy
+        live = (Tp >= 0.004) & (q <= 9.0)
+        a = op * _exp_series(np.where(live, -0.5 * q, 0.0))
+        live &= a >= 1.0e-4
+        a = np.minimum(a, 0.99)
+        w = Tp * a
+        cb = C[y0:y1, x0:x1]
+        for ch, col in enumerate((cl[9, i], cl[10, i], cl[11, i])):
+            cb[..., ch] = np.where(live, cb[..., ch] + w * col, cb[..., ch])
+        T[y0:y1, x0:x1] = np.where(live, Tp * (1.0 - a), Tp)
+    for ch in range(3):
+        C[..., ch] = C[..., ch] + T * background[ch]
+    return C.reshape(-1)

@@ -99,3 +99,19 @@ def test_write_obj_text_matches_reference(sv, name, tmp_path):
     nv, nf = sv.write_obj(path, pts2, tris, _pts3d(name), G[name + "_rgb"], W, H)
     assert (nv, nf) == tuple(G[name + "_obj_counts"].tolist())
     assert open(path, "rb").read() == G[name + "_obj"].tobytes()
+
+
+@pytest.mark.parametrize("name", SCENES)
+def test_oracle_splat_render_matches_reference(name):
+    """The numpy restatement of SplatRenderer.render (the splat workload's
+    CPU baseline) against the reference's renders, original and novel view
+    with a background."""
+    W, H, _ = (int(v) for v in G[name + "_wh"])
+    cam = _cam(W, H)
+    ident = [1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0]
+    got = oracle.splat_render(G[name + "_gc"], cam, W, H, ident, [0.0, 0.0, 0.0])
+    assert np.array_equal(got, G[name + "_render"])
+    v = G[name + "_views"][0]
+    got = oracle.splat_render(G[name + "_adam1"], cam, W, H, list(v[1:10]), list(v[10:13]),
+                              background=(0.1, 0.2, 0.3))
+    assert np.array_equal(got, G[name + "_render_novel"])
