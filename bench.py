@@ -8,6 +8,11 @@ predict_auto with the trained v5 StereoNet (probe + up to 3 passes, FP32
 network) followed by refine_disparity(planes=False): census 9x7, band SGM
 (R=10, P1=12, P2=150, 8 paths, both views), LR check (tol 2), 3x3 median.
 
+The step is timed as individual launches, replayed as one HIP graph, and
+replayed pipelined over batches (the network of batch k and the refinement of
+batch k-1 as two branches of one graph); value is the fastest of the graph
+replays, each checked equal to the individual launches' output.
+
 Printed (one JSON line): pairs/s (value), the full-range SGM control on the
 same pairs (eval_real.py:143-146 rule, one launch for all pairs, replayed as
 a HIP graph like the headline), accuracy of both (bad2 / EPE vs the
@@ -108,6 +113,10 @@ def main():
                     help="graph replay: cut the batch into this many equal parts, each on its "
                          "own stream with its own network buffers, captured together so that one "
                          "part's kernels fill the other's gaps (1: the eager step as it is)")
+    ap.add_argument("--no-pipeline", action="store_true",
+                    help="graph replay: time only the sequential step (default: also replay a "
+                         "pipelined step, the network of batch k beside the refinement of batch "
+                         "k-1 as two graph branches, and report the faster of the two)")
     ap.add_argument("--tune-only", action="store_true",
                     help="autotune, write the choices to --tune-file and exit (the default "
                          "run tunes this way in a child process, then loads the file)")
@@ -300,6 +309,7 @@ def main():
     net_ms = sum(a.elapsed_time(b) for a, b in net_events)
     eager_pairs_s = B * args.steps / dt
     pairs_s, execution = eager_pairs_s, "eager"
+    seq_graph_pairs_s = pipe_pairs_s = None
     if not args.eager:
         # the same step replayed as one HIP graph (every launch recorded
         # once, the kernel variants chosen above baked in); its output must
@@ -328,6 +338,15 @@ def main():
         pairs_s = B * args.steps / dt
         execution = "hip graph" if S == 1 else "hip graph, %d streams x %d pairs" % (S, hb)
         log("graph replay: %.1f pairs/s (eager %.1f)" % (pairs_s, eager_pairs_s))
+        seq_graph_pairs_s = pairs_s
+        if S == 1 and not args.no_pipeline:
+            pipe_pairs_s = pipelined_rate(args, sv, N, L, R, net, L8, R8, d0, lv, out, st)
+            log("pipelined graph replay: %.1f pairs/s" % pipe_pairs_s)
+            if pipe_pairs_s > pairs_s:
+                pairs_s = pipe_pairs_s
+                dt = B * args.steps / pipe_pairs_s
+                execution = ("hip graph, pipelined: the network of batch k and the refinement "
+                             "of batch k-1 as two branches of one graph per step")
 
     # accuracy of net + band refinement
     pred = out.reshape(B, -1).cpu().numpy()
@@ -418,6 +437,8 @@ def main():
                                "refine_disparity(planes=False), R=10 census 9x7",
                    "pairs_per_step": B, "parallelism": "single GPU", "execution": execution},
         "eager_pairs_per_s": round(eager_pairs_s, 2),
+        "sequential_graph_pairs_per_s": round(seq_graph_pairs_s, 2) if seq_graph_pairs_s else None,
+        "pipelined_graph_pairs_per_s": round(pipe_pairs_s, 2) if pipe_pairs_s else None,
         "accuracy": {"bad2_pct": round(bad2, 3), "epe_px": round(epe, 4),
                      "network_only_bad2_pct": round(net_bad2, 3)},
         "fullrange": fullrange,
@@ -442,6 +463,54 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         log("copy probe: 268435456 bytes read + 268435456 bytes written")
+
+
+def pipelined_rate(args, sv, N, L, Rmod, net, L8, R8, d0, lv, out, st):
+    """The step software-pipelined over batches: each replay runs the
+    network (predict_auto + widening) of batch k on one stream and the
+    refinement of batch k-1, whose disparity the previous replay left, on
+    another; the two are branches of one HIP graph, so the network's small
+    launches fill the gaps beside the SGM.  Two graphs alternate the pair of
+    disparity buffers.  Every pair is still fully processed once per batch;
+    each graph's refinement must equal the eager step's output."""
+    import torch
+    B = L8.shape[0]
+    sN, sS = torch.cuda.Stream(), torch.cuda.Stream()
+    bufs = [d0, torch.empty_like(d0)]
+
+    def net_part(k):
+        res = N.predict_auto_batch(net, L8, R8, W, H)
+        N._call("sv_widen", N._p(res.full), N._p(lv.table), H * W, N._p(bufs[k % 2]), H * W,
+                None, B, H * W, L.stream())
+
+    def piped(k):
+        cur = torch.cuda.current_stream()
+        sN.wait_stream(cur)
+        sS.wait_stream(cur)
+        with torch.cuda.stream(sN):
+            net_part(k)
+        with torch.cuda.stream(sS):
+            o = sv.refine_disparity(L8, R8, None, bufs[(k + 1) % 2], W, H, planes=False)
+        cur.wait_stream(sN)
+        cur.wait_stream(sS)
+        return o
+
+    net_part(1)     # the disparity of the batch before the first
+    torch.cuda.synchronize()
+    G = [Rmod.CapturedPipeline(lambda a, b, k=k: piped(k), L8, R8, warmup=1) for k in (0, 1)]
+    for k in range(4):
+        go, gs = G[k % 2].replay()
+        torch.cuda.synchronize()
+        if not (torch.equal(go, out) and torch.equal(gs, st)):
+            raise RuntimeError("pipelined graph replay differs from the eager step")
+    for k in range(args.warmup):
+        G[k % 2].replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        G[k % 2].replay()
+    torch.cuda.synchronize()
+    return B * args.steps / (time.perf_counter() - t0)
 
 
 def fullrange_control(args, sv, L, gt, valid, L8, R8):
