@@ -252,7 +252,7 @@ def main():
         # rest (every variant leaves the results unchanged)
         layers = A.tune_layers(net, lambda: step()[:2], reps=2)
         tuned = A.tune(lambda: step()[:2],
-                       knobs=["sgm_split", "sgm_prefetch", "sgm_htile", "sgm_overlap", "wta_reads"], reps=2,
+                       knobs=["sgm_split", "sgm_prefetch", "sgm_htile", "sgm_overlap", "wta_reads", "sgm_order"], reps=2,
                        log=log)
         tuned["conv_layers_tuned"] = len(layers)
     # the parts share the per-layer choices (keyed by launch shape, batch
@@ -1573,7 +1573,7 @@ def ragged_main(args):
     tuned = None
     if not args.no_autotune:
         layers = A.tune_layers(net, run, reps=2)
-        tuned = A.tune(run, knobs=["sgm_split", "sgm_prefetch", "sgm_htile", "sgm_overlap", "wta_reads"], reps=2, log=log)
+        tuned = A.tune(run, knobs=["sgm_split", "sgm_prefetch", "sgm_htile", "sgm_overlap", "wta_reads", "sgm_order"], reps=2, log=log)
         tuned["conv_layers_tuned"] = len(layers)
     # eager reference run (stage profile of the refinement) and capture
     L.lib().sv_profile_enable(1)

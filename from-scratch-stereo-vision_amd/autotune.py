@@ -35,7 +35,8 @@ KNOBS = {"sgm_prefetch": (0, (2, 4, 8)),
          "conv_mgroups": (7, (1, 2, 4)),
          "sgm_split": (8, (1, 2)),
          "wta_reads": (9, (1, 2)),
-         "conv_dchunk": (10, (12, 6, 4))}
+         "conv_dchunk": (10, (12, 6, 4)),
+         "sgm_order": (11, (1, 2))}
 
 
 def set_knob(name, value):

@@ -32,7 +32,8 @@
 #define SV_TUNE_SGM_SPLIT 8     // packed SGM kernels: 1 one lane per scanline, 2 two lanes per scanline
 #define SV_TUNE_WTA_READS 9     // band epilogue (K = 21, u8): 1 one lane per pixel, 2 coalesced runs through LDS
 #define SV_TUNE_CONV_DCHUNK 10  // depth-column 3D convolution: output depths per tile, 4, 6, 12
-#define SV_TUNE_COUNT 11
+#define SV_TUNE_SGM_ORDER 11   // vertical/diagonal SGM blocks: 1 in launch order across the XCDs, 2 one contiguous run of (pair, direction, columns) per XCD
+#define SV_TUNE_COUNT 12
 
 namespace sv {
 

@@ -794,6 +794,37 @@ struct LinePos {
     }
 };
 
+// LinePos::at for consecutive steps without its modulo: the pixel index
+// and column are carried from one step to the next, a diagonal wrapping
+// from one image edge to the other (dx is -1, 0 or 1).
+struct LineCursor {
+    int x, p, dx, dyW, W;
+    __device__ __forceinline__ void start(const LinePos& lp, int i) {
+        bool bd;
+        p = (int)lp.at(i, bd);
+        dx = lp.dx;
+        dyW = lp.dy * lp.W;
+        W = lp.W;
+        x = lp.horiz ? (lp.dx > 0 ? i : lp.W - 1 - i) : p - (lp.dy > 0 ? i : lp.H - 1 - i) * lp.W;
+    }
+    __device__ __forceinline__ void next() {
+        x += dx;
+        p += dyW + dx;
+        if (x >= W) {
+            x = 0;
+            p -= W;
+        } else if (x < 0) {
+            x = W - 1;
+            p += W;
+        }
+    }
+    // border pixel (the recurrence restarts): the first step, or where a
+    // diagonal re-enters at the opposite edge
+    __device__ __forceinline__ bool border(int i) const {
+        return i == 0 || (dx > 0 && x == 0) || (dx < 0 && x == W - 1);
+    }
+};
+
 // Band geometry of the packed kernels for band length K.
 template <int K>
 struct PackedBand {
@@ -1253,18 +1284,36 @@ __device__ __forceinline__ void packed_step2c(u16x2 (&L)[PackedBand2<K>::NPH],
     mprev = m;
 }
 
+// The block a k_sgm_p2 workgroup works on.  Workgroups go to the eight
+// XCDs in turn (workgroup b to XCD b mod 8), each XCD with its own L2.  With
+// xcd set, XCD k takes one contiguous run of the (column block, direction,
+// pair) blocks instead: the blocks of one pair's directions then share an
+// L2, and its three downward (and three upward) paths, which read the same
+// cost row at the same step, find it there.
+__device__ __forceinline__ uint3 sgm_block(int xcd) {
+    const unsigned gx = gridDim.x, gy = gridDim.y;
+    unsigned b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    if (xcd) {
+        const unsigned nb = gx * gy * gridDim.z, q = nb / 8, r = nb % 8, k = b % 8, i = b / 8;
+        b = (k < r ? k * (q + 1) : r * (q + 1) + (k - r) * q) + i;
+    }
+    return make_uint3(b % gx, (b / gx) % gy, b / (gx * gy));
+}
+
 // k_sgm_p with two lanes per scanline (32 lines per wave).  CB:
 // constant band centre (packed_step2c; the band centres are not read).
 template <int K, int PF, bool CB = false>
 __global__ void __launch_bounds__(64) k_sgm_p2(const uint8_t* __restrict__ C,
                                                const int* __restrict__ base,
                                                uint8_t* __restrict__ Lout, Geom g, int Kp,
-                                               int P1, int P2, int dir0, long long dirStride) {
+                                               int P1, int P2, int dir0, long long dirStride,
+                                               int xcd) {
     using PB = PackedBand2<K>;
     constexpr int NPH = PB::NPH, NW = PB::NW, SD = PB::SD;
     __shared__ uint32_t lds[32 * SD];
-    const int W = g.w(blockIdx.z), H = g.h(blockIdx.z);
-    const int dir = dir0 + (int)blockIdx.y;
+    const uint3 bk = sgm_block(xcd);
+    const int W = g.w(bk.z), H = g.h(bk.z);
+    const int dir = dir0 + (int)bk.y;
     LinePos lp;
     lp.dx = c_dx[dir];
     lp.dy = c_dy[dir];
@@ -1274,16 +1323,16 @@ __global__ void __launch_bounds__(64) k_sgm_p2(const uint8_t* __restrict__ C,
     const int lane = threadIdx.x, h = lane & 1, ln = lane >> 1;
     const int nLines = lp.horiz ? H : W;
     const int len = lp.horiz ? W : H;
-    lp.line = blockIdx.x * 32 + ln;
+    lp.line = bk.x * 32 + ln;
     if (lp.line >= nLines) return;  // both lanes of a pair leave together
     u32_alias* sl32 = reinterpret_cast<u32_alias*>(lds + ln * SD);
     u16_alias* s16 = reinterpret_cast<u16_alias*>(sl32);
     if constexpr (!CB)
         for (int i = h; i < SD; i += 2) sl32[i] = 0xFFFFFFFFu;
     const long long plane = g.slot;
-    const uint8_t* Cb = C + (long long)blockIdx.z * plane * Kp + 4 * NW * h;
-    const int* Bb = base + (long long)blockIdx.z * plane;
-    uint8_t* Lb = Lout + (long long)blockIdx.y * dirStride + (long long)blockIdx.z * plane * Kp + 4 * NW * h;
+    const uint8_t* Cb = C + (long long)bk.z * plane * Kp + 4 * NW * h;
+    const int* Bb = base + (long long)bk.z * plane;
+    uint8_t* Lb = Lout + (long long)bk.y * dirStride + (long long)bk.z * plane * Kp + 4 * NW * h;
     uint32_t ring[PF][NW];
     int rb[PF];
 #pragma unroll
@@ -1294,13 +1343,20 @@ __global__ void __launch_bounds__(64) k_sgm_p2(const uint8_t* __restrict__ C,
         for (int q = 0; q < NW; ++q) ring[u][q] = *reinterpret_cast<const uint32_t*>(Cb + pq * Kp + 4 * q);
         rb[u] = CB ? 0 : Bb[pq];
     }
+    // cur: the pixel of the step being made; pre: the next one to prefetch
+    // (steps PF, PF + 1, ..., then the last step again)
+    LineCursor cur, pre;
+    cur.start(lp, 0);
+    int ipre = min(PF, len - 1);
+    pre.start(lp, ipre);
     u16x2 L[NPH];
     u16x2 M[CB ? NPH : 1];
     uint32_t mprev = 0;
     int bprev = 0;
     auto step = [&](int i, const uint32_t (&cr)[NW], int b) {
-        bool border;
-        const long long pix = lp.at(i, border);
+        const bool border = cur.border(i);
+        const long long pix = cur.p;
+        cur.next();
         uint32_t cw[NW];
 #pragma unroll
         for (int q = 0; q < NW; ++q) cw[q] = cr[q];
@@ -1312,9 +1368,12 @@ __global__ void __launch_bounds__(64) k_sgm_p2(const uint8_t* __restrict__ C,
 #pragma unroll
         for (int q = 0; q < NW; ++q) *reinterpret_cast<uint32_t*>(Lb + pix * Kp + 4 * q) = o[q];
     };
-    auto refill = [&](int u, int inext) {
-        bool bd;
-        const long long pq = lp.at(inext, bd);
+    auto refill = [&](int u) {
+        const long long pq = pre.p;
+        if (ipre < len - 1) {
+            pre.next();
+            ++ipre;
+        }
 #pragma unroll
         for (int q = 0; q < NW; ++q) ring[u][q] = *reinterpret_cast<const uint32_t*>(Cb + pq * Kp + 4 * q);
         rb[u] = CB ? 0 : Bb[pq];
@@ -1324,7 +1383,7 @@ __global__ void __launch_bounds__(64) k_sgm_p2(const uint8_t* __restrict__ C,
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
             step(i0 + u, ring[u], rb[u]);
-            refill(u, min(i0 + u + PF, len - 1));
+            refill(u);
         }
     }
 #pragma unroll
@@ -2465,21 +2524,22 @@ int sgm_k_t(const Plan& pl, const void* C, const int* base, void* L, int BV, con
                                    base, (uint8_t*)L, g, pl.Kp, (int)P1, (int)P2, dir0, dirStride);
         }
         hipError_t lerr = hipGetLastError();
+        const int xcd = sv::tune(SV_TUNE_SGM_ORDER) == 2;
         if (vEnd > vBeg && lerr == hipSuccess && split) {
             dim3 vgrid((g.W + 31) / 32, vEnd - vBeg, BV);
             uint8_t* Lv = (uint8_t*)L + (long long)(vBeg - dir0) * dirStride;
             switch (sv::tune(SV_TUNE_SGM_PREFETCH)) {
                 case 2:
                     hipLaunchKernelGGL((k_sgm_p2<K, 2>), vgrid, dim3(64), 0, st, (const uint8_t*)C,
-                                       base, Lv, g, pl.Kp, (int)P1, (int)P2, vBeg, dirStride);
+                                       base, Lv, g, pl.Kp, (int)P1, (int)P2, vBeg, dirStride, xcd);
                     break;
                 case 8:
                     hipLaunchKernelGGL((k_sgm_p2<K, 8>), vgrid, dim3(64), 0, st, (const uint8_t*)C,
-                                       base, Lv, g, pl.Kp, (int)P1, (int)P2, vBeg, dirStride);
+                                       base, Lv, g, pl.Kp, (int)P1, (int)P2, vBeg, dirStride, xcd);
                     break;
                 default:
                     hipLaunchKernelGGL((k_sgm_p2<K, 4>), vgrid, dim3(64), 0, st, (const uint8_t*)C,
-                                       base, Lv, g, pl.Kp, (int)P1, (int)P2, vBeg, dirStride);
+                                       base, Lv, g, pl.Kp, (int)P1, (int)P2, vBeg, dirStride, xcd);
             }
             lerr = hipGetLastError();
         } else if (vEnd > vBeg && lerr == hipSuccess) {
@@ -2749,7 +2809,8 @@ int sgm_cb_t(const uint8_t* C, uint8_t* L, int BV, const Geom& g, int Kp, int P1
                        P2, 0, dirStride);
     dim3 vgrid((g.W + 31) / 32, ndir - 2, BV);
     hipLaunchKernelGGL((k_sgm_p2<KB, 4, true>), vgrid, dim3(64), 0, st, C, nullptr,
-                       L + 2 * dirStride, g, Kp, P1, P2, 2, dirStride);
+                       L + 2 * dirStride, g, Kp, P1, P2, 2, dirStride,
+                       (int)(sv::tune(SV_TUNE_SGM_ORDER) == 2));
     SV_CHECK_LAUNCH();
     return SV_OK;
 }

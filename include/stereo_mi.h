@@ -225,7 +225,10 @@ int sv_refine_ragged(int mode, const void* left, const void* right, int dtype, c
  * SGM kernels (1, 2); 9: path-cost reads of the band epilogue (K = 21, u8
  * path costs): one lane per pixel (1) or coalesced 64-pixel runs summed per
  * lane and handed out through LDS (2); 10: output depths per tile of the
- * depth-column 3D kernel (4, 6, 12).  Value 0 restores the built-in
+ * depth-column 3D kernel (4, 6, 12); 11: workgroup order of the two-lane
+ * vertical/diagonal SGM kernel: in launch order, spread over the XCDs (1),
+ * or one contiguous run of (columns, direction, pair) blocks per XCD, so a
+ * pair's paths share that XCD's L2 (2).  Value 0 restores the built-in
  * heuristic (LDS weights; horizontal paths after the others; matrix cores
  * for 4 to 16 output channels, 3D layers over depth columns of 6, or 4 when
  * 6 leaves CUs idle; the per-channel vector kernel when the input channels
