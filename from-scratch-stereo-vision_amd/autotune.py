@@ -143,7 +143,7 @@ def _conv_variants(kd=1, oc=16):
     groups per wave x tile width, the 4-channel-block matrix-core
     kernel (2D layers) under 1 or 2 pixel groups x tile width, for 3D
     layers (kd = 3) the depth-column matrix-core kernel under each tile
-    width x depth chunk, and for layers of at most 4 output channels the
+    width x depth chunk, and for layers of at most 16 output channels the
     direct kernel."""
     K = {k: v[0] for k, v in KNOBS.items()}
     out = []
@@ -166,7 +166,7 @@ def _conv_variants(kd=1, oc=16):
         for t in KNOBS["conv_tile_width"][1]:
             for dc in KNOBS["conv_dchunk"][1]:
                 out.append(((K["conv_engine"], 6), (K["conv_tile_width"], t), (K["conv_dchunk"], dc)))
-    if oc <= 4:
+    if oc <= 16:
         out.append(((K["conv_engine"], 7),))
     return out
 

@@ -87,12 +87,13 @@ int conv_fwd_col(const float* x, const float* xr, const int* idims, long long xs
                  float* y, const int* odims, long long ystride, int N, int max_ho, int max_wo,
                  int mode, float slope, const float* resid, long long rstride, hipStream_t st);
 
-// direct convolution, one output position per thread (conv_direct.hip);
-// returns SV_EUNSUPPORTED when the shape does not fit it
-int conv_fwd_direct(const float* x, const int* idims, long long xstride, int IC, int D, int KD,
-                    const float* w, const float* b, int OC, int s, int p, int dil, float* y,
-                    const int* odims, long long ystride, int N, int max_ho, int max_wo, int mode,
-                    float slope, const float* resid, long long rstride, hipStream_t st);
+// direct convolution, one output position per thread (conv_direct.hip;
+// xr: the right features of a first 3D layer that forms the cost volume on
+// the fly, else null); returns SV_EUNSUPPORTED when the shape does not fit
+int conv_fwd_direct(const float* x, const float* xr, const int* idims, long long xstride, int IC,
+                    int D, int KD, const float* w, const float* b, int OC, int s, int p, int dil,
+                    float* y, const int* odims, long long ystride, int N, int max_ho, int max_wo,
+                    int mode, float slope, const float* resid, long long rstride, hipStream_t st);
 
 }  // namespace sv
 

@@ -1677,9 +1677,9 @@ int sv_conv_fwd(const float* x, const float* xr, const int* idims, long long xst
                                            st);
             if (rc != SV_EUNSUPPORTED) return rc;
         }
-        if (eng == 7 && !src) {
-            const int rc = sv::conv_fwd_direct(x, idims, xstride, IC, D, KD, w, b, OC, s, p, dil, y,
-                                               odims, ystride, N, max_ho, max_wo, mode, slope,
+        if (eng == 7) {
+            const int rc = sv::conv_fwd_direct(x, xr, idims, xstride, IC, D, KD, w, b, OC, s, p, dil,
+                                               y, odims, ystride, N, max_ho, max_wo, mode, slope,
                                                resid, rstride, st);
             if (rc != SV_EUNSUPPORTED) return rc;
         }

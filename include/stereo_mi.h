@@ -219,7 +219,8 @@ int sv_refine_ragged(int mode, const void* left, const void* right, int dtype, c
  * tile prefetched, 2D layers of 2 to 12 output channels; 6 matrix cores
  * over a column of output depths per tile, 3D layers of up to 16 output
  * channels; 7 direct, one output position per thread without LDS staging,
- * layers of up to 4 output channels; other shapes fall back to 2/3); 7:
+ * any output channel count, the first 3D layer's cost volume formed on the
+ * fly; other shapes fall back to 2/3); 7:
  * 64-pixel groups per wave of the
  * matrix-core convolution (1, 2, 4); 8: lanes per scanline of the packed
  * SGM kernels (1, 2); 9: path-cost reads of the band epilogue (K = 21, u8
