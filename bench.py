@@ -50,9 +50,11 @@ def log(*a):
 # Measured HBM traffic of the band-SGM stage (k_sgm_ph2 + k_sgm_p2, both
 # views) per 460x370 pair: rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 (the
 # factors a 256 MiB device copy reads under the same counters), from
-# profiles/sgm_460x370_pmc.md at sgm_split=2, sgm_prefetch=8, sgm_htile=16.
+# profiles/sgm_460x370_pmc.md at sgm_split=2, sgm_prefetch=8, sgm_htile=16, per
+# workgroup order of the vertical/diagonal kernel (sgm_order 1: launch order;
+# 2: one run of blocks per XCD, whose L2 then serves a pair's directions).
 # A counter reading, not measured live: bench.py cannot read PMC counters.
-SGM_TRAFFIC_PROFILE = ("profiles/sgm_460x370_pmc.md", 5215.6e6 / 32)
+SGM_TRAFFIC_PROFILE = ("profiles/sgm_460x370_pmc.md", {1: 5083.5e6 / 32, 2: 4056.6e6 / 32})
 
 
 def canonical_bytes_per_pair(views, K, HW):
@@ -377,7 +379,8 @@ def main():
     achieved = canon / t_kern / 1e9 if t_kern > 0 else 0.0
     A_ = importlib.import_module("from-scratch-stereo-vision_amd.autotune")
     split = A_.get_knob("sgm_split") == 2
-    traffic = SGM_TRAFFIC_PROFILE[1] * B if split else None
+    order = 2 if A_.get_knob("sgm_order") == 2 else 1
+    traffic = SGM_TRAFFIC_PROFILE[1][order] * B if split else None
     roofline = {"bound": "hbm",
                 "kernel": "band SGM aggregation + epilogue: %s + %s (8 paths, both views, %d pairs)"
                           % ("k_sgm_ph2 + k_sgm_p2" if split else "k_sgm_ph + k_sgm_p",
@@ -386,8 +389,9 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": round(traffic) if traffic else None,
                 "traffic_source": ("rocprofv3 FETCH_SIZE+WRITE_SIZE of k_sgm_ph2 + k_sgm_p2 from %s "
-                                   "(sgm_split=2, sgm_prefetch=8, sgm_htile=16), a constant: not "
-                                   "read live" % SGM_TRAFFIC_PROFILE[0]) if traffic else None,
+                                   "(sgm_split=2, sgm_prefetch=8, sgm_htile=16, sgm_order=%d), a "
+                                   "constant: not read live" % (SGM_TRAFFIC_PROFILE[0], order))
+                if traffic else None,
                 "avg_launch_ms": round(t_kern * 1e3, 4),
                 "algorithmic_bytes_per_launch": canon,
                 "algorithmic_bytes": "canonical dataflow V*HW*K*(8*(1+2+2)+2), V=2, K=21",
