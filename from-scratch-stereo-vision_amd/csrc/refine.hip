@@ -2406,32 +2406,46 @@ int launch_costs(const Plan& pl, bool right, const uint64_t* cA, const uint64_t*
     return sv::set_error(SV_EINVAL, "unknown image dtype %d", dtype);
 }
 
-// A second stream per device for work that can overlap the caller's: the
-// horizontal SGM paths run there while the vertical and diagonal ones run
-// on the caller's stream (fork and join through events, so the caller's
-// stream order is kept and a graph capture of the caller's stream records
-// both branches).  Created on first use, kept until the process ends.
+// A second stream for work that can overlap the caller's: the horizontal
+// SGM paths run there while the vertical and diagonal ones run on the
+// caller's stream (fork and join through events, so the caller's stream
+// order is kept).  One per (device, caller stream), so callers on different
+// streams never wait on each other's work; created on first use and kept
+// until the process ends.  Past kSidePerDevice caller streams on one device
+// the work stays on the caller's stream.
 struct SideStream {
+    hipStream_t caller = nullptr;
     hipStream_t s = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
 };
 
+constexpr int kSidePerDevice = 16;
 std::mutex g_side_mu;
-SideStream g_side[64];
+SideStream g_side[64][kSidePerDevice];
 
-int side_stream(SideStream** out) {
+// *out = the side stream of caller (null when the table is full)
+int side_stream(hipStream_t caller, SideStream** out) {
+    *out = nullptr;
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess || dev < 0 || dev >= 64)
         return sv::set_error(SV_EHIP, "side stream: no current device");
-    SideStream& ss = g_side[dev];
-    if (!ss.s) {
-        if ((e = hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking)) != hipSuccess ||
-            (e = hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming)) != hipSuccess ||
-            (e = hipEventCreateWithFlags(&ss.join, hipEventDisableTiming)) != hipSuccess)
-            return sv::set_error(SV_EHIP, "side stream: %s", hipGetErrorString(e));
+    for (int i = 0; i < kSidePerDevice; ++i) {
+        SideStream& ss = g_side[dev][i];
+        if (ss.s && ss.caller == caller) {
+            *out = &ss;
+            return SV_OK;
+        }
+        if (!ss.s) {
+            if ((e = hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking)) != hipSuccess ||
+                (e = hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming)) != hipSuccess ||
+                (e = hipEventCreateWithFlags(&ss.join, hipEventDisableTiming)) != hipSuccess)
+                return sv::set_error(SV_EHIP, "side stream: %s", hipGetErrorString(e));
+            ss.caller = caller;
+            *out = &ss;
+            return SV_OK;
+        }
     }
-    *out = &ss;
     return SV_OK;
 }
 
@@ -2483,10 +2497,10 @@ int sgm_k_t(const Plan& pl, const void* C, const int* base, void* L, int BV, con
     if (sizeof(LT) == 1 && pl.invalid + P2 + P1 < 65000.0 && pl.Kp >= 8 * ((K + 1 + 7) / 8)) {
         const int hEnd = min(dir0 + ndir, 2);
         const int vBeg = max(dir0, 2), vEnd = dir0 + ndir;
-        // both kinds of path in this call: horizontal ones on the side stream,
-        // except while the caller's stream is being captured into a graph
-        // (the one side stream per device would then join every capturing
-        // stream that reaches here; such captures run both kinds in order)
+        // both kinds of path in this call: horizontal ones on the caller
+        // stream's side stream, except while the caller's stream is being
+        // captured into a graph (captured work keeps to the capturing
+        // stream and runs both kinds in order)
         bool fork = hEnd > dir0 && vEnd > vBeg && sv::tune(SV_TUNE_SGM_OVERLAP) == 2 && !t_noalloc;
         if (fork) {
             hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
@@ -2498,8 +2512,11 @@ int sgm_k_t(const Plan& pl, const void* C, const int* base, void* L, int BV, con
         hipStream_t hst = st;
         if (fork) {
             lock.lock();
-            int rc = side_stream(&ss);
+            int rc = side_stream(st, &ss);
             if (rc) return rc;
+            fork = ss != nullptr;
+        }
+        if (fork) {
             hipError_t e = hipEventRecord(ss->fork, st);
             if (e == hipSuccess) e = hipStreamWaitEvent(ss->s, ss->fork, 0);
             if (e != hipSuccess) return sv::set_error(SV_EHIP, "sgm fork: %s", hipGetErrorString(e));
