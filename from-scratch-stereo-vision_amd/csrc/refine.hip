@@ -1436,44 +1436,60 @@ __global__ void __launch_bounds__(64) k_sgm_ph2(const uint8_t* __restrict__ C,
     constexpr int CU = T * NB;          // 8-byte cost units of a full row run
     constexpr int TOT = ROWS * CU;      // units of a full tile
     constexpr int TB = ROWS * T;        // band centres of a full tile
+    constexpr int NIT = (TOT + 511) / 512, NITB = CB ? 1 : (TB + 255) / 256;
+    // the next tile's costs and band centres, loaded into registers while
+    // this tile is swept and stored into LDS after it is written out.  The
+    // staging covers the (row, unit) grid, clamped onto the last valid
+    // element (a duplicate load and an identical LDS write).
+    uint2 pv[NIT][8];
+    int poff[NIT][8];
+    int pb[NITB][4], pboff[NITB][4];
+    auto fetch = [&](int c0n) {
+        const int n = min(T, W - c0n);
+        const int xlo = dx > 0 ? c0n : W - c0n - n;
+        const int units = n * NB;
+        const uint8_t* Cx = Cb + (long long)xlo * Kp;
+        const int* Bx = Bb + xlo;
+#pragma unroll
+        for (int it = 0; it < NIT; ++it)
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int idx = it * 512 + u * 64 + lane;
+                const int r = min(idx / CU, rows - 1);
+                const int q = min(idx % CU, units - 1);
+                poff[it][u] = r * RS + 2 * q;
+                pv[it][u] = *reinterpret_cast<const uint2*>(Cx + r * rowBytes + 8 * q);
+            }
+        if constexpr (!CB) {
+#pragma unroll
+            for (int it = 0; it < NITB; ++it)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int idx = it * 256 + u * 64 + lane;
+                    const int r = min(idx / T, rows - 1);
+                    const int c = min(idx % T, n - 1);
+                    pboff[it][u] = r * BS + c;
+                    pb[it][u] = Bx[(long long)r * W + c];
+                }
+        }
+    };
+    fetch(0);
     for (int c0 = 0; c0 < W; c0 += T) {
         const int n = min(T, W - c0);
         const int xlo = dx > 0 ? c0 : W - c0 - n;
         const int units = n * NB;
-        const uint8_t* Cx = Cb + (long long)xlo * Kp;
-        const int* Bx = Bb + xlo;
-        // staging over the (row, unit) grid, clamped onto the last valid
-        // element (a duplicate load and an identical LDS write)
-#pragma unroll 1
-        for (int it = 0; it < TOT; it += 64 * 8) {
-            uint2 v[8];
-            int off[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int idx = it + u * 64 + lane;
-                const int r = min(idx / CU, rows - 1);
-                const int q = min(idx % CU, units - 1);
-                off[u] = r * RS + 2 * q;
-                v[u] = *reinterpret_cast<const uint2*>(Cx + r * rowBytes + 8 * q);
-            }
+        for (int it = 0; it < NIT; ++it)
 #pragma unroll
-            for (int u = 0; u < 8; ++u) *reinterpret_cast<uint2*>(tile + off[u]) = v[u];
-        }
+            for (int u = 0; u < 8; ++u) *reinterpret_cast<uint2*>(tile + poff[it][u]) = pv[it][u];
+        if constexpr (!CB) {
 #pragma unroll
-        for (int it = 0; it < (CB ? 0 : TB); it += 64 * 4) {
-            int v[4], off[4];
+            for (int it = 0; it < NITB; ++it)
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int idx = it + u * 64 + lane;
-                const int r = min(idx / T, rows - 1);
-                const int c = min(idx % T, n - 1);
-                off[u] = r * BS + c;
-                v[u] = Bx[(long long)r * W + c];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) btile[off[u]] = v[u];
+                for (int u = 0; u < 4; ++u) btile[pboff[it][u]] = pb[it][u];
         }
         __syncthreads();
+        if (c0 + T < W) fetch(c0 + T);
         for (int j = 0; j < n; ++j) {
             const int col = dx > 0 ? j : n - 1 - j;
             uint32_t cw[NW];
