@@ -808,15 +808,11 @@ struct LineCursor {
         x = lp.horiz ? (lp.dx > 0 ? i : lp.W - 1 - i) : p - (lp.dy > 0 ? i : lp.H - 1 - i) * lp.W;
     }
     __device__ __forceinline__ void next() {
-        x += dx;
-        p += dyW + dx;
-        if (x >= W) {
-            x = 0;
-            p -= W;
-        } else if (x < 0) {
-            x = W - 1;
-            p += W;
-        }
+        const int xn = x + dx;  // in [-1, W]
+        int adj = xn >= W ? W : 0;
+        adj = xn < 0 ? -W : adj;
+        x = xn - adj;
+        p += dyW + dx - adj;
     }
     // border pixel (the recurrence restarts): the first step, or where a
     // diagonal re-enters at the opposite edge
