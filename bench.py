@@ -254,7 +254,7 @@ def main():
         # rest (every variant leaves the results unchanged)
         layers = A.tune_layers(net, lambda: step()[:2], reps=2)
         tuned = A.tune(lambda: step()[:2],
-                       knobs=["sgm_split", "sgm_prefetch", "sgm_htile", "sgm_overlap", "wta_reads", "sgm_order"], reps=2,
+                       knobs=["sgm_split", "sgm_prefetch", "sgm_htile", "sgm_overlap", "wta_reads", "sgm_order", "sgm_queue"], reps=2,
                        log=log)
         tuned["conv_layers_tuned"] = len(layers)
     # the parts share the per-layer choices (keyed by launch shape, batch
@@ -1545,6 +1545,51 @@ def mesh_main(args):
     print(json.dumps(line), flush=True)
 
 
+def ragged_fullrange(args, sv, b, l, r, pairs, spec):
+    """The full-range control of eval_real.py:143-146 on the same 64 uneven
+    pairs: each pair's own R = int(max visible truth + 2) // 2 + 1 (K = 2R +
+    1, up to 245 here), P1=12, P2=150, census 9x7, one sv.full_range call
+    per pair (their sizes differ), the 64 calls captured in one HIP graph
+    and replayed like the net + band step."""
+    import numpy as np
+    import torch
+    RG = importlib.import_module("from-scratch-stereo-vision_amd.ragged")
+    Rs = [int(float(np.asarray(p["disp"])[np.asarray(p["valid"])].max()) + 2) // 2 + 1
+          for p in pairs]
+    ls, rs = b.unpack(l), b.unpack(r)
+    dims = [(w, h) for w, h, _ in spec]
+
+    def f(a, c):
+        A_, C_ = b.unpack(a), b.unpack(c)
+        return [sv.full_range(A_[n], C_[n], w, h, Rs[n], P1=12.0, P2=150.0, rx=4, ry=3)
+                for n, (w, h) in enumerate(dims)]
+
+    log("full-range control: 64 calls, K up to %d" % (2 * max(Rs) + 1))
+    eager = [o.clone() for o in f(l, r)]
+    torch.cuda.synchronize()
+    g = RG.CapturedPipeline(f, l, r, warmup=1)
+    outs = g.replay()
+    torch.cuda.synchronize()
+    if not all(torch.equal(u, v) for u, v in zip(outs, eager)):
+        raise RuntimeError("full-range graph replay differs from the eager run")
+    steps = max(2, args.steps // 2)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.replay()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    bad = []
+    for o, p in zip(outs, pairs):
+        e = np.abs(o.cpu().numpy().reshape(np.asarray(p["disp"]).shape) - p["disp"])
+        bad.append(float((e > 2.0)[p["valid"]].mean() * 100.0))
+    return {"pairs_per_s": round(64 / dt, 2), "ms_per_step": round(dt * 1e3, 3),
+            "execution": "hip graph of 64 sv.full_range calls, one per pair",
+            "K_max": 2 * max(Rs) + 1, "K_mean": round(float(np.mean([2 * v + 1 for v in Rs])), 1),
+            "bad2_pct_mean": round(float(np.mean(bad)), 3),
+            "bad2_pct_max": round(float(np.max(bad)), 3),
+            "kernels": "constant-centre packed kernels for K <= 125, the general band kernels above"}
+
+
 def ragged_main(args):
     """64 synthetic pairs of uneven size (200x150 up to 1390x1110, maximum
     disparity 16-240 px) through net + band SGM as one ragged batch: the
@@ -1577,7 +1622,7 @@ def ragged_main(args):
     tuned = None
     if not args.no_autotune:
         layers = A.tune_layers(net, run, reps=2)
-        tuned = A.tune(run, knobs=["sgm_split", "sgm_prefetch", "sgm_htile", "sgm_overlap", "wta_reads", "sgm_order"], reps=2, log=log)
+        tuned = A.tune(run, knobs=["sgm_split", "sgm_prefetch", "sgm_htile", "sgm_overlap", "wta_reads", "sgm_order", "sgm_queue"], reps=2, log=log)
         tuned["conv_layers_tuned"] = len(layers)
     # eager reference run (stage profile of the refinement) and capture
     L.lib().sv_profile_enable(1)
@@ -1608,6 +1653,7 @@ def ragged_main(args):
     same = all(torch.equal(u, v) for u, v in zip(b.unpack(out), b.unpack(eout)))
     outs = [o.cpu().numpy() for o in b.unpack(out)]
     bad = [float((np.abs(o - p["disp"]) > 2.0)[p["valid"]].mean() * 100.0) for o, p in zip(outs, pairs)]
+    fr = None if args.no_fullrange else ragged_fullrange(args, sv, b, l, r, pairs, spec)
     px = sum(w * h for w, h, _ in spec)
     K = 2 * R_BAND + 1
     canon = sum(2 * w * h * K * (1 + 8 * 5 + 2) + 2 * w * h * 10 + w * h for w, h, _ in spec)
@@ -1629,6 +1675,8 @@ def ragged_main(args):
         "accuracy": {"bad2_pct_mean": round(float(np.mean(bad)), 3),
                      "bad2_pct_max": round(float(np.max(bad)), 3)},
         "stage_ms_eager": stages,
+        "fullrange": fr,
+        "speedup_vs_fullrange": round(fr["ms_per_step"] / (dt * 1e3), 3) if fr else None,
         "roofline": {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
                      "pipeline_canonical_bytes": canon,
                      "pipeline_achieved": round(canon / dt / 1e9, 1),

@@ -36,7 +36,8 @@ KNOBS = {"sgm_prefetch": (0, (2, 4, 8)),
          "sgm_split": (8, (1, 2)),
          "wta_reads": (9, (1, 2)),
          "conv_dchunk": (10, (12, 6, 4)),
-         "sgm_order": (11, (1, 2))}
+         "sgm_order": (11, (1, 2)),
+         "sgm_queue": (12, (1, 2, 3))}
 
 
 def set_knob(name, value):

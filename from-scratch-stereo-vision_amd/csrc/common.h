@@ -33,7 +33,8 @@
 #define SV_TUNE_WTA_READS 9     // band epilogue (K = 21, u8): 1 one lane per pixel, 2 coalesced runs through LDS
 #define SV_TUNE_CONV_DCHUNK 10  // depth-column 3D convolution: output depths per tile, 4, 6, 12
 #define SV_TUNE_SGM_ORDER 11   // vertical/diagonal SGM blocks: 1 in launch order across the XCDs, 2 one contiguous run of (pair, direction, columns) per XCD
-#define SV_TUNE_COUNT 12
+#define SV_TUNE_SGM_QUEUE 12   // packed SGM kernels with two lanes per scanline: 1 one workgroup per block of scanlines, 2 persistent workgroups pulling (image, direction, scanline block) items from per-XCD queues, longest scanlines first, 3 the same with one queue for the whole device
+#define SV_TUNE_COUNT 13
 
 namespace sv {
 

@@ -48,7 +48,8 @@ int sv_tune_set(int knob, int value) {
         {2, 4, 8, -1, -1, -1, -1}, {8, 16, -1, -1, -1, -1, -1}, {1, 4, -1, -1, -1, -1, -1},
         {16, 32, 64, -1, -1, -1, -1}, {1, 2, -1, -1, -1, -1, -1}, {1, 2, -1, -1, -1, -1, -1},
         {1, 2, 3, 4, 5, 6, 7}, {1, 2, 4, -1, -1, -1, -1}, {1, 2, -1, -1, -1, -1, -1},
-        {1, 2, -1, -1, -1, -1, -1}, {4, 6, 12, -1, -1, -1, -1}, {1, 2, -1, -1, -1, -1, -1}};
+        {1, 2, -1, -1, -1, -1, -1}, {4, 6, 12, -1, -1, -1, -1}, {1, 2, -1, -1, -1, -1, -1},
+        {1, 2, 3, -1, -1, -1, -1}};
     if (knob < 0 || knob >= SV_TUNE_COUNT)
         return sv::set_error(SV_EINVAL, "tuning knob %d does not exist", knob);
     bool ok = value == 0;

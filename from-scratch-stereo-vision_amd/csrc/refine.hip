@@ -1296,18 +1296,18 @@ __device__ __forceinline__ uint3 sgm_block(int xcd) {
     return make_uint3(b % gx, (b / gx) % gy, b / (gx * gy));
 }
 
-// k_sgm_p with two lanes per scanline (32 lines per wave).  CB:
-// constant band centre (packed_step2c; the band centres are not read).
-template <int K, int PF, bool CB = false>
-__global__ void __launch_bounds__(64) k_sgm_p2(const uint8_t* __restrict__ C,
-                                               const int* __restrict__ base,
-                                               uint8_t* __restrict__ Lout, Geom g, int Kp,
-                                               int P1, int P2, int dir0, long long dirStride,
-                                               int xcd) {
+// k_sgm_p with two lanes per scanline (32 lines per wave): one block of 32
+// scanlines (bk.x) of direction dir0 + bk.y of image bk.z.  CB: constant
+// band centre (packed_step2c; the band centres are not read).  lds: the
+// wave's 32 * SD words.
+template <int K, int PF, bool CB>
+__device__ __forceinline__ void sgm_p2_block(const uint8_t* __restrict__ C,
+                                             const int* __restrict__ base,
+                                             uint8_t* __restrict__ Lout, const Geom& g, int Kp,
+                                             int P1, int P2, int dir0, long long dirStride,
+                                             uint3 bk, uint32_t* lds) {
     using PB = PackedBand2<K>;
     constexpr int NPH = PB::NPH, NW = PB::NW, SD = PB::SD;
-    __shared__ uint32_t lds[32 * SD];
-    const uint3 bk = sgm_block(xcd);
     const int W = g.w(bk.z), H = g.h(bk.z);
     const int dir = dir0 + (int)bk.y;
     LinePos lp;
@@ -1387,38 +1387,159 @@ __global__ void __launch_bounds__(64) k_sgm_p2(const uint8_t* __restrict__ C,
         if (i0 + u < len) step(i0 + u, ring[u], rb[u]);
 }
 
-// k_sgm_ph with two lanes per row: 32 rows per wave, the row tiles staged
-// through LDS as in k_sgm_ph.
-template <int K, int TC, bool CB = false>
-__global__ void __launch_bounds__(64) k_sgm_ph2(const uint8_t* __restrict__ C,
+template <int K, int PF, bool CB = false>
+__global__ void __launch_bounds__(64) k_sgm_p2(const uint8_t* __restrict__ C,
+                                               const int* __restrict__ base,
+                                               uint8_t* __restrict__ Lout, Geom g, int Kp,
+                                               int P1, int P2, int dir0, long long dirStride,
+                                               int xcd) {
+    __shared__ uint32_t lds[32 * PackedBand2<K>::SD];
+    sgm_p2_block<K, PF, CB>(C, base, Lout, g, Kp, P1, P2, dir0, dirStride, sgm_block(xcd), lds);
+}
+
+// ------------------------------------------------ persistent work queue
+// A ragged batch's scanline blocks differ in length by up to the ratio of
+// its largest to smallest image, and a grid in launch order starts the
+// longest ones wherever they happen to fall.  k_sgm_queue orders the
+// (image, direction, scanline block) items of one SGM kernel longest
+// first, deals the images round-robin to eight queues (one per XCD, so
+// that an image's directions, which read the same cost rows, share an L2)
+// and clears the queue heads; the persistent kernels below then pull items
+// until every queue is empty, a workgroup emptying its own queue going on
+// to the others.  Queue layout (ints): [0, 8) heads, [8, 16) item counts,
+// [16, 24) first item, [32, 32 + BV) image order, [32 + BV, 32 + 2 BV)
+// image offsets within their queue, then the items (image << 16 | j with
+// j = block * ndir + direction).
+constexpr int SGM_Q_HDR = 32;
+
+__device__ __forceinline__ int sgm_q_len(const Geom& g, int z, int horiz) {
+    return horiz ? g.w(z) : g.h(z);
+}
+__device__ __forceinline__ int sgm_q_items(const Geom& g, int z, int horiz, int ndir) {
+    return ((horiz ? g.h(z) : g.w(z)) + 31) / 32 * ndir;
+}
+
+// blockIdx.x: which queue of `qs` (0: vertical/diagonal kernel, 1:
+// horizontal kernel), each of qints ints.
+__global__ void __launch_bounds__(256) k_sgm_queue(Geom g, int BV, int ndirV, int ndirH, int nq,
+                                                   int* __restrict__ qs, long long qints) {
+    const int horiz = blockIdx.x;
+    const int ndir = horiz ? ndirH : ndirV;
+    int* q = qs + horiz * qints;
+    int* order = q + SGM_Q_HDR;
+    int* off = order + BV;
+    int* items = off + BV;
+    // rank by scanline length, longest first, ties by image index
+    for (int z = threadIdx.x; z < BV; z += blockDim.x) {
+        const int lz = sgm_q_len(g, z, horiz);
+        int r = 0;
+        for (int u = 0; u < BV; ++u) {
+            const int lu = sgm_q_len(g, u, horiz);
+            r += lu > lz || (lu == lz && u < z);
+        }
+        order[r] = z;
+    }
+    __syncthreads();
+    __shared__ int cnt[8];
+    if (threadIdx.x < 8) {
+        int acc = 0;
+        for (int s = threadIdx.x; s < BV && (int)threadIdx.x < nq; s += nq) {
+            const int z = order[s];
+            off[z] = acc;
+            acc += sgm_q_items(g, z, horiz, ndir);
+        }
+        cnt[threadIdx.x] = acc;
+    }
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        int first = 0;
+        for (int k = 0; k < (int)threadIdx.x; ++k) first += cnt[k];
+        q[threadIdx.x] = 0;
+        q[8 + threadIdx.x] = cnt[threadIdx.x];
+        q[16 + threadIdx.x] = first;
+    }
+    __syncthreads();
+    for (int s = threadIdx.x; s < BV; s += blockDim.x) {
+        const int z = order[s];
+        const int n = sgm_q_items(g, z, horiz, ndir);
+        int* dst = items + q[16 + s % nq] + off[z];
+        for (int j = 0; j < n; ++j) dst[j] = z << 16 | j;
+    }
+}
+
+// The next item of queue k for this workgroup (one wave), -1 when empty.
+__device__ __forceinline__ int sgm_q_pull(int* __restrict__ q, int k) {
+    int i = 0;
+    if (threadIdx.x == 0) i = atomicAdd(q + k, 1);
+    i = __builtin_amdgcn_readfirstlane(__shfl(i, 0));
+    if (i >= q[8 + k]) return -1;
+    return q[SGM_Q_HDR + 0] * 0 + i;
+}
+
+template <int K, int PF, bool CB = false>
+__global__ void __launch_bounds__(64) k_sgm_p2q(const uint8_t* __restrict__ C,
                                                 const int* __restrict__ base,
                                                 uint8_t* __restrict__ Lout, Geom g, int Kp,
-                                                int P1, int P2, int dir0, long long dirStride) {
+                                                int P1, int P2, int dir0, long long dirStride,
+                                                int* __restrict__ q, int BV, int ndir, int nq) {
+    __shared__ uint32_t lds[32 * PackedBand2<K>::SD];
+    const int* items = q + SGM_Q_HDR + 2 * BV;
+    for (int k = 0; k < nq; ++k) {
+        const int qk = (blockIdx.x + k) % nq;
+        for (;;) {
+            const int i = sgm_q_pull(q, qk);
+            if (i < 0) break;
+            const int it = __builtin_amdgcn_readfirstlane(items[q[16 + qk] + i]);
+            const int j = it & 0xFFFF;
+            sgm_p2_block<K, PF, CB>(C, base, Lout, g, Kp, P1, P2, dir0, dirStride,
+                                    make_uint3(j / ndir, j % ndir, it >> 16), lds);
+        }
+    }
+}
+
+// k_sgm_ph with two lanes per row: 32 rows per wave, the row tiles staged
+// through LDS as in k_sgm_ph.
+// LDS of one k_sgm_ph2 workgroup
+template <int K, int TC, bool CB>
+struct Ph2Lds {
+    using HT = HorizTile<K, TC>;
+    __align__(16) uint32_t tile[32 * HT::RS];
+    int btile[CB ? 1 : 32 * HT::BS];
+    uint32_t lds[CB ? 1 : 32 * PackedBand2<K>::SD];
+};
+
+// One block of 32 rows (bk.x) of direction dir0 + bk.y of image bk.z.
+template <int K, int TC, bool CB>
+__device__ __forceinline__ void sgm_ph2_block(const uint8_t* __restrict__ C,
+                                              const int* __restrict__ base,
+                                              uint8_t* __restrict__ Lout, const Geom& g, int Kp,
+                                              int P1, int P2, int dir0, long long dirStride,
+                                              uint3 bk, Ph2Lds<K, TC, CB>& sh) {
     using PB = PackedBand2<K>;
     using HT = HorizTile<K, TC>;
     constexpr int ROWS = 32;
-    const int W = g.w(blockIdx.z), H = g.h(blockIdx.z);
-    if ((int)blockIdx.x * ROWS >= H) return;  // whole block below this image
+    const int W = g.w(bk.z), H = g.h(bk.z);
+    if ((int)bk.x * ROWS >= H) return;  // whole block below this image
     constexpr int NPH = PB::NPH, NW = PB::NW, SD = PB::SD;
     constexpr int NB = PackedBand<K>::NB;
     constexpr int T = HT::T, RS = HT::RS, BS = HT::BS;
     static_assert(RS % 2 == 0, "8-byte aligned rows");
     static_assert(4 * NW * 2 <= 8 * NB, "a lane pair's words fit the pixel's chunks");
-    __shared__ __align__(16) uint32_t tile[ROWS * RS];
-    __shared__ int btile[CB ? 1 : ROWS * BS];
-    __shared__ uint32_t lds[CB ? 1 : ROWS * SD];
-    const int dx = c_dx[dir0 + (int)blockIdx.y];
+    uint32_t* tile = sh.tile;
+    int* btile = sh.btile;
+    uint32_t* lds = sh.lds;
+    const int dx = c_dx[dir0 + (int)bk.y];
     const int lane = threadIdx.x, h = lane & 1, ln = lane >> 1;
-    const int y0 = blockIdx.x * ROWS;
+    const int y0 = bk.x * ROWS;
     const int rows = min(ROWS, H - y0);
     u32_alias* sl32 = reinterpret_cast<u32_alias*>(lds + (CB ? 0 : ln * SD));
     u16_alias* s16 = reinterpret_cast<u16_alias*>(sl32);
     if constexpr (!CB)
         for (int i = h; i < SD; i += 2) sl32[i] = 0xFFFFFFFFu;
     const long long plane = g.slot;
-    const uint8_t* Cb = C + (long long)blockIdx.z * plane * Kp + (long long)y0 * W * Kp;
-    const int* Bb = base + (long long)blockIdx.z * plane + (long long)y0 * W;
-    uint8_t* Lb = Lout + (long long)blockIdx.y * dirStride + (long long)blockIdx.z * plane * Kp +
+    const uint8_t* Cb = C + (long long)bk.z * plane * Kp + (long long)y0 * W * Kp;
+    const int* Bb = base + (long long)bk.z * plane + (long long)y0 * W;
+    uint8_t* Lb = Lout + (long long)bk.y * dirStride + (long long)bk.z * plane * Kp +
                   (long long)y0 * W * Kp;
     const long long rowBytes = (long long)W * Kp;
     u16x2 L[NPH];
@@ -1438,8 +1559,7 @@ __global__ void __launch_bounds__(64) k_sgm_ph2(const uint8_t* __restrict__ C,
     // staging covers the (row, unit) grid, clamped onto the last valid
     // element (a duplicate load and an identical LDS write).
     uint2 pv[NIT][8];
-    int poff[NIT][8];
-    int pb[NITB][4], pboff[NITB][4];
+    int pb[NITB][4];
     auto fetch = [&](int c0n) {
         const int n = min(T, W - c0n);
         const int xlo = dx > 0 ? c0n : W - c0n - n;
@@ -1453,7 +1573,6 @@ __global__ void __launch_bounds__(64) k_sgm_ph2(const uint8_t* __restrict__ C,
                 const int idx = it * 512 + u * 64 + lane;
                 const int r = min(idx / CU, rows - 1);
                 const int q = min(idx % CU, units - 1);
-                poff[it][u] = r * RS + 2 * q;
                 pv[it][u] = *reinterpret_cast<const uint2*>(Cx + r * rowBytes + 8 * q);
             }
         if constexpr (!CB) {
@@ -1464,7 +1583,6 @@ __global__ void __launch_bounds__(64) k_sgm_ph2(const uint8_t* __restrict__ C,
                     const int idx = it * 256 + u * 64 + lane;
                     const int r = min(idx / T, rows - 1);
                     const int c = min(idx % T, n - 1);
-                    pboff[it][u] = r * BS + c;
                     pb[it][u] = Bx[(long long)r * W + c];
                 }
         }
@@ -1477,12 +1595,23 @@ __global__ void __launch_bounds__(64) k_sgm_ph2(const uint8_t* __restrict__ C,
 #pragma unroll
         for (int it = 0; it < NIT; ++it)
 #pragma unroll
-            for (int u = 0; u < 8; ++u) *reinterpret_cast<uint2*>(tile + poff[it][u]) = pv[it][u];
+            for (int u = 0; u < 8; ++u) {
+                // the fetch's (row, unit) of this element, recomputed
+                const int idx = it * 512 + u * 64 + lane;
+                const int r = min(idx / CU, rows - 1);
+                const int q = min(idx % CU, units - 1);
+                *reinterpret_cast<uint2*>(tile + r * RS + 2 * q) = pv[it][u];
+            }
         if constexpr (!CB) {
 #pragma unroll
             for (int it = 0; it < NITB; ++it)
 #pragma unroll
-                for (int u = 0; u < 4; ++u) btile[pboff[it][u]] = pb[it][u];
+                for (int u = 0; u < 4; ++u) {
+                    const int idx = it * 256 + u * 64 + lane;
+                    const int r = min(idx / T, rows - 1);
+                    const int c = min(idx % T, n - 1);
+                    btile[r * BS + c] = pb[it][u];
+                }
         }
         __syncthreads();
         if (c0 + T < W) fetch(c0 + T);
@@ -1514,6 +1643,39 @@ __global__ void __launch_bounds__(64) k_sgm_ph2(const uint8_t* __restrict__ C,
             }
         }
         __syncthreads();
+    }
+}
+
+template <int K, int TC, bool CB = false>
+__global__ void __launch_bounds__(64) k_sgm_ph2(const uint8_t* __restrict__ C,
+                                                const int* __restrict__ base,
+                                                uint8_t* __restrict__ Lout, Geom g, int Kp,
+                                                int P1, int P2, int dir0, long long dirStride) {
+    __shared__ Ph2Lds<K, TC, CB> sh;
+    sgm_ph2_block<K, TC, CB>(C, base, Lout, g, Kp, P1, P2, dir0, dirStride,
+                             make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), sh);
+}
+
+// k_sgm_ph2 as a persistent workgroup over the horizontal kernel's queue
+template <int K, int TC, bool CB = false>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_sgm_ph2q(const uint8_t* __restrict__ C,
+                                                 const int* __restrict__ base,
+                                                 uint8_t* __restrict__ Lout, Geom g, int Kp,
+                                                 int P1, int P2, int dir0, long long dirStride,
+                                                 int* __restrict__ q, int BV, int ndir, int nq) {
+    __shared__ Ph2Lds<K, TC, CB> sh;
+    const int* items = q + SGM_Q_HDR + 2 * BV;
+    for (int k = 0; k < nq; ++k) {
+        const int qk = (blockIdx.x + k) % nq;
+        for (;;) {
+            const int i = sgm_q_pull(q, qk);
+            if (i < 0) break;
+            const int it = __builtin_amdgcn_readfirstlane(items[q[16 + qk] + i]);
+            const int j = it & 0xFFFF;
+            sgm_ph2_block<K, TC, CB>(C, base, Lout, g, Kp, P1, P2, dir0, dirStride,
+                                     make_uint3(j / ndir, j % ndir, it >> 16), sh);
+            __syncthreads();  // the next item restages the tiles
+        }
     }
 }
 
@@ -2303,6 +2465,8 @@ int make_plan(int W, int H, int R, double P1, double P2, double ad_weight, doubl
     return choose_layout(pl->K, !pl->integer, &pl->G, &pl->KC);
 }
 
+long long sgm_queue_ints(int BV, int W, int H);
+
 struct RefineBufs {
     uint64_t* cen;   // [2][B][HW]
     int* base;       // [V][B][HW]
@@ -2311,10 +2475,13 @@ struct RefineBufs {
     double* d0r;     // [B][HW]            (lr only)
     double* disp;    // [V][B][HW]         (lr only: dl, dr)
     double* tmp;     // [B][HW]            (median)
+    int* queue;      // 2 persistent SGM queues (sgm_queue_ints each)
     size_t bytes;
 };
 
-RefineBufs carve(void* ws, const Plan& pl, int V, int B, long long slot, bool lr, bool median) {
+// W, H: bounds of every image (the SGM queues are sized by them)
+RefineBufs carve(void* ws, const Plan& pl, int V, int B, long long slot, bool lr, bool median,
+                 int W, int H) {
     sv::Carver cv(ws);
     const size_t hw = (size_t)slot;
     RefineBufs r;
@@ -2326,6 +2493,7 @@ RefineBufs carve(void* ws, const Plan& pl, int V, int B, long long slot, bool lr
     r.d0r = lr ? cv.take<double>((size_t)B * hw) : nullptr;
     r.disp = lr ? cv.take<double>((size_t)V * B * hw) : nullptr;
     r.tmp = median ? cv.take<double>((size_t)B * hw) : nullptr;
+    r.queue = cv.take<int>(2 * (size_t)sgm_queue_ints(V * B, W, H));
     r.bytes = cv.used;
     return r;
 }
@@ -2497,9 +2665,53 @@ int wta_t(const Plan& pl, const void* L, int BV, int mirrorFrom, const int* base
     }                                                                          \
     return sv::set_error(SV_EUNSUPPORTED, "band chunk %d", KCV);
 
+// Ints of one persistent-queue area for BV images of at most W x H.
+long long sgm_queue_ints(int BV, int W, int H) {
+    return SGM_Q_HDR + 2LL * BV + (long long)BV * (((W > H ? W : H) + 31) / 32) * 8;
+}
+
+// Workgroups of a persistent kernel that fit the device at once.
+template <typename F>
+int resident_blocks(F kernel, int* cache) {
+    if (*cache > 0) return *cache;
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 64, 0) != hipSuccess)
+        return 0;
+    *cache = cus * (per > 0 ? per : 1);
+    return *cache;
+}
+
+template <int K, int PF>
+int launch_p2q(const void* C, const int* base, void* L, const Geom& g, int Kp, int P1, int P2,
+               int dir0, long long dirStride, int* q, int BV, int ndir, int nq, long long items,
+               hipStream_t st) {
+    static int cache = 0;
+    const int rb = resident_blocks(k_sgm_p2q<K, PF>, &cache);
+    if (rb <= 0) return sv::set_error(SV_EHIP, "sgm queue: occupancy query failed");
+    const unsigned nb = (unsigned)(items < rb ? items : rb);
+    hipLaunchKernelGGL((k_sgm_p2q<K, PF>), dim3(nb), dim3(64), 0, st, (const uint8_t*)C, base,
+                       (uint8_t*)L, g, Kp, P1, P2, dir0, dirStride, q, BV, ndir, nq);
+    return SV_OK;
+}
+
+template <int K, int TC>
+int launch_ph2q(const void* C, const int* base, void* L, const Geom& g, int Kp, int P1, int P2,
+                int dir0, long long dirStride, int* q, int BV, int ndir, int nq, long long items,
+                hipStream_t st) {
+    static int cache = 0;
+    const int rb = resident_blocks(k_sgm_ph2q<K, TC>, &cache);
+    if (rb <= 0) return sv::set_error(SV_EHIP, "sgm queue: occupancy query failed");
+    const unsigned nb = (unsigned)(items < rb ? items : rb);
+    hipLaunchKernelGGL((k_sgm_ph2q<K, TC>), dim3(nb), dim3(64), 0, st, (const uint8_t*)C, base,
+                       (uint8_t*)L, g, Kp, P1, P2, dir0, dirStride, q, BV, ndir, nq);
+    return SV_OK;
+}
+
 template <int K, typename LT>
 int sgm_k_t(const Plan& pl, const void* C, const int* base, void* L, int BV, const Geom& g,
-            double P1, double P2, int dir0, int ndir, hipStream_t st) {
+            double P1, double P2, int dir0, int ndir, int* queue, hipStream_t st) {
     const int nl = g.W > g.H ? g.W : g.H;
     dim3 grid((nl + 63) / 64, ndir, BV);
     const long long dirStride = (long long)BV * g.slot * pl.Kp;
@@ -2513,6 +2725,19 @@ int sgm_k_t(const Plan& pl, const void* C, const int* base, void* L, int BV, con
         // stream's side stream, except while the caller's stream is being
         // captured into a graph (captured work keeps to the capturing
         // stream and runs both kinds in order)
+        // persistent workgroups over work queues (tuning knob 12 = 2): the
+        // queues are built on the caller's stream before anything forks
+        const bool split0 = sv::tune(SV_TUNE_SGM_SPLIT) == 2;
+        const int qmode = sv::tune(SV_TUNE_SGM_QUEUE);
+        const int nq = qmode == 3 ? 1 : 8;
+        const bool useq = queue && split0 && qmode >= 2 && BV < 32768 &&
+                          ((g.W > g.H ? g.W : g.H) + 31) / 32 * 8 <= 0xFFFF;
+        const long long qints = sgm_queue_ints(BV, g.W, g.H);
+        if (useq) {
+            hipLaunchKernelGGL(k_sgm_queue, dim3(2), dim3(256), 0, st, g, BV, max(vEnd - vBeg, 1),
+                               max(hEnd - dir0, 1), nq, queue, qints);
+            SV_CHECK_LAUNCH();
+        }
         bool fork = hEnd > dir0 && vEnd > vBeg && sv::tune(SV_TUNE_SGM_OVERLAP) == 2 && !t_noalloc;
         if (fork) {
             hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
@@ -2535,7 +2760,17 @@ int sgm_k_t(const Plan& pl, const void* C, const int* base, void* L, int BV, con
             hst = ss->s;
         }
         const bool split = sv::tune(SV_TUNE_SGM_SPLIT) == 2;
-        if (hEnd > dir0 && split) {
+        int qrc = SV_OK;
+        if (hEnd > dir0 && useq) {
+            const long long items = (long long)BV * ((g.H + 31) / 32) * (hEnd - dir0);
+            int* qh = queue + qints;
+            if (sv::tune(SV_TUNE_SGM_HTILE) == 8)
+                qrc = launch_ph2q<K, 8>(C, base, L, g, pl.Kp, (int)P1, (int)P2, dir0, dirStride, qh,
+                                        BV, hEnd - dir0, nq, items, hst);
+            else
+                qrc = launch_ph2q<K, 16>(C, base, L, g, pl.Kp, (int)P1, (int)P2, dir0, dirStride, qh,
+                                         BV, hEnd - dir0, nq, items, hst);
+        } else if (hEnd > dir0 && split) {
             dim3 hgrid((g.H + 31) / 32, hEnd - dir0, BV);
             if (sv::tune(SV_TUNE_SGM_HTILE) == 8)
                 hipLaunchKernelGGL((k_sgm_ph2<K, 8>), hgrid, dim3(64), 0, hst, (const uint8_t*)C,
@@ -2554,7 +2789,24 @@ int sgm_k_t(const Plan& pl, const void* C, const int* base, void* L, int BV, con
         }
         hipError_t lerr = hipGetLastError();
         const int xcd = sv::tune(SV_TUNE_SGM_ORDER) == 2;
-        if (vEnd > vBeg && lerr == hipSuccess && split) {
+        if (vEnd > vBeg && lerr == hipSuccess && qrc == SV_OK && useq) {
+            const long long items = (long long)BV * ((g.W + 31) / 32) * (vEnd - vBeg);
+            uint8_t* Lv = (uint8_t*)L + (long long)(vBeg - dir0) * dirStride;
+            switch (sv::tune(SV_TUNE_SGM_PREFETCH)) {
+                case 2:
+                    qrc = launch_p2q<K, 2>(C, base, Lv, g, pl.Kp, (int)P1, (int)P2, vBeg, dirStride,
+                                           queue, BV, vEnd - vBeg, nq, items, st);
+                    break;
+                case 8:
+                    qrc = launch_p2q<K, 8>(C, base, Lv, g, pl.Kp, (int)P1, (int)P2, vBeg, dirStride,
+                                           queue, BV, vEnd - vBeg, nq, items, st);
+                    break;
+                default:
+                    qrc = launch_p2q<K, 4>(C, base, Lv, g, pl.Kp, (int)P1, (int)P2, vBeg, dirStride,
+                                           queue, BV, vEnd - vBeg, nq, items, st);
+            }
+            lerr = hipGetLastError();
+        } else if (vEnd > vBeg && lerr == hipSuccess && qrc == SV_OK && split) {
             dim3 vgrid((g.W + 31) / 32, vEnd - vBeg, BV);
             uint8_t* Lv = (uint8_t*)L + (long long)(vBeg - dir0) * dirStride;
             switch (sv::tune(SV_TUNE_SGM_PREFETCH)) {
@@ -2595,7 +2847,7 @@ int sgm_k_t(const Plan& pl, const void* C, const int* base, void* L, int BV, con
             if (e != hipSuccess) return sv::set_error(SV_EHIP, "sgm join: %s", hipGetErrorString(e));
         }
         if (lerr != hipSuccess) return sv::set_error(SV_EHIP, "sgm launch: %s", hipGetErrorString(lerr));
-        return SV_OK;
+        return qrc;
     }
     hipLaunchKernelGGL((k_sgm_k<K, LT>), grid, dim3(64), 0, st, (const uint8_t*)C, base, (LT*)L, g,
                        pl.Kp, (int)P1, (int)P2, dir0, dirStride);
@@ -2608,14 +2860,18 @@ inline bool has_k_specialisation(const Plan& pl) {
     return pl.integer && pl.G == 1 && (pl.K == 21 || pl.K == 7) && pl.Kp >= ((pl.K + 7) / 8) * 8;
 }
 
+// queue: sgm_queue_ints(BV, g.W, g.H) * 2 ints for the persistent kernels,
+// or nullptr (grid launches only)
 int launch_sgm(const Plan& pl, const void* C, const int* base, void* L, int BV, const Geom& g,
-               double P1, double P2, int dir0, int ndir, hipStream_t st) {
+               double P1, double P2, int dir0, int ndir, hipStream_t st, int* queue = nullptr) {
     if (has_k_specialisation(pl)) {
         if (pl.K == 21)
-            return pl.lt == 1 ? sgm_k_t<21, uint8_t>(pl, C, base, L, BV, g, P1, P2, dir0, ndir, st)
-                              : sgm_k_t<21, uint16_t>(pl, C, base, L, BV, g, P1, P2, dir0, ndir, st);
-        return pl.lt == 1 ? sgm_k_t<7, uint8_t>(pl, C, base, L, BV, g, P1, P2, dir0, ndir, st)
-                          : sgm_k_t<7, uint16_t>(pl, C, base, L, BV, g, P1, P2, dir0, ndir, st);
+            return pl.lt == 1
+                       ? sgm_k_t<21, uint8_t>(pl, C, base, L, BV, g, P1, P2, dir0, ndir, queue, st)
+                       : sgm_k_t<21, uint16_t>(pl, C, base, L, BV, g, P1, P2, dir0, ndir, queue, st);
+        return pl.lt == 1
+                   ? sgm_k_t<7, uint8_t>(pl, C, base, L, BV, g, P1, P2, dir0, ndir, queue, st)
+                   : sgm_k_t<7, uint16_t>(pl, C, base, L, BV, g, P1, P2, dir0, ndir, queue, st);
     }
     if (!pl.integer) {
         switch (pl.KC) {
@@ -2759,7 +3015,8 @@ int refine_views(const Plan& pl, const RefineBufs& bf, const void* left, const v
     tc.stop();
     {
         StageTimer t(ST_SGM, st);
-        if ((rc = launch_sgm(pl, bf.C, bf.base, bf.L, V * B, g, P1, P2, 0, pl.ndir, st))) return rc;
+        if ((rc = launch_sgm(pl, bf.C, bf.base, bf.L, V * B, g, P1, P2, 0, pl.ndir, st, bf.queue)))
+            return rc;
     }
     StageTimer t(ST_WTA, st);
     return launch_wta(pl, bf.L, V * B, V == 2 ? B : V * B, bf.base, disp0, d0r, g, R, uniq,
@@ -3285,7 +3542,7 @@ int sv_refine_workspace_bytes(int mode, int B, int W, int H, int R, double P1, d
     if (rc) return rc;
     pl.Kp = pl.G * pl.KC;
     const int V = mode == 0 ? 1 : 2;
-    RefineBufs r = carve(nullptr, pl, V, B, (long long)W * H, mode != 0, mode == 2);
+    RefineBufs r = carve(nullptr, pl, V, B, (long long)W * H, mode != 0, mode == 2, W, H);
     *bytes = r.bytes;
     return SV_OK;
 }
@@ -3300,7 +3557,7 @@ int sv_refine_fullres(const void* left, const void* right, int dtype, const doub
     int rc = make_plan(W, H, R, P1, P2, ad_weight, prior_w, ad_trunc, prior_t, paths, rx, ry, &pl);
     if (rc) return rc;
     pl.Kp = pl.G * pl.KC;
-    RefineBufs bf = carve(ws, pl, 1, B, (long long)W * H, false, false);
+    RefineBufs bf = carve(ws, pl, 1, B, (long long)W * H, false, false, W, H);
     if ((rc = check_ws(bf.bytes, ws, ws_bytes))) return rc;
     return refine_views(pl, bf, left, right, dtype, disp0, nullptr, 1, uniform_geom(B, W, H), R,
                         P1, P2, ad_weight, ad_trunc, uniq, rx, ry, prior_w, prior_t, out, conf,
@@ -3325,7 +3582,7 @@ static int lr_core(const void* left, const void* right, int dtype, const double*
     if (rc) return rc;
     pl.Kp = pl.G * pl.KC;
     const int B = g.B;
-    RefineBufs bf = carve(ws, pl, 2, B, g.slot, true, median);
+    RefineBufs bf = carve(ws, pl, 2, B, g.slot, true, median, g.W, g.H);
     if ((rc = check_ws(bf.bytes, ws, ws_bytes))) return rc;
     {
         StageTimer t(ST_LR, st);
@@ -3390,7 +3647,7 @@ int sv_full_range_workspace_bytes(int mode, int B, int W, int H, int R, double P
     }
     pl.Kp = pl.G * pl.KC;
     // the general path plus the constant centre map
-    *bytes = carve(nullptr, pl, V, B, (long long)W * H, lr, false).bytes +
+    *bytes = carve(nullptr, pl, V, B, (long long)W * H, lr, false, W, H).bytes +
              sv::align_up((size_t)B * W * H * sizeof(double));
     return SV_OK;
 }
@@ -3428,7 +3685,7 @@ int sv_full_range(int mode, const void* left, const void* right, int dtype, int 
             return sv::set_error(SV_EUNSUPPORTED, "per-pair centres need the constant-centre "
                                  "kernels (integer u8 path, 2R + 1 <= 125, W >= R in lr mode)");
         pl.Kp = pl.G * pl.KC;
-        RefineBufs bf = carve(ws, pl, V, B, (long long)W * H, lr, false);
+        RefineBufs bf = carve(ws, pl, V, B, (long long)W * H, lr, false, W, H);
         const size_t need = bf.bytes + sv::align_up((size_t)n * sizeof(double));
         if ((rc = check_ws(need, ws, ws_bytes))) return rc;
         double* d0 = reinterpret_cast<double*>((char*)ws + bf.bytes);
@@ -3478,7 +3735,7 @@ int sv_refine_disparity_noalloc(const void* left, const void* right, int dtype,
     int rc = make_plan(W, H, R, P1, P2, 0.0, 0.0, 20.0, 4.0, 8, rx, ry, &pl);
     if (rc) return rc;
     pl.Kp = pl.G * pl.KC;
-    const size_t need = carve(nullptr, pl, 2, B, (long long)W * H, true, true).bytes;
+    const size_t need = carve(nullptr, pl, 2, B, (long long)W * H, true, true, W, H).bytes;
     if (need > kScratchBytes)
         return sv::set_error(SV_EWORKSPACE,
                              "sv_refine_disparity_noalloc: %zu bytes of workspace needed, the "
@@ -3507,7 +3764,7 @@ int sv_refine_ragged_workspace_bytes(int mode, int B, long long slot, int maxW, 
     if (rc) return rc;
     SV_REQUIRE(pl.integer, "ragged batches need integral P1, P2");
     pl.Kp = pl.G * pl.KC;
-    *bytes = carve(nullptr, pl, mode == 0 ? 1 : 2, B, slot, mode != 0, mode == 2).bytes;
+    *bytes = carve(nullptr, pl, mode == 0 ? 1 : 2, B, slot, mode != 0, mode == 2, maxW, maxH).bytes;
     return SV_OK;
 }
 
@@ -3535,7 +3792,7 @@ int sv_refine_ragged(int mode, const void* left, const void* right, int dtype, c
     if (rc) return rc;
     SV_REQUIRE(pl.integer, "ragged batches need integral P1, P2");
     pl.Kp = pl.G * pl.KC;
-    RefineBufs bf = carve(ws, pl, 1, B, slot, false, false);
+    RefineBufs bf = carve(ws, pl, 1, B, slot, false, false, maxW, maxH);
     if ((rc = check_ws(bf.bytes, ws, ws_bytes))) return rc;
     return refine_views(pl, bf, left, right, dtype, disp0, nullptr, 1, g, R, P1, P2, 0.0, 20.0, 0.0,
                         rx, ry, 0.0, 4.0, out, nullptr, st);

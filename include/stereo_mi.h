@@ -229,7 +229,14 @@ int sv_refine_ragged(int mode, const void* left, const void* right, int dtype, c
  * depth-column 3D kernel (4, 6, 12); 11: workgroup order of the two-lane
  * vertical/diagonal SGM kernel: in launch order, spread over the XCDs (1),
  * or one contiguous run of (columns, direction, pair) blocks per XCD, so a
- * pair's paths share that XCD's L2 (2).  Value 0 restores the built-in
+ * pair's paths share that XCD's L2 (2); 12: scheduling of the two-lane SGM
+ * kernels: one workgroup per block of 32 scanlines (1), or persistent
+ * workgroups, as many as fit the device at once, pulling (image,
+ * direction, scanline block) items from eight queues, one per XCD, built
+ * on the device before each launch with the longest scanlines first and the
+ * images dealt round-robin, a workgroup that empties its own queue going on
+ * to the others (2), or the same with one queue for the whole device (3).
+ * Value 0 restores the built-in
  * heuristic (LDS weights; horizontal paths after the others; matrix cores
  * for 4 to 16 output channels, 3D layers over depth columns of 6, or 4 when
  * 6 leaves CUs idle; the per-channel vector kernel when the input channels

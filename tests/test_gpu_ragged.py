@@ -235,7 +235,104 @@ def test_side_streams_of_two_callers_match_sequential():
             for s in streams:
                 cur.wait_stream(s)
             torch.cuda.synchronize()
-            for o, e in zip(outs, ref):
-                assert torch.equal(o, e)
+            for o, e in zip(outs, ref):  # the padding past each pair carries no result
+                assert all(torch.equal(u, v) for u, v in zip(b.unpack(o), b.unpack(e)))
+    finally:
+        A.reset()
+
+
+QSIZES = [(460, 370), (97, 64), (33, 17), (600, 90), (80, 500), (256, 180)]
+
+
+def _qpairs():
+    sv = package()
+    return [sv.synth.make_pair(w, h, 70 + i, dmin=2.0, dmax=min(30.0, w / 3.0), n_boxes=1)
+            for i, (w, h) in enumerate(QSIZES)]
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_sgm_work_queue_equals_grid(mode):
+    """Persistent SGM workgroups over the per-XCD work queues (knob
+    sgm_queue 2) give the grid launches' results bit for bit, under every
+    prefetch depth, horizontal tile and overlap setting."""
+    R, _ = _mods()
+    A = importlib.import_module("from-scratch-stereo-vision_amd.autotune")
+    ps = _qpairs()
+    b = R.RaggedBatch(QSIZES)
+    l = b.pack([p["left"] for p in ps])
+    r = b.pack([p["right"] for p in ps])
+    d0 = b.pack([p["disp0"].astype(np.float64) for p in ps], torch.float64)
+    try:
+        A.set_knob("sgm_split", 2)
+        A.set_knob("sgm_queue", 1)
+        ref, rst = R.refine_ragged(b, l, r, d0, mode=mode, R=10)
+        ref = ref.clone()
+        rst = rst.clone() if rst is not None else None
+        for pf in (2, 8):
+            for ht in (8, 16):
+                for ov in (1, 2):
+                    print("sgm queue mode=%d prefetch=%d htile=%d overlap=%d" % (mode, pf, ht, ov),
+                          flush=True)
+                    A.apply({"sgm_queue": 2, "sgm_prefetch": pf, "sgm_htile": ht,
+                             "sgm_overlap": ov})
+                    out, st = R.refine_ragged(b, l, r, d0, mode=mode, R=10)
+                    torch.cuda.synchronize()
+                    for n, (u, v) in enumerate(zip(b.unpack(out), b.unpack(ref))):
+                        assert torch.equal(u, v), (pf, ht, ov, n)
+                    if st is not None:
+                        for n, (u, v) in enumerate(zip(b.unpack(st), b.unpack(rst))):
+                            assert torch.equal(u, v), (pf, ht, ov, n)
+    finally:
+        A.reset()
+
+
+def test_sgm_work_queue_graph_replays():
+    """The queue build and the persistent kernels captured in a HIP graph:
+    every replay, with eager work between them, equals the eager run (the
+    queue heads are cleared by a kernel inside the graph on each replay)."""
+    print("sgm queue graph replays", flush=True)
+    R, _ = _mods()
+    A = importlib.import_module("from-scratch-stereo-vision_amd.autotune")
+    ps = _qpairs()
+    b = R.RaggedBatch(QSIZES)
+    l = b.pack([p["left"] for p in ps])
+    r = b.pack([p["right"] for p in ps])
+    d0 = b.pack([p["disp0"].astype(np.float64) for p in ps], torch.float64)
+    try:
+        A.apply({"sgm_split": 2, "sgm_queue": 1})
+        eager, est = R.refine_ragged(b, l, r, d0, mode=2)
+        eager, est = eager.clone(), est.clone()
+        A.set_knob("sgm_queue", 2)
+        cap = R.CapturedPipeline(lambda a, c: R.refine_ragged(b, a, c, d0, mode=2), l, r)
+        for k in range(3):
+            out, st = cap.replay()
+            torch.cuda.synchronize()
+            assert all(torch.equal(u, v) for u, v in zip(b.unpack(out), b.unpack(eager))), k
+            assert all(torch.equal(u, v) for u, v in zip(b.unpack(st), b.unpack(est))), k
+            R.refine_ragged(b, l, r, d0, mode=0)  # eager work between replays
+            torch.cuda.synchronize()
+    finally:
+        A.reset()
+
+
+def test_sgm_work_queue_uniform_batch():
+    """A uniform batch through the queue kernels equals the grid launches."""
+    print("sgm queue uniform batch", flush=True)
+    sv = package()
+    A = importlib.import_module("from-scratch-stereo-vision_amd.autotune")
+    ps = [sv.synth.band_pair(s) for s in range(4)]
+    W, H = ps[0]["W"], ps[0]["H"]
+    l = torch.stack([torch.as_tensor(p["left"]) for p in ps]).cuda()
+    r = torch.stack([torch.as_tensor(p["right"]) for p in ps]).cuda()
+    d0 = torch.stack([torch.as_tensor(p["disp0"].astype(np.float64)) for p in ps]).cuda()
+    try:
+        A.apply({"sgm_split": 2, "sgm_queue": 1})
+        ref, rst = sv.lr_refine(l, r, d0, W, H, tol=2.0, R=10, P1=12.0, P2=150.0, rx=4, ry=3)
+        ref, rst = ref.clone(), rst.clone()
+        A.set_knob("sgm_queue", 2)
+        out, st = sv.lr_refine(l, r, d0, W, H, tol=2.0, R=10, P1=12.0, P2=150.0, rx=4, ry=3)
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref)
+        assert torch.equal(st, rst)
     finally:
         A.reset()
