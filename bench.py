@@ -1590,6 +1590,39 @@ def ragged_fullrange(args, sv, b, l, r, pairs, spec):
             "kernels": "constant-centre packed kernels for K <= 125, the general band kernels above"}
 
 
+def ragged_fallback_sweep(args, RG, b, l, r, cap, pairs, spec):
+    """Net + band with full-range fallback (RG.fallback_ragged) on the
+    pairs whose share of occluded or mismatched pixels exceeds tau, whole
+    pairs replaced, each pair's full range from its declared maximum
+    disparity: bad2 and time per step (graph replay of net + band, then the
+    host's choice of pairs and their full-range calls) for a range of tau."""
+    import numpy as np
+    import torch
+    md = [d for _, _, d in spec]
+    res = []
+    for tau in (1.01, 0.3, 0.2, 0.15, 0.1, 0.05, -1.0):
+        best = None
+        for _ in range(2):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            out, st = cap.replay()
+            picked = RG.fallback_ragged(b, l, r, out, st, md, tau=tau)
+            torch.cuda.synchronize()
+            t = time.perf_counter() - t0
+            best = t if best is None or t < best else best
+        outs = [o.cpu().numpy() for o in b.unpack(out)]
+        bad = [float((np.abs(o - p["disp"]) > 2.0)[p["valid"]].mean() * 100.0)
+               for o, p in zip(outs, pairs)]
+        res.append({"tau": tau, "fallback_pairs": len(picked),
+                    "fallback_px_frac": round(sum(spec[n][0] * spec[n][1] for n in picked) /
+                                              float(sum(w * h for w, h, _ in spec)), 4),
+                    "ms_per_step": round(best * 1e3, 3), "pairs_per_s": round(64 / best, 2),
+                    "bad2_pct_mean": round(float(np.mean(bad)), 3)})
+        log("fallback tau=%.2f: %d pairs, %.1f ms, bad2 %.2f%%" % (
+            tau, len(picked), best * 1e3, res[-1]["bad2_pct_mean"]))
+    return res
+
+
 def ragged_main(args):
     """64 synthetic pairs of uneven size (200x150 up to 1390x1110, maximum
     disparity 16-240 px) through net + band SGM as one ragged batch: the
@@ -1620,10 +1653,16 @@ def ragged_main(args):
         return b.unpack(o) + b.unpack(s_)
 
     tuned = None
-    if not args.no_autotune:
+    if args.tune_file and os.path.exists(args.tune_file):
+        tuned = A.load_choices(net, args.tune_file)
+        log("kernel variants from %s" % args.tune_file)
+    elif not args.no_autotune:
         layers = A.tune_layers(net, run, reps=2)
         tuned = A.tune(run, knobs=["sgm_split", "sgm_prefetch", "sgm_htile", "sgm_overlap", "wta_reads", "sgm_order", "sgm_queue"], reps=2, log=log)
         tuned["conv_layers_tuned"] = len(layers)
+        if args.tune_file:
+            A.save_choices(net, {k: v for k, v in tuned.items() if k in A.KNOBS}, args.tune_file)
+            log("kernel variants written to %s" % args.tune_file)
     # eager reference run (stage profile of the refinement) and capture
     L.lib().sv_profile_enable(1)
     eout, est, eres = RG.net_band_ragged(net, b, l, r)
@@ -1654,6 +1693,7 @@ def ragged_main(args):
     outs = [o.cpu().numpy() for o in b.unpack(out)]
     bad = [float((np.abs(o - p["disp"]) > 2.0)[p["valid"]].mean() * 100.0) for o, p in zip(outs, pairs)]
     fr = None if args.no_fullrange else ragged_fullrange(args, sv, b, l, r, pairs, spec)
+    sweep = None if args.no_fullrange else ragged_fallback_sweep(args, RG, b, l, r, cap, pairs, spec)
     px = sum(w * h for w, h, _ in spec)
     K = 2 * R_BAND + 1
     canon = sum(2 * w * h * K * (1 + 8 * 5 + 2) + 2 * w * h * 10 + w * h for w, h, _ in spec)
@@ -1677,6 +1717,7 @@ def ragged_main(args):
         "stage_ms_eager": stages,
         "fullrange": fr,
         "speedup_vs_fullrange": round(fr["ms_per_step"] / (dt * 1e3), 3) if fr else None,
+        "fallback_sweep": sweep,
         "roofline": {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
                      "pipeline_canonical_bytes": canon,
                      "pipeline_achieved": round(canon / dt / 1e9, 1),

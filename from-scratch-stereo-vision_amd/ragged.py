@@ -125,6 +125,38 @@ def net_band_ragged(net, batch, left, right, R=10, P1=12.0, P2=150.0, rx=4, ry=3
     return out, st, res
 
 
+def fallback_ragged(batch, left, right, out, status, max_disps, tau=0.1, statuses=(1, 2),
+                    whole=True, P1=12.0, P2=150.0, rx=4, ry=3):
+    """Full-range fallback after net + band SGM over a ragged batch: the
+    pairs whose share of pixels with a status in `statuses` (1 occluded, 2
+    left-right mismatch, from refine_ragged) exceeds tau are solved again
+    with full-range SGM (each pair's R from its maximum disparity,
+    eval_real.py:143-146), one call per pair since their sizes differ.
+    whole=True replaces the pair's disparity map, else only its flagged
+    pixels.  out is updated in place; returns the indices of the pairs that
+    fell back.  The reference has no counterpart: it fills inconsistent
+    pixels from disp0 and its neighbours (stereonet.py:5372-5390)."""
+    from . import refine as RF
+    if len(max_disps) != batch.B:
+        raise ValueError("%d maximum disparities for %d pairs" % (len(max_disps), batch.B))
+    flags = [torch.zeros_like(s_, dtype=torch.bool) for s_ in batch.unpack(status)]
+    for f, s_ in zip(flags, batch.unpack(status)):
+        for v in statuses:
+            f |= s_ == v
+    frac = torch.stack([f.float().mean() for f in flags]).cpu().numpy()
+    picked = [n for n in range(batch.B) if frac[n] > tau]
+    ls, rs, os_ = batch.unpack(left), batch.unpack(right), batch.unpack(out)
+    for n in picked:
+        w, h = batch.sizes[n]
+        R = int(float(max_disps[n]) + 2) // 2 + 1
+        fr = RF.full_range(ls[n], rs[n], w, h, R, P1=P1, P2=P2, rx=rx, ry=ry).reshape(h, w)
+        if whole:
+            os_[n].copy_(fr)
+        else:
+            os_[n][flags[n]] = fr[flags[n]]
+    return picked
+
+
 class CapturedPipeline:
     """fn(*static inputs) captured once in a HIP graph (torch.cuda.CUDAGraph)
     after `warmup` eager runs; replay() reruns the whole recorded launch

@@ -336,3 +336,32 @@ def test_sgm_work_queue_uniform_batch():
         assert torch.equal(st, rst)
     finally:
         A.reset()
+
+
+def test_fallback_ragged_whole_and_flagged_pixels():
+    """fallback_ragged: tau above 1 changes nothing; tau below 0 replaces
+    every pair by its full-range solution; flagged-pixel mode takes the
+    full-range value exactly on the occluded/mismatched pixels."""
+    print("fallback_ragged", flush=True)
+    R, _ = _mods()
+    sv = package()
+    ps = _pairs()
+    b = R.RaggedBatch(SIZES)
+    l = b.pack([p["left"] for p in ps])
+    r = b.pack([p["right"] for p in ps])
+    d0 = b.pack([p["disp0"].astype(np.float64) for p in ps], torch.float64)
+    md = [36.0] * len(SIZES)
+    out, st = R.refine_ragged(b, l, r, d0, mode=2)
+    base = [t.clone() for t in b.unpack(out)]
+    assert R.fallback_ragged(b, l, r, out, st, md, tau=1.01) == []
+    assert all(torch.equal(u, v) for u, v in zip(b.unpack(out), base))
+    fulls = [sv.full_range(li, ri, w, h, 20).reshape(h, w)
+             for li, ri, (w, h) in zip(b.unpack(l), b.unpack(r), SIZES)]
+    part = out.clone()
+    picked = R.fallback_ragged(b, l, r, part, st, md, tau=-1.0, whole=False)
+    assert picked == list(range(len(SIZES)))
+    for o, bs, s_, f in zip(b.unpack(part), base, b.unpack(st), fulls):
+        flag = (s_ == 1) | (s_ == 2)
+        assert torch.equal(o[flag], f[flag]) and torch.equal(o[~flag], bs[~flag])
+    R.fallback_ragged(b, l, r, out, st, md, tau=-1.0)
+    assert all(torch.equal(u, f) for u, f in zip(b.unpack(out), fulls))
