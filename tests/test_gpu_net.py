@@ -427,6 +427,49 @@ def test_conv_variants_bit_identical_and_match_float64(sv, N, case):
 
 
 @pytest.mark.parametrize("wide", [False, True])
+def test_bf16_mfma_error_vs_float64_at_460x370(sv, N, wide):
+    """The BF16 matrix-core network against the float64 oracle at the
+    headline size: predict_at_scale of a 460x370 pipeline pair at working
+    width 208 (predict_auto's usual second pass there), v5 and the widened
+    net; FP32 stays within 2e-3 px, BF16's mean error and EPE shift are
+    stated and bounded."""
+    print("bf16 vs float64 at 460x370, wide=%s" % wide, flush=True)
+    net = _v5(N)
+    if wide:
+        net = N.widen(net, 48, 24, 32)
+        g = np.random.default_rng(1)
+        host = net.params.detach().cpu().double().numpy()
+        for c in net.convs():
+            w = host[c.w_off:c.b_off].reshape(c.oc, c.ic, c.kd * 9)
+            old_ic = {48: 12, 24: 6, 32: 8}.get(c.ic, c.ic)
+            w[:, old_ic:, :] += g.normal(size=w[:, old_ic:, :].shape) * 0.02
+        N.set_flat_params(net, host)
+    p = sv.synth.pipeline_pair(0)
+    W, H = p["W"], p["H"]
+    gl, gr = sv.synth.gray(p["left"]), sv.synth.gray(p["right"])
+    ref, _, Ww, Hw = oracle.predict_at_scale(_arch(net), _host_params(net), gl, gr, W, H, 208)
+    l = torch.from_numpy(gl.astype(np.float32)).cuda()
+    r = torch.from_numpy(gr.astype(np.float32)).cuda()
+    f32 = N.predict_at_scale(net, l, r, W, H, 208, verbose=False)[0].double().cpu().numpy().ravel()
+    net.use_bf16(True)
+    b16 = N.predict_at_scale(net, l, r, W, H, 208, verbose=False)[0].double().cpu().numpy().ravel()
+    net.use_bf16(False)
+    gt, v = np.asarray(p["disp"]).ravel(), np.asarray(p["valid"]).ravel()
+    e32, e16 = np.abs(f32 - ref), np.abs(b16 - ref)
+    epe = lambda d: float(np.abs(d - gt)[v].mean())
+    print("%s 460x370 at %dx%d: fp32 max/mean |err| %.2e/%.2e px; bf16 max/mean %.3f/%.4f px; "
+          "EPE fp64 %.4f fp32 %.4f bf16 %.4f" % ("widened" if wide else "v5", Ww, Hw, e32.max(),
+                                                 e32.mean(), e16.max(), e16.mean(), epe(ref),
+                                                 epe(f32), epe(b16)))
+    assert (Ww, Hw) == (208, 167)
+    assert e32.max() <= 2e-3
+    # measured: v5 0.037 px mean (EPE shift -0.005); the widened net with its
+    # perturbed new channels (EPE 11.9 px, disparities spread over the whole
+    # range) 0.072 px mean (EPE shift -0.002)
+    assert e16.mean() <= (0.1 if wide else 0.05) and abs(epe(b16) - epe(ref)) <= 0.05
+
+
+@pytest.mark.parametrize("wide", [False, True])
 def test_fused_launches_bit_identical(sv, N, wide):
     """The fused launches (pass input + guides; last cost-filter layer +
     soft-argmin; refinement upsampling + first convolution) give the
