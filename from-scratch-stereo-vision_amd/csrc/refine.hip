@@ -2730,7 +2730,9 @@ int sgm_k_t(const Plan& pl, const void* C, const int* base, void* L, int BV, con
         const bool split0 = sv::tune(SV_TUNE_SGM_SPLIT) == 2;
         const int qmode = sv::tune(SV_TUNE_SGM_QUEUE);
         const int nq = qmode == 3 ? 1 : 8;
-        const bool useq = queue && split0 && qmode >= 2 && BV < 32768 &&
+        // (the queue build ranks the images in O(BV^2) on one block: up to
+        // 4096 images, a few hundred microseconds at most)
+        const bool useq = queue && split0 && qmode >= 2 && BV <= 4096 &&
                           ((g.W > g.H ? g.W : g.H) + 31) / 32 * 8 <= 0xFFFF;
         const long long qints = sgm_queue_ints(BV, g.W, g.H);
         if (useq) {

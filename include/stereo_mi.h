@@ -235,7 +235,8 @@ int sv_refine_ragged(int mode, const void* left, const void* right, int dtype, c
  * direction, scanline block) items from eight queues, one per XCD, built
  * on the device before each launch with the longest scanlines first and the
  * images dealt round-robin, a workgroup that empties its own queue going on
- * to the others (2), or the same with one queue for the whole device (3).
+ * to the others (2), or the same with one queue for the whole device (3);
+ * batches of more than 4096 images (pairs x views) keep the grid.
  * Value 0 restores the built-in
  * heuristic (LDS weights; horizontal paths after the others; matrix cores
  * for 4 to 16 output channels, 3D layers over depth columns of 6, or 4 when
