@@ -285,18 +285,16 @@ __global__ void __launch_bounds__(256) k_census8(const uint8_t* __restrict__ img
 // band centre, stereonet.py:5254-5257: max(0, int(disp0 + 0.5)), int()
 // truncating toward zero.  Values beyond 2^30 (and NaN) put the whole band
 // outside the image, which is what the reference's huge centre does.
+__device__ __forceinline__ int band_centre(double d0) {
+    const double t = d0 + 0.5;
+    if (!(t < 1073741824.0)) return 1 << 30;
+    return t < 1.0 ? 0 : (int)t;
+}
+
 __global__ void k_base(const double* __restrict__ d0, long long n, int* __restrict__ base) {
     const long long i = blockIdx.x * 256LL + threadIdx.x;
     if (i >= n) return;
-    const double t = d0[i] + 0.5;
-    int v;
-    if (!(t < 1073741824.0))
-        v = 1 << 30;
-    else if (t < 1.0)
-        v = 0;
-    else
-        v = (int)t;
-    base[i] = v;
+    base[i] = band_centre(d0[i]);
 }
 
 // ------------------------------------------------------------- band costs
@@ -306,10 +304,13 @@ __global__ void k_base(const double* __restrict__ d0, long long n, int* __restri
 // KT > 0 fixes the band length (Kp = KT rounded up to 8): the partner words
 // are read from clamped columns, all KT loads in flight, and the candidates
 // outside the image take `invalid` afterwards.
+// d0 != nullptr: the band centres are formed here from d0 (k_base's rule)
+// and written to base for the later stages, instead of being read.
 template <bool RIGHT, int KT>
 __global__ void __launch_bounds__(256) k_costs_int(const uint64_t* __restrict__ cA,
                                                    const uint64_t* __restrict__ cB,
-                                                   const int* __restrict__ base, Geom g, int R,
+                                                   int* __restrict__ base,
+                                                   const double* __restrict__ d0, Geom g, int R,
                                                    int K, int Kp, int invalid,
                                                    uint8_t* __restrict__ C) {
     // each wave: 64 pixels of one row (blockDim.y rows per block) x Kp bytes,
@@ -342,7 +343,14 @@ __global__ void __launch_bounds__(256) k_costs_int(const uint64_t* __restrict__ 
         const int xc = live ? x : W - 1;
         const long long pix = row0 + xc;
         const uint64_t a = cA[pix];
-        const int b0 = base[pix] - R;
+        int bc;
+        if (d0) {
+            bc = band_centre(d0[pix]);
+            if (live) base[pix] = bc;
+        } else {
+            bc = base[pix];
+        }
+        const int b0 = bc - R;
         const uint64_t* rowB = cB + row0;
         const int e0 = RIGHT ? xc + b0 : xc - b0 - (KT - 1);
         int wlo = min(max(e0, 0), W - 1), whi = min(max(e0 + KT - 1, 0), W - 1);
@@ -395,7 +403,14 @@ __global__ void __launch_bounds__(256) k_costs_int(const uint64_t* __restrict__ 
     } else if (x < W) {
         const long long pix = row0 + x;
         const uint64_t a = cA[pix];
-        const int b0 = base[pix] - R;
+        int bc;
+        if (d0) {
+            bc = band_centre(d0[pix]);
+            base[pix] = bc;
+        } else {
+            bc = base[pix];
+        }
+        const int b0 = bc - R;
         const uint64_t* rowB = cB + row0;
         for (int k0 = 0; k0 < Kp; k0 += 4) {
             uint32_t packed = 0;
@@ -2546,8 +2561,11 @@ int costs_f64_t(bool right, const uint64_t* cA, const uint64_t* cB, const void* 
     return SV_OK;
 }
 
+// Integer path: the band centres are formed from `prior` (the centre
+// estimate) inside the costs kernel and written to base; the float path
+// reads base, which launch_base filled.
 int launch_costs(const Plan& pl, bool right, const uint64_t* cA, const uint64_t* cB,
-                 const void* gA, const void* gB, int dtype, const int* base, const double* prior,
+                 const void* gA, const void* gB, int dtype, int* base, const double* prior,
                  int B, const Geom& g, int R, double aw, double at, double pw, double pt, void* C,
                  hipStream_t st) {
     if (pl.integer) {
@@ -2559,16 +2577,16 @@ int launch_costs(const Plan& pl, bool right, const uint64_t* cA, const uint64_t*
         const size_t shm = (size_t)rb * 64 * pl.Kp + (k21 ? (size_t)rb * 256 * 8 : 0);
         if (right && k21)
             hipLaunchKernelGGL((k_costs_int<true, 21>), grid, dim3(64, rb), shm, st, cA, cB, base,
-                               g, R, pl.K, pl.Kp, (int)pl.invalid, (uint8_t*)C);
+                               prior, g, R, pl.K, pl.Kp, (int)pl.invalid, (uint8_t*)C);
         else if (k21)
             hipLaunchKernelGGL((k_costs_int<false, 21>), grid, dim3(64, rb), shm, st, cA, cB, base,
-                               g, R, pl.K, pl.Kp, (int)pl.invalid, (uint8_t*)C);
+                               prior, g, R, pl.K, pl.Kp, (int)pl.invalid, (uint8_t*)C);
         else if (right)
             hipLaunchKernelGGL((k_costs_int<true, 0>), grid, dim3(64, rb), shm, st, cA, cB, base,
-                               g, R, pl.K, pl.Kp, (int)pl.invalid, (uint8_t*)C);
+                               prior, g, R, pl.K, pl.Kp, (int)pl.invalid, (uint8_t*)C);
         else
             hipLaunchKernelGGL((k_costs_int<false, 0>), grid, dim3(64, rb), shm, st, cA, cB, base,
-                               g, R, pl.K, pl.Kp, (int)pl.invalid, (uint8_t*)C);
+                               prior, g, R, pl.K, pl.Kp, (int)pl.invalid, (uint8_t*)C);
         SV_CHECK_LAUNCH();
         return SV_OK;
     }
@@ -3006,7 +3024,7 @@ int refine_views(const Plan& pl, const RefineBufs& bf, const void* left, const v
     for (int v = 0; v < V; ++v) {
         const double* d0 = v == 0 ? disp0 : d0r;
         int* base = bf.base + (size_t)v * B * hw;
-        if ((rc = launch_base(d0, (long long)B * hw, base, st))) return rc;
+        if (!pl.integer && (rc = launch_base(d0, (long long)B * hw, base, st))) return rc;
         void* C = (char*)bf.C + (size_t)v * B * hw * pl.Kp * csz;
         if (v == 0)
             rc = launch_costs(pl, false, cL, cR, left, right, dtype, base, d0, B, g, R, aw, at, pw, pt, C, st);
