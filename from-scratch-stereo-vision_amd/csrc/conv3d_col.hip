@@ -48,16 +48,43 @@ namespace {
 
 typedef float v4f32 __attribute__((ext_vector_type(4)));
 
+// LDS row pitches: with 16-column tiles a wave's 64 lanes read 4 rows of 16,
+// and a half-wave's two rows hit the same LDS banks unless their pitch is 16
+// (mod 32) words apart; the cost-volume layer (SRC == 1), whose staging is
+// small, pads its rows to 48 words for that.  Other shapes keep packed rows
+// (a half-wave reads within one row).
 template <int SRC, int TW, int DC>
 struct ColGeom {
     static constexpr int GR = 64 / TW;                // rows of a 64-pixel group
     static constexpr int TT_H = 4 * GR;               // tile rows (one group per wave)
     static constexpr int IHt = TT_H + 2, IWt = TW + 2;  // input halo tile of one plane
-    static constexpr int PLANE = IHt * IWt;
+    static constexpr int PLANE = IHt * IWt;           // staged values of one plane
     static constexpr int FW = IWt + DC + 1;           // right-feature tile columns (SRC == 1)
-    static constexpr int NE = SRC == 1 ? PLANE + IHt * FW : (DC + 2) * PLANE;  // staged floats
+    static constexpr bool PAD = SRC == 1 && TW == 16;
+    static constexpr int IP = PAD ? 48 : IWt;         // LDS row pitch of a plane
+    static constexpr int FP = PAD ? 48 : FW;          // LDS row pitch of the right features
+    static constexpr int PLANEP = IHt * IP;           // LDS words of a plane
+    static constexpr int NE = SRC == 1 ? PLANE + IHt * FW : (DC + 2) * PLANE;  // staged values
+    static constexpr int NL = SRC == 1 ? PLANEP + IHt * FP : (DC + 2) * PLANEP;  // LDS words
     static constexpr int KE = (NE + 255) / 256;       // per thread
-    static constexpr int BUF = (NE + 3) / 4 * 4;      // one LDS buffer (floats)
+    static constexpr int BUF = (NL + 3) / 4 * 4;      // one LDS buffer (floats)
+    static_assert(!PAD || (IWt <= IP && FW <= FP), "padded pitches hold the rows");
+    // LDS word of staged value idx
+    __device__ __forceinline__ static int at(int idx) {
+        if constexpr (!PAD) {
+            return idx;
+        } else if (SRC == 1) {
+            if (idx < PLANE) {
+                const int yy = idx / IWt;
+                return yy * IP + (idx - yy * IWt);
+            }
+            const int j = idx - PLANE, yy = j / FW;
+            return PLANEP + yy * FP + (j - yy * FW);
+        } else {
+            const int e = idx / PLANE, pp = idx - e * PLANE, yy = pp / IWt;
+            return e * PLANEP + yy * IP + (pp - yy * IWt);
+        }
+    }
 };
 
 // Global loads of input channel ic for this tile into registers (zero
@@ -104,7 +131,7 @@ __device__ __forceinline__ void col_store(float* buf, const float* v, int tid) {
 #pragma unroll
     for (int k = 0; k < G::KE; ++k) {
         const int idx = tid + 256 * k;
-        if (idx < G::NE) buf[idx] = v[k];
+        if (idx < G::NE) buf[G::at(idx)] = v[k];
     }
 }
 
@@ -191,7 +218,7 @@ __global__ void __launch_bounds__(256) k_conv3_col(
             float flv[9];
             if (SRC == 1) {
 #pragma unroll
-                for (int tp = 0; tp < 9; ++tp) flv[tp] = cb[(yy0 + tp / 3) * G::IWt + lc + tp % 3];
+                for (int tp = 0; tp < 9; ++tp) flv[tp] = cb[(yy0 + tp / 3) * G::IP + lc + tp % 3];
             }
             // input planes d' = od0 - 1 + e, e = 0 .. DC + 1
             auto plane = [&](auto ec) {
@@ -203,12 +230,12 @@ __global__ void __launch_bounds__(256) k_conv3_col(
                     constexpr int ky = t9 / 3, kx = t9 % 3;
                     float v;
                     if (SRC == 1) {
-                        const float fr = cb[G::PLANE + (yy0 + ky) * G::FW + lc + kx + DC + 1 - e];
+                        const float fr = cb[G::PLANEP + (yy0 + ky) * G::FP + lc + kx + DC + 1 - e];
                         v = fabsf(flv[t9] - fr);
                         v = xq[t9] < e ? 2.0f : v;
                         v = inb[t9] ? v : 0.f;
                     } else {
-                        v = cb[e * G::PLANE + (yy0 + ky) * G::IWt + lc + kx];
+                        v = cb[e * G::PLANEP + (yy0 + ky) * G::IP + lc + kx];
                     }
                     // output depths od0 + o with depth tap kd = e - o
                     auto out = [&](auto oc_) {
