@@ -137,6 +137,8 @@ def fallback_ragged(batch, left, right, out, status, max_disps, tau=0.1, statuse
     fell back.  The reference has no counterpart: it fills inconsistent
     pixels from disp0 and its neighbours (stereonet.py:5372-5390)."""
     from . import refine as RF
+    if status is None:
+        raise ValueError("fallback_ragged needs the statuses of refine_ragged mode 1 or 2")
     if len(max_disps) != batch.B:
         raise ValueError("%d maximum disparities for %d pairs" % (len(max_disps), batch.B))
     flags = [torch.zeros_like(s_, dtype=torch.bool) for s_ in batch.unpack(status)]
