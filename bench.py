@@ -141,7 +141,7 @@ def main():
         return cpu_worker(args.cpu_worker)
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(pairs are independent; see DESIGN.md for how it would shard)", file=sys.stderr)
+              "(pairs are independent: several GPUs would each take their own share of a batch)", file=sys.stderr)
         sys.exit(2)
     if args.workload == "ragged64":
         return ragged_main(args)
