@@ -140,7 +140,7 @@ def main():
     if args.cpu_worker:
         return cpu_worker(args.cpu_worker)
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
+        print("bench.py measures one GPU: the library has no multi-GPU path "
               "(pairs are independent: several GPUs would each take their own share of a batch)", file=sys.stderr)
         sys.exit(2)
     if args.workload == "ragged64":
