@@ -1482,6 +1482,11 @@ __global__ void __launch_bounds__(256) k_sgm_queue(Geom g, int BV, int ndirV, in
     }
 }
 
+// The XCD this workgroup runs on (hardware register XCC_ID, bits 0-3).
+__device__ __forceinline__ int xcd_id() {
+    return __builtin_amdgcn_s_getreg((3 << 11) | 20) & 7;
+}
+
 // The next item of queue k for this workgroup (one wave), -1 when empty.
 __device__ __forceinline__ int sgm_q_pull(int* __restrict__ q, int k) {
     int i = 0;
@@ -1499,8 +1504,9 @@ __global__ void __launch_bounds__(64) k_sgm_p2q(const uint8_t* __restrict__ C,
                                                 int* __restrict__ q, int BV, int ndir, int nq) {
     __shared__ uint32_t lds[32 * PackedBand2<K>::SD];
     const int* items = q + SGM_Q_HDR + 2 * BV;
+    const int home = nq == 8 ? xcd_id() : 0;
     for (int k = 0; k < nq; ++k) {
-        const int qk = (blockIdx.x + k) % nq;
+        const int qk = (home + k) % nq;
         for (;;) {
             const int i = sgm_q_pull(q, qk);
             if (i < 0) break;
@@ -1680,8 +1686,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
                                                  int* __restrict__ q, int BV, int ndir, int nq) {
     __shared__ Ph2Lds<K, TC, CB> sh;
     const int* items = q + SGM_Q_HDR + 2 * BV;
+    const int home = nq == 8 ? xcd_id() : 0;
     for (int k = 0; k < nq; ++k) {
-        const int qk = (blockIdx.x + k) % nq;
+        const int qk = (home + k) % nq;
         for (;;) {
             const int i = sgm_q_pull(q, qk);
             if (i < 0) break;

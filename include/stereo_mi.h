@@ -232,7 +232,8 @@ int sv_refine_ragged(int mode, const void* left, const void* right, int dtype, c
  * pair's paths share that XCD's L2 (2); 12: scheduling of the two-lane SGM
  * kernels: one workgroup per block of 32 scanlines (1), or persistent
  * workgroups, as many as fit the device at once, pulling (image,
- * direction, scanline block) items from eight queues, one per XCD, built
+ * direction, scanline block) items from eight queues, one per XCD (a
+ * workgroup starts on the queue of the XCD it runs on), built
  * on the device before each launch with the longest scanlines first and the
  * images dealt round-robin, a workgroup that empties its own queue going on
  * to the others (2), or the same with one queue for the whole device (3);
