@@ -1417,11 +1417,12 @@ __global__ void __launch_bounds__(64) k_sgm_p2(const uint8_t* __restrict__ C,
 // its largest to smallest image, and a grid in launch order starts the
 // longest ones wherever they happen to fall.  k_sgm_queue orders the
 // (image, direction, scanline block) items of one SGM kernel longest
-// first, deals the images round-robin to eight queues (one per XCD, so
-// that an image's directions, which read the same cost rows, share an L2)
-// and clears the queue heads; the persistent kernels below then pull items
-// until every queue is empty, a workgroup emptying its own queue going on
-// to the others.  Queue layout (ints): [0, 8) heads, [8, 16) item counts,
+// first, deals the images in snake order to nq queues (nq = 8: one per XCD,
+// so that an image's directions, which read the same cost rows, share an
+// L2; nq = 1: one queue for the device) and clears the queue heads; the
+// persistent kernels below then pull items until every queue is empty, a
+// workgroup starting on its own XCD's queue and going on to the others.
+// Queue layout (ints): [0, 8) heads, [8, 16) item counts,
 // [16, 24) first item, [32, 32 + BV) image order, [32 + BV, 32 + 2 BV)
 // image offsets within their queue, then the items (image << 16 | j with
 // j = block * ndir + direction).
@@ -1432,6 +1433,13 @@ __device__ __forceinline__ int sgm_q_len(const Geom& g, int z, int horiz) {
 }
 __device__ __forceinline__ int sgm_q_items(const Geom& g, int z, int horiz, int ndir) {
     return ((horiz ? g.h(z) : g.w(z)) + 31) / 32 * ndir;
+}
+
+// Queue of the image at rank s: dealt in snake order (0..7, 7..0, ...), so
+// that the longest images of each round of eight do not all go to queue 0.
+__device__ __forceinline__ int sgm_q_of(int s, int nq) {
+    const int i = s % nq;
+    return (s / nq) & 1 ? nq - 1 - i : i;
 }
 
 // blockIdx.x: which queue of `qs` (0: vertical/diagonal kernel, 1:
@@ -1458,7 +1466,8 @@ __global__ void __launch_bounds__(256) k_sgm_queue(Geom g, int BV, int ndirV, in
     __shared__ int cnt[8];
     if (threadIdx.x < 8) {
         int acc = 0;
-        for (int s = threadIdx.x; s < BV && (int)threadIdx.x < nq; s += nq) {
+        for (int s = 0; s < BV && (int)threadIdx.x < nq; ++s) {
+            if (sgm_q_of(s, nq) != (int)threadIdx.x) continue;
             const int z = order[s];
             off[z] = acc;
             acc += sgm_q_items(g, z, horiz, ndir);
@@ -1477,7 +1486,7 @@ __global__ void __launch_bounds__(256) k_sgm_queue(Geom g, int BV, int ndirV, in
     for (int s = threadIdx.x; s < BV; s += blockDim.x) {
         const int z = order[s];
         const int n = sgm_q_items(g, z, horiz, ndir);
-        int* dst = items + q[16 + s % nq] + off[z];
+        int* dst = items + q[16 + sgm_q_of(s, nq)] + off[z];
         for (int j = 0; j < n; ++j) dst[j] = z << 16 | j;
     }
 }
